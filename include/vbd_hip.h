@@ -1,0 +1,180 @@
+/*
+ * vbd_hip.h -- C ABI of libvbd_hip.so, the MI355X (gfx950) implementation of
+ * the particle Vertex Block Descent solver step of Newton's SolverVBD.
+ *
+ * Every pointer named in the descriptors is a DEVICE pointer unless its name
+ * ends in _host.  Arrays use the reference's layouts: vec3 = 3 contiguous
+ * float32 (AoS), mat22/mat33 row-major, indices int32.  No call allocates
+ * device memory after vbd_create / vbd_collide_create, none synchronises the
+ * stream, and every kernel is enqueued on the stream passed in, so a whole
+ * substep can be captured into a HIP graph.  Functions return 0 on success or
+ * a negative VBD_ERR_* code; vbd_last_error() returns the message of the
+ * calling thread's last failure.  Nothing throws across this ABI.
+ *
+ * Reference interfaces replaced (CSYSLAM/newton, paths relative to the
+ * repository root):
+ *   SolverVBD.__init__ particle part  newton/_src/solvers/vbd/solver_vbd.py:221-618 -> vbd_create
+ *   SolverVBD.step                    newton/_src/solvers/vbd/solver_vbd.py:1773-1825 -> vbd_step
+ *   _initialize_particles/forward_step solver_vbd.py:2065-2100, particle_vbd_kernels.py:1350-1375 -> vbd_forward
+ *   init_body_particle_contacts        rigid_vbd_kernels.py:3276-3321 (via solver_vbd.py:2493-2514) -> vbd_contact_prepare
+ *   update_duals_body_particle_contacts rigid_vbd_kernels.py:4861-4926 -> vbd_iteration
+ *   _solve_particle_iteration (one color: accumulate_particle_body_contact_force_and_hessian
+ *     + solve_elasticity(_tile) + apply_truncation_ts) solver_vbd.py:2552-2702 -> vbd_color_pass
+ *   _finalize_particles/update_velocity solver_vbd.py:3158-3169, particle_vbd_kernels.py:1468-1471 -> vbd_finalize
+ *   create_soft_contacts (particle vs static plane/sphere) newton/_src/sim/collide.py:1553-1592,
+ *     newton/_src/geometry/kernels.py:1020-1184 -> vbd_collide_generate
+ */
+#ifndef VBD_HIP_H
+#define VBD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VBD_OK 0
+#define VBD_ERR_ARG -1
+#define VBD_ERR_HIP -2
+#define VBD_ERR_STATE -3
+
+typedef struct VbdContext VbdContext;
+typedef struct VbdCollide VbdCollide;
+
+/* Static model arrays (Model fields of newton/_src/sim/model.py:745-1019). */
+typedef struct VbdModelDesc {
+    int32_t particle_count;
+    int32_t tri_count;
+    int32_t edge_count;
+    int32_t tet_count;
+    const float* particle_mass;
+    const float* particle_inv_mass;
+    const float* particle_radius;
+    const int32_t* particle_flags;
+    const int32_t* tri_indices;            /* [T,3] */
+    const float* tri_poses;                /* [T,2,2] */
+    const float* tri_materials;            /* [T,5] */
+    const float* tri_areas;                /* [T] */
+    const int32_t* edge_indices;           /* [E,4] (o0,o1,v0,v1) */
+    const float* edge_rest_angle;          /* [E] */
+    const float* edge_rest_length;         /* [E] */
+    const float* edge_bending_properties;  /* [E,2] */
+    const int32_t* tet_indices;            /* [K,4] */
+    const float* tet_poses;                /* [K,3,3] */
+    const float* tet_materials;            /* [K,3] */
+    const int32_t* v_adj_tris;             /* CSR of (element, slot) pairs */
+    const int32_t* v_adj_tris_offsets;     /* [N+1] (counts ints, 2 per entry) */
+    const int32_t* v_adj_edges;
+    const int32_t* v_adj_edges_offsets;
+    const int32_t* v_adj_tets;
+    const int32_t* v_adj_tets_offsets;
+    int32_t color_count;
+    const int32_t* color_offsets_host;     /* [C+1] host */
+    const int32_t* color_particles;        /* concatenated color groups */
+    int32_t batch_count;                   /* color-pass tile schedule */
+    const int32_t* batch_color_offsets_host; /* [C+1] host: batches of color c */
+    const int32_t* batch_start;            /* [B+1]: first index into color_particles */
+    const float* gravity;                  /* vec3; world 0 is used for every particle */
+    int32_t shape_count;
+    const float* shape_margin;             /* [S] */
+    const float* shape_material_ke;
+    const float* shape_material_kd;
+    const float* shape_material_mu;
+    float soft_contact_ke;
+    float soft_contact_kd;
+    float soft_contact_mu;
+} VbdModelDesc;
+
+typedef struct VbdSolverParams {
+    int32_t iterations;
+    float friction_epsilon;
+    float avbd_beta;            /* rigid_avbd_linear_beta (0: penalty fixed at the averaged ke) */
+    float k_start;              /* < 0: penalty seeded at the averaged ke */
+    int32_t fp64_ill_conditioned; /* 1: blocks with Frobenius condition > 1e6 are re-solved in FP64 */
+    int32_t soft_contact_capacity; /* largest soft_contact_max the solver will be given */
+} VbdSolverParams;
+
+/* Contacts.soft_contact_* (newton/_src/sim/contacts.py:327-371). */
+typedef struct VbdContactsDesc {
+    const int32_t* soft_contact_count;  /* [1] device */
+    int32_t soft_contact_max;
+    const int32_t* soft_contact_indices; /* [S,3] (p,-1,-1) particle records */
+    const int32_t* soft_contact_shape;
+    const float* soft_contact_body_pos;
+    const float* soft_contact_body_vel;
+    const float* soft_contact_normal;
+} VbdContactsDesc;
+
+/* Writable soft-contact outputs of the collision step. */
+typedef struct VbdContactsOut {
+    int32_t* soft_contact_count;        /* [1] */
+    int32_t soft_contact_max;
+    int32_t* soft_contact_particle;     /* [S] */
+    int32_t* soft_contact_indices;      /* [S,3] */
+    float* soft_contact_barycentric;    /* [S,3] */
+    int32_t* soft_contact_shape;        /* [S] */
+    float* soft_contact_body_pos;       /* [S,3] */
+    float* soft_contact_body_vel;       /* [S,3] */
+    float* soft_contact_normal;         /* [S,3] */
+} VbdContactsOut;
+
+/* Particle and static-shape data for soft-contact generation. */
+typedef struct VbdCollideDesc {
+    int32_t particle_count;
+    int32_t pair_count;
+    const int32_t* pairs;               /* [P,2] (particle, shape), world-compatible order */
+    const float* particle_radius;
+    const int32_t* particle_flags;
+    const int32_t* particle_world;
+    int32_t shape_count;
+    const int32_t* shape_type;          /* 1 plane, 3 sphere */
+    const int32_t* shape_flags;
+    const int32_t* shape_world;
+    const float* shape_transform;       /* [S,7] (p, q xyzw) */
+    const float* shape_scale;           /* [S,3] */
+    const float* shape_margin;          /* [S] */
+} VbdCollideDesc;
+
+const char* vbd_last_error(void);
+int32_t vbd_version(void);
+
+/* SolverVBD.__init__ (particle part): stores the model pointers (the caller
+ * keeps them alive), allocates every scratch buffer. */
+int32_t vbd_create(const VbdModelDesc* model, const VbdSolverParams* params, VbdContext** out);
+int32_t vbd_destroy(VbdContext* ctx);
+
+/* Full substep: contact prepare, forward, iterations x colors, finalize.
+ * q_in is overwritten with the iterate (as the reference does); q_out/qd_out
+ * receive the result.  contacts may be NULL (no soft contacts). */
+int32_t vbd_step(VbdContext* ctx, void* stream, float* q_in, const float* qd_in, const float* f_ext,
+                 float* q_out, float* qd_out, const VbdContactsDesc* contacts, float dt);
+
+/* The pieces of vbd_step, for callers that interleave their own work. */
+int32_t vbd_contact_prepare(VbdContext* ctx, void* stream, const VbdContactsDesc* contacts);
+int32_t vbd_forward(VbdContext* ctx, void* stream, float* q, const float* qd, const float* f_ext, float dt);
+int32_t vbd_iteration(VbdContext* ctx, void* stream, float* q, float dt);
+int32_t vbd_color_pass(VbdContext* ctx, void* stream, int32_t color, float* q, float dt);
+int32_t vbd_finalize(VbdContext* ctx, void* stream, const float* q, float* q_out, float* qd_out, float dt);
+
+/* Incremental-potential force residual at q: g_i = m_i/h^2 (y_i - q_i) +
+ * elastic + damping + contact forces (the force the block solve assembles),
+ * zero for inactive particles.  g_out (optional) receives [N,3]; norm_sq_out
+ * (device, 1 double) receives sum |g_i|^2.  Uses the inertia and contact
+ * state of the current substep (call after vbd_forward / vbd_contact_prepare). */
+int32_t vbd_residual(VbdContext* ctx, void* stream, const float* q, float dt, float* g_out, double* norm_sq_out);
+
+/* Copies the solver's scratch state of the current substep ([N,3] floats)
+ * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */
+int32_t vbd_copy_scratch(VbdContext* ctx, void* stream, int32_t which, float* dst);
+
+/* Soft-contact generation (CollisionPipeline.collide, particle part). */
+int32_t vbd_collide_create(const VbdCollideDesc* desc, VbdCollide** out);
+int32_t vbd_collide_destroy(VbdCollide* col);
+int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* particle_q, float margin,
+                             const VbdContactsOut* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,231 @@
+"""ctypes bindings for the native libraries built by ``newton_amd.build``.
+
+``libvbd_hip.so`` is the product compute path; it is loaded lazily and any
+failure to find or load it raises -- there is no fallback implementation.
+The Structure layouts mirror ``include/vbd_hip.h`` field for field.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+_lock = threading.Lock()
+_hip = None
+_host = None
+
+c_int32 = ctypes.c_int32
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+
+class VbdModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", c_int32),
+        ("tri_count", c_int32),
+        ("edge_count", c_int32),
+        ("tet_count", c_int32),
+        ("particle_mass", c_void_p),
+        ("particle_inv_mass", c_void_p),
+        ("particle_radius", c_void_p),
+        ("particle_flags", c_void_p),
+        ("tri_indices", c_void_p),
+        ("tri_poses", c_void_p),
+        ("tri_materials", c_void_p),
+        ("tri_areas", c_void_p),
+        ("edge_indices", c_void_p),
+        ("edge_rest_angle", c_void_p),
+        ("edge_rest_length", c_void_p),
+        ("edge_bending_properties", c_void_p),
+        ("tet_indices", c_void_p),
+        ("tet_poses", c_void_p),
+        ("tet_materials", c_void_p),
+        ("v_adj_tris", c_void_p),
+        ("v_adj_tris_offsets", c_void_p),
+        ("v_adj_edges", c_void_p),
+        ("v_adj_edges_offsets", c_void_p),
+        ("v_adj_tets", c_void_p),
+        ("v_adj_tets_offsets", c_void_p),
+        ("color_count", c_int32),
+        ("color_offsets_host", c_void_p),
+        ("color_particles", c_void_p),
+        ("batch_count", c_int32),
+        ("batch_color_offsets_host", c_void_p),
+        ("batch_start", c_void_p),
+        ("gravity", c_void_p),
+        ("shape_count", c_int32),
+        ("shape_margin", c_void_p),
+        ("shape_material_ke", c_void_p),
+        ("shape_material_kd", c_void_p),
+        ("shape_material_mu", c_void_p),
+        ("soft_contact_ke", c_float),
+        ("soft_contact_kd", c_float),
+        ("soft_contact_mu", c_float),
+    ]
+
+
+class VbdSolverParams(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_int32),
+        ("friction_epsilon", c_float),
+        ("avbd_beta", c_float),
+        ("k_start", c_float),
+        ("fp64_ill_conditioned", c_int32),
+        ("soft_contact_capacity", c_int32),
+    ]
+
+
+class VbdContactsDesc(ctypes.Structure):
+    _fields_ = [
+        ("soft_contact_count", c_void_p),
+        ("soft_contact_max", c_int32),
+        ("soft_contact_indices", c_void_p),
+        ("soft_contact_shape", c_void_p),
+        ("soft_contact_body_pos", c_void_p),
+        ("soft_contact_body_vel", c_void_p),
+        ("soft_contact_normal", c_void_p),
+    ]
+
+
+class VbdContactsOut(ctypes.Structure):
+    _fields_ = [
+        ("soft_contact_count", c_void_p),
+        ("soft_contact_max", c_int32),
+        ("soft_contact_particle", c_void_p),
+        ("soft_contact_indices", c_void_p),
+        ("soft_contact_barycentric", c_void_p),
+        ("soft_contact_shape", c_void_p),
+        ("soft_contact_body_pos", c_void_p),
+        ("soft_contact_body_vel", c_void_p),
+        ("soft_contact_normal", c_void_p),
+    ]
+
+
+class VbdCollideDesc(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", c_int32),
+        ("pair_count", c_int32),
+        ("pairs", c_void_p),
+        ("particle_radius", c_void_p),
+        ("particle_flags", c_void_p),
+        ("particle_world", c_void_p),
+        ("shape_count", c_int32),
+        ("shape_type", c_void_p),
+        ("shape_flags", c_void_p),
+        ("shape_world", c_void_p),
+        ("shape_transform", c_void_p),
+        ("shape_scale", c_void_p),
+        ("shape_margin", c_void_p),
+    ]
+
+
+# entry points of include/vbd_hip.h: name -> (restype, argtypes)
+HIP_EXPORTS = {
+    "vbd_last_error": (ctypes.c_char_p, []),
+    "vbd_version": (c_int32, []),
+    "vbd_create": (c_int32, [ctypes.POINTER(VbdModelDesc), ctypes.POINTER(VbdSolverParams), ctypes.POINTER(c_void_p)]),
+    "vbd_destroy": (c_int32, [c_void_p]),
+    "vbd_step": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc), c_float],
+    ),
+    "vbd_contact_prepare": (c_int32, [c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc)]),
+    "vbd_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float]),
+    "vbd_iteration": (c_int32, [c_void_p, c_void_p, c_void_p, c_float]),
+    "vbd_color_pass": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_float]),
+    "vbd_finalize": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float]),
+    "vbd_residual": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
+    "vbd_collide_create": (c_int32, [ctypes.POINTER(VbdCollideDesc), ctypes.POINTER(c_void_p)]),
+    "vbd_collide_destroy": (c_int32, [c_void_p]),
+    "vbd_collide_generate": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),
+}
+
+HOST_EXPORTS = {
+    "vbd_host_color_graph": (
+        c_int32,
+        [
+            c_int32,
+            ctypes.c_int64,
+            ctypes.POINTER(c_int32),
+            c_int32,
+            c_int32,
+            c_float,
+            ctypes.POINTER(c_int32),
+            ctypes.POINTER(c_float),
+        ],
+    ),
+    "vbd_host_build_batches": (
+        ctypes.c_int64,
+        [
+            c_int32,
+            ctypes.POINTER(c_int32),
+            ctypes.POINTER(c_int32),
+            ctypes.POINTER(c_int32),
+            c_int32,
+            c_int32,
+            ctypes.c_int64,
+            ctypes.POINTER(c_int32),
+            ctypes.POINTER(c_int32),
+        ],
+    ),
+}
+
+
+def _load(path: Path, exports: dict):
+    if not path.exists():
+        raise RuntimeError(
+            f"native library {path} is missing; build it with `python -m newton_amd.build` "
+            "(there is no non-native fallback)"
+        )
+    lib = ctypes.CDLL(str(path))
+    for name, (restype, argtypes) in exports.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib
+
+
+def hip_lib():
+    """The HIP compute library (raises if it is not built)."""
+    global _hip
+    with _lock:
+        if _hip is None:
+            _hip = _load(LIB_DIR / "libvbd_hip.so", HIP_EXPORTS)
+        return _hip
+
+
+def host_lib():
+    """The host model-construction library (raises if it is not built)."""
+    global _host
+    with _lock:
+        if _host is None:
+            _host = _load(LIB_DIR / "libvbd_host.so", HOST_EXPORTS)
+        return _host
+
+
+class VbdError(RuntimeError):
+    """A libvbd_hip call returned an error status."""
+
+
+def check(rc: int, what: str) -> None:
+    """Turns a non-zero libvbd_hip status into the reference's exception types."""
+    if rc == 0:
+        return
+    msg = hip_lib().vbd_last_error()
+    text = msg.decode() if msg else ""
+    if rc == -1:
+        raise ValueError(f"{what}: {text}")
+    if rc == -3:
+        raise RuntimeError(f"{what}: {text}")
+    raise VbdError(f"{what} failed ({rc}): {text}")
+
+
+def ptr(t) -> int | None:
+    """Device address of a torch tensor (None for None / empty tensors)."""
+    if t is None or t.numel() == 0:
+        return None
+    return t.data_ptr()

@@ -1,0 +1,1472 @@
+// libvbd_hip.so -- particle Vertex Block Descent step for MI355X (gfx950).
+//
+// Kernels:
+//   k_forward          inertial predictor + first truncation   (particle_vbd_kernels.py:1350-1375, :2266-2286)
+//   k_contact_*        soft-contact material averaging, penalty seed and a
+//                      per-particle contact list in contact-index order
+//                      (rigid_vbd_kernels.py:799-815, :3276-3321)
+//   k_dual_update      per-iteration penalty ramp   (rigid_vbd_kernels.py:4861-4926)
+//   k_solve_color      one color pass: element forces/Hessians gathered through
+//                      the CSR adjacency, contact forces, guarded 3x3 solve,
+//                      position update   (particle_vbd_kernels.py:2289-2401, :2568-2706)
+//   k_finalize         velocity update   (particle_vbd_kernels.py:1468-1471)
+//   k_collide_*        particle vs static plane/sphere soft contacts (geometry/kernels.py:1020-1184)
+//
+// Numerics: every FP32 expression follows the reference's CPU kernel order with
+// no FMA contraction (built with -ffp-contract=off), IEEE division and square
+// root, and the portable atan2 below, so results are bit-identical to the FP32
+// CPU oracle.  The per-vertex force is summed serially in CSR order (inertia,
+// triangles, bending edges, tetrahedra, then the separately summed contacts),
+// exactly as the reference's CPU solve does; the parallelism comes from
+// evaluating each vertex's incident elements on separate lanes.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vbd_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define VBD_HIP_TRY(expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(VBD_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_));      \
+    } while (0)
+
+#define VBD_LAUNCH_CHECK(name)                                                                     \
+    do {                                                                                           \
+        hipError_t e_ = hipGetLastError();                                                         \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(VBD_ERR_HIP, std::string("launch of " name " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;     // threads per color-pass workgroup (4 wavefronts)
+constexpr int kComp = 12;       // f (3) + H (9) per element contribution
+constexpr int kMaxV = kBlock / kComp;  // vertices per batch: one lane per (vertex, component)
+
+// ---------------------------------------------------------------------------
+// small vector algebra; each helper evaluates in the reference's order
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+struct M3 {
+    float m[3][3];
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 ld3(const float* __restrict__ p, int i) {
+    return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+}
+__device__ __forceinline__ void st3(float* __restrict__ p, int i, V3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 scl(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+__device__ __forceinline__ M3 mzero() {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = 0.0f;
+    return r;
+}
+__device__ __forceinline__ M3 madd(M3 a, M3 b) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[i][j] + b.m[i][j];
+    return r;
+}
+__device__ __forceinline__ M3 mscl(M3 a, float s) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = s * a.m[i][j];
+    return r;
+}
+__device__ __forceinline__ M3 outer(V3 a, V3 b) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = comp(a, i) * comp(b, j);
+    return r;
+}
+__device__ __forceinline__ V3 mvec(const M3& a, V3 v) {
+    return V3{a.m[0][0] * v.x + a.m[0][1] * v.y + a.m[0][2] * v.z, a.m[1][0] * v.x + a.m[1][1] * v.y + a.m[1][2] * v.z,
+              a.m[2][0] * v.x + a.m[2][1] * v.y + a.m[2][2] * v.z};
+}
+// transpose(a) * v
+__device__ __forceinline__ V3 mtvec(const M3& a, V3 v) {
+    return V3{a.m[0][0] * v.x + a.m[1][0] * v.y + a.m[2][0] * v.z, a.m[0][1] * v.x + a.m[1][1] * v.y + a.m[2][1] * v.z,
+              a.m[0][2] * v.x + a.m[1][2] * v.y + a.m[2][2] * v.z};
+}
+// determinant = row0 . (row1 x row2)
+__device__ __forceinline__ float mdet(const M3& a) {
+    V3 r0 = v3(a.m[0][0], a.m[0][1], a.m[0][2]);
+    V3 r1 = v3(a.m[1][0], a.m[1][1], a.m[1][2]);
+    V3 r2 = v3(a.m[2][0], a.m[2][1], a.m[2][2]);
+    return dot(r0, cross(r1, r2));
+}
+
+// Products with skew matrices.  skew(v) = [[0,-vz,vy],[vz,0,-vx],[-vy,vx,0]];
+// the reference multiplies the full 3x3, whose zero entries add signed zeros
+// only, so the two non-zero terms per entry are kept in the same order.
+// A * skew(v):
+__device__ __forceinline__ M3 mul_skew_right(const M3& A, V3 v) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        r.m[i][0] = A.m[i][1] * v.z + A.m[i][2] * (-v.y);
+        r.m[i][1] = A.m[i][0] * (-v.z) + A.m[i][2] * v.x;
+        r.m[i][2] = A.m[i][0] * v.y + A.m[i][1] * (-v.x);
+    }
+    return r;
+}
+// skew(a) * B:
+__device__ __forceinline__ M3 mul_skew_left(V3 a, const M3& B) {
+    M3 r;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        r.m[0][j] = (-a.z) * B.m[1][j] + a.y * B.m[2][j];
+        r.m[1][j] = a.z * B.m[0][j] + (-a.x) * B.m[2][j];
+        r.m[2][j] = (-a.y) * B.m[0][j] + a.x * B.m[1][j];
+    }
+    return r;
+}
+
+// Portable FP32 atan2 (Cephes atanf reduction, S. L. Moshier), same steps as the oracle's.
+__device__ __forceinline__ float portable_atan2f(float y, float x) {
+    const float PI_F = 3.14159265358979323846f;
+    const float PIO2_F = 1.57079632679489661923f;
+    const float PIO4_F = 0.78539816339744830962f;
+    float ax = fabsf(x), ay = fabsf(y);
+    float r;
+    if (ax == 0.0f && ay == 0.0f) {
+        r = signbit(x) ? PI_F : 0.0f;
+    } else {
+        bool swap = ay > ax;
+        float t = swap ? (ax / ay) : (ay / ax);
+        float base = 0.0f;
+        if (t > 0.41421356237309504880f) {
+            base = PIO4_F;
+            t = (t - 1.0f) / (t + 1.0f);
+        }
+        float z = t * t;
+        float poly = (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) *
+                         z * t +
+                     t;
+        r = base + poly;
+        if (swap) r = PIO2_F - r;
+        if (signbit(x)) r = PI_F - r;
+    }
+    return signbit(y) ? -r : r;
+}
+
+// ---------------------------------------------------------------------------
+// element force / Hessian contributions for one (vertex, element) pair
+// ---------------------------------------------------------------------------
+
+// Stable Neo-Hookean membrane, particle_vbd_kernels.py:502-632.
+__device__ void eval_membrane(V3 X0, V3 X1, V3 X2, const float* __restrict__ qp, int i0, int i1, int i2,
+                              const float* __restrict__ pose, float area, float mu, float lmbd, float damping, float dt,
+                              int v_order, V3& f_out, M3& h_out) {
+    V3 x0 = X0;
+    V3 x01 = sub(X1, x0);
+    V3 x02 = sub(X2, x0);
+    float D00 = pose[0], D01 = pose[1], D10 = pose[2], D11 = pose[3];
+    V3 f0 = add(scl(x01, D00), scl(x02, D10));
+    V3 f1 = add(scl(x01, D01), scl(x02, D11));
+    float f0f0 = dot(f0, f0), f1f1 = dot(f1, f1), f0f1 = dot(f0, f1);
+    float Jsq = f0f0 * f1f1 - f0f1 * f0f1;
+    Jsq = fmaxf(Jsq, 1.0e-20f);
+    float Js = sqrtf(Jsq);
+    float invJ = 1.0f / Js;
+    float mu_nh = mu;
+    float l_nh = lmbd + mu;
+    float l_safe = (l_nh < 0.0f ? -1.0f : 1.0f) * fmaxf(fabsf(l_nh), 1.0e-6f);
+    float alpha = 1.0f + mu_nh / l_safe;
+    V3 g0 = scl(sub(scl(f0, f1f1), scl(f1, f0f1)), invJ);
+    V3 g1 = scl(sub(scl(f1, f0f0), scl(f0, f0f1)), invJ);
+    float s = l_nh * (Js - alpha);
+    V3 P0 = add(scl(f0, mu_nh), scl(g0, s));
+    V3 P1 = add(scl(f1, mu_nh), scl(g1, s));
+    float m0 = (float)(v_order == 0), m1 = (float)(v_order == 1), m2 = (float)(v_order == 2);
+    float df0 = D00 * (m1 - m0) + D10 * (m2 - m0);
+    float df1 = D01 * (m1 - m0) + D11 * (m2 - m0);
+    V3 force = neg(add(scl(P0, df0), scl(P1, df1)));
+    float s_clamp = fmaxf(0.0f, s);
+    float r = s_clamp * invJ;
+    float c1 = l_nh - r;
+    float df0sq = df0 * df0, df1sq = df1 * df1;
+    V3 dJ = add(scl(g0, df0), scl(g1, df1));
+    V3 w = sub(scl(f1, df0), scl(f0, df1));
+    float I_coeff = mu_nh * (df0sq + df1sq) + r * (df0sq * f1f1 + df1sq * f0f0 - 2.0f * df0 * df1 * f0f1);
+    M3 hess;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            hess.m[i][j] = (I_coeff * (i == j ? 1.0f : 0.0f) + c1 * (comp(dJ, i) * comp(dJ, j))) -
+                           r * (comp(w, i) * comp(w, j));
+    if (damping > 0.0f) {
+        float inv_dt = 1.0f / dt;
+        V3 A0 = ld3(qp, i0), A1 = ld3(qp, i1), A2 = ld3(qp, i2);
+        V3 x01p = sub(A1, A0);
+        V3 x02p = sub(A2, A0);
+        V3 f0p = add(scl(x01p, D00), scl(x02p, D10));
+        V3 f1p = add(scl(x01p, D01), scl(x02p, D11));
+        float c00 = (f0f0 - dot(f0p, f0p)) * inv_dt;
+        float c01 = (f0f1 - dot(f0p, f1p)) * inv_dt;
+        float c11 = (f1f1 - dot(f1p, f1p)) * inv_dt;
+        V3 dc00 = scl(f0, 2.0f * df0);
+        V3 dc01 = add(scl(f1, df0), scl(f0, df1));
+        V3 dc11 = scl(f1, 2.0f * df1);
+        V3 f_damp = add(add(scl(dc00, c00), scl(dc01, 2.0f * c01)), scl(dc11, c11));
+        force = add(force, scl(f_damp, -damping));
+        float dscale = damping * inv_dt;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float sum = (comp(dc00, i) * comp(dc00, j) + 2.0f * (comp(dc01, i) * comp(dc01, j))) +
+                            comp(dc11, i) * comp(dc11, j);
+                hess.m[i][j] = hess.m[i][j] + dscale * sum;
+            }
+    }
+    f_out = scl(force, area);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) h_out.m[i][j] = area * hess.m[i][j];
+}
+
+// ((1/len) * (I - n n^T)) * (sign * skew(v)), where sign*skew(v) == skew(sign*v)
+__device__ __forceinline__ M3 nvd_skew(float len, V3 nh, V3 v) {
+    float s = 1.0f / len;
+    M3 P;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P.m[i][j] = s * ((i == j ? 1.0f : 0.0f) - comp(nh, i) * comp(nh, j));
+    return mul_skew_right(P, v);
+}
+
+// Dihedral-angle bending, particle_vbd_kernels.py:635-816.  Only the angle
+// derivative of the vertex being solved is formed (the reference forms all
+// four and selects one with 0/1 masks, which adds signed zeros only).
+__device__ void eval_bending(const float* __restrict__ q, const float* __restrict__ qp, const int* __restrict__ ei,
+                             float rest_angle, float rest_length, float stiffness, float damping, float dt, int v_order,
+                             V3& f_out, M3& h_out) {
+    f_out = v3(0.0f, 0.0f, 0.0f);
+    h_out = mzero();
+    int i0 = ei[0], i1 = ei[1], i2 = ei[2], i3 = ei[3];
+    if (i0 == -1 || i1 == -1) return;
+    const float eps = 1.0e-6f;
+    V3 x0 = ld3(q, i0), x1 = ld3(q, i1), x2 = ld3(q, i2), x3 = ld3(q, i3);
+    V3 x02 = sub(x2, x0), x03 = sub(x3, x0), x13 = sub(x3, x1), x12 = sub(x2, x1), e = sub(x3, x2);
+    V3 n1 = cross(x02, x03), n2 = cross(x13, x12);
+    float n1n = length(n1), n2n = length(n2), en = length(e);
+    if (n1n < eps || n2n < eps || en < eps) return;
+    V3 n1h = divs(n1, n1n), n2h = divs(n2, n2n), eh = divs(e, en);
+    float sin_t = dot(cross(n1h, n2h), eh);
+    float cos_t = dot(n1h, n2h);
+    float theta = portable_atan2f(sin_t, cos_t);
+    float k = stiffness * rest_length;
+    float dE = k * (theta - rest_angle);
+    V3 dsin, dcos;
+    if (v_order == 0) {
+        M3 dn1 = nvd_skew(n1n, n1h, e);
+        M3 t2 = mul_skew_left(n2h, dn1);
+        M3 Mm;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Mm.m[i][j] = -t2.m[i][j];
+        dsin = mtvec(Mm, eh);
+        dcos = mtvec(dn1, n2h);
+    } else if (v_order == 1) {
+        M3 dn2 = nvd_skew(n2n, n2h, neg(e));
+        M3 Mm = mul_skew_left(n1h, dn2);
+        dsin = mtvec(Mm, eh);
+        dcos = mtvec(dn2, n1h);
+    } else {
+        M3 dn1, dn2;
+        if (v_order == 2) {
+            dn1 = nvd_skew(n1n, n1h, neg(x03));
+            dn2 = nvd_skew(n2n, n2h, x13);
+        } else {
+            dn1 = nvd_skew(n1n, n1h, x02);
+            dn2 = nvd_skew(n2n, n2h, neg(x12));
+        }
+        M3 a = mul_skew_left(n1h, dn2);
+        M3 b = mul_skew_left(n2h, dn1);
+        M3 Mm;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Mm.m[i][j] = a.m[i][j] - b.m[i][j];
+        dsin = mtvec(Mm, eh);
+        dcos = add(mtvec(dn1, n2h), mtvec(dn2, n1h));
+    }
+    V3 dth = sub(scl(dsin, cos_t), scl(dcos, sin_t));
+    V3 bf = scl(dth, -dE);
+    M3 bh;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) bh.m[i][j] = k * (comp(dth, i) * comp(dth, j));
+    if (damping > 0.0f) {
+        float inv_dt = 1.0f / dt;
+        V3 p0 = ld3(qp, i0), p1 = ld3(qp, i1), p2 = ld3(qp, i2), p3 = ld3(qp, i3);
+        V3 x02p = sub(p2, p0), x03p = sub(p3, p0), x13p = sub(p3, p1), x12p = sub(p2, p1), ep = sub(p3, p2);
+        V3 n1pr = cross(x02p, x03p), n2pr = cross(x13p, x12p);
+        float n1pn = length(n1pr), n2pn = length(n2pr), epn = length(ep);
+        if (!(n1pn < eps || n2pn < eps || epn < eps)) {
+            V3 n1p = divs(n1pr, n1pn), n2p = divs(n2pr, n2pn), ehp = divs(ep, epn);
+            float sinp = dot(cross(n1p, n2p), ehp);
+            float cosp = dot(n1p, n2p);
+            float thetap = portable_atan2f(sinp, cosp);
+            float dtheta = theta - thetap;
+            if (dtheta > 3.141592653589793f)
+                dtheta = dtheta - 6.283185307179586f;
+            else if (dtheta < -3.141592653589793f)
+                dtheta = dtheta + 6.283185307179586f;
+            float dtheta_dt = dtheta * inv_dt;
+            bf = add(bf, scl(dth, -damping * rest_length * dtheta_dt));
+            float hs = damping * rest_length * inv_dt;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) bh.m[i][j] = bh.m[i][j] + hs * (comp(dth, i) * comp(dth, j));
+        }
+    }
+    f_out = bf;
+    h_out = bh;
+}
+
+// Volumetric stable Neo-Hookean tetrahedron, particle_vbd_kernels.py:76-292, :347-367.
+__device__ void eval_tet(const float* __restrict__ q, const float* __restrict__ qp, const int* __restrict__ ti,
+                         const float* __restrict__ dm, float mu, float lmbd, float damping, float dt, int v_order,
+                         V3& f_out, M3& h_out) {
+    M3 Dm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Dm.m[i][j] = dm[3 * i + j];
+    float rest_volume = 1.0f / (mdet(Dm) * 6.0f);
+    V3 X0 = ld3(q, ti[0]), X1 = ld3(q, ti[1]), X2 = ld3(q, ti[2]), X3 = ld3(q, ti[3]);
+    V3 c0 = sub(X1, X0), c1 = sub(X2, X0), c2 = sub(X3, X0);
+    M3 Ds;
+    Ds.m[0][0] = c0.x; Ds.m[0][1] = c1.x; Ds.m[0][2] = c2.x;
+    Ds.m[1][0] = c0.y; Ds.m[1][1] = c1.y; Ds.m[1][2] = c2.y;
+    Ds.m[2][0] = c0.z; Ds.m[2][1] = c1.z; Ds.m[2][2] = c2.z;
+    M3 F;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F.m[i][j] = Ds.m[i][0] * Dm.m[0][j] + Ds.m[i][1] * Dm.m[1][j] + Ds.m[i][2] * Dm.m[2][j];
+    float f9[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) f9[3 * c + r] = F.m[r][c];
+    float J = mdet(F);
+    float mu_nh = mu;
+    float l_nh = lmbd + mu;
+    float l_safe = (l_nh < 0.0f ? -1.0f : 1.0f) * fmaxf(fabsf(l_nh), 1.0e-6f);
+    float alpha = 1.0f + mu_nh / l_safe;
+    float F11 = F.m[0][0], F21 = F.m[1][0], F31 = F.m[2][0];
+    float F12 = F.m[0][1], F22 = F.m[1][1], F32 = F.m[2][1];
+    float F13 = F.m[0][2], F23 = F.m[1][2], F33 = F.m[2][2];
+    M3 cof;
+    cof.m[0][0] = F22 * F33 - F23 * F32;
+    cof.m[0][1] = F23 * F31 - F21 * F33;
+    cof.m[0][2] = F21 * F32 - F22 * F31;
+    cof.m[1][0] = F13 * F32 - F12 * F33;
+    cof.m[1][1] = F11 * F33 - F13 * F31;
+    cof.m[1][2] = F12 * F31 - F11 * F32;
+    cof.m[2][0] = F12 * F23 - F13 * F22;
+    cof.m[2][1] = F13 * F21 - F11 * F23;
+    cof.m[2][2] = F11 * F22 - F12 * F21;
+    float c9[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) c9[3 * c + r] = cof.m[r][c];
+    float s = l_nh * (J - alpha);
+    float Pv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Pv[i] = rest_volume * (mu_nh * f9[i] + s * c9[i]);
+    float m1, m2, m3;
+    if (v_order == 0) {
+        m1 = -(Dm.m[0][0] + Dm.m[1][0] + Dm.m[2][0]);
+        m2 = -(Dm.m[0][1] + Dm.m[1][1] + Dm.m[2][1]);
+        m3 = -(Dm.m[0][2] + Dm.m[1][2] + Dm.m[2][2]);
+    } else if (v_order == 1) {
+        m1 = Dm.m[0][0]; m2 = Dm.m[0][1]; m3 = Dm.m[0][2];
+    } else if (v_order == 2) {
+        m1 = Dm.m[1][0]; m2 = Dm.m[1][1]; m3 = Dm.m[1][2];
+    } else {
+        m1 = Dm.m[2][0]; m2 = Dm.m[2][1]; m3 = Dm.m[2][2];
+    }
+    V3 force = v3(-(Pv[0] * m1 + Pv[3] * m2 + Pv[6] * m3), -(Pv[1] * m1 + Pv[4] * m2 + Pv[7] * m3),
+                  -(Pv[2] * m1 + Pv[5] * m2 + Pv[8] * m3));
+    M3 hess;
+#define VBD_H9(i, j) (rest_volume * (mu_nh * ((i) == (j) ? 1.0f : 0.0f) + l_nh * (c9[i] * c9[j])))
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            hess.m[r][c] = m1 * (VBD_H9(r, c) * m1 + VBD_H9(r + 3, c) * m2 + VBD_H9(r + 6, c) * m3) +
+                           m2 * (VBD_H9(r, c + 3) * m1 + VBD_H9(r + 3, c + 3) * m2 + VBD_H9(r + 6, c + 3) * m3) +
+                           m3 * (VBD_H9(r, c + 6) * m1 + VBD_H9(r + 3, c + 6) * m2 + VBD_H9(r + 6, c + 6) * m3);
+#undef VBD_H9
+    if (damping > 0.0f) {
+        float inv_dt = 1.0f / dt;
+        V3 P0 = ld3(qp, ti[0]), P1 = ld3(qp, ti[1]), P2 = ld3(qp, ti[2]), P3 = ld3(qp, ti[3]);
+        V3 d0 = sub(P1, P0), d1 = sub(P2, P0), d2 = sub(P3, P0);
+        M3 Dsp;
+        Dsp.m[0][0] = d0.x; Dsp.m[0][1] = d1.x; Dsp.m[0][2] = d2.x;
+        Dsp.m[1][0] = d0.y; Dsp.m[1][1] = d1.y; Dsp.m[1][2] = d2.y;
+        Dsp.m[2][0] = d0.z; Dsp.m[2][1] = d1.z; Dsp.m[2][2] = d2.z;
+        M3 Fp;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Fp.m[i][j] = Dsp.m[i][0] * Dm.m[0][j] + Dsp.m[i][1] * Dm.m[1][j] + Dsp.m[i][2] * Dm.m[2][j];
+        V3 f0 = v3(F.m[0][0], F.m[1][0], F.m[2][0]);
+        V3 f1 = v3(F.m[0][1], F.m[1][1], F.m[2][1]);
+        V3 f2 = v3(F.m[0][2], F.m[1][2], F.m[2][2]);
+        V3 f0p = v3(Fp.m[0][0], Fp.m[1][0], Fp.m[2][0]);
+        V3 f1p = v3(Fp.m[0][1], Fp.m[1][1], Fp.m[2][1]);
+        V3 f2p = v3(Fp.m[0][2], Fp.m[1][2], Fp.m[2][2]);
+        float c00 = (dot(f0, f0) - dot(f0p, f0p)) * inv_dt;
+        float c01 = (dot(f0, f1) - dot(f0p, f1p)) * inv_dt;
+        float c02 = (dot(f0, f2) - dot(f0p, f2p)) * inv_dt;
+        float c11 = (dot(f1, f1) - dot(f1p, f1p)) * inv_dt;
+        float c12 = (dot(f1, f2) - dot(f1p, f2p)) * inv_dt;
+        float c22 = (dot(f2, f2) - dot(f2p, f2p)) * inv_dt;
+        V3 dc00 = scl(f0, 2.0f * m1);
+        V3 dc01 = add(scl(f1, m1), scl(f0, m2));
+        V3 dc02 = add(scl(f2, m1), scl(f0, m3));
+        V3 dc11 = scl(f1, 2.0f * m2);
+        V3 dc12 = add(scl(f2, m2), scl(f1, m3));
+        V3 dc22 = scl(f2, 2.0f * m3);
+        V3 fd = scl(dc00, c00);
+        fd = add(fd, scl(dc01, 2.0f * c01));
+        fd = add(fd, scl(dc02, 2.0f * c02));
+        fd = add(fd, scl(dc11, c11));
+        fd = add(fd, scl(dc12, 2.0f * c12));
+        fd = add(fd, scl(dc22, c22));
+        force = sub(force, scl(fd, rest_volume * damping));
+        float hsc = rest_volume * damping * inv_dt;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float hs = comp(dc00, i) * comp(dc00, j);
+                hs = hs + 2.0f * (comp(dc01, i) * comp(dc01, j));
+                hs = hs + 2.0f * (comp(dc02, i) * comp(dc02, j));
+                hs = hs + comp(dc11, i) * comp(dc11, j);
+                hs = hs + 2.0f * (comp(dc12, i) * comp(dc12, j));
+                hs = hs + comp(dc22, i) * comp(dc22, j);
+                hess.m[i][j] = hess.m[i][j] + hsc * hs;
+            }
+    }
+    f_out = force;
+    h_out = hess;
+}
+
+// Particle vs static shape contact: penalty + damping + IPC friction,
+// rigid_vbd_kernels.py:1331-1439, :1603-1645.
+__device__ void eval_contact(V3 x, V3 x_prev, V3 bx, V3 body_vel, V3 n, float radius, float margin, float ke, float kd,
+                             float mu, float friction_epsilon, float dt, V3& f_out, M3& h_out) {
+    float pen = -(dot(n, sub(x, bx)) - radius - margin);
+    if (!(pen > 0.0f)) {
+        f_out = v3(0.0f, 0.0f, 0.0f);
+        h_out = mzero();
+        return;
+    }
+    V3 dx = sub(x, x_prev);
+    V3 zero = v3(0.0f, 0.0f, 0.0f);
+    V3 bv = add(add(zero, cross(zero, bx)), body_vel);
+    V3 u = sub(dx, scl(bv, dt));
+    float f_n = pen * ke;
+    V3 force = scl(n, f_n);
+    M3 nn = outer(n, n);
+    M3 hess = mscl(nn, ke);
+    if (dot(n, u) < 0.0f) {
+        M3 dh = mscl(nn, kd / dt);
+        hess = madd(hess, dh);
+        force = sub(force, mvec(dh, u));
+    }
+    float eps_u = friction_epsilon * dt;
+    float dot_nu = dot(n, u);
+    V3 u_t = sub(u, scl(n, dot_nu));
+    float u_norm = length(u_t);
+    if (u_norm > 0.0f) {
+        float f1 = (u_norm > eps_u) ? 1.0f / u_norm : (-u_norm / eps_u + 2.0f) / eps_u;
+        float scale = mu * f_n * f1;
+        force = add(force, neg(scl(u_t, scale)));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) hess.m[i][j] = hess.m[i][j] + scale * ((i == j ? 1.0f : 0.0f) - nn.m[i][j]);
+    } else {
+        force = add(force, zero);
+        hess = madd(hess, mzero());
+    }
+    f_out = force;
+    h_out = hess;
+}
+
+// Guarded 3x3 solve, particle_vbd_kernels.py:2701-2706: returns false when |det H| <= 1e-8.
+__device__ bool block_solve(const M3& h, V3 f, int fp64_ill, V3& dx) {
+    float det = mdet(h);
+    if (!(fabsf(det) > 1e-8f)) return false;
+    M3 b;
+    b.m[0][0] = h.m[1][1] * h.m[2][2] - h.m[1][2] * h.m[2][1];
+    b.m[1][0] = h.m[1][2] * h.m[2][0] - h.m[1][0] * h.m[2][2];
+    b.m[2][0] = h.m[1][0] * h.m[2][1] - h.m[1][1] * h.m[2][0];
+    b.m[0][1] = h.m[0][2] * h.m[2][1] - h.m[0][1] * h.m[2][2];
+    b.m[1][1] = h.m[0][0] * h.m[2][2] - h.m[0][2] * h.m[2][0];
+    b.m[2][1] = h.m[0][1] * h.m[2][0] - h.m[0][0] * h.m[2][1];
+    b.m[0][2] = h.m[0][1] * h.m[1][2] - h.m[0][2] * h.m[1][1];
+    b.m[1][2] = h.m[0][2] * h.m[1][0] - h.m[0][0] * h.m[1][2];
+    b.m[2][2] = h.m[0][0] * h.m[1][1] - h.m[0][1] * h.m[1][0];
+    float inv_det = 1.0f / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b.m[i][j] = b.m[i][j] * inv_det;
+    if (fp64_ill) {
+        float n1 = 0.0f, n2 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                n1 = n1 + h.m[i][j] * h.m[i][j];
+                n2 = n2 + b.m[i][j] * b.m[i][j];
+            }
+        float cond = sqrtf(n1) * sqrtf(n2);
+        if (!(cond <= 1.0e6f)) {
+            double a[3][3], c[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) a[i][j] = (double)h.m[i][j];
+            double det_d = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) +
+                           a[0][1] * (a[1][2] * a[2][0] - a[1][0] * a[2][2]) +
+                           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+            c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+            c[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+            c[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+            c[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+            c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+            c[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+            c[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+            c[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+            c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+            double inv = 1.0 / det_d;
+            double fd0 = (double)f.x, fd1 = (double)f.y, fd2 = (double)f.z;
+            double r0 = (c[0][0] * inv) * fd0 + (c[0][1] * inv) * fd1 + (c[0][2] * inv) * fd2;
+            double r1 = (c[1][0] * inv) * fd0 + (c[1][1] * inv) * fd1 + (c[1][2] * inv) * fd2;
+            double r2 = (c[2][0] * inv) * fd0 + (c[2][1] * inv) * fd1 + (c[2][2] * inv) * fd2;
+            dx = v3((float)r0, (float)r1, (float)r2);
+            return true;
+        }
+    }
+    dx = mvec(b, f);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct KParams {
+    int N;
+    const float* mass;
+    const float* inv_mass;
+    const float* radius;
+    const int* flags;
+    const int* tri_idx;
+    const float* tri_pose;
+    const float* tri_mat;
+    const float* tri_area;
+    const int* edge_idx;
+    const float* edge_rest_angle;
+    const float* edge_rest_len;
+    const float* edge_bend;
+    const int* tet_idx;
+    const float* tet_pose;
+    const float* tet_mat;
+    const int* adj_tri;
+    const int* adj_tri_off;
+    const int* adj_edge;
+    const int* adj_edge_off;
+    const int* adj_tet;
+    const int* adj_tet_off;
+    int has_tri, has_edge, has_tet;
+    const int* color_particles;
+    const int* batch_start;
+    const float* gravity;
+    float* q_prev;
+    float* inertia;
+    float* disp;
+    // per-particle contact lists (contact-index order) and per-contact state
+    const int* c_off;
+    const int* c_ids;
+    const float* c_pk;
+    const float* c_kd;
+    const float* c_mu;
+    const int* c_shape;
+    const float* c_body_pos;
+    const float* c_body_vel;
+    const float* c_normal;
+    const float* shape_margin;
+    int shape_count;
+    float friction_epsilon;
+    int fp64_ill;
+};
+
+__device__ __forceinline__ bool particle_active(const KParams& P, int v) {
+    return (P.flags[v] & 1) && P.mass[v] != 0.0f;
+}
+
+// Work-group -> batch mapping that keeps consecutive batches (neighbouring
+// vertices of one color) on the same XCD, whose L2 then serves their shared
+// neighbours.  Hardware dispatches work-group i to XCD i % 8.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int kXcd = 8;
+    int xcd = bid % kXcd, idx = bid / kXcd;
+    int qn = nb / kXcd, rn = nb % kXcd;
+    return xcd * qn + (xcd < rn ? xcd : rn) + idx;
+}
+
+// mode 0: solve and update the color's vertices; mode 1: write the assembled force g.
+__global__ __launch_bounds__(kBlock) void k_solve_color(KParams P, float* __restrict__ q, int batch_base, int nb,
+                                                        float dt, int mode, float* __restrict__ g_out) {
+    __shared__ int s_v[kMaxV];
+    __shared__ int s_cnt[3][kMaxV];
+    __shared__ int s_beg[3][kMaxV];
+    __shared__ int s_start[kMaxV + 1];
+    __shared__ float s_item[kBlock * kComp];
+    __shared__ float s_acc[kMaxV * kComp];
+
+    const int t = threadIdx.x;
+    const int b = batch_base + xcd_remap(blockIdx.x, nb);
+    const int vs = P.batch_start[b];
+    const int nv = P.batch_start[b + 1] - vs;
+    const float dtr = 1.0f / (dt * dt);
+
+    if (t < nv) {
+        int v = P.color_particles[vs + t];
+        s_v[t] = v;
+        int nt = 0, ne = 0, nk = 0, bt = 0, be = 0, bk = 0;
+        if (particle_active(P, v)) {
+            if (P.has_tri) {
+                bt = P.adj_tri_off[v];
+                nt = (P.adj_tri_off[v + 1] - bt) >> 1;
+            }
+            if (P.has_edge) {
+                be = P.adj_edge_off[v];
+                ne = (P.adj_edge_off[v + 1] - be) >> 1;
+            }
+            if (P.has_tet) {
+                bk = P.adj_tet_off[v];
+                nk = (P.adj_tet_off[v + 1] - bk) >> 1;
+            }
+        }
+        s_cnt[0][t] = nt; s_cnt[1][t] = ne; s_cnt[2][t] = nk;
+        s_beg[0][t] = bt; s_beg[1][t] = be; s_beg[2][t] = bk;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int s = 0; s < nv; ++s) {
+            s_start[s] = acc;
+            acc += s_cnt[0][s] + s_cnt[1][s] + s_cnt[2][s];
+        }
+        s_start[nv] = acc;
+    }
+    __syncthreads();
+    const int total = s_start[nv];
+
+    // lane (cs, cc) owns component cc of vertex cs's force (cc < 3) or Hessian
+    const int cs = t / kComp;
+    const int cc = t - cs * kComp;
+    const bool comp_lane = cs < nv;
+    float acc = 0.0f;
+    if (comp_lane) {
+        int v = s_v[cs];
+        float m = P.mass[v];
+        if (cc < 3) {
+            acc = (m * (P.inertia[3 * v + cc] - q[3 * v + cc])) * dtr;
+        } else {
+            int r = (cc - 3) / 3, c = (cc - 3) % 3;
+            acc = (m * dtr) * (r == c ? 1.0f : 0.0f);
+        }
+    }
+
+    for (int lo = 0; lo < total; lo += kBlock) {
+        const int item = lo + t;
+        if (item < total) {
+            int s = 0;
+            while (s_start[s + 1] <= item) ++s;
+            int k = item - s_start[s];
+            V3 fe = v3(0.0f, 0.0f, 0.0f);
+            M3 he = mzero();
+            if (k < s_cnt[0][s]) {
+                int base = s_beg[0][s] + 2 * k;
+                int tri = P.adj_tri[base], slot = P.adj_tri[base + 1];
+                const float* mat = P.tri_mat + 5 * tri;
+                float mu = mat[0], la = mat[1];
+                if (mu > 0.0f || la > 0.0f) {
+                    const int* ti = P.tri_idx + 3 * tri;
+                    int i0 = ti[0], i1 = ti[1], i2 = ti[2];
+                    eval_membrane(ld3(q, i0), ld3(q, i1), ld3(q, i2), P.q_prev, i0, i1, i2, P.tri_pose + 4 * tri,
+                                  P.tri_area[tri], mu, la, mat[2], dt, slot, fe, he);
+                }
+            } else if (k < s_cnt[0][s] + s_cnt[1][s]) {
+                int base = s_beg[1][s] + 2 * (k - s_cnt[0][s]);
+                int ed = P.adj_edge[base], slot = P.adj_edge[base + 1];
+                float ke = P.edge_bend[2 * ed];
+                if (ke > 0.0f)
+                    eval_bending(q, P.q_prev, P.edge_idx + 4 * ed, P.edge_rest_angle[ed], P.edge_rest_len[ed], ke,
+                                 P.edge_bend[2 * ed + 1], dt, slot, fe, he);
+            } else {
+                int base = s_beg[2][s] + 2 * (k - s_cnt[0][s] - s_cnt[1][s]);
+                int tet = P.adj_tet[base], slot = P.adj_tet[base + 1];
+                const float* mat = P.tet_mat + 3 * tet;
+                if (mat[0] > 0.0f || mat[1] > 0.0f)
+                    eval_tet(q, P.q_prev, P.tet_idx + 4 * tet, P.tet_pose + 9 * tet, mat[0], mat[1], mat[2], dt, slot,
+                             fe, he);
+            }
+            float* row = s_item + t * kComp;
+            row[0] = fe.x; row[1] = fe.y; row[2] = fe.z;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) row[3 + 3 * i + j] = he.m[i][j];
+        }
+        __syncthreads();
+        if (comp_lane) {
+            int a = max(lo, s_start[cs]);
+            int e = min(lo + kBlock, s_start[cs + 1]);
+            for (int i = a; i < e; ++i) acc = acc + s_item[(i - lo) * kComp + cc];
+        }
+        __syncthreads();
+    }
+    if (comp_lane) s_acc[cs * kComp + cc] = acc;
+    __syncthreads();
+
+    if (t < nv) {
+        const int v = s_v[t];
+        if (!particle_active(P, v)) {
+            if (mode == 0) {
+                V3 z = v3(0.0f, 0.0f, 0.0f);
+                st3(P.disp, v, z);
+                st3(q, v, add(ld3(P.q_prev, v), scl(z, 1.0f)));
+            } else {
+                st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
+            }
+            return;
+        }
+        const float* a = s_acc + t * kComp;
+        V3 f = v3(a[0], a[1], a[2]);
+        M3 h;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
+        V3 cf = v3(0.0f, 0.0f, 0.0f);
+        M3 ch = mzero();
+        const int c0 = P.c_off[v], c1 = P.c_off[v + 1];
+        if (c1 > c0) {
+            V3 x = ld3(q, v), xp = ld3(P.q_prev, v);
+            float rad = P.radius[v];
+            for (int j = c0; j < c1; ++j) {
+                int cid = P.c_ids[j];
+                int sh = P.c_shape[cid];
+                float margin = P.shape_count > 0 ? P.shape_margin[sh] : 0.0f;
+                V3 fc;
+                M3 hc;
+                eval_contact(x, xp, ld3(P.c_body_pos, cid), ld3(P.c_body_vel, cid), ld3(P.c_normal, cid), rad, margin,
+                             P.c_pk[cid], P.c_kd[cid], P.c_mu[cid], P.friction_epsilon, dt, fc, hc);
+                cf = add(cf, fc);
+                ch = madd(ch, hc);
+            }
+        }
+        h = madd(h, ch);
+        f = add(f, cf);
+        if (mode == 1) {
+            st3(g_out, v, f);
+            return;
+        }
+        V3 dx;
+        if (block_solve(h, f, P.fp64_ill, dx)) {
+            V3 d = add(ld3(P.disp, v), dx);
+            st3(P.disp, v, d);
+            st3(q, v, add(ld3(P.q_prev, v), scl(d, 1.0f)));
+        }
+    }
+}
+
+// forward_step + apply_truncation_ts(t = 1): particle_vbd_kernels.py:1350-1375, :2266-2286
+__global__ void k_forward(KParams P, float* __restrict__ q, const float* __restrict__ qd, const float* __restrict__ f_ext,
+                          float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    V3 x = ld3(q, i);
+    st3(P.q_prev, i, x);
+    V3 y, d;
+    if (!(P.flags[i] & 1) || P.inv_mass[i] == 0.0f) {
+        y = x;
+        d = v3(0.0f, 0.0f, 0.0f);
+    } else {
+        V3 g = v3(P.gravity[0], P.gravity[1], P.gravity[2]);
+        V3 fe = f_ext ? ld3(f_ext, i) : v3(0.0f, 0.0f, 0.0f);
+        V3 vnew = add(ld3(qd, i), scl(add(g, scl(fe, P.inv_mass[i])), dt));
+        y = add(x, scl(vnew, dt));
+        d = scl(vnew, dt);
+    }
+    st3(P.inertia, i, y);
+    st3(P.disp, i, d);
+    st3(q, i, add(x, scl(d, 1.0f)));
+}
+
+// update_velocity: particle_vbd_kernels.py:1468-1471 (plus the final q copy, solver_vbd.py:2704)
+__global__ void k_finalize(KParams P, const float* __restrict__ q, float* __restrict__ q_out, float* __restrict__ qd_out,
+                           float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    V3 x = ld3(q, i);
+    if (q_out != q) st3(q_out, i, x);
+    st3(qd_out, i, divs(sub(x, ld3(P.q_prev, i)), dt));
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, double value, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + per-particle counts
+__global__ void k_contact_init(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
+                               const int* __restrict__ c_shape, float soft_ke, float soft_kd, float soft_mu,
+                               const float* __restrict__ shape_ke, const float* __restrict__ shape_kd,
+                               const float* __restrict__ shape_mu, float k_start, float* __restrict__ pk,
+                               float* __restrict__ mke, float* __restrict__ mkd, float* __restrict__ mmu,
+                               int* __restrict__ cnt, int n_particles, int n_shapes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = min(*count_ptr, cmax);
+    if (i >= nc) return;
+    int s = c_shape[i];
+    if (s < 0 || s >= n_shapes) {
+        // a record naming no shape of this model contributes nothing (it is never gathered)
+        pk[i] = 0.0f;
+        mke[i] = 0.0f;
+        mkd[i] = 0.0f;
+        mmu[i] = 0.0f;
+        return;
+    }
+    float ke = 0.5f * (soft_ke + shape_ke[s]);
+    float kd = 0.5f * (soft_kd + shape_kd[s]);
+    float mu = sqrtf(soft_mu * shape_mu[s]);
+    mke[i] = ke;
+    mkd[i] = kd;
+    mmu[i] = mu;
+    pk[i] = k_start < 0.0f ? ke : fminf(k_start, ke);
+    int p = c_indices[3 * i];
+    if (c_indices[3 * i + 1] < 0 && p >= 0 && p < n_particles) atomicAdd(&cnt[p], 1);
+}
+
+// same record filter as k_contact_init, so each particle's fill count ends at its counted total
+__global__ void k_contact_scatter(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
+                                  const int* __restrict__ c_shape, const int* __restrict__ off, int* __restrict__ fillc,
+                                  int* __restrict__ ids, int n_particles, int n_shapes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = min(*count_ptr, cmax);
+    if (i >= nc) return;
+    if (c_indices[3 * i + 1] >= 0) return;
+    int p = c_indices[3 * i];
+    if (p < 0 || p >= n_particles) return;
+    int sh = c_shape[i];
+    if (sh < 0 || sh >= n_shapes) return;
+    int slot = atomicAdd(&fillc[p], 1);
+    ids[off[p] + slot] = i;
+}
+
+// sort each particle's contact ids ascending (the serial accumulation order)
+__global__ void k_contact_sort(int n, const int* __restrict__ off, int* __restrict__ ids) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int b = off[p], e = off[p + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int key = ids[i];
+        int j = i - 1;
+        while (j >= b && ids[j] > key) {
+            ids[j + 1] = ids[j];
+            --j;
+        }
+        ids[j + 1] = key;
+    }
+}
+
+// update_duals_body_particle_contacts, rigid_vbd_kernels.py:4861-4926 (particle records)
+__global__ void k_dual_update(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
+                              const int* __restrict__ c_shape, const float* __restrict__ body_pos,
+                              const float* __restrict__ normal, const float* __restrict__ q,
+                              const float* __restrict__ radius, const float* __restrict__ shape_margin, int shape_count,
+                              const float* __restrict__ mke, float beta, float* __restrict__ pk, int n_particles) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = min(*count_ptr, cmax);
+    if (i >= nc) return;
+    if (c_indices[3 * i + 1] >= 0) return;
+    int p = c_indices[3 * i];
+    if (p < 0 || p >= n_particles) return;
+    int s = c_shape[i];
+    float margin = shape_count > 0 ? shape_margin[s] : 0.0f;
+    V3 n = ld3(normal, i);
+    float pen = -(dot(n, sub(ld3(q, p), ld3(body_pos, i))) - radius[p] - margin);
+    pen = fmaxf(0.0f, pen);
+    pk[i] = fminf(pk[i] + beta * pen, mke[i]);
+}
+
+__global__ void k_residual_norm(int n, const float* __restrict__ g, double* __restrict__ out) {
+    __shared__ double s_part[kBlock / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        double a = g[3 * i], b = g[3 * i + 1], c = g[3 * i + 2];
+        v = a * a + b * b + c * c;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += s_part[w];
+        atomicAdd(out, s);
+    }
+}
+
+// ---- soft-contact generation (geometry/kernels.py:1020-1184) -------------
+struct ColParams {
+    int P;
+    const int* pairs;
+    const float* radius;
+    const int* pflags;
+    const int* pworld;
+    const int* stype;
+    const int* sflags;
+    const int* sworld;
+    const float* sxf;
+    const float* sscale;
+    const float* smargin;
+};
+
+__device__ __forceinline__ V3 quat_rotate(float qx, float qy, float qz, float qw, V3 x) {
+    float c = 2.0f * qw * qw - 1.0f;
+    float d = 2.0f * (qx * x.x + qy * x.y + qz * x.z);
+    return v3(x.x * c + qx * d + (qy * x.z - qz * x.y) * qw * 2.0f, x.y * c + qy * d + (qz * x.x - qx * x.z) * qw * 2.0f,
+              x.z * c + qz * d + (qx * x.y - qy * x.x) * qw * 2.0f);
+}
+
+struct ContactGeo {
+    bool hit;
+    int particle, shape;
+    V3 body_pos, body_vel, normal;
+};
+
+__device__ ContactGeo soft_contact_geometry(const ColParams& C, const float* __restrict__ q, float margin, int tid) {
+    ContactGeo g;
+    g.hit = false;
+    int p = C.pairs[2 * tid], s = C.pairs[2 * tid + 1];
+    g.particle = p;
+    g.shape = s;
+    if ((C.pflags[p] & 1) == 0) return g;
+    if ((C.sflags[s] & 4) == 0) return g;
+    int pw = C.pworld[p], sw = C.sworld[s];
+    if (pw != -1 && sw != -1 && pw != sw) return g;
+    V3 px = ld3(q, p);
+    float radius = C.radius[p];
+    const float* tf = C.sxf + 7 * s;
+    V3 bp = v3(tf[0], tf[1], tf[2]);
+    float qx = tf[3], qy = tf[4], qz = tf[5], qw = tf[6];
+    // X_ws == X_bs for static shapes; X_sw = inverse(X_ws)
+    float iqx = -qx, iqy = -qy, iqz = -qz, iqw = qw;
+    V3 ip = neg(quat_rotate(iqx, iqy, iqz, iqw, bp));
+    V3 xl = add(quat_rotate(iqx, iqy, iqz, iqw, px), ip);
+    const float* sc = C.sscale + 3 * s;
+    float s_margin = C.smargin[s];
+    float dd = 1.0e6f;
+    V3 n = v3(0.0f, 0.0f, 0.0f);
+    int type = C.stype[s];
+    if (type == 3) {
+        float pl = length(xl);
+        dd = pl - sc[0];
+        n = pl > 1.0e-8f ? divs(xl, pl) : v3(0.0f, 0.0f, 1.0f);
+    } else if (type == 1) {
+        float width = sc[0] * 0.5f, len = sc[1] * 0.5f;
+        if (width > 0.0f && len > 0.0f) {
+            float a = fmaxf(fabsf(xl.x) - width, fabsf(xl.y) - len);
+            dd = fmaxf(a, fabsf(xl.z));
+        } else {
+            dd = xl.z;
+        }
+        n = v3(0.0f, 0.0f, 1.0f);
+    } else {
+        return g;
+    }
+    if (dd < margin + s_margin + radius) {
+        g.hit = true;
+        g.body_pos = add(quat_rotate(qx, qy, qz, qw, sub(xl, scl(n, dd))), bp);
+        g.body_vel = quat_rotate(qx, qy, qz, qw, v3(0.0f, 0.0f, 0.0f));
+        g.normal = quat_rotate(qx, qy, qz, qw, n);
+    }
+    return g;
+}
+
+__global__ void k_collide_flag(ColParams C, const float* __restrict__ q, float margin, int* __restrict__ flag) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= C.P) return;
+    flag[tid] = soft_contact_geometry(C, q, margin, tid).hit ? 1 : 0;
+}
+
+__global__ void k_collide_write(ColParams C, const float* __restrict__ q, float margin, const int* __restrict__ flag,
+                                const int* __restrict__ index, VbdContactsOut out) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= C.P) return;
+    if (tid == C.P - 1) *out.soft_contact_count = index[tid] + flag[tid];
+    if (!flag[tid]) return;
+    int k = index[tid];
+    if (k >= out.soft_contact_max) return;
+    ContactGeo g = soft_contact_geometry(C, q, margin, tid);
+    out.soft_contact_shape[k] = g.shape;
+    st3(out.soft_contact_body_pos, k, g.body_pos);
+    st3(out.soft_contact_body_vel, k, g.body_vel);
+    if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
+    out.soft_contact_indices[3 * k] = g.particle;
+    out.soft_contact_indices[3 * k + 1] = -1;
+    out.soft_contact_indices[3 * k + 2] = -1;
+    if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, v3(1.0f, 0.0f, 0.0f));
+    st3(out.soft_contact_normal, k, g.normal);
+}
+
+inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct VbdContext {
+    VbdModelDesc model;
+    VbdSolverParams params;
+    std::vector<int32_t> color_offsets;
+    std::vector<int32_t> batch_color_offsets;
+    KParams kp;
+    float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
+    int* c_cnt = nullptr;       // [N+1]
+    int* c_off = nullptr;       // [N+1]
+    int* c_ids = nullptr;       // [cap]
+    float* c_state = nullptr;   // pk | mke | mkd | mmu, each [cap]
+    float* g_tmp = nullptr;     // [N,3] residual scratch
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    int capacity = 0;
+    const VbdContactsDesc* last_contacts = nullptr;
+    VbdContactsDesc contacts{};
+    bool have_contacts = false;
+};
+
+struct VbdCollide {
+    VbdCollideDesc desc;
+    ColParams cp;
+    int* flag = nullptr;
+    int* index = nullptr;
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+};
+
+extern "C" {
+
+const char* vbd_last_error(void) { return g_last_error.c_str(); }
+
+int32_t vbd_version(void) { return 1; }
+
+int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext** out) {
+    if (!m || !prm || !out) return fail(VBD_ERR_ARG, "vbd_create: null argument");
+    *out = nullptr;
+    const int N = m->particle_count;
+    if (N <= 0) return fail(VBD_ERR_ARG, "vbd_create: particle_count must be positive");
+    if (m->color_count <= 0 || !m->color_offsets_host || !m->color_particles)
+        return fail(VBD_ERR_ARG, "vbd_create: model.particle_color_groups is empty (color the model first)");
+    if (m->batch_count <= 0 || !m->batch_color_offsets_host || !m->batch_start)
+        return fail(VBD_ERR_ARG, "vbd_create: missing color-pass tile schedule");
+    if (!m->particle_mass || !m->particle_inv_mass || !m->particle_flags || !m->particle_radius || !m->gravity)
+        return fail(VBD_ERR_ARG, "vbd_create: missing particle arrays");
+    if (m->tri_count > 0 && (!m->v_adj_tris || !m->v_adj_tris_offsets || !m->tri_indices))
+        return fail(VBD_ERR_ARG, "vbd_create: triangle adjacency missing");
+    if (m->edge_count > 0 && (!m->v_adj_edges || !m->v_adj_edges_offsets || !m->edge_indices))
+        return fail(VBD_ERR_ARG, "vbd_create: edge adjacency missing");
+    if (m->tet_count > 0 && (!m->v_adj_tets || !m->v_adj_tets_offsets || !m->tet_indices))
+        return fail(VBD_ERR_ARG, "vbd_create: tet adjacency missing");
+    if (prm->iterations < 0) return fail(VBD_ERR_ARG, "vbd_create: iterations must be >= 0");
+    if (prm->soft_contact_capacity < 0) return fail(VBD_ERR_ARG, "vbd_create: negative soft_contact_capacity");
+
+    VbdContext* c = new VbdContext();
+    c->model = *m;
+    c->params = *prm;
+    c->color_offsets.assign(m->color_offsets_host, m->color_offsets_host + m->color_count + 1);
+    c->batch_color_offsets.assign(m->batch_color_offsets_host, m->batch_color_offsets_host + m->color_count + 1);
+    if (c->batch_color_offsets[m->color_count] != m->batch_count) {
+        delete c;
+        return fail(VBD_ERR_ARG, "vbd_create: batch_color_offsets does not end at batch_count");
+    }
+    // the kernel assumes <= kMaxV vertices per batch: check the schedule once here
+    {
+        std::vector<int32_t> bs((size_t)m->batch_count + 1);
+        hipError_t e = hipMemcpy(bs.data(), m->batch_start, sizeof(int32_t) * bs.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(VBD_ERR_HIP, std::string("vbd_create: reading batch_start: ") + hipGetErrorString(e));
+        }
+        for (int b = 0; b < m->batch_count; ++b) {
+            int nv = bs[b + 1] - bs[b];
+            if (nv <= 0 || nv > kMaxV) {
+                delete c;
+                return fail(VBD_ERR_ARG, "vbd_create: batch " + std::to_string(b) + " has " + std::to_string(nv) +
+                                             " vertices (allowed 1.." + std::to_string(kMaxV) + ")");
+            }
+        }
+        if (bs[m->batch_count] != c->color_offsets[m->color_count]) {
+            delete c;
+            return fail(VBD_ERR_ARG, "vbd_create: batch_start does not cover the color groups");
+        }
+    }
+    c->capacity = prm->soft_contact_capacity > 0 ? prm->soft_contact_capacity : 1;
+    auto cleanup = [&](const char* what, hipError_t e) {
+        std::string msg = std::string("vbd_create: ") + what + ": " + hipGetErrorString(e);
+        vbd_destroy(c);
+        return fail(VBD_ERR_HIP, msg);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&c->scratch3, sizeof(float) * 9 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_cnt, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_off, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_ids, sizeof(int) * (size_t)c->capacity)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * (size_t)c->capacity)) != hipSuccess)
+        return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->g_tmp, sizeof(float) * 3 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->c_off, N + 1)) != hipSuccess)
+        return cleanup("scan sizing", e);
+    if ((e = hipMalloc(&c->cub_tmp, c->cub_bytes > 0 ? c->cub_bytes : 16)) != hipSuccess)
+        return cleanup("hipMalloc", e);
+    // no contacts until vbd_contact_prepare: every particle's list is empty
+    if ((e = hipMemset(c->c_off, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return cleanup("sync", e);
+
+    KParams& k = c->kp;
+    std::memset(&k, 0, sizeof(k));
+    k.N = N;
+    k.mass = m->particle_mass;
+    k.inv_mass = m->particle_inv_mass;
+    k.radius = m->particle_radius;
+    k.flags = m->particle_flags;
+    k.tri_idx = m->tri_indices;
+    k.tri_pose = m->tri_poses;
+    k.tri_mat = m->tri_materials;
+    k.tri_area = m->tri_areas;
+    k.edge_idx = m->edge_indices;
+    k.edge_rest_angle = m->edge_rest_angle;
+    k.edge_rest_len = m->edge_rest_length;
+    k.edge_bend = m->edge_bending_properties;
+    k.tet_idx = m->tet_indices;
+    k.tet_pose = m->tet_poses;
+    k.tet_mat = m->tet_materials;
+    k.adj_tri = m->v_adj_tris;
+    k.adj_tri_off = m->v_adj_tris_offsets;
+    k.adj_edge = m->v_adj_edges;
+    k.adj_edge_off = m->v_adj_edges_offsets;
+    k.adj_tet = m->v_adj_tets;
+    k.adj_tet_off = m->v_adj_tets_offsets;
+    k.has_tri = m->tri_count > 0;
+    k.has_edge = m->edge_count > 0;
+    k.has_tet = m->tet_count > 0;
+    k.color_particles = m->color_particles;
+    k.batch_start = m->batch_start;
+    k.gravity = m->gravity;
+    k.q_prev = c->scratch3;
+    k.inertia = c->scratch3 + 3 * (size_t)N;
+    k.disp = c->scratch3 + 6 * (size_t)N;
+    k.c_off = c->c_off;
+    k.c_ids = c->c_ids;
+    k.c_pk = c->c_state;
+    k.c_kd = c->c_state + 2 * (size_t)c->capacity;
+    k.c_mu = c->c_state + 3 * (size_t)c->capacity;
+    k.shape_margin = m->shape_margin;
+    k.shape_count = m->shape_count;
+    k.friction_epsilon = prm->friction_epsilon;
+    k.fp64_ill = prm->fp64_ill_conditioned;
+    *out = c;
+    return VBD_OK;
+}
+
+int32_t vbd_destroy(VbdContext* c) {
+    if (!c) return VBD_OK;
+    hipFree(c->scratch3);
+    hipFree(c->c_cnt);
+    hipFree(c->c_off);
+    hipFree(c->c_ids);
+    hipFree(c->c_state);
+    hipFree(c->g_tmp);
+    hipFree(c->cub_tmp);
+    delete c;
+    return VBD_OK;
+}
+
+int32_t vbd_copy_scratch(VbdContext* c, void* stream, int32_t which, float* dst) {
+    if (!c || !dst || which < 0 || which > 2) return fail(VBD_ERR_ARG, "vbd_copy_scratch: bad argument");
+    const size_t n3 = 3 * (size_t)c->model.particle_count;
+    // scratch3 holds q_prev | inertia | displacement
+    const float* src = which == 0 ? c->kp.inertia : (which == 1 ? c->kp.q_prev : c->kp.disp);
+    VBD_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float) * n3, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return VBD_OK;
+}
+
+int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* cd) {
+    if (!c) return fail(VBD_ERR_ARG, "vbd_contact_prepare: null context");
+    hipStream_t s = (hipStream_t)stream;
+    const int N = c->model.particle_count;
+    if (!cd || cd->soft_contact_max <= 0) {
+        c->have_contacts = false;
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_off, 0, N + 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        return VBD_OK;
+    }
+    if (cd->soft_contact_max > c->capacity)
+        return fail(VBD_ERR_STATE, "vbd_contact_prepare: soft_contact_max " + std::to_string(cd->soft_contact_max) +
+                                       " exceeds the capacity " + std::to_string(c->capacity) +
+                                       " given at creation (buffers cannot grow inside a step)");
+    if (!cd->soft_contact_count || !cd->soft_contact_indices || !cd->soft_contact_shape || !cd->soft_contact_body_pos ||
+        !cd->soft_contact_body_vel || !cd->soft_contact_normal)
+        return fail(VBD_ERR_ARG, "vbd_contact_prepare: missing contact arrays");
+    if (c->model.shape_count <= 0 || !c->model.shape_material_ke)
+        return fail(VBD_ERR_ARG, "vbd_contact_prepare: contacts given but the model has no shapes");
+    c->contacts = *cd;
+    c->have_contacts = true;
+    const int cap = c->capacity;
+    float* pk = c->c_state;
+    float* mke = c->c_state + cap;
+    float* mkd = c->c_state + 2 * (size_t)cap;
+    float* mmu = c->c_state + 3 * (size_t)cap;
+    const int cb = blocks_for(cd->soft_contact_max, 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_cnt, 0, N + 1);
+    VBD_LAUNCH_CHECK("k_fill_i32");
+    hipLaunchKernelGGL(k_contact_init, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
+                       cd->soft_contact_indices, cd->soft_contact_shape, c->model.soft_contact_ke,
+                       c->model.soft_contact_kd, c->model.soft_contact_mu, c->model.shape_material_ke,
+                       c->model.shape_material_kd, c->model.shape_material_mu, c->params.k_start, pk, mke, mkd, mmu,
+                       c->c_cnt, N, c->model.shape_count);
+    VBD_LAUNCH_CHECK("k_contact_init");
+    size_t bytes = c->cub_bytes;
+    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, c->c_cnt, c->c_off, N + 1, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_cnt, 0, N + 1);
+    VBD_LAUNCH_CHECK("k_fill_i32");
+    hipLaunchKernelGGL(k_contact_scatter, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
+                       cd->soft_contact_indices, cd->soft_contact_shape, c->c_off, c->c_cnt, c->c_ids, N,
+                       c->model.shape_count);
+    VBD_LAUNCH_CHECK("k_contact_scatter");
+    hipLaunchKernelGGL(k_contact_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->c_off, c->c_ids);
+    VBD_LAUNCH_CHECK("k_contact_sort");
+    KParams& k = c->kp;
+    k.c_shape = cd->soft_contact_shape;
+    k.c_body_pos = cd->soft_contact_body_pos;
+    k.c_body_vel = cd->soft_contact_body_vel;
+    k.c_normal = cd->soft_contact_normal;
+    return VBD_OK;
+}
+
+int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, const float* f_ext, float dt) {
+    if (!c || !q || !qd) return fail(VBD_ERR_ARG, "vbd_forward: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_forward, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0, s, c->kp, q, qd, f_ext,
+                       dt);
+    VBD_LAUNCH_CHECK("k_forward");
+    return VBD_OK;
+}
+
+int32_t vbd_color_pass(VbdContext* c, void* stream, int32_t color, float* q, float dt) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_color_pass: null argument");
+    if (color < 0 || color >= c->model.color_count) return fail(VBD_ERR_ARG, "vbd_color_pass: color out of range");
+    int b0 = c->batch_color_offsets[color], b1 = c->batch_color_offsets[color + 1];
+    if (b1 <= b0) return VBD_OK;
+    hipLaunchKernelGGL(k_solve_color, dim3(b1 - b0), dim3(kBlock), 0, (hipStream_t)stream, c->kp, q, b0, b1 - b0, dt,
+                       0, (float*)nullptr);
+    VBD_LAUNCH_CHECK("k_solve_color");
+    return VBD_OK;
+}
+
+int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_iteration: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->have_contacts && c->params.avbd_beta != 0.0f) {
+        const VbdContactsDesc& cd = c->contacts;
+        hipLaunchKernelGGL(k_dual_update, dim3(blocks_for(cd.soft_contact_max, 256)), dim3(256), 0, s,
+                           cd.soft_contact_count, cd.soft_contact_max, cd.soft_contact_indices, cd.soft_contact_shape,
+                           cd.soft_contact_body_pos, cd.soft_contact_normal, q, c->model.particle_radius,
+                           c->model.shape_margin, c->model.shape_count, c->c_state + c->capacity,
+                           c->params.avbd_beta, c->c_state, c->model.particle_count);
+        VBD_LAUNCH_CHECK("k_dual_update");
+    }
+    for (int color = 0; color < c->model.color_count; ++color) {
+        int32_t rc = vbd_color_pass(c, stream, color, q, dt);
+        if (rc) return rc;
+    }
+    return VBD_OK;
+}
+
+int32_t vbd_finalize(VbdContext* c, void* stream, const float* q, float* q_out, float* qd_out, float dt) {
+    if (!c || !q || !q_out || !qd_out) return fail(VBD_ERR_ARG, "vbd_finalize: null argument");
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0, (hipStream_t)stream,
+                       c->kp, q, q_out, qd_out, dt);
+    VBD_LAUNCH_CHECK("k_finalize");
+    return VBD_OK;
+}
+
+int32_t vbd_step(VbdContext* c, void* stream, float* q_in, const float* qd_in, const float* f_ext, float* q_out,
+                 float* qd_out, const VbdContactsDesc* contacts, float dt) {
+    if (!c) return fail(VBD_ERR_ARG, "vbd_step: null context");
+    if (!(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_step: dt must be positive");
+    int32_t rc = vbd_contact_prepare(c, stream, contacts);
+    if (rc) return rc;
+    rc = vbd_forward(c, stream, q_in, qd_in, f_ext, dt);
+    if (rc) return rc;
+    for (int it = 0; it < c->params.iterations; ++it) {
+        rc = vbd_iteration(c, stream, q_in, dt);
+        if (rc) return rc;
+    }
+    return vbd_finalize(c, stream, q_in, q_out, qd_out, dt);
+}
+
+int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, float* g_out, double* norm_sq_out) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_residual: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    float* g = g_out ? g_out : c->g_tmp;
+    hipLaunchKernelGGL(k_solve_color, dim3(c->model.batch_count), dim3(kBlock), 0, s, c->kp, (float*)q, 0,
+                       c->model.batch_count, dt, 1, g);
+    VBD_LAUNCH_CHECK("k_solve_color(residual)");
+    if (norm_sq_out) {
+        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, s, norm_sq_out, 0.0, 1);
+        VBD_LAUNCH_CHECK("k_fill_f64");
+        hipLaunchKernelGGL(k_residual_norm, dim3(blocks_for(c->model.particle_count, kBlock)), dim3(kBlock), 0, s,
+                           c->model.particle_count, g, norm_sq_out);
+        VBD_LAUNCH_CHECK("k_residual_norm");
+    }
+    return VBD_OK;
+}
+
+int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
+    if (!d || !out) return fail(VBD_ERR_ARG, "vbd_collide_create: null argument");
+    *out = nullptr;
+    if (d->pair_count < 0) return fail(VBD_ERR_ARG, "vbd_collide_create: negative pair_count");
+    if (d->pair_count > 0 && (!d->pairs || !d->particle_radius || !d->particle_flags || !d->particle_world ||
+                              !d->shape_type || !d->shape_flags || !d->shape_world || !d->shape_transform ||
+                              !d->shape_scale || !d->shape_margin))
+        return fail(VBD_ERR_ARG, "vbd_collide_create: missing arrays");
+    VbdCollide* col = new VbdCollide();
+    col->desc = *d;
+    ColParams& cp = col->cp;
+    cp.P = d->pair_count;
+    cp.pairs = d->pairs;
+    cp.radius = d->particle_radius;
+    cp.pflags = d->particle_flags;
+    cp.pworld = d->particle_world;
+    cp.stype = d->shape_type;
+    cp.sflags = d->shape_flags;
+    cp.sworld = d->shape_world;
+    cp.sxf = d->shape_transform;
+    cp.sscale = d->shape_scale;
+    cp.smargin = d->shape_margin;
+    int P = d->pair_count > 0 ? d->pair_count : 1;
+    hipError_t e;
+    if ((e = hipMalloc(&col->flag, sizeof(int) * (size_t)P)) != hipSuccess ||
+        (e = hipMalloc(&col->index, sizeof(int) * (size_t)P)) != hipSuccess ||
+        (e = hipcub::DeviceScan::ExclusiveSum(nullptr, col->cub_bytes, col->flag, col->index, P)) != hipSuccess ||
+        (e = hipMalloc(&col->cub_tmp, col->cub_bytes > 0 ? col->cub_bytes : 16)) != hipSuccess) {
+        std::string msg = std::string("vbd_collide_create: ") + hipGetErrorString(e);
+        vbd_collide_destroy(col);
+        return fail(VBD_ERR_HIP, msg);
+    }
+    *out = col;
+    return VBD_OK;
+}
+
+int32_t vbd_collide_destroy(VbdCollide* col) {
+    if (!col) return VBD_OK;
+    hipFree(col->flag);
+    hipFree(col->index);
+    hipFree(col->cub_tmp);
+    delete col;
+    return VBD_OK;
+}
+
+int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* q, float margin, const VbdContactsOut* out) {
+    if (!col || !q || !out || !out->soft_contact_count) return fail(VBD_ERR_ARG, "vbd_collide_generate: null argument");
+    if (col->cp.P > 0 && (!out->soft_contact_indices || !out->soft_contact_shape || !out->soft_contact_body_pos ||
+                          !out->soft_contact_body_vel || !out->soft_contact_normal))
+        return fail(VBD_ERR_ARG, "vbd_collide_generate: missing output arrays");
+    hipStream_t s = (hipStream_t)stream;
+    if (col->cp.P == 0 || out->soft_contact_max <= 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out->soft_contact_count, 0, 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        return VBD_OK;
+    }
+    const int P = col->cp.P;
+    hipLaunchKernelGGL(k_collide_flag, dim3(blocks_for(P, 256)), dim3(256), 0, s, col->cp, q, margin, col->flag);
+    VBD_LAUNCH_CHECK("k_collide_flag");
+    size_t bytes = col->cub_bytes;
+    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(col->cub_tmp, bytes, col->flag, col->index, P, s));
+    hipLaunchKernelGGL(k_collide_write, dim3(blocks_for(P, 256)), dim3(256), 0, s, col->cp, q, margin, col->flag,
+                       col->index, *out);
+    VBD_LAUNCH_CHECK("k_collide_write");
+    return VBD_OK;
+}
+
+}  // extern "C"

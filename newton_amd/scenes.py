@@ -1,0 +1,137 @@
+"""Synthetic cloth / soft-body scenes of the VBD workloads.
+
+Each function returns ``(model, state, info)`` where ``state`` carries the
+seeded initial positions and ``info`` the step parameters.  Shared choices:
+
+* up axis Y, gravity -9.81, frame dt 1/60 s split into 10 substeps;
+* cloth grids are built as ``add_cloth_grid`` builds them, rotated +90 deg
+  about x so the grid lies in the XZ plane, with lumped mass from an areal
+  density (``add_cloth_mesh``: density * area / 3 per triangle corner);
+* contact material is averaged between ``model.soft_contact_*`` and each
+  shape (arithmetic mean of ke / kd, geometric mean of mu), so the shape mu is
+  chosen to give the stated effective friction: ground plane mu 0.5 with
+  ``soft_contact_mu`` 0.5, sphere mu 0.18 for an effective 0.3.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .geometry import ParticleFlags
+from .sim.builder import ModelBuilder, cloth_grid_mesh, quat_from_axis_angle
+
+FRAME_DT = 1.0 / 60.0
+SUBSTEPS = 10
+
+
+def _finish(builder: ModelBuilder, device, jitter_sigma: float, seed: int, soft_ke: float, soft_kd: float,
+            soft_mu: float):
+    model = builder.finalize(device)
+    model.soft_contact_ke = soft_ke
+    model.soft_contact_kd = soft_kd
+    model.soft_contact_mu = soft_mu
+    state = model.state()
+    if jitter_sigma > 0.0:
+        rng = np.random.default_rng(seed)
+        jitter = rng.normal(0.0, jitter_sigma, size=(model.particle_count, 3)).astype(np.float32)
+        state.particle_q += torch.as_tensor(jitter, device=model.device)
+    return model, state
+
+
+def cloth_hanging(device="cuda:0", verts_per_side: int = 32, size: float = 1.0, height: float = 1.0,
+                  seed: int = 0, iterations: int = 5):
+    """32x32 cloth, two top corners pinned, ground plane (the 1,024-vertex CPU-runnable case).
+
+    Areal density 0.1 kg/m^2, tri_ke = tri_ka = 1e4, tri_kd = 1e-5, edge_ke = 1,
+    contact ke 1e4 / mu 0.5, positions jittered by N(0, 1e-4) from ``seed``.
+    """
+    n = verts_per_side - 1
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
+    b.add_cloth_mesh(pos=(-0.5 * size, height, -0.5 * size), rot=rot, scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
+                     edge_ke=1.0, edge_kd=0.0)
+    top = [n * (n + 1), n * (n + 1) + n]
+    b.particle_mass[top] = 0.0
+    b.particle_flags[top] = int(ParticleFlags.ACTIVE) & ~int(ParticleFlags.ACTIVE)
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 1.0e-4, seed, 1.0e4, 0.0, 0.5)
+    info = {"workload": f"cloth_hanging_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations}
+    return model, state, info
+
+
+def cloth_drop_on_sphere(device="cuda:0", verts_per_side: int = 256, size: float = 2.0, height: float = 1.5,
+                         seed: int = 1, iterations: int = 10):
+    """256x256 cloth over 2 m falling onto a static sphere (r 0.4 m) and the ground (65,536 vertices).
+
+    Same material as :func:`cloth_hanging` but tri_ke per triangle from
+    log-uniform[5e3, 2e4] (``seed``); sphere contact ke 1e4, effective mu 0.3.
+    """
+    n = verts_per_side - 1
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    b.add_shape_sphere(-1, radius=0.4, cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.18))
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
+    b.add_cloth_mesh(pos=(-0.5 * size, height, -0.5 * size), rot=rot, scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
+                     edge_ke=1.0, edge_kd=0.0)
+    rng = np.random.default_rng(seed)
+    ke = np.exp(rng.uniform(math.log(5.0e3), math.log(2.0e4), size=b.tri_count))
+    b.tri_materials[:, 0] = ke
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+    info = {"workload": f"cloth_drop_on_sphere_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations}
+    return model, state, info
+
+
+def cloth_strip_pinned(device="cuda:0", nx: int = 6, ny: int = 4, seed: int = 0, tri_kd: float = 1e-2,
+                       edge_kd: float = 1e-3, iterations: int = 3):
+    """Tiny cloth (default 6x4 vertices) with membrane and bending damping on, one corner pinned.
+
+    Used by parity tests that want every element type and damping term active.
+    """
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=1.0, mu=0.5))
+    b.add_shape_sphere(-1, xform=(np.array([0.1, 0.2, 0.1], np.float32), np.array([0, 0, 0, 1], np.float32)),
+                       radius=0.15, cfg=ModelBuilder.ShapeConfig(ke=5.0e3, kd=2.0, mu=0.4))
+    verts, tris = cloth_grid_mesh(nx - 1, ny - 1, 0.05, 0.05)
+    rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
+    b.add_cloth_mesh(pos=(0.0, 0.35, 0.0), rot=rot, scale=1.0, vel=(0.0, -0.5, 0.0), vertices=verts, indices=tris,
+                     density=0.2, tri_ke=2.0e3, tri_ka=3.0e3, tri_kd=tri_kd, edge_ke=0.5, edge_kd=edge_kd,
+                     particle_radius=0.02)
+    b.particle_mass[0] = 0.0
+    b.particle_flags[0] = 0
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 2.0e-3, seed, 2.0e3, 1.0, 0.6)
+    info = {"workload": f"cloth_strip_{nx}x{ny}", "dt": FRAME_DT / SUBSTEPS, "substeps": SUBSTEPS,
+            "iterations": iterations}
+    return model, state, info
+
+
+def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 4, k_damp: float = 1e-3):
+    """Small tetrahedral cube (5 tets per hex cell) above the ground, for tet-path parity."""
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e5, kd=0.0, mu=0.5))
+    e, nu = 3.0e4, 0.35
+    mu = e / (2 * (1 + nu))
+    lam = e * nu / ((1 + nu) * (1 - 2 * nu))
+    rot = quat_from_axis_angle((0.3, 1.0, 0.2), 0.4)
+    b.add_soft_grid(pos=(0.0, 0.12, 0.0), rot=rot, vel=(0.0, -1.0, 0.0), dim_x=cells, dim_y=cells, dim_z=cells,
+                    cell_x=0.05, cell_y=0.05, cell_z=0.05, density=1000.0, k_mu=mu, k_lambda=lam, k_damp=k_damp,
+                    particle_radius=0.01)
+    b.color(include_bending=False)
+    model, state = _finish(b, device, 1.0e-4, seed, 1.0e5, 0.0, 0.5)
+    info = {"workload": f"soft_cube_{cells}", "dt": FRAME_DT / SUBSTEPS, "substeps": SUBSTEPS,
+            "iterations": iterations}
+    return model, state, info
+
+
+__all__ = ["FRAME_DT", "SUBSTEPS", "cloth_drop_on_sphere", "cloth_hanging", "cloth_strip_pinned", "soft_cube"]

@@ -1,0 +1,148 @@
+"""Soft-contact generation between particles and static shapes.
+
+``CollisionPipeline`` mirrors the particle part of the reference's
+``newton/_src/sim/collide.py`` (``CollisionPipeline`` :710, soft-contact sizing
+:1110-1136, ``collide`` :1236, launch :1553-1592).  Candidate pairs are built on
+the host once, in the order of ``_world_compatible_pairs`` (:480-546); the
+contacts are generated on the GPU by ``libvbd_hip`` (``vbd_collide_generate``),
+which writes records in pair order (a stable compaction), where the reference
+takes slots from an atomic counter.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _native
+from .model import Contacts, Model, State
+
+
+def world_compatible_pairs(feature_world: np.ndarray, shape_world: np.ndarray, world_count: int) -> np.ndarray:
+    """``(feature, shape)`` pairs whose worlds are compatible (``collide.py:480-546``)."""
+    feature_world = np.asarray(feature_world)
+    shape_world = np.asarray(shape_world)
+    n_features, n_shapes = len(feature_world), len(shape_world)
+    if n_features == 0 or n_shapes == 0:
+        return np.empty((0, 2), dtype=np.int32)
+    features = np.arange(n_features)
+    shapes = np.arange(n_shapes)
+    f_local = (feature_world >= 0) & (feature_world < world_count)
+    s_local = (shape_world >= 0) & (shape_world < world_count)
+    f_cols, s_cols = [], []
+    global_features = features[feature_world < 0]
+    if len(global_features):
+        f_cols.append(np.repeat(global_features, len(shapes)))
+        s_cols.append(np.tile(shapes, len(global_features)))
+    local_features = features[f_local]
+    global_shapes = shapes[shape_world < 0]
+    if len(local_features) and len(global_shapes):
+        f_cols.append(np.repeat(local_features, len(global_shapes)))
+        s_cols.append(np.tile(global_shapes, len(local_features)))
+    local_feature_world = feature_world[f_local]
+    shapes_per_world = np.bincount(shape_world[s_local], minlength=world_count)
+    reps = shapes_per_world[local_feature_world] if len(local_feature_world) else np.zeros(0, np.intp)
+    if reps.sum():
+        shapes_by_world = shapes[s_local][np.argsort(shape_world[s_local], kind="stable")]
+        world_start = np.cumsum(shapes_per_world) - shapes_per_world
+        within = np.arange(reps.sum()) - np.repeat(np.cumsum(reps) - reps, reps)
+        f_cols.append(np.repeat(local_features, reps))
+        s_cols.append(shapes_by_world[np.repeat(world_start[local_feature_world], reps) + within])
+    if not f_cols:
+        return np.empty((0, 2), dtype=np.int32)
+    return np.column_stack((np.concatenate(f_cols), np.concatenate(s_cols))).astype(np.int32)
+
+
+def current_stream_handle(device) -> int:
+    """Raw hipStream_t of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class CollisionPipeline:
+    """Particle-vs-static-shape soft contacts (plane, sphere)."""
+
+    def __init__(self, model: Model, *, soft_contact_max: int | None = None, soft_contact_margin: float = 0.01):
+        self.model = model
+        self.device = model.device
+        pairs = world_compatible_pairs(
+            model.particle_world.cpu().numpy() if model.particle_count else np.empty(0, np.int32),
+            model.shape_world.cpu().numpy() if model.shape_count else np.empty(0, np.int32),
+            model.world_count,
+        )
+        self.soft_rigid_contact_pairs_host = pairs
+        self.soft_rigid_contact_pairs = torch.as_tensor(np.ascontiguousarray(pairs), device=self.device)
+        self.soft_rigid_contact_pair_count = int(pairs.shape[0])
+        if soft_contact_max is None:
+            soft_contact_max = self.soft_rigid_contact_pair_count
+        self.soft_contact_max = int(soft_contact_max)
+        self.soft_contact_margin = float(soft_contact_margin)
+        if model.shape_count:
+            body = model.shape_body.cpu().numpy()
+            if np.any(body >= 0):
+                raise NotImplementedError("only static shapes (body=-1) are supported on this path")
+            types = model.shape_type.cpu().numpy()
+            if np.any((types != 1) & (types != 3)):
+                raise NotImplementedError("particle contact supports plane and sphere shapes only")
+        self._handle = None
+        if self.device.type == "cuda" and self.soft_rigid_contact_pair_count > 0:
+            lib = _native.hip_lib()
+            d = _native.VbdCollideDesc()
+            d.particle_count = model.particle_count
+            d.pair_count = self.soft_rigid_contact_pair_count
+            d.pairs = _native.ptr(self.soft_rigid_contact_pairs)
+            d.particle_radius = _native.ptr(model.particle_radius)
+            d.particle_flags = _native.ptr(model.particle_flags)
+            d.particle_world = _native.ptr(model.particle_world)
+            d.shape_count = model.shape_count
+            d.shape_type = _native.ptr(model.shape_type)
+            d.shape_flags = _native.ptr(model.shape_flags)
+            d.shape_world = _native.ptr(model.shape_world)
+            d.shape_transform = _native.ptr(model.shape_transform)
+            d.shape_scale = _native.ptr(model.shape_scale)
+            d.shape_margin = _native.ptr(model.shape_margin)
+            h = ctypes.c_void_p()
+            _native.check(lib.vbd_collide_create(ctypes.byref(d), ctypes.byref(h)), "vbd_collide_create")
+            self._handle = h
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _native.hip_lib().vbd_collide_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def contacts(self) -> Contacts:
+        """Allocate a :class:`Contacts` buffer sized for this pipeline."""
+        return Contacts(self.soft_contact_max, self.device)
+
+    def collide(self, state: State, contacts: Contacts, *, soft_contact_margin: float | None = None) -> None:
+        """Generate soft contacts for ``state.particle_q`` into ``contacts``."""
+        if self.device.type != "cuda":
+            raise RuntimeError("CollisionPipeline.collide runs on the GPU (libvbd_hip); the model is on the CPU")
+        margin = self.soft_contact_margin if soft_contact_margin is None else float(soft_contact_margin)
+        if contacts.soft_contact_max > self.soft_contact_max:
+            raise ValueError("contacts buffer is larger than the pipeline's soft_contact_max")
+        if self._handle is None:
+            contacts.soft_contact_count.zero_()
+            return
+        out = _native.VbdContactsOut()
+        out.soft_contact_count = _native.ptr(contacts.soft_contact_count)
+        out.soft_contact_max = contacts.soft_contact_max
+        out.soft_contact_particle = _native.ptr(contacts.soft_contact_particle)
+        out.soft_contact_indices = _native.ptr(contacts.soft_contact_indices)
+        out.soft_contact_barycentric = _native.ptr(contacts.soft_contact_barycentric)
+        out.soft_contact_shape = _native.ptr(contacts.soft_contact_shape)
+        out.soft_contact_body_pos = _native.ptr(contacts.soft_contact_body_pos)
+        out.soft_contact_body_vel = _native.ptr(contacts.soft_contact_body_vel)
+        out.soft_contact_normal = _native.ptr(contacts.soft_contact_normal)
+        rc = _native.hip_lib().vbd_collide_generate(
+            self._handle, current_stream_handle(self.device), _native.ptr(state.particle_q), margin, ctypes.byref(out)
+        )
+        _native.check(rc, "vbd_collide_generate")
+
+
+__all__ = ["CollisionPipeline", "world_compatible_pairs"]

@@ -1,0 +1,356 @@
+"""SolverVBD: drop-in particle Vertex Block Descent solver running on libvbd_hip.
+
+Mirrors the reference's ``newton/_src/solvers/vbd/solver_vbd.py`` for the
+particle path: the constructor signature (keyword-only after ``model``,
+``:221-269``), ``step(state_in, state_out, control, contacts, dt)``
+(``:1773-1825``), the in-place mutation of ``state_in.particle_q`` with the
+iterate, and the exception types raised at construction (``:403-404``,
+``:558-559``, ``:610-614``).  The compute path is the HIP library; there is no
+Python or CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _native
+from ..sim.collide import current_stream_handle
+from ..sim.model import Contacts, Control, Model, State
+
+# must match kBlock / kMaxV of csrc/vbd_hip.hip
+TILE_ITEM_CAPACITY = 256
+TILE_MAX_VERTICES = 21
+
+
+def build_tile_schedule(model: Model, item_cap: int = TILE_ITEM_CAPACITY, max_vertices: int = TILE_MAX_VERTICES):
+    """Per-color batches of vertices for the color-pass kernel.
+
+    Returns ``(color_offsets [C+1], color_particles [n], batch_start [B+1],
+    batch_color_offsets [C+1], item_count [N])`` as int32 numpy arrays; a
+    batch's vertices are consecutive in ``color_particles`` and their summed
+    element counts fit ``item_cap`` (a vertex with more elements than that has
+    a batch of its own).
+    """
+    adj = model.soft_mesh_adjacency
+    n = model.particle_count
+
+    def counts(offsets):
+        if offsets is None or len(offsets) == 0:
+            return np.zeros(n, dtype=np.int64)
+        o = np.asarray(offsets, dtype=np.int64)
+        return (o[1:] - o[:-1]) >> 1
+
+    flags = model.particle_flags.cpu().numpy()
+    mass = model.particle_mass.cpu().numpy()
+    active = ((flags & 1) != 0) & (mass != 0.0)
+    items = counts(adj.v_adj_tris_offsets) + counts(adj.v_adj_edges_offsets) + counts(adj.v_adj_tets_offsets)
+    items = np.where(active, items, 0).astype(np.int32)
+    groups = [g.cpu().numpy().astype(np.int32) for g in model.particle_color_groups]
+    color_offsets = np.zeros(len(groups) + 1, dtype=np.int32)
+    color_offsets[1:] = np.cumsum([len(g) for g in groups])
+    color_particles = np.ascontiguousarray(np.concatenate(groups) if groups else np.empty(0, np.int32), dtype=np.int32)
+    cap = int(color_offsets[-1]) + len(groups) + 1
+    batch_start = np.zeros(cap + 1, dtype=np.int32)
+    batch_color_offsets = np.zeros(len(groups) + 1, dtype=np.int32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    nb = _native.host_lib().vbd_host_build_batches(
+        len(groups),
+        color_offsets.ctypes.data_as(i32p),
+        color_particles.ctypes.data_as(i32p),
+        items.ctypes.data_as(i32p),
+        int(item_cap),
+        int(max_vertices),
+        cap,
+        batch_start.ctypes.data_as(i32p),
+        batch_color_offsets.ctypes.data_as(i32p),
+    )
+    if nb < 0:
+        raise RuntimeError(f"tile schedule construction failed ({nb})")
+    return color_offsets, color_particles, batch_start[: nb + 1].copy(), batch_color_offsets, items
+
+
+class SolverVBD:
+    """Vertex Block Descent for particles (cloth triangles, bending edges, tetrahedra) with soft contact."""
+
+    def __init__(
+        self,
+        model: Model,
+        *,
+        iterations: int = 10,
+        friction_epsilon: float = 1e-2,
+        integrate_with_external_rigid_solver: bool = False,
+        particle_enable_self_contact: bool = False,
+        particle_enable_tile_solve: bool = True,
+        rigid_avbd_beta: float = 0.0,
+        rigid_avbd_linear_beta: float | None = None,
+        rigid_contact_k_start: float = 1.0e2,
+        fp64_ill_conditioned: bool = False,
+        soft_contact_capacity: int | None = None,
+        **unsupported,
+    ):
+        if rigid_avbd_beta < 0:
+            raise ValueError(f"rigid_avbd_beta must be >= 0, got {rigid_avbd_beta}")
+        linear_beta = rigid_avbd_beta if rigid_avbd_linear_beta is None else rigid_avbd_linear_beta
+        if linear_beta < 0:
+            raise ValueError(f"rigid_avbd_linear_beta must be >= 0, got {linear_beta}")
+        if rigid_contact_k_start < 0:
+            raise ValueError(f"rigid_contact_k_start must be >= 0, got {rigid_contact_k_start}")
+        if particle_enable_self_contact:
+            raise NotImplementedError("particle self-contact is not part of this VBD path")
+        for key in unsupported:
+            if not (key.startswith("particle_") or key.startswith("rigid_") or key == "deterministic"):
+                raise TypeError(f"SolverVBD got an unexpected keyword argument {key!r}")
+        if model.body_count > 0:
+            raise NotImplementedError("rigid bodies are not part of this VBD path")
+        self.model = model
+        self.iterations = int(iterations)
+        self.friction_epsilon = float(friction_epsilon)
+        self.integrate_with_external_rigid_solver = integrate_with_external_rigid_solver
+        self.particle_enable_self_contact = False
+        self.particle_enable_tile_solve = particle_enable_tile_solve
+        self.rigid_linear_beta = float(linear_beta)
+        # fixed penalty (k = averaged ke) when the linear beta is 0 (solver_vbd.py:671)
+        self.rigid_contact_k_start_value = -1.0 if linear_beta == 0.0 else float(rigid_contact_k_start)
+        self.fp64_ill_conditioned = bool(fp64_ill_conditioned)
+        self._ctx = None
+        if model.particle_count == 0:
+            return
+        if model.soft_mesh_adjacency_device is None or model.soft_mesh_adjacency is None:
+            raise ValueError("model.soft_mesh_adjacency_device is missing; finalize the model with ModelBuilder.")
+        if len(model.particle_color_groups) == 0:
+            raise ValueError(
+                "model.particle_color_groups is empty! When using the SolverVBD you must call ModelBuilder.color() "
+                "or ModelBuilder.set_coloring() before calling ModelBuilder.finalize()."
+            )
+        if model.device.type != "cuda":
+            raise RuntimeError("SolverVBD runs on the GPU through libvbd_hip; finalize the model on a cuda device")
+        co, cp, bs, bco, items = build_tile_schedule(model)
+        if int(co[-1]) != model.particle_count or np.unique(cp).size != model.particle_count:
+            raise ValueError("particle_color_groups must contain every particle exactly once")
+        self.color_offsets = co
+        self.batch_color_offsets = bco
+        self.item_count = items
+        self.color_particles = torch.as_tensor(cp, device=model.device)
+        self.batch_start = torch.as_tensor(bs, device=model.device)
+        self.batch_count = int(len(bs) - 1)
+        if soft_contact_capacity is None:
+            soft_contact_capacity = model.particle_count * max(model.shape_count, 1)
+        self._capacity = int(soft_contact_capacity)
+        self._create_context()
+
+    # ------------------------------------------------------------- internals
+    def _create_context(self):
+        m = self.model
+        adj = m.soft_mesh_adjacency_device
+        d = _native.VbdModelDesc()
+        d.particle_count = m.particle_count
+        d.tri_count = m.tri_count
+        d.edge_count = m.edge_count
+        d.tet_count = m.tet_count
+        p = _native.ptr
+        d.particle_mass = p(m.particle_mass)
+        d.particle_inv_mass = p(m.particle_inv_mass)
+        d.particle_radius = p(m.particle_radius)
+        d.particle_flags = p(m.particle_flags)
+        d.tri_indices = p(m.tri_indices)
+        d.tri_poses = p(m.tri_poses)
+        d.tri_materials = p(m.tri_materials)
+        d.tri_areas = p(m.tri_areas)
+        d.edge_indices = p(m.edge_indices)
+        d.edge_rest_angle = p(m.edge_rest_angle)
+        d.edge_rest_length = p(m.edge_rest_length)
+        d.edge_bending_properties = p(m.edge_bending_properties)
+        d.tet_indices = p(m.tet_indices)
+        d.tet_poses = p(m.tet_poses)
+        d.tet_materials = p(m.tet_materials)
+        d.v_adj_tris = p(adj.v_adj_tris)
+        d.v_adj_tris_offsets = p(adj.v_adj_tris_offsets)
+        d.v_adj_edges = p(adj.v_adj_edges)
+        d.v_adj_edges_offsets = p(adj.v_adj_edges_offsets)
+        d.v_adj_tets = p(adj.v_adj_tets)
+        d.v_adj_tets_offsets = p(adj.v_adj_tets_offsets)
+        d.color_count = len(self.color_offsets) - 1
+        d.color_offsets_host = self.color_offsets.ctypes.data
+        d.color_particles = p(self.color_particles)
+        d.batch_count = self.batch_count
+        d.batch_color_offsets_host = self.batch_color_offsets.ctypes.data
+        d.batch_start = p(self.batch_start)
+        d.gravity = p(m.gravity)
+        d.shape_count = m.shape_count
+        d.shape_margin = p(m.shape_margin)
+        d.shape_material_ke = p(m.shape_material_ke)
+        d.shape_material_kd = p(m.shape_material_kd)
+        d.shape_material_mu = p(m.shape_material_mu)
+        d.soft_contact_ke = float(m.soft_contact_ke)
+        d.soft_contact_kd = float(m.soft_contact_kd)
+        d.soft_contact_mu = float(m.soft_contact_mu)
+        prm = _native.VbdSolverParams()
+        prm.iterations = self.iterations
+        prm.friction_epsilon = self.friction_epsilon
+        prm.avbd_beta = self.rigid_linear_beta
+        prm.k_start = self.rigid_contact_k_start_value
+        prm.fp64_ill_conditioned = int(self.fp64_ill_conditioned)
+        prm.soft_contact_capacity = self._capacity
+        self._desc, self._params = d, prm
+        lib = _native.hip_lib()
+        h = ctypes.c_void_p()
+        with torch.cuda.device(m.device):
+            _native.check(lib.vbd_create(ctypes.byref(d), ctypes.byref(prm), ctypes.byref(h)), "vbd_create")
+        self._ctx = h
+        self._material_key = self._contact_material_key()
+
+    def _contact_material_key(self):
+        m = self.model
+        return (float(m.soft_contact_ke), float(m.soft_contact_kd), float(m.soft_contact_mu), self.iterations)
+
+    def _destroy_context(self):
+        if self._ctx is not None and self._ctx.value:
+            _native.hip_lib().vbd_destroy(self._ctx)
+        self._ctx = None
+
+    def __del__(self):
+        try:
+            self._destroy_context()
+        except Exception:
+            pass
+
+    def _ensure_context(self, contacts: Contacts | None):
+        capturing = torch.cuda.is_current_stream_capturing()
+        need = contacts.soft_contact_max if contacts is not None else 0
+        rebuild = need > self._capacity or self._contact_material_key() != self._material_key
+        if rebuild:
+            if capturing:
+                raise RuntimeError(
+                    "SolverVBD: solver buffers would need to be rebuilt (larger soft_contact_max or changed "
+                    "contact material / iterations) during graph capture; run one step before capturing"
+                )
+            torch.cuda.synchronize(self.model.device)
+            self._capacity = max(self._capacity, need)
+            self._destroy_context()
+            self._create_context()
+
+    @staticmethod
+    def _contacts_desc(contacts: Contacts | None):
+        if contacts is None or contacts.soft_contact_max <= 0:
+            return None
+        c = _native.VbdContactsDesc()
+        c.soft_contact_count = _native.ptr(contacts.soft_contact_count)
+        c.soft_contact_max = contacts.soft_contact_max
+        c.soft_contact_indices = _native.ptr(contacts.soft_contact_indices)
+        c.soft_contact_shape = _native.ptr(contacts.soft_contact_shape)
+        c.soft_contact_body_pos = _native.ptr(contacts.soft_contact_body_pos)
+        c.soft_contact_body_vel = _native.ptr(contacts.soft_contact_body_vel)
+        c.soft_contact_normal = _native.ptr(contacts.soft_contact_normal)
+        return c
+
+    # ------------------------------------------------------------ public API
+    @property
+    def device(self):
+        return self.model.device
+
+    def step(self, state_in: State, state_out: State, control: Control | None, contacts: Contacts | None,
+             dt: float) -> None:
+        """One VBD step (``solver_vbd.py:1773-1825``).
+
+        ``state_in.particle_q`` is overwritten with the iterate;
+        ``state_out.particle_q`` / ``particle_qd`` receive the result.
+        """
+        if self.model.particle_count == 0:
+            return
+        self._ensure_context(contacts)
+        cd = self._contacts_desc(contacts)
+        f_ext = state_in.particle_f
+        rc = _native.hip_lib().vbd_step(
+            self._ctx,
+            current_stream_handle(self.model.device),
+            _native.ptr(state_in.particle_q),
+            _native.ptr(state_in.particle_qd),
+            _native.ptr(f_ext),
+            _native.ptr(state_out.particle_q),
+            _native.ptr(state_out.particle_qd),
+            ctypes.byref(cd) if cd is not None else None,
+            float(dt),
+        )
+        _native.check(rc, "SolverVBD.step")
+
+    # pieces of step(), for drivers that interleave residual checks
+    def begin_step(self, state_in: State, contacts: Contacts | None, dt: float) -> None:
+        """Contact preparation and the inertial forward step (``solver_vbd.py:2065-2100``, ``:2493-2514``)."""
+        self._ensure_context(contacts)
+        cd = self._contacts_desc(contacts)
+        lib = _native.hip_lib()
+        s = current_stream_handle(self.model.device)
+        _native.check(lib.vbd_contact_prepare(self._ctx, s, ctypes.byref(cd) if cd is not None else None),
+                      "vbd_contact_prepare")
+        _native.check(
+            lib.vbd_forward(self._ctx, s, _native.ptr(state_in.particle_q), _native.ptr(state_in.particle_qd),
+                            _native.ptr(state_in.particle_f), float(dt)),
+            "vbd_forward",
+        )
+
+    def iterate(self, state_in: State, dt: float, iterations: int = 1) -> None:
+        """Run ``iterations`` VBD iterations (all colors) on ``state_in.particle_q``."""
+        lib = _native.hip_lib()
+        s = current_stream_handle(self.model.device)
+        for _ in range(int(iterations)):
+            _native.check(lib.vbd_iteration(self._ctx, s, _native.ptr(state_in.particle_q), float(dt)),
+                          "vbd_iteration")
+
+    def color_pass(self, state_in: State, color: int, dt: float) -> None:
+        _native.check(
+            _native.hip_lib().vbd_color_pass(self._ctx, current_stream_handle(self.model.device), int(color),
+                                             _native.ptr(state_in.particle_q), float(dt)),
+            "vbd_color_pass",
+        )
+
+    def end_step(self, state_in: State, state_out: State, dt: float) -> None:
+        """Final copy and velocity update (``solver_vbd.py:3158-3169``)."""
+        _native.check(
+            _native.hip_lib().vbd_finalize(self._ctx, current_stream_handle(self.model.device),
+                                           _native.ptr(state_in.particle_q), _native.ptr(state_out.particle_q),
+                                           _native.ptr(state_out.particle_qd), float(dt)),
+            "vbd_finalize",
+        )
+
+    def residual(self, state_in: State, dt: float, g_out: torch.Tensor | None = None) -> torch.Tensor:
+        """Squared force-residual norm (device double tensor of shape [1]) at ``state_in.particle_q``."""
+        out = torch.empty(1, dtype=torch.float64, device=self.model.device)
+        _native.check(
+            _native.hip_lib().vbd_residual(self._ctx, current_stream_handle(self.model.device),
+                                           _native.ptr(state_in.particle_q), float(dt), _native.ptr(g_out),
+                                           _native.ptr(out)),
+            "vbd_residual",
+        )
+        return out
+
+    def scratch(self, which: str) -> torch.Tensor:
+        """Copy of the current substep's ``inertia`` / ``q_prev`` / ``displacement`` array ([N, 3])."""
+        idx = {"inertia": 0, "q_prev": 1, "displacement": 2}[which]
+        out = torch.empty((self.model.particle_count, 3), dtype=torch.float32, device=self.model.device)
+        _native.check(
+            _native.hip_lib().vbd_copy_scratch(self._ctx, current_stream_handle(self.model.device), idx,
+                                               _native.ptr(out)),
+            "vbd_copy_scratch",
+        )
+        return out
+
+    def notify_model_changed(self, flags: int = 0) -> None:
+        """Re-read model scalars (contact material) on the next step."""
+        self._material_key = None
+
+    def reset(self, state: State, world_mask=None, flags=None) -> None:
+        """The particle path keeps no state across steps (``CHANGELOG.md:23``): nothing to reset."""
+        return None
+
+    def update_contacts(self, contacts: Contacts) -> None:
+        return None
+
+    @classmethod
+    def register_custom_attributes(cls, builder) -> None:
+        return None
+
+
+__all__ = ["SolverVBD", "build_tile_schedule"]

@@ -1,0 +1,413 @@
+"""ctypes front end of the CPU oracle (``oracle/libvbd_oracle.so``) -- TEST INFRASTRUCTURE.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg use this
+module; the product path never imports it.  ``OracleModel.from_model`` copies a
+:class:`newton_amd.Model` (on any device) into host arrays laid out as the
+reference's Model, and the ``step`` / ``simulate`` / element functions run the
+C restatement in FP32 ("f32") or FP64 ("f64").
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libvbd_oracle.so"
+
+i32 = ctypes.c_int32
+f32 = ctypes.c_float
+vp = ctypes.c_void_p
+
+
+class _OracleModel(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", i32),
+        ("tri_count", i32),
+        ("edge_count", i32),
+        ("tet_count", i32),
+        ("particle_mass", vp),
+        ("particle_inv_mass", vp),
+        ("particle_radius", vp),
+        ("particle_flags", vp),
+        ("particle_colors", vp),
+        ("particle_world", vp),
+        ("tri_indices", vp),
+        ("tri_poses", vp),
+        ("tri_materials", vp),
+        ("tri_areas", vp),
+        ("edge_indices", vp),
+        ("edge_rest_angle", vp),
+        ("edge_rest_length", vp),
+        ("edge_bending_properties", vp),
+        ("tet_indices", vp),
+        ("tet_poses", vp),
+        ("tet_materials", vp),
+        ("v_adj_tris", vp),
+        ("v_adj_tris_offsets", vp),
+        ("v_adj_edges", vp),
+        ("v_adj_edges_offsets", vp),
+        ("v_adj_tets", vp),
+        ("v_adj_tets_offsets", vp),
+        ("color_count", i32),
+        ("color_offsets", vp),
+        ("color_particles", vp),
+        ("gravity", f32 * 3),
+        ("shape_count", i32),
+        ("shape_type", vp),
+        ("shape_body", vp),
+        ("shape_flags", vp),
+        ("shape_world", vp),
+        ("shape_transform", vp),
+        ("shape_scale", vp),
+        ("shape_margin", vp),
+        ("shape_material_ke", vp),
+        ("shape_material_kd", vp),
+        ("shape_material_mu", vp),
+        ("world_count", i32),
+        ("soft_contact_ke", f32),
+        ("soft_contact_kd", f32),
+        ("soft_contact_mu", f32),
+    ]
+
+
+class _OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("iterations", i32),
+        ("friction_epsilon", f32),
+        ("avbd_beta", f32),
+        ("k_start", f32),
+        ("use_libm_atan2", i32),
+        ("fp64_ill_conditioned", i32),
+        ("num_threads", i32),
+    ]
+
+
+class _OraclePairs(ctypes.Structure):
+    _fields_ = [("pair_count", i32), ("pairs", vp)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (gcc, no FMA contraction)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        for sfx in ("f32", "f64"):
+            for name in ("step", "create_soft_contacts", "simulate", "residual", "block_solve"):
+                getattr(L, f"vbd_oracle_{name}_{sfx}").restype = i32
+            for name in ("membrane", "bending", "tet", "body_contact"):
+                getattr(L, f"vbd_oracle_{name}_{sfx}").restype = None
+        L.vbd_oracle_atan2_f32.restype = f32
+        L.vbd_oracle_atan2_f32.argtypes = [f32, f32]
+        L.vbd_oracle_atan2_f64.restype = ctypes.c_double
+        L.vbd_oracle_atan2_f64.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _np(t, dtype):
+    if t is None:
+        return np.zeros(0, dtype=dtype)
+    if hasattr(t, "detach"):
+        t = t.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(t), dtype=dtype)
+
+
+def _p(a: np.ndarray):
+    """Address of a numpy array as a ctypes pointer argument (NULL when empty)."""
+    return ctypes.c_void_p(a.ctypes.data) if a.size else None
+
+
+def _addr(a: np.ndarray):
+    """Address for a c_void_p Structure field."""
+    return a.ctypes.data if a.size else None
+
+
+def _real(precision: str):
+    return (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
+
+
+class OracleModel:
+    """Host copy of a model in the oracle's layout (keeps the numpy arrays alive)."""
+
+    def __init__(self, model, color_groups=None):
+        adj = model.soft_mesh_adjacency
+        self.N = model.particle_count
+        a = {}
+        a["mass"] = _np(model.particle_mass, np.float32)
+        a["inv_mass"] = _np(model.particle_inv_mass, np.float32)
+        a["radius"] = _np(model.particle_radius, np.float32)
+        a["flags"] = _np(model.particle_flags, np.int32)
+        a["world"] = _np(model.particle_world, np.int32)
+        groups = color_groups if color_groups is not None else [_np(g, np.int32) for g in model.particle_color_groups]
+        groups = [np.asarray(g, dtype=np.int32) for g in groups]
+        colors = np.full(self.N, -1, dtype=np.int32)
+        for c, g in enumerate(groups):
+            colors[g] = c
+        a["colors"] = colors
+        a["color_offsets"] = np.concatenate([[0], np.cumsum([len(g) for g in groups])]).astype(np.int32)
+        a["color_particles"] = np.ascontiguousarray(np.concatenate(groups) if groups else np.zeros(0), dtype=np.int32)
+        a["tri_indices"] = _np(model.tri_indices, np.int32)
+        a["tri_poses"] = _np(model.tri_poses, np.float32)
+        a["tri_materials"] = _np(model.tri_materials, np.float32)
+        a["tri_areas"] = _np(model.tri_areas, np.float32)
+        a["edge_indices"] = _np(model.edge_indices, np.int32)
+        a["edge_rest_angle"] = _np(model.edge_rest_angle, np.float32)
+        a["edge_rest_length"] = _np(model.edge_rest_length, np.float32)
+        a["edge_bend"] = _np(model.edge_bending_properties, np.float32)
+        a["tet_indices"] = _np(model.tet_indices, np.int32)
+        a["tet_poses"] = _np(model.tet_poses, np.float32)
+        a["tet_materials"] = _np(model.tet_materials, np.float32)
+        for key in ("tris", "edges", "tets"):
+            vals = getattr(adj, f"v_adj_{key}")
+            offs = getattr(adj, f"v_adj_{key}_offsets")
+            a[f"adj_{key}"] = np.ascontiguousarray(vals if vals is not None else np.zeros(0), dtype=np.int32)
+            offs = offs if offs is not None and len(offs) else np.zeros(self.N + 1)
+            a[f"adj_{key}_off"] = np.ascontiguousarray(offs, dtype=np.int32)
+        a["shape_type"] = _np(model.shape_type, np.int32)
+        a["shape_body"] = _np(model.shape_body, np.int32)
+        a["shape_flags"] = _np(model.shape_flags, np.int32)
+        a["shape_world"] = _np(model.shape_world, np.int32)
+        a["shape_transform"] = _np(model.shape_transform, np.float32)
+        a["shape_scale"] = _np(model.shape_scale, np.float32)
+        a["shape_margin"] = _np(model.shape_margin, np.float32)
+        a["shape_ke"] = _np(model.shape_material_ke, np.float32)
+        a["shape_kd"] = _np(model.shape_material_kd, np.float32)
+        a["shape_mu"] = _np(model.shape_material_mu, np.float32)
+        self.arrays = a
+        s = _OracleModel()
+        s.particle_count = self.N
+        s.tri_count = model.tri_count
+        s.edge_count = model.edge_count
+        s.tet_count = model.tet_count
+        s.particle_mass = _addr(a["mass"])
+        s.particle_inv_mass = _addr(a["inv_mass"])
+        s.particle_radius = _addr(a["radius"])
+        s.particle_flags = _addr(a["flags"])
+        s.particle_colors = _addr(a["colors"])
+        s.particle_world = _addr(a["world"])
+        s.tri_indices = _addr(a["tri_indices"])
+        s.tri_poses = _addr(a["tri_poses"])
+        s.tri_materials = _addr(a["tri_materials"])
+        s.tri_areas = _addr(a["tri_areas"])
+        s.edge_indices = _addr(a["edge_indices"])
+        s.edge_rest_angle = _addr(a["edge_rest_angle"])
+        s.edge_rest_length = _addr(a["edge_rest_length"])
+        s.edge_bending_properties = _addr(a["edge_bend"])
+        s.tet_indices = _addr(a["tet_indices"])
+        s.tet_poses = _addr(a["tet_poses"])
+        s.tet_materials = _addr(a["tet_materials"])
+        s.v_adj_tris = _addr(a["adj_tris"])
+        s.v_adj_tris_offsets = _addr(a["adj_tris_off"])
+        s.v_adj_edges = _addr(a["adj_edges"])
+        s.v_adj_edges_offsets = _addr(a["adj_edges_off"])
+        s.v_adj_tets = _addr(a["adj_tets"])
+        s.v_adj_tets_offsets = _addr(a["adj_tets_off"])
+        s.color_count = len(groups)
+        s.color_offsets = _addr(a["color_offsets"])
+        s.color_particles = _addr(a["color_particles"])
+        g = _np(model.gravity, np.float32).reshape(-1, 3)[0]
+        s.gravity = (f32 * 3)(*[float(v) for v in g])
+        s.shape_count = model.shape_count
+        s.shape_type = _addr(a["shape_type"])
+        s.shape_body = _addr(a["shape_body"])
+        s.shape_flags = _addr(a["shape_flags"])
+        s.shape_world = _addr(a["shape_world"])
+        s.shape_transform = _addr(a["shape_transform"])
+        s.shape_scale = _addr(a["shape_scale"])
+        s.shape_margin = _addr(a["shape_margin"])
+        s.shape_material_ke = _addr(a["shape_ke"])
+        s.shape_material_kd = _addr(a["shape_kd"])
+        s.shape_material_mu = _addr(a["shape_mu"])
+        s.world_count = model.world_count
+        s.soft_contact_ke = float(model.soft_contact_ke)
+        s.soft_contact_kd = float(model.soft_contact_kd)
+        s.soft_contact_mu = float(model.soft_contact_mu)
+        self.struct = s
+        self.world_count = model.world_count
+        from newton_amd.sim.collide import world_compatible_pairs  # noqa: PLC0415
+
+        pairs = world_compatible_pairs(a["world"], a["shape_world"], model.world_count)
+        self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        self.pairs_struct = _OraclePairs(int(pairs.shape[0]), _addr(self.pairs))
+
+
+def params(iterations: int, friction_epsilon: float = 1e-2, avbd_beta: float = 0.0, k_start: float = -1.0,
+           use_libm_atan2: bool = False, fp64_ill_conditioned: bool = False, num_threads: int = 1) -> _OracleParams:
+    return _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
+                         int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads))
+
+
+class Contacts:
+    """Host soft-contact arrays in the oracle's REAL type."""
+
+    def __init__(self, capacity: int, precision: str = "f32"):
+        dt, _ = _real(precision)
+        self.max = int(capacity)
+        n = max(self.max, 1)
+        self.count = np.zeros(1, dtype=np.int32)
+        self.particle = np.zeros(n, dtype=np.int32)
+        self.indices = np.full((n, 3), -1, dtype=np.int32)
+        self.shape = np.zeros(n, dtype=np.int32)
+        self.body_pos = np.zeros((n, 3), dtype=dt)
+        self.body_vel = np.zeros((n, 3), dtype=dt)
+        self.normal = np.zeros((n, 3), dtype=dt)
+
+    @classmethod
+    def from_device(cls, contacts, precision: str = "f32"):
+        c = cls(contacts.soft_contact_max, precision)
+        dt, _ = _real(precision)
+        n = contacts.soft_contact_max
+        if n:
+            c.count[:] = contacts.soft_contact_count.cpu().numpy()
+            c.particle[:n] = contacts.soft_contact_particle.cpu().numpy()
+            c.indices[:n] = contacts.soft_contact_indices.cpu().numpy()
+            c.shape[:n] = contacts.soft_contact_shape.cpu().numpy()
+            c.body_pos[:n] = contacts.soft_contact_body_pos.cpu().numpy().astype(dt)
+            c.body_vel[:n] = contacts.soft_contact_body_vel.cpu().numpy().astype(dt)
+            c.normal[:n] = contacts.soft_contact_normal.cpu().numpy().astype(dt)
+        return c
+
+
+def create_soft_contacts(om: OracleModel, q: np.ndarray, margin: float, contacts: Contacts, precision: str = "f32"):
+    dt, ct = _real(precision)
+    q = np.ascontiguousarray(q, dtype=dt)
+    fn = getattr(lib(), f"vbd_oracle_create_soft_contacts_{precision}")
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(om.pairs_struct), _p(q), ct(margin), i32(contacts.max),
+            _p(contacts.count), _p(contacts.particle), _p(contacts.indices), _p(contacts.shape),
+            _p(contacts.body_pos), _p(contacts.body_vel), _p(contacts.normal))
+    if rc != 0:
+        raise RuntimeError(f"oracle create_soft_contacts failed ({rc})")
+    return int(contacts.count[0])
+
+
+def step(om: OracleModel, prm: _OracleParams, q, qd, f_ext, contacts: Contacts | None, dt: float,
+         precision: str = "f32", residual_history: bool = False):
+    """One SolverVBD.step; returns (q_iterate, q_out, qd_out[, residual norms per iteration])."""
+    rt, ct = _real(precision)
+    q = np.array(q, dtype=rt, copy=True).reshape(-1, 3)
+    qd = np.ascontiguousarray(qd, dtype=rt).reshape(-1, 3)
+    fe = None if f_ext is None else np.ascontiguousarray(f_ext, dtype=rt).reshape(-1, 3)
+    q_out = np.zeros_like(q)
+    qd_out = np.zeros_like(q)
+    hist = np.zeros(prm.iterations + 1, dtype=np.float64) if residual_history else None
+    if contacts is None:
+        count, cmax, ci, cs, cbp, cbv, cn = 0, 0, None, None, None, None, None
+    else:
+        count, cmax = int(contacts.count[0]), contacts.max
+        ci, cs = _p(contacts.indices), _p(contacts.shape)
+        keep = [np.ascontiguousarray(x, dtype=rt) for x in (contacts.body_pos, contacts.body_vel, contacts.normal)]
+        cbp, cbv, cn = (_p(x) for x in keep)
+    fn = getattr(lib(), f"vbd_oracle_step_{precision}")
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(prm), i32(count), i32(cmax), ci, cs, cbp, cbv, cn, _p(q), _p(qd),
+            None if fe is None else _p(fe), _p(q_out), _p(qd_out), ct(dt), None if hist is None else _p(hist))
+    if rc != 0:
+        raise RuntimeError(f"oracle step failed ({rc})")
+    if residual_history:
+        return q, q_out, qd_out, hist
+    return q, q_out, qd_out
+
+
+def simulate(om: OracleModel, prm: _OracleParams, q, qd, substeps: int, dt: float, margin: float, contact_max: int,
+             record_every: int = 0, f_ext=None, precision: str = "f32"):
+    """Frame loop clear_forces -> collide -> step -> swap; returns (q, qd, recorded q every `record_every`)."""
+    rt, ct = _real(precision)
+    q = np.array(q, dtype=rt, copy=True).reshape(-1, 3)
+    qd = np.array(qd, dtype=rt, copy=True).reshape(-1, 3)
+    fe = None if f_ext is None else np.ascontiguousarray(f_ext, dtype=rt).reshape(-1, 3)
+    nrec = substeps // record_every if record_every > 0 else 0
+    rec = np.zeros((max(nrec, 1), om.N, 3), dtype=rt)
+    fn = getattr(lib(), f"vbd_oracle_simulate_{precision}")
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(prm), ctypes.byref(om.pairs_struct), _p(q), _p(qd),
+            None if fe is None else _p(fe), i32(substeps), ct(dt), ct(margin), i32(contact_max), i32(record_every),
+            _p(rec))
+    if rc != 0:
+        raise RuntimeError(f"oracle simulate failed ({rc})")
+    return q, qd, rec[:nrec]
+
+
+def residual(om: OracleModel, prm: _OracleParams, contacts: Contacts | None, q, q_prev, inertia, dt: float,
+             precision: str = "f64"):
+    rt, ct = _real(precision)
+    q, q_prev, inertia = (np.ascontiguousarray(x, dtype=rt).reshape(-1, 3) for x in (q, q_prev, inertia))
+    g = np.zeros_like(q)
+    norm = np.zeros(1, dtype=np.float64)
+    if contacts is None:
+        args = (i32(0), i32(0), None, None, None, None, None)
+    else:
+        keep = [np.ascontiguousarray(x, dtype=rt) for x in (contacts.body_pos, contacts.body_vel, contacts.normal)]
+        args = (i32(int(contacts.count[0])), i32(contacts.max), _p(contacts.indices), _p(contacts.shape),
+                *(_p(x) for x in keep))
+    fn = getattr(lib(), f"vbd_oracle_residual_{precision}")
+    fn(ctypes.byref(om.struct), ctypes.byref(prm), *args, _p(q), _p(q_prev), _p(inertia), ct(dt), _p(g), _p(norm))
+    return g, float(norm[0])
+
+
+# ---- element-level entry points (for unit tests) ---------------------------
+def membrane(x, x_anchor, pose, area, mu, lmbd, kd, dt, v_order, precision="f64"):
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(3, 3)
+    xa = np.ascontiguousarray(x_anchor, dtype=rt).reshape(3, 3)
+    pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(4)
+    f, h = np.zeros(3, dtype=rt), np.zeros(9, dtype=rt)
+    getattr(lib(), f"vbd_oracle_membrane_{precision}")(_p(x), _p(xa), _p(pose), f32(area), f32(mu), f32(lmbd),
+                                                       f32(kd), ct(dt), i32(v_order), _p(f), _p(h))
+    return f, h.reshape(3, 3)
+
+
+def bending(x, x_anchor, rest_angle, rest_length, ke, kd, dt, v_order, precision="f64", use_libm_atan2=False):
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(4, 3)
+    xa = np.ascontiguousarray(x_anchor, dtype=rt).reshape(4, 3)
+    f, h = np.zeros(3, dtype=rt), np.zeros(9, dtype=rt)
+    getattr(lib(), f"vbd_oracle_bending_{precision}")(_p(x), _p(xa), f32(rest_angle), f32(rest_length), f32(ke),
+                                                      f32(kd), ct(dt), i32(v_order), i32(int(use_libm_atan2)),
+                                                      _p(f), _p(h))
+    return f, h.reshape(3, 3)
+
+
+def tet(x, x_prev, dm_inv, mu, lmbd, kd, dt, v_order, precision="f64"):
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(4, 3)
+    xp = np.ascontiguousarray(x_prev, dtype=rt).reshape(4, 3)
+    dm = np.ascontiguousarray(dm_inv, dtype=np.float32).reshape(9)
+    f, h = np.zeros(3, dtype=rt), np.zeros(9, dtype=rt)
+    getattr(lib(), f"vbd_oracle_tet_{precision}")(_p(x), _p(xp), _p(dm), f32(mu), f32(lmbd), f32(kd), ct(dt),
+                                                  i32(v_order), _p(f), _p(h))
+    return f, h.reshape(3, 3)
+
+
+def body_contact(x, x_prev, body_pos, body_vel, normal, radius, margin, ke, kd, mu, friction_epsilon, dt,
+                 precision="f64"):
+    rt, ct = _real(precision)
+    arrs = [np.ascontiguousarray(v, dtype=rt).reshape(3) for v in (x, x_prev, body_pos, body_vel, normal)]
+    f, h = np.zeros(3, dtype=rt), np.zeros(9, dtype=rt)
+    getattr(lib(), f"vbd_oracle_body_contact_{precision}")(*[_p(a) for a in arrs], ct(radius), ct(margin), ct(ke),
+                                                           ct(kd), ct(mu), ct(friction_epsilon), ct(dt), _p(f), _p(h))
+    return f, h.reshape(3, 3)
+
+
+def block_solve(h, f, fp64_ill_conditioned=False, precision="f32"):
+    rt, _ = _real(precision)
+    h = np.ascontiguousarray(h, dtype=rt).reshape(9)
+    f = np.ascontiguousarray(f, dtype=rt).reshape(3)
+    dx = np.zeros(3, dtype=rt)
+    ok = getattr(lib(), f"vbd_oracle_block_solve_{precision}")(_p(h), _p(f), i32(int(fp64_ill_conditioned)), _p(dx))
+    return bool(ok), dx
+
+
+def atan2(y, x, precision="f32"):
+    return getattr(lib(), f"vbd_oracle_atan2_{precision}")(y, x)
