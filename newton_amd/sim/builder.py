@@ -416,6 +416,12 @@ class ModelBuilder:
         self._edge_len.extend(np.linalg.norm(x4_minus_x3, axis=1))
         self._edge_bend.extend(np.stack([ke, kd], axis=1))
 
+    def add_edge(self, i: int, j: int, k: int, l: int, rest: float | None = None, edge_ke: float | None = None,
+                 edge_kd: float | None = None) -> None:
+        """Single bending hinge (``builder.py:8725-8796``)."""
+        self.add_edges([i], [j], [k], [l], rest=None if rest is None else [rest],
+                       edge_ke=None if edge_ke is None else [edge_ke], edge_kd=None if edge_kd is None else [edge_kd])
+
     def _add_soft_mesh_edges_from_triangles(self, start_tri: int, end_tri: int, *, edge_ke=None, edge_kd=None) -> range:
         """Register the triangle range's unique edges as bending hinges (``builder.py:8909-8941``)."""
         start = self.edge_count
@@ -572,6 +578,51 @@ class ModelBuilder:
         for v in faces.values():
             self.add_triangle(v[0], v[1], v[2], tri_ke=tri_ke, tri_ka=tri_ka, tri_kd=tri_kd, tri_drag=tri_drag,
                               tri_lift=tri_lift)
+        end_tri = self.tri_count
+        if add_surface_mesh_edges and end_tri > start_tri:
+            self._add_soft_mesh_edges_from_triangles(start_tri, end_tri, edge_ke=edge_ke, edge_kd=edge_kd)
+
+    def add_soft_mesh(self, *, pos, rot, scale: float, vel, vertices, indices, density: float | None = None,
+                      k_mu=None, k_lambda=None, k_damp=None, tri_ke: float = 0.0, tri_ka: float = 0.0,
+                      tri_kd: float = 0.0, tri_drag: float = 0.0, tri_lift: float = 0.0,
+                      add_surface_mesh_edges: bool = True, edge_ke: float = 0.0, edge_kd: float = 0.0,
+                      particle_radius=None) -> None:
+        """Tetrahedral soft body from vertices / tet indices (``builder.py:9497-9700``).
+
+        Mass is lumped as density * volume / 4 per tet corner; the surface
+        triangles (faces used by exactly one tet, in first-seen order) get the
+        given (default zero) triangle material and optional bending edges.
+        """
+        density = 1000.0 if density is None else density
+        k_mu = self.default_tet_k_mu if k_mu is None else k_mu
+        k_lambda = self.default_tet_k_lambda if k_lambda is None else k_lambda
+        k_damp = self.default_tet_k_damp if k_damp is None else k_damp
+        ind = np.asarray(indices, dtype=np.int64).reshape(-1, 4)
+        verts = np.asarray(vertices, dtype=np.float32).reshape(-1, 3) * np.float32(scale)
+        p = (quat_rotate(rot, verts) + np.asarray(pos, dtype=np.float32)).astype(np.float32)
+        start = self.particle_count
+        n = p.shape[0]
+        self.add_particles(p.astype(np.float64), np.tile(np.asarray(vel, dtype=np.float32), (n, 1)), np.zeros(n),
+                           None if particle_radius is None else np.full(n, particle_radius))
+        t = ind + start
+        vols = self.add_tetrahedra(t[:, 0], t[:, 1], t[:, 2], t[:, 3], np.float32(k_mu), np.float32(k_lambda),
+                                   np.float32(k_damp))
+        mass = self._pmass.array()
+        ok = vols > 0.0
+        contrib = density * vols[ok] / 4.0
+        np.add.at(mass, t[ok].reshape(-1), np.repeat(contrib, 4))
+        faces: dict = {}
+        for i, j, k, l in ind.tolist():
+            for f in ((i, k, j), (j, k, l), (i, j, l), (i, l, k)):
+                key = tuple(sorted(f))
+                if key in faces:
+                    del faces[key]
+                else:
+                    faces[key] = f
+        start_tri = self.tri_count
+        for f in faces.values():
+            self.add_triangle(start + f[0], start + f[1], start + f[2], tri_ke=tri_ke, tri_ka=tri_ka, tri_kd=tri_kd,
+                              tri_drag=tri_drag, tri_lift=tri_lift)
         end_tri = self.tri_count
         if add_surface_mesh_edges and end_tri > start_tri:
             self._add_soft_mesh_edges_from_triangles(start_tri, end_tri, edge_ke=edge_ke, edge_kd=edge_kd)
