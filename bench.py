@@ -32,7 +32,91 @@ def parse():
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time budget (0 disables)")
     ap.add_argument("--roofline-reps", type=int, default=20)
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per color pass")
+    ap.add_argument("--pmc-mode", action="store_true",
+                    help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
+                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     return ap.parse_args()
+
+
+def pmc_mode(args):
+    """Minimal workload for rocprofv3 --pmc: `reps` launches of each color pass and one known-size copy."""
+    import numpy as np
+    import torch
+
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.roofline import color_pass_bytes
+    from newton_amd.solvers import SolverVBD
+
+    dev = "cuda:0"
+    model, state, info = scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
+    solver = SolverVBD(model, iterations=args.iterations)
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    state.clear_forces()
+    pipe.collide(state, contacts)
+    solver.begin_step(state, contacts, info["dt"])
+    ncol = len(model.particle_color_groups)
+    for _ in range(args.roofline_reps):
+        for c in range(ncol):
+            solver.color_pass(state, c, info["dt"])
+    nbytes = 256 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    counts = np.bincount(contacts.soft_contact_particle.cpu().numpy()[: int(contacts.soft_contact_count.item())],
+                         minlength=model.particle_count)
+    b = color_pass_bytes(model, contacts_per_particle=counts)
+    print(json.dumps({"pmc_mode": True, "color_pass_launches": args.roofline_reps * ncol,
+                      "algorithmic_bytes_per_color_pass": [int(x) for x in b], "copy_bytes": nbytes}))
+
+
+def measure_traffic(args, timeout_s: float = 240.0):
+    """HBM bytes per color-pass launch from two rocprofv3 --pmc child runs (FETCH_SIZE, then WRITE_SIZE).
+
+    Each counter is scaled by what it reads for the 256 MiB device copy made in
+    the same run (bytes copied / counter bytes), so the figure is in real bytes.
+    Returns (bytes_per_launch, details) or (None, reason).
+    """
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+                   str(ROOT / "bench.py"), "--pmc-mode", "--verts", str(args.verts), "--iterations",
+                   str(args.iterations), "--roofline-reps", "5"]
+            try:
+                proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
+                                      env={**os.environ, "TMPDIR": "/tmp"})
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            if proc.returncode != 0:
+                return None, f"rocprofv3 --pmc {counter} failed ({proc.returncode})"
+            path = os.path.join(d, "pmc_counter_collection.csv")
+            if not os.path.exists(path):
+                return None, f"no counter output for {counter}"
+            rows = list(csv.DictReader(open(path)))
+            solve = [float(r["Counter_Value"]) for r in rows if "k_solve_color" in r["Kernel_Name"]]
+            copies = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == "__amd_rocclr_copyBuffer"]
+            if not solve or not copies:
+                return None, f"counter rows missing for {counter}"
+            scale = (256 << 20) / (max(copies) * 1024.0)
+            out[counter] = (sum(solve) / len(solve) * 1024.0, scale)
+    fetch, fs = out["FETCH_SIZE"]
+    write, ws = out["WRITE_SIZE"]
+    total = fetch * fs + write * ws
+    return total, {"fetch_bytes": round(fetch * fs), "write_bytes": round(write * ws), "fetch_scale": round(fs, 4),
+                   "write_scale": round(ws, 4)}
 
 
 def main():
@@ -41,6 +125,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (worlds would shard "
                           "across GPUs with no per-step exchange, see DESIGN.md)"}))
         sys.exit(2)
+    if args.pmc_mode:
+        pmc_mode(args)
+        return
     import numpy as np
     import torch
 
@@ -132,10 +219,16 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "traffic_detail": None,
         "bytes_per_launch": round(total_bytes / ncol, 1),
         "avg_launch_us": round(float(color_ms.mean()) * 1e3, 3),
         "launches_per_substep": ncol * args.iterations,
     }
+
+    if args.traffic:
+        traffic, detail = measure_traffic(args)
+        roofline["traffic"] = None if traffic is None else round(traffic, 1)
+        roofline["traffic_detail"] = detail
 
     # ---- CPU baseline: the FP32 CPU oracle on a bounded sample of the same workload
     cpu = None
