@@ -106,7 +106,7 @@ def measure_traffic(args, timeout_s: float = 240.0):
             if not os.path.exists(path):
                 return None, f"no counter output for {counter}"
             rows = list(csv.DictReader(open(path)))
-            solve = [float(r["Counter_Value"]) for r in rows if "k_solve_color" in r["Kernel_Name"]]
+            solve = [float(r["Counter_Value"]) for r in rows if "k_solve_fast" in r["Kernel_Name"]]
             copies = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"] == "__amd_rocclr_copyBuffer"]
             if not solve or not copies:
                 return None, f"counter rows missing for {counter}"
@@ -212,7 +212,7 @@ def main():
     total_s = float(color_ms.sum()) / 1e3
     achieved = total_bytes / total_s / 1e9
     roofline = {
-        "kernel": "k_solve_color",
+        "kernel": "k_solve_fast",
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,

@@ -70,10 +70,7 @@ typedef struct VbdModelDesc {
     const int32_t* v_adj_tets_offsets;
     int32_t color_count;
     const int32_t* color_offsets_host;     /* [C+1] host */
-    const int32_t* color_particles;        /* concatenated color groups */
-    int32_t batch_count;                   /* color-pass tile schedule */
-    const int32_t* batch_color_offsets_host; /* [C+1] host: batches of color c */
-    const int32_t* batch_start;            /* [B+1]: first index into color_particles */
+    const int32_t* color_particles;        /* concatenated color groups (every particle once) */
     const float* gravity;                  /* vec3; world 0 is used for every particle */
     int32_t shape_count;
     const float* shape_margin;             /* [S] */
@@ -139,9 +136,19 @@ const char* vbd_last_error(void);
 int32_t vbd_version(void);
 
 /* SolverVBD.__init__ (particle part): stores the model pointers (the caller
- * keeps them alive), allocates every scratch buffer. */
+ * keeps them alive), allocates every scratch buffer, reads the static model
+ * arrays back once and builds the color-pass layout (element records packed
+ * per workgroup batch).  Synchronises the device; not for use during capture. */
 int32_t vbd_create(const VbdModelDesc* model, const VbdSolverParams* params, VbdContext** out);
 int32_t vbd_destroy(VbdContext* ctx);
+
+/* Rebuilds the color-pass layout after the caller changed static model arrays
+ * (element materials, rest data, flags, masses).  SolverVBD.notify_model_changed
+ * (newton/_src/solvers/solver.py:375).  Allocates; not for use during capture. */
+int32_t vbd_rebuild(VbdContext* ctx);
+
+/* Layout statistics: out4 = {fast batches, generic-path vertices, packed element items, colors}. */
+int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
 
 /* Full substep: contact prepare, forward, iterations x colors, finalize.
  * q_in is overwritten with the iterate (as the reference does); q_out/qd_out

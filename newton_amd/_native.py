@@ -52,9 +52,6 @@ class VbdModelDesc(ctypes.Structure):
         ("color_count", c_int32),
         ("color_offsets_host", c_void_p),
         ("color_particles", c_void_p),
-        ("batch_count", c_int32),
-        ("batch_color_offsets_host", c_void_p),
-        ("batch_start", c_void_p),
         ("gravity", c_void_p),
         ("shape_count", c_int32),
         ("shape_margin", c_void_p),
@@ -128,6 +125,8 @@ HIP_EXPORTS = {
     "vbd_version": (c_int32, []),
     "vbd_create": (c_int32, [ctypes.POINTER(VbdModelDesc), ctypes.POINTER(VbdSolverParams), ctypes.POINTER(c_void_p)]),
     "vbd_destroy": (c_int32, [c_void_p]),
+    "vbd_rebuild": (c_int32, [c_void_p]),
+    "vbd_layout_stats": (c_int32, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "vbd_step": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc), c_float],
@@ -156,20 +155,6 @@ HOST_EXPORTS = {
             c_float,
             ctypes.POINTER(c_int32),
             ctypes.POINTER(c_float),
-        ],
-    ),
-    "vbd_host_build_batches": (
-        ctypes.c_int64,
-        [
-            c_int32,
-            ctypes.POINTER(c_int32),
-            ctypes.POINTER(c_int32),
-            ctypes.POINTER(c_int32),
-            c_int32,
-            c_int32,
-            ctypes.c_int64,
-            ctypes.POINTER(c_int32),
-            ctypes.POINTER(c_int32),
         ],
     ),
 }

@@ -6,9 +6,11 @@
 //                      per-particle contact list in contact-index order
 //                      (rigid_vbd_kernels.py:799-815, :3276-3321)
 //   k_dual_update      per-iteration penalty ramp   (rigid_vbd_kernels.py:4861-4926)
-//   k_solve_color      one color pass: element forces/Hessians gathered through
-//                      the CSR adjacency, contact forces, guarded 3x3 solve,
-//                      position update   (particle_vbd_kernels.py:2289-2401, :2568-2706)
+//   k_solve_fast       one color pass over the packed pass layout (vbd_layout.h):
+//                      element and contact forces/Hessians evaluated lane-parallel,
+//                      serial canonical-order sums, guarded 3x3 solve, position
+//                      update   (particle_vbd_kernels.py:2289-2401, :2568-2706)
+//   k_solve_generic    the same for vertices with more than 256 elements (CSR walk)
 //   k_finalize         velocity update   (particle_vbd_kernels.py:1468-1471)
 //   k_collide_*        particle vs static plane/sphere soft contacts (geometry/kernels.py:1020-1184)
 //
@@ -28,6 +30,7 @@
 #include <vector>
 
 #include "vbd_hip.h"
+#include "vbd_layout.h"
 
 namespace {
 
@@ -279,15 +282,17 @@ __device__ __forceinline__ M3 nvd_skew(float len, V3 nh, V3 v) {
     return mul_skew_right(P, v);
 }
 
-// Dihedral-angle bending, particle_vbd_kernels.py:635-816.  Only the angle
-// derivative of the vertex being solved is formed (the reference forms all
-// four and selects one with 0/1 masks, which adds signed zeros only).
-__device__ void eval_bending(const float* __restrict__ q, const float* __restrict__ qp, const int* __restrict__ ei,
-                             float rest_angle, float rest_length, float stiffness, float damping, float dt, int v_order,
-                             V3& f_out, M3& h_out) {
+// Dihedral-angle bending, particle_vbd_kernels.py:635-816.  Only the solved
+// vertex's angle derivative is formed, in one branch-free form for all four
+// slots: with s1, s2 the vectors whose skew matrices enter dn1/dx and dn2/dx
+// (zero where the reference uses a zero matrix, :730-741) the reference's
+// products are reproduced term for term; the terms it adds for the other
+// slots, and its 0/1 mask selection, contribute signed zeros only.
+__device__ void eval_bending(const float* __restrict__ q, const float* __restrict__ qp, int i0, int i1, int i2,
+                             int i3, float rest_angle, float rest_length, float stiffness, float damping, float dt,
+                             int v_order, V3& f_out, M3& h_out) {
     f_out = v3(0.0f, 0.0f, 0.0f);
     h_out = mzero();
-    int i0 = ei[0], i1 = ei[1], i2 = ei[2], i3 = ei[3];
     if (i0 == -1 || i1 == -1) return;
     const float eps = 1.0e-6f;
     V3 x0 = ld3(q, i0), x1 = ld3(q, i1), x2 = ld3(q, i2), x3 = ld3(q, i3);
@@ -301,41 +306,20 @@ __device__ void eval_bending(const float* __restrict__ q, const float* __restric
     float theta = portable_atan2f(sin_t, cos_t);
     float k = stiffness * rest_length;
     float dE = k * (theta - rest_angle);
-    V3 dsin, dcos;
-    if (v_order == 0) {
-        M3 dn1 = nvd_skew(n1n, n1h, e);
-        M3 t2 = mul_skew_left(n2h, dn1);
-        M3 Mm;
+    const V3 zero = v3(0.0f, 0.0f, 0.0f);
+    V3 s1 = v_order == 0 ? e : (v_order == 1 ? zero : (v_order == 2 ? neg(x03) : x02));
+    V3 s2 = v_order == 0 ? zero : (v_order == 1 ? neg(e) : (v_order == 2 ? x13 : neg(x12)));
+    M3 dn1 = nvd_skew(n1n, n1h, s1);
+    M3 dn2 = nvd_skew(n2n, n2h, s2);
+    M3 a = mul_skew_left(n1h, dn2);
+    M3 b = mul_skew_left(n2h, dn1);
+    M3 Mm;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) Mm.m[i][j] = -t2.m[i][j];
-        dsin = mtvec(Mm, eh);
-        dcos = mtvec(dn1, n2h);
-    } else if (v_order == 1) {
-        M3 dn2 = nvd_skew(n2n, n2h, neg(e));
-        M3 Mm = mul_skew_left(n1h, dn2);
-        dsin = mtvec(Mm, eh);
-        dcos = mtvec(dn2, n1h);
-    } else {
-        M3 dn1, dn2;
-        if (v_order == 2) {
-            dn1 = nvd_skew(n1n, n1h, neg(x03));
-            dn2 = nvd_skew(n2n, n2h, x13);
-        } else {
-            dn1 = nvd_skew(n1n, n1h, x02);
-            dn2 = nvd_skew(n2n, n2h, neg(x12));
-        }
-        M3 a = mul_skew_left(n1h, dn2);
-        M3 b = mul_skew_left(n2h, dn1);
-        M3 Mm;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) Mm.m[i][j] = a.m[i][j] - b.m[i][j];
-        dsin = mtvec(Mm, eh);
-        dcos = add(mtvec(dn1, n2h), mtvec(dn2, n1h));
-    }
+        for (int j = 0; j < 3; ++j) Mm.m[i][j] = a.m[i][j] - b.m[i][j];
+    V3 dsin = mtvec(Mm, eh);
+    V3 dcos = add(mtvec(dn1, n2h), mtvec(dn2, n1h));
     V3 dth = sub(scl(dsin, cos_t), scl(dcos, sin_t));
     V3 bf = scl(dth, -dE);
     M3 bh;
@@ -637,11 +621,19 @@ struct KParams {
     const int* adj_tet_off;
     int has_tri, has_edge, has_tet;
     const int* color_particles;
-    const int* batch_start;
     const float* gravity;
     float* q_prev;
     float* inertia;
     float* disp;
+    // pass layout (vbd_layout.h)
+    const int* batch_rec;
+    const int* vert_rec;
+    const int* tri_rec;
+    const int* edge_rec;
+    const int* edge_meta;
+    const int* tet_rec;
+    const int* tet_meta;
+    const int* slow_pos;
     // per-particle contact lists (contact-index order) and per-contact state
     const int* c_off;
     const int* c_ids;
@@ -652,11 +644,19 @@ struct KParams {
     const float* c_body_pos;
     const float* c_body_vel;
     const float* c_normal;
+    // contacts packed per color position (k_contact_pack)
+    const int* pos_coff;
+    const float* cpack;
     const float* shape_margin;
     int shape_count;
     float friction_epsilon;
     int fp64_ill;
 };
+
+// packed contact record: 4 x float4
+//   {particle, local slot, ke, kd} {mu, margin, radius, bx} {by, bz, vx, vy} {vz, nx, ny, nz}
+constexpr int kContactWords = 16;
+constexpr int kRowsMax = 512;  // LDS rows of one batch: static items (<= 256) + contacts
 
 __device__ __forceinline__ bool particle_active(const KParams& P, int v) {
     return (P.flags[v] & 1) && P.mass[v] != 0.0f;
@@ -672,56 +672,103 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     return xcd * qn + (xcd < rn ? xcd : rn) + idx;
 }
 
-// mode 0: solve and update the color's vertices; mode 1: write the assembled force g.
-__global__ __launch_bounds__(kBlock) void k_solve_color(KParams P, float* __restrict__ q, int batch_base, int nb,
-                                                        float dt, int mode, float* __restrict__ g_out) {
+__device__ __forceinline__ void store_row(float* __restrict__ row, V3 f, const M3& h) {
+    row[0] = f.x;
+    row[1] = f.y;
+    row[2] = f.z;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) row[3 + 3 * i + j] = h.m[i][j];
+}
+
+__device__ __forceinline__ void eval_packed_contact(const KParams& P, const float* __restrict__ q, int rec, float dt,
+                                                    V3& f, M3& h) {
+    const float4* c4 = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)rec;
+    float4 A = c4[0], B = c4[1], C = c4[2], D = c4[3];
+    int p = __float_as_int(A.x);
+    eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z, B.y,
+                 A.z, A.w, B.x, P.friction_epsilon, dt, f, h);
+}
+
+// Finish one vertex: contacts (when not already summed), then the block solve
+// and position update (mode 0) or the force output (mode 1).
+__device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 f, M3 h,
+                                              bool add_contacts, int crec0, int crec1, float dt, int mode,
+                                              float* __restrict__ g_out) {
+    if (!active) {
+        if (mode == 0) {
+            V3 z = v3(0.0f, 0.0f, 0.0f);
+            st3(P.disp, v, z);
+            st3(q, v, add(ld3(P.q_prev, v), scl(z, 1.0f)));
+        } else {
+            st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
+        }
+        return;
+    }
+    if (add_contacts) {
+        V3 cf = v3(0.0f, 0.0f, 0.0f);
+        M3 ch = mzero();
+        for (int j = crec0; j < crec1; ++j) {
+            V3 fc;
+            M3 hc;
+            eval_packed_contact(P, q, j, dt, fc, hc);
+            cf = add(cf, fc);
+            ch = madd(ch, hc);
+        }
+        h = madd(h, ch);
+        f = add(f, cf);
+    }
+    if (mode == 1) {
+        st3(g_out, v, f);
+        return;
+    }
+    V3 dx;
+    if (block_solve(h, f, P.fp64_ill, dx)) {
+        V3 d = add(ld3(P.disp, v), dx);
+        st3(P.disp, v, d);
+        st3(q, v, add(ld3(P.q_prev, v), scl(d, 1.0f)));
+    }
+}
+
+// One color pass over packed batches (mode 0: solve and update; mode 1: write the assembled force g).
+//   phase 0  batch record (uniform) and vertex records
+//   phase 1  every lane evaluates one triangle / edge / tet / contact item into its canonical LDS row
+//   phase 2  lane (vertex s, component c) sums component c of s's rows serially, in canonical order,
+//            starting from the inertia term; contacts are summed from zero and added afterwards
+//   phase 3  the vertex's lane solves the 3x3 block and writes displacement and position
+__global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
+                                                       float dt, int mode, float* __restrict__ g_out) {
     __shared__ int s_v[kMaxV];
-    __shared__ int s_cnt[3][kMaxV];
-    __shared__ int s_beg[3][kMaxV];
-    __shared__ int s_start[kMaxV + 1];
-    __shared__ float s_item[kBlock * kComp];
+    __shared__ int s_row[kMaxV];
+    __shared__ int s_nrow[kMaxV];
+    __shared__ int s_act[kMaxV];
+    __shared__ int s_cb[kMaxV + 1];
+    __shared__ __align__(16) float s_rows[kRowsMax * kComp];
     __shared__ float s_acc[kMaxV * kComp];
 
     const int t = threadIdx.x;
     const int b = batch_base + xcd_remap(blockIdx.x, nb);
-    const int vs = P.batch_start[b];
-    const int nv = P.batch_start[b + 1] - vs;
+    const int4* br = reinterpret_cast<const int4*>(P.batch_rec) + 2 * (size_t)b;
+    const int4 b0 = br[0], b1 = br[1];
+    const int vb = b0.x, nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
+    const int nstatic = ntri + nedge + ntet;
+    const int cbeg = P.pos_coff[vb];
+    const int ncont = P.pos_coff[vb + nv] - cbeg;
+    const bool par_contacts = nstatic + ncont <= kRowsMax;
     const float dtr = 1.0f / (dt * dt);
 
     if (t < nv) {
-        int v = P.color_particles[vs + t];
-        s_v[t] = v;
-        int nt = 0, ne = 0, nk = 0, bt = 0, be = 0, bk = 0;
-        if (particle_active(P, v)) {
-            if (P.has_tri) {
-                bt = P.adj_tri_off[v];
-                nt = (P.adj_tri_off[v + 1] - bt) >> 1;
-            }
-            if (P.has_edge) {
-                be = P.adj_edge_off[v];
-                ne = (P.adj_edge_off[v + 1] - be) >> 1;
-            }
-            if (P.has_tet) {
-                bk = P.adj_tet_off[v];
-                nk = (P.adj_tet_off[v + 1] - bk) >> 1;
-            }
-        }
-        s_cnt[0][t] = nt; s_cnt[1][t] = ne; s_cnt[2][t] = nk;
-        s_beg[0][t] = bt; s_beg[1][t] = be; s_beg[2][t] = bk;
+        int4 vr = reinterpret_cast<const int4*>(P.vert_rec)[vb + t];
+        s_v[t] = vr.x;
+        s_row[t] = vr.y;
+        s_nrow[t] = vr.z;
+        s_act[t] = vr.w >> 16;
+        s_cb[t] = P.pos_coff[vb + t] - cbeg;
     }
+    if (t == kBlock - 1) s_cb[nv] = ncont;
     __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int s = 0; s < nv; ++s) {
-            s_start[s] = acc;
-            acc += s_cnt[0][s] + s_cnt[1][s] + s_cnt[2][s];
-        }
-        s_start[nv] = acc;
-    }
-    __syncthreads();
-    const int total = s_start[nv];
 
-    // lane (cs, cc) owns component cc of vertex cs's force (cc < 3) or Hessian
     const int cs = t / kComp;
     const int cc = t - cs * kComp;
     const bool comp_lane = cs < nv;
@@ -737,70 +784,63 @@ __global__ __launch_bounds__(kBlock) void k_solve_color(KParams P, float* __rest
         }
     }
 
-    for (int lo = 0; lo < total; lo += kBlock) {
-        const int item = lo + t;
-        if (item < total) {
-            int s = 0;
-            while (s_start[s + 1] <= item) ++s;
-            int k = item - s_start[s];
-            V3 fe = v3(0.0f, 0.0f, 0.0f);
-            M3 he = mzero();
-            if (k < s_cnt[0][s]) {
-                int base = s_beg[0][s] + 2 * k;
-                int tri = P.adj_tri[base], slot = P.adj_tri[base + 1];
-                const float* mat = P.tri_mat + 5 * tri;
-                float mu = mat[0], la = mat[1];
-                if (mu > 0.0f || la > 0.0f) {
-                    const int* ti = P.tri_idx + 3 * tri;
-                    int i0 = ti[0], i1 = ti[1], i2 = ti[2];
-                    eval_membrane(ld3(q, i0), ld3(q, i1), ld3(q, i2), P.q_prev, i0, i1, i2, P.tri_pose + 4 * tri,
-                                  P.tri_area[tri], mu, la, mat[2], dt, slot, fe, he);
-                }
-            } else if (k < s_cnt[0][s] + s_cnt[1][s]) {
-                int base = s_beg[1][s] + 2 * (k - s_cnt[0][s]);
-                int ed = P.adj_edge[base], slot = P.adj_edge[base + 1];
-                float ke = P.edge_bend[2 * ed];
-                if (ke > 0.0f)
-                    eval_bending(q, P.q_prev, P.edge_idx + 4 * ed, P.edge_rest_angle[ed], P.edge_rest_len[ed], ke,
-                                 P.edge_bend[2 * ed + 1], dt, slot, fe, he);
-            } else {
-                int base = s_beg[2][s] + 2 * (k - s_cnt[0][s] - s_cnt[1][s]);
-                int tet = P.adj_tet[base], slot = P.adj_tet[base + 1];
-                const float* mat = P.tet_mat + 3 * tet;
-                if (mat[0] > 0.0f || mat[1] > 0.0f)
-                    eval_tet(q, P.q_prev, P.tet_idx + 4 * tet, P.tet_pose + 9 * tet, mat[0], mat[1], mat[2], dt, slot,
-                             fe, he);
-            }
-            float* row = s_item + t * kComp;
-            row[0] = fe.x; row[1] = fe.y; row[2] = fe.z;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) row[3 + 3 * i + j] = he.m[i][j];
+    const int nitems = nstatic + (par_contacts ? ncont : 0);
+    for (int it = t; it < nitems; it += kBlock) {
+        V3 fe = v3(0.0f, 0.0f, 0.0f);
+        M3 he = mzero();
+        int row;
+        if (it < ntri) {
+            const int4* r4 = reinterpret_cast<const int4*>(P.tri_rec) + 3 * (size_t)(tri_b + it);
+            int4 a = r4[0];
+            float4 pz = __builtin_bit_cast(float4, r4[1]);
+            float4 mt = __builtin_bit_cast(float4, r4[2]);
+            row = a.w & 0xffff;
+            float pose[4] = {pz.x, pz.y, pz.z, pz.w};
+            eval_membrane(ld3(q, a.x), ld3(q, a.y), ld3(q, a.z), P.q_prev, a.x, a.y, a.z, pose, mt.x, mt.y, mt.z, mt.w,
+                          dt, a.w >> 16, fe, he);
+        } else if (it < ntri + nedge) {
+            const int k = it - ntri;
+            const int4* r4 = reinterpret_cast<const int4*>(P.edge_rec) + 2 * (size_t)(edge_b + k);
+            int4 a = r4[0];
+            float4 pr = __builtin_bit_cast(float4, r4[1]);
+            int mt = P.edge_meta[edge_b + k];
+            row = mt & 0xffff;
+            eval_bending(q, P.q_prev, a.x, a.y, a.z, a.w, pr.x, pr.y, pr.z, pr.w, dt, mt >> 16, fe, he);
+        } else if (it < nstatic) {
+            const int k = it - ntri - nedge;
+            const int4* r4 = reinterpret_cast<const int4*>(P.tet_rec) + 4 * (size_t)(tet_b + k);
+            int4 a = r4[0];
+            float4 d0 = __builtin_bit_cast(float4, r4[1]);
+            float4 d1 = __builtin_bit_cast(float4, r4[2]);
+            float4 d2 = __builtin_bit_cast(float4, r4[3]);
+            int ti[4] = {a.x, a.y, a.z, a.w};
+            float dm[9] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, d2.x};
+            int mt = P.tet_meta[tet_b + k];
+            row = mt & 0xffff;
+            eval_tet(q, P.q_prev, ti, dm, d2.y, d2.z, d2.w, dt, mt >> 16, fe, he);
+        } else {
+            const int j = it - nstatic;
+            eval_packed_contact(P, q, cbeg + j, dt, fe, he);
+            row = nstatic + j;
         }
-        __syncthreads();
-        if (comp_lane) {
-            int a = max(lo, s_start[cs]);
-            int e = min(lo + kBlock, s_start[cs + 1]);
-            for (int i = a; i < e; ++i) acc = acc + s_item[(i - lo) * kComp + cc];
-        }
-        __syncthreads();
+        store_row(s_rows + row * kComp, fe, he);
     }
-    if (comp_lane) s_acc[cs * kComp + cc] = acc;
+    __syncthreads();
+
+    if (comp_lane) {
+        const int r0 = s_row[cs], r1 = r0 + s_nrow[cs];
+        for (int r = r0; r < r1; ++r) acc = acc + s_rows[r * kComp + cc];
+        if (par_contacts) {
+            const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
+            float ca = 0.0f;
+            for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
+            acc = acc + ca;
+        }
+        s_acc[cs * kComp + cc] = acc;
+    }
     __syncthreads();
 
     if (t < nv) {
-        const int v = s_v[t];
-        if (!particle_active(P, v)) {
-            if (mode == 0) {
-                V3 z = v3(0.0f, 0.0f, 0.0f);
-                st3(P.disp, v, z);
-                st3(q, v, add(ld3(P.q_prev, v), scl(z, 1.0f)));
-            } else {
-                st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
-            }
-            return;
-        }
         const float* a = s_acc + t * kComp;
         V3 f = v3(a[0], a[1], a[2]);
         M3 h;
@@ -808,36 +848,128 @@ __global__ __launch_bounds__(kBlock) void k_solve_color(KParams P, float* __rest
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
-        V3 cf = v3(0.0f, 0.0f, 0.0f);
-        M3 ch = mzero();
-        const int c0 = P.c_off[v], c1 = P.c_off[v + 1];
-        if (c1 > c0) {
-            V3 x = ld3(q, v), xp = ld3(P.q_prev, v);
-            float rad = P.radius[v];
-            for (int j = c0; j < c1; ++j) {
-                int cid = P.c_ids[j];
-                int sh = P.c_shape[cid];
-                float margin = P.shape_count > 0 ? P.shape_margin[sh] : 0.0f;
-                V3 fc;
-                M3 hc;
-                eval_contact(x, xp, ld3(P.c_body_pos, cid), ld3(P.c_body_vel, cid), ld3(P.c_normal, cid), rad, margin,
-                             P.c_pk[cid], P.c_kd[cid], P.c_mu[cid], P.friction_epsilon, dt, fc, hc);
-                cf = add(cf, fc);
-                ch = madd(ch, hc);
+        finish_vertex(P, q, s_v[t], s_act[t] != 0, f, h, !par_contacts, cbeg + s_cb[t], cbeg + s_cb[t + 1], dt, mode,
+                      g_out);
+    }
+}
+
+// Generic color pass for vertices with more than 256 element items: one
+// vertex per work-group, the CSR walked in chunks of 256 items, same
+// canonical summation order; contacts from the per-particle lists.
+__global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __restrict__ q, int slow_base, float dt,
+                                                          int mode, float* __restrict__ g_out) {
+    __shared__ int s_cnt[3];
+    __shared__ int s_beg[3];
+    __shared__ __align__(16) float s_item[kBlock * kComp];
+    __shared__ float s_acc[kComp];
+
+    const int t = threadIdx.x;
+    const int v = P.color_particles[P.slow_pos[slow_base + blockIdx.x]];
+    const bool active = particle_active(P, v);
+    const float dtr = 1.0f / (dt * dt);
+    if (t == 0) {
+        int nt = 0, ne = 0, nk = 0, bt = 0, be = 0, bk = 0;
+        if (active) {
+            if (P.has_tri) {
+                bt = P.adj_tri_off[v];
+                nt = (P.adj_tri_off[v + 1] - bt) >> 1;
+            }
+            if (P.has_edge) {
+                be = P.adj_edge_off[v];
+                ne = (P.adj_edge_off[v + 1] - be) >> 1;
+            }
+            if (P.has_tet) {
+                bk = P.adj_tet_off[v];
+                nk = (P.adj_tet_off[v + 1] - bk) >> 1;
             }
         }
-        h = madd(h, ch);
-        f = add(f, cf);
-        if (mode == 1) {
-            st3(g_out, v, f);
-            return;
+        s_cnt[0] = nt; s_cnt[1] = ne; s_cnt[2] = nk;
+        s_beg[0] = bt; s_beg[1] = be; s_beg[2] = bk;
+    }
+    __syncthreads();
+    const int total = s_cnt[0] + s_cnt[1] + s_cnt[2];
+    float acc = 0.0f;
+    if (t < kComp) {
+        float m = P.mass[v];
+        if (t < 3) {
+            acc = (m * (P.inertia[3 * v + t] - q[3 * v + t])) * dtr;
+        } else {
+            int r = (t - 3) / 3, c = (t - 3) % 3;
+            acc = (m * dtr) * (r == c ? 1.0f : 0.0f);
         }
-        V3 dx;
-        if (block_solve(h, f, P.fp64_ill, dx)) {
-            V3 d = add(ld3(P.disp, v), dx);
-            st3(P.disp, v, d);
-            st3(q, v, add(ld3(P.q_prev, v), scl(d, 1.0f)));
+    }
+    for (int lo = 0; lo < total; lo += kBlock) {
+        const int k = lo + t;
+        if (k < total) {
+            V3 fe = v3(0.0f, 0.0f, 0.0f);
+            M3 he = mzero();
+            if (k < s_cnt[0]) {
+                int base = s_beg[0] + 2 * k;
+                int tri = P.adj_tri[base], slot = P.adj_tri[base + 1];
+                const float* mat = P.tri_mat + 5 * tri;
+                if (mat[0] > 0.0f || mat[1] > 0.0f) {
+                    const int* ti = P.tri_idx + 3 * tri;
+                    eval_membrane(ld3(q, ti[0]), ld3(q, ti[1]), ld3(q, ti[2]), P.q_prev, ti[0], ti[1], ti[2],
+                                  P.tri_pose + 4 * tri, P.tri_area[tri], mat[0], mat[1], mat[2], dt, slot, fe, he);
+                }
+            } else if (k < s_cnt[0] + s_cnt[1]) {
+                int base = s_beg[1] + 2 * (k - s_cnt[0]);
+                int ed = P.adj_edge[base], slot = P.adj_edge[base + 1];
+                float ke = P.edge_bend[2 * ed];
+                if (ke > 0.0f) {
+                    const int* ei = P.edge_idx + 4 * ed;
+                    eval_bending(q, P.q_prev, ei[0], ei[1], ei[2], ei[3], P.edge_rest_angle[ed], P.edge_rest_len[ed],
+                                 ke, P.edge_bend[2 * ed + 1], dt, slot, fe, he);
+                }
+            } else {
+                int base = s_beg[2] + 2 * (k - s_cnt[0] - s_cnt[1]);
+                int tet = P.adj_tet[base], slot = P.adj_tet[base + 1];
+                const float* mat = P.tet_mat + 3 * tet;
+                if (mat[0] > 0.0f || mat[1] > 0.0f)
+                    eval_tet(q, P.q_prev, P.tet_idx + 4 * tet, P.tet_pose + 9 * tet, mat[0], mat[1], mat[2], dt, slot,
+                             fe, he);
+            }
+            store_row(s_item + t * kComp, fe, he);
         }
+        __syncthreads();
+        if (t < kComp) {
+            int e = min(kBlock, total - lo);
+            for (int i = 0; i < e; ++i) acc = acc + s_item[i * kComp + t];
+        }
+        __syncthreads();
+    }
+    if (t < kComp) s_acc[t] = acc;
+    __syncthreads();
+    if (t == 0) {
+        V3 f = v3(s_acc[0], s_acc[1], s_acc[2]);
+        M3 h;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) h.m[i][j] = s_acc[3 + 3 * i + j];
+        if (active) {
+            V3 cf = v3(0.0f, 0.0f, 0.0f);
+            M3 ch = mzero();
+            const int c0 = P.c_off[v], c1 = P.c_off[v + 1];
+            if (c1 > c0) {
+                V3 x = ld3(q, v), xp = ld3(P.q_prev, v);
+                float rad = P.radius[v];
+                for (int j = c0; j < c1; ++j) {
+                    int cid = P.c_ids[j];
+                    int sh = P.c_shape[cid];
+                    float margin = P.shape_count > 0 ? P.shape_margin[sh] : 0.0f;
+                    V3 fc;
+                    M3 hc;
+                    eval_contact(x, xp, ld3(P.c_body_pos, cid), ld3(P.c_body_vel, cid), ld3(P.c_normal, cid), rad,
+                                 margin, P.c_pk[cid], P.c_kd[cid], P.c_mu[cid], P.friction_epsilon, dt, fc, hc);
+                    cf = add(cf, fc);
+                    ch = madd(ch, hc);
+                }
+            }
+            h = madd(h, ch);
+            f = add(f, cf);
+        }
+        finish_vertex(P, q, v, active, f, h, false, 0, 0, dt, mode, g_out);
     }
 }
 
@@ -946,12 +1078,57 @@ __global__ void k_contact_sort(int n, const int* __restrict__ off, int* __restri
     }
 }
 
+// contacts per color position (input of the position-order scan)
+__global__ void k_pos_contact_count(int n, const int* __restrict__ color_particles, const int* __restrict__ c_off,
+                                    int* __restrict__ pos_cnt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    if (k == n) {
+        pos_cnt[n] = 0;
+        return;
+    }
+    int v = color_particles[k];
+    pos_cnt[k] = c_off[v + 1] - c_off[v];
+}
+
+// Packs every particle's contacts (in contact-index order) at its color
+// position, so a batch's contacts are one contiguous, vertex-ordered range.
+__global__ void k_contact_pack(int n, const int* __restrict__ color_particles, const int* __restrict__ vert_rec,
+                               const int* __restrict__ c_off, const int* __restrict__ c_ids,
+                               const int* __restrict__ pos_coff, const float* __restrict__ pk,
+                               const float* __restrict__ mkd, const float* __restrict__ mmu,
+                               const int* __restrict__ c_shape, const float* __restrict__ body_pos,
+                               const float* __restrict__ body_vel, const float* __restrict__ normal,
+                               const float* __restrict__ shape_margin, int shape_count,
+                               const float* __restrict__ radius, float* __restrict__ cpack,
+                               int* __restrict__ cid_to_pack) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int v = color_particles[k];
+    int c0 = c_off[v], c1 = c_off[v + 1];
+    if (c1 == c0) return;
+    int base = pos_coff[k];
+    int local = vert_rec[4 * k + 3] & 0xffff;
+    float rad = radius[v];
+    for (int j = 0; j < c1 - c0; ++j) {
+        int cid = c_ids[c0 + j];
+        int sh = c_shape[cid];
+        float4* o = reinterpret_cast<float4*>(cpack) + 4 * (size_t)(base + j);
+        o[0] = make_float4(__int_as_float(v), __int_as_float(local), pk[cid], mkd[cid]);
+        o[1] = make_float4(mmu[cid], shape_count > 0 ? shape_margin[sh] : 0.0f, rad, body_pos[3 * cid]);
+        o[2] = make_float4(body_pos[3 * cid + 1], body_pos[3 * cid + 2], body_vel[3 * cid], body_vel[3 * cid + 1]);
+        o[3] = make_float4(body_vel[3 * cid + 2], normal[3 * cid], normal[3 * cid + 1], normal[3 * cid + 2]);
+        cid_to_pack[cid] = base + j;
+    }
+}
+
 // update_duals_body_particle_contacts, rigid_vbd_kernels.py:4861-4926 (particle records)
 __global__ void k_dual_update(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
                               const int* __restrict__ c_shape, const float* __restrict__ body_pos,
                               const float* __restrict__ normal, const float* __restrict__ q,
                               const float* __restrict__ radius, const float* __restrict__ shape_margin, int shape_count,
-                              const float* __restrict__ mke, float beta, float* __restrict__ pk, int n_particles) {
+                              const float* __restrict__ mke, float beta, float* __restrict__ pk, int n_particles,
+                              const int* __restrict__ cid_to_pack, float* __restrict__ cpack) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
@@ -959,11 +1136,15 @@ __global__ void k_dual_update(const int* __restrict__ count_ptr, int cmax, const
     int p = c_indices[3 * i];
     if (p < 0 || p >= n_particles) return;
     int s = c_shape[i];
+    if (s < 0 || s >= shape_count) return;
     float margin = shape_count > 0 ? shape_margin[s] : 0.0f;
     V3 n = ld3(normal, i);
     float pen = -(dot(n, sub(ld3(q, p), ld3(body_pos, i))) - radius[p] - margin);
     pen = fmaxf(0.0f, pen);
-    pk[i] = fminf(pk[i] + beta * pen, mke[i]);
+    float k = fminf(pk[i] + beta * pen, mke[i]);
+    pk[i] = k;
+    int slot = cid_to_pack[i];
+    if (slot >= 0) cpack[16 * (size_t)slot + 2] = k;
 }
 
 __global__ void k_residual_norm(int n, const float* __restrict__ g, double* __restrict__ out) {
@@ -1099,18 +1280,31 @@ struct VbdContext {
     VbdModelDesc model;
     VbdSolverParams params;
     std::vector<int32_t> color_offsets;
-    std::vector<int32_t> batch_color_offsets;
+    std::vector<int32_t> fast_color_off;
+    std::vector<int32_t> slow_color_off;
     KParams kp;
     float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
     int* c_cnt = nullptr;       // [N+1]
     int* c_off = nullptr;       // [N+1]
     int* c_ids = nullptr;       // [cap]
     float* c_state = nullptr;   // pk | mke | mkd | mmu, each [cap]
+    int* pos_coff = nullptr;    // [N+1]
+    float* cpack = nullptr;     // [cap * 16]
+    int* cid_to_pack = nullptr; // [cap]
     float* g_tmp = nullptr;     // [N,3] residual scratch
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int capacity = 0;
-    const VbdContactsDesc* last_contacts = nullptr;
+    // pass layout on the device
+    int* lay_batch = nullptr;
+    int* lay_vert = nullptr;
+    int* lay_tri = nullptr;
+    int* lay_edge = nullptr;
+    int* lay_edge_meta = nullptr;
+    int* lay_tet = nullptr;
+    int* lay_tet_meta = nullptr;
+    int* lay_slow = nullptr;
+    int64_t fast_items = 0;
     VbdContactsDesc contacts{};
     bool have_contacts = false;
 };
@@ -1124,11 +1318,114 @@ struct VbdCollide {
     size_t cub_bytes = 0;
 };
 
+namespace {
+
+template <typename T>
+std::string d2h(std::vector<T>& dst, const void* src, size_t count) {
+    dst.resize(count);
+    if (count == 0) return std::string();
+    if (!src) return "null device array";
+    hipError_t e = hipMemcpy(dst.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? std::string() : std::string(hipGetErrorString(e));
+}
+
+template <typename T>
+std::string h2d(T** dst, const std::vector<T>& src) {
+    hipFree(*dst);
+    *dst = nullptr;
+    size_t n = src.empty() ? 1 : src.size();
+    hipError_t e = hipMalloc(dst, sizeof(T) * n);
+    if (e != hipSuccess) return hipGetErrorString(e);
+    if (!src.empty()) e = hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+    return e == hipSuccess ? std::string() : std::string(hipGetErrorString(e));
+}
+
+void free_layout(VbdContext* c) {
+    for (int** p : {&c->lay_batch, &c->lay_vert, &c->lay_tri, &c->lay_edge, &c->lay_edge_meta, &c->lay_tet,
+                    &c->lay_tet_meta, &c->lay_slow}) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+// Reads the model arrays back once and builds the color-pass layout on the host (vbd_layout.h).
+int32_t build_layout(VbdContext* c) {
+    const VbdModelDesc* m = &c->model;
+    vbd_layout::HostModel hm;
+    hm.N = m->particle_count;
+    hm.T = m->tri_count;
+    hm.E = m->edge_count;
+    hm.K = m->tet_count;
+    std::string err;
+    auto rd = [&](auto& vec, const void* src, size_t n) {
+        if (err.empty()) err = d2h(vec, src, n);
+    };
+    rd(hm.mass, m->particle_mass, (size_t)hm.N);
+    rd(hm.flags, m->particle_flags, (size_t)hm.N);
+    rd(hm.tri_idx, m->tri_indices, 3 * (size_t)hm.T);
+    rd(hm.tri_pose, m->tri_poses, 4 * (size_t)hm.T);
+    rd(hm.tri_mat, m->tri_materials, 5 * (size_t)hm.T);
+    rd(hm.tri_area, m->tri_areas, (size_t)hm.T);
+    rd(hm.edge_idx, m->edge_indices, 4 * (size_t)hm.E);
+    rd(hm.edge_rest, m->edge_rest_angle, (size_t)hm.E);
+    rd(hm.edge_len, m->edge_rest_length, (size_t)hm.E);
+    rd(hm.edge_bend, m->edge_bending_properties, 2 * (size_t)hm.E);
+    rd(hm.tet_idx, m->tet_indices, 4 * (size_t)hm.K);
+    rd(hm.tet_pose, m->tet_poses, 9 * (size_t)hm.K);
+    rd(hm.tet_mat, m->tet_materials, 3 * (size_t)hm.K);
+    if (hm.T > 0) rd(hm.adj_tri_off, m->v_adj_tris_offsets, (size_t)hm.N + 1);
+    if (hm.E > 0) rd(hm.adj_edge_off, m->v_adj_edges_offsets, (size_t)hm.N + 1);
+    if (hm.K > 0) rd(hm.adj_tet_off, m->v_adj_tets_offsets, (size_t)hm.N + 1);
+    if (err.empty() && hm.T > 0) rd(hm.adj_tri, m->v_adj_tris, (size_t)hm.adj_tri_off[hm.N]);
+    if (err.empty() && hm.E > 0) rd(hm.adj_edge, m->v_adj_edges, (size_t)hm.adj_edge_off[hm.N]);
+    if (err.empty() && hm.K > 0) rd(hm.adj_tet, m->v_adj_tets, (size_t)hm.adj_tet_off[hm.N]);
+    hm.color_off = c->color_offsets;
+    rd(hm.color_particles, m->color_particles, (size_t)c->color_offsets.back());
+    if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: reading the model: " + err);
+    // validate the color groups: every particle exactly once
+    {
+        std::vector<char> seen((size_t)hm.N, 0);
+        if ((int)hm.color_particles.size() != hm.N)
+            return fail(VBD_ERR_ARG, "vbd_create: color groups must contain every particle exactly once");
+        for (int32_t v : hm.color_particles) {
+            if (v < 0 || v >= hm.N || seen[v])
+                return fail(VBD_ERR_ARG, "vbd_create: color groups must contain every particle exactly once");
+            seen[v] = 1;
+        }
+    }
+    vbd_layout::PassLayout L;
+    err = vbd_layout::build_pass_layout(hm, L);
+    if (!err.empty()) return fail(VBD_ERR_ARG, "vbd_create: " + err);
+    free_layout(c);
+    for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_rec), std::make_pair(&c->lay_vert, &L.vert_rec),
+                    std::make_pair(&c->lay_tri, &L.tri_rec), std::make_pair(&c->lay_edge, &L.edge_rec),
+                    std::make_pair(&c->lay_edge_meta, &L.edge_meta), std::make_pair(&c->lay_tet, &L.tet_rec),
+                    std::make_pair(&c->lay_tet_meta, &L.tet_meta), std::make_pair(&c->lay_slow, &L.slow_pos)}) {
+        err = h2d(pr.first, *pr.second);
+        if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: uploading the layout: " + err);
+    }
+    c->fast_color_off = L.fast_color_off;
+    c->slow_color_off = L.slow_color_off;
+    c->fast_items = L.fast_items;
+    KParams& k = c->kp;
+    k.batch_rec = c->lay_batch;
+    k.vert_rec = c->lay_vert;
+    k.tri_rec = c->lay_tri;
+    k.edge_rec = c->lay_edge;
+    k.edge_meta = c->lay_edge_meta;
+    k.tet_rec = c->lay_tet;
+    k.tet_meta = c->lay_tet_meta;
+    k.slow_pos = c->lay_slow;
+    return VBD_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* vbd_last_error(void) { return g_last_error.c_str(); }
 
-int32_t vbd_version(void) { return 1; }
+int32_t vbd_version(void) { return 2; }
 
 int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext** out) {
     if (!m || !prm || !out) return fail(VBD_ERR_ARG, "vbd_create: null argument");
@@ -1137,16 +1434,17 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if (N <= 0) return fail(VBD_ERR_ARG, "vbd_create: particle_count must be positive");
     if (m->color_count <= 0 || !m->color_offsets_host || !m->color_particles)
         return fail(VBD_ERR_ARG, "vbd_create: model.particle_color_groups is empty (color the model first)");
-    if (m->batch_count <= 0 || !m->batch_color_offsets_host || !m->batch_start)
-        return fail(VBD_ERR_ARG, "vbd_create: missing color-pass tile schedule");
     if (!m->particle_mass || !m->particle_inv_mass || !m->particle_flags || !m->particle_radius || !m->gravity)
         return fail(VBD_ERR_ARG, "vbd_create: missing particle arrays");
-    if (m->tri_count > 0 && (!m->v_adj_tris || !m->v_adj_tris_offsets || !m->tri_indices))
-        return fail(VBD_ERR_ARG, "vbd_create: triangle adjacency missing");
-    if (m->edge_count > 0 && (!m->v_adj_edges || !m->v_adj_edges_offsets || !m->edge_indices))
-        return fail(VBD_ERR_ARG, "vbd_create: edge adjacency missing");
-    if (m->tet_count > 0 && (!m->v_adj_tets || !m->v_adj_tets_offsets || !m->tet_indices))
-        return fail(VBD_ERR_ARG, "vbd_create: tet adjacency missing");
+    if (m->tri_count > 0 && (!m->v_adj_tris || !m->v_adj_tris_offsets || !m->tri_indices || !m->tri_poses ||
+                             !m->tri_materials || !m->tri_areas))
+        return fail(VBD_ERR_ARG, "vbd_create: triangle arrays or adjacency missing");
+    if (m->edge_count > 0 && (!m->v_adj_edges || !m->v_adj_edges_offsets || !m->edge_indices ||
+                              !m->edge_rest_angle || !m->edge_rest_length || !m->edge_bending_properties))
+        return fail(VBD_ERR_ARG, "vbd_create: edge arrays or adjacency missing");
+    if (m->tet_count > 0 && (!m->v_adj_tets || !m->v_adj_tets_offsets || !m->tet_indices || !m->tet_poses ||
+                             !m->tet_materials))
+        return fail(VBD_ERR_ARG, "vbd_create: tet arrays or adjacency missing");
     if (prm->iterations < 0) return fail(VBD_ERR_ARG, "vbd_create: iterations must be >= 0");
     if (prm->soft_contact_capacity < 0) return fail(VBD_ERR_ARG, "vbd_create: negative soft_contact_capacity");
 
@@ -1154,31 +1452,9 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     c->model = *m;
     c->params = *prm;
     c->color_offsets.assign(m->color_offsets_host, m->color_offsets_host + m->color_count + 1);
-    c->batch_color_offsets.assign(m->batch_color_offsets_host, m->batch_color_offsets_host + m->color_count + 1);
-    if (c->batch_color_offsets[m->color_count] != m->batch_count) {
+    if (c->color_offsets[0] != 0 || c->color_offsets.back() != N) {
         delete c;
-        return fail(VBD_ERR_ARG, "vbd_create: batch_color_offsets does not end at batch_count");
-    }
-    // the kernel assumes <= kMaxV vertices per batch: check the schedule once here
-    {
-        std::vector<int32_t> bs((size_t)m->batch_count + 1);
-        hipError_t e = hipMemcpy(bs.data(), m->batch_start, sizeof(int32_t) * bs.size(), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            delete c;
-            return fail(VBD_ERR_HIP, std::string("vbd_create: reading batch_start: ") + hipGetErrorString(e));
-        }
-        for (int b = 0; b < m->batch_count; ++b) {
-            int nv = bs[b + 1] - bs[b];
-            if (nv <= 0 || nv > kMaxV) {
-                delete c;
-                return fail(VBD_ERR_ARG, "vbd_create: batch " + std::to_string(b) + " has " + std::to_string(nv) +
-                                             " vertices (allowed 1.." + std::to_string(kMaxV) + ")");
-            }
-        }
-        if (bs[m->batch_count] != c->color_offsets[m->color_count]) {
-            delete c;
-            return fail(VBD_ERR_ARG, "vbd_create: batch_start does not cover the color groups");
-        }
+        return fail(VBD_ERR_ARG, "vbd_create: color_offsets must run from 0 to particle_count");
     }
     c->capacity = prm->soft_contact_capacity > 0 ? prm->soft_contact_capacity : 1;
     auto cleanup = [&](const char* what, hipError_t e) {
@@ -1187,12 +1463,15 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
         return fail(VBD_ERR_HIP, msg);
     };
     hipError_t e;
+    const size_t cap = (size_t)c->capacity;
     if ((e = hipMalloc(&c->scratch3, sizeof(float) * 9 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_cnt, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_off, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->c_ids, sizeof(int) * (size_t)c->capacity)) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * (size_t)c->capacity)) != hipSuccess)
-        return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->pos_coff, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_ids, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->cid_to_pack, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->g_tmp, sizeof(float) * 3 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->c_off, N + 1)) != hipSuccess)
         return cleanup("scan sizing", e);
@@ -1200,6 +1479,8 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
         return cleanup("hipMalloc", e);
     // no contacts until vbd_contact_prepare: every particle's list is empty
     if ((e = hipMemset(c->c_off, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
+    if ((e = hipMemset(c->pos_coff, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
+    if ((e = hipMemset(c->cid_to_pack, 0xff, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipDeviceSynchronize()) != hipSuccess) return cleanup("sync", e);
 
     KParams& k = c->kp;
@@ -1230,7 +1511,6 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.has_edge = m->edge_count > 0;
     k.has_tet = m->tet_count > 0;
     k.color_particles = m->color_particles;
-    k.batch_start = m->batch_start;
     k.gravity = m->gravity;
     k.q_prev = c->scratch3;
     k.inertia = c->scratch3 + 3 * (size_t)N;
@@ -1238,14 +1518,29 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.c_off = c->c_off;
     k.c_ids = c->c_ids;
     k.c_pk = c->c_state;
-    k.c_kd = c->c_state + 2 * (size_t)c->capacity;
-    k.c_mu = c->c_state + 3 * (size_t)c->capacity;
+    k.c_kd = c->c_state + 2 * cap;
+    k.c_mu = c->c_state + 3 * cap;
+    k.pos_coff = c->pos_coff;
+    k.cpack = c->cpack;
     k.shape_margin = m->shape_margin;
     k.shape_count = m->shape_count;
     k.friction_epsilon = prm->friction_epsilon;
     k.fp64_ill = prm->fp64_ill_conditioned;
+    int32_t rc = build_layout(c);
+    if (rc) {
+        std::string msg = g_last_error;
+        vbd_destroy(c);
+        return fail(rc, msg);
+    }
     *out = c;
     return VBD_OK;
+}
+
+int32_t vbd_rebuild(VbdContext* c) {
+    if (!c) return fail(VBD_ERR_ARG, "vbd_rebuild: null context");
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(VBD_ERR_HIP, std::string("vbd_rebuild: ") + hipGetErrorString(e));
+    return build_layout(c);
 }
 
 int32_t vbd_destroy(VbdContext* c) {
@@ -1253,10 +1548,14 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->scratch3);
     hipFree(c->c_cnt);
     hipFree(c->c_off);
+    hipFree(c->pos_coff);
     hipFree(c->c_ids);
+    hipFree(c->cid_to_pack);
     hipFree(c->c_state);
+    hipFree(c->cpack);
     hipFree(c->g_tmp);
     hipFree(c->cub_tmp);
+    free_layout(c);
     delete c;
     return VBD_OK;
 }
@@ -1270,6 +1569,15 @@ int32_t vbd_copy_scratch(VbdContext* c, void* stream, int32_t which, float* dst)
     return VBD_OK;
 }
 
+int32_t vbd_layout_stats(VbdContext* c, int64_t* out4) {
+    if (!c || !out4) return fail(VBD_ERR_ARG, "vbd_layout_stats: null argument");
+    out4[0] = c->fast_color_off.back();
+    out4[1] = c->slow_color_off.back();
+    out4[2] = c->fast_items;
+    out4[3] = c->model.color_count;
+    return VBD_OK;
+}
+
 int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* cd) {
     if (!c) return fail(VBD_ERR_ARG, "vbd_contact_prepare: null context");
     hipStream_t s = (hipStream_t)stream;
@@ -1277,6 +1585,8 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
     if (!cd || cd->soft_contact_max <= 0) {
         c->have_contacts = false;
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_off, 0, N + 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->pos_coff, 0, N + 1);
         VBD_LAUNCH_CHECK("k_fill_i32");
         return VBD_OK;
     }
@@ -1315,6 +1625,22 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
     VBD_LAUNCH_CHECK("k_contact_scatter");
     hipLaunchKernelGGL(k_contact_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->c_off, c->c_ids);
     VBD_LAUNCH_CHECK("k_contact_sort");
+    // position-order packing for the color-pass kernel
+    hipLaunchKernelGGL(k_pos_contact_count, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, N,
+                       c->model.color_particles, c->c_off, c->c_cnt);
+    VBD_LAUNCH_CHECK("k_pos_contact_count");
+    bytes = c->cub_bytes;
+    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, c->c_cnt, c->pos_coff, N + 1, s));
+    if (c->params.avbd_beta != 0.0f) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(cap, 256)), dim3(256), 0, s, c->cid_to_pack, -1, cap);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+    }
+    hipLaunchKernelGGL(k_contact_pack, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->model.color_particles,
+                       c->lay_vert, c->c_off, c->c_ids, c->pos_coff, pk, mkd, mmu, cd->soft_contact_shape,
+                       cd->soft_contact_body_pos, cd->soft_contact_body_vel, cd->soft_contact_normal,
+                       c->model.shape_margin, c->model.shape_count, c->model.particle_radius, c->cpack,
+                       c->cid_to_pack);
+    VBD_LAUNCH_CHECK("k_contact_pack");
     KParams& k = c->kp;
     k.c_shape = cd->soft_contact_shape;
     k.c_body_pos = cd->soft_contact_body_pos;
@@ -1332,15 +1658,26 @@ int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, cons
     return VBD_OK;
 }
 
+namespace {
+int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt, int mode, float* g) {
+    int b0 = c->fast_color_off[color], b1 = c->fast_color_off[color + 1];
+    if (b1 > b0) {
+        hipLaunchKernelGGL(k_solve_fast, dim3(b1 - b0), dim3(kBlock), 0, s, c->kp, q, b0, b1 - b0, dt, mode, g);
+        VBD_LAUNCH_CHECK("k_solve_fast");
+    }
+    int s0 = c->slow_color_off[color], s1 = c->slow_color_off[color + 1];
+    if (s1 > s0) {
+        hipLaunchKernelGGL(k_solve_generic, dim3(s1 - s0), dim3(kBlock), 0, s, c->kp, q, s0, dt, mode, g);
+        VBD_LAUNCH_CHECK("k_solve_generic");
+    }
+    return VBD_OK;
+}
+}  // namespace
+
 int32_t vbd_color_pass(VbdContext* c, void* stream, int32_t color, float* q, float dt) {
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_color_pass: null argument");
     if (color < 0 || color >= c->model.color_count) return fail(VBD_ERR_ARG, "vbd_color_pass: color out of range");
-    int b0 = c->batch_color_offsets[color], b1 = c->batch_color_offsets[color + 1];
-    if (b1 <= b0) return VBD_OK;
-    hipLaunchKernelGGL(k_solve_color, dim3(b1 - b0), dim3(kBlock), 0, (hipStream_t)stream, c->kp, q, b0, b1 - b0, dt,
-                       0, (float*)nullptr);
-    VBD_LAUNCH_CHECK("k_solve_color");
-    return VBD_OK;
+    return launch_color(c, (hipStream_t)stream, color, q, dt, 0, nullptr);
 }
 
 int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
@@ -1352,11 +1689,11 @@ int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
                            cd.soft_contact_count, cd.soft_contact_max, cd.soft_contact_indices, cd.soft_contact_shape,
                            cd.soft_contact_body_pos, cd.soft_contact_normal, q, c->model.particle_radius,
                            c->model.shape_margin, c->model.shape_count, c->c_state + c->capacity,
-                           c->params.avbd_beta, c->c_state, c->model.particle_count);
+                           c->params.avbd_beta, c->c_state, c->model.particle_count, c->cid_to_pack, c->cpack);
         VBD_LAUNCH_CHECK("k_dual_update");
     }
     for (int color = 0; color < c->model.color_count; ++color) {
-        int32_t rc = vbd_color_pass(c, stream, color, q, dt);
+        int32_t rc = launch_color(c, s, color, q, dt, 0, nullptr);
         if (rc) return rc;
     }
     return VBD_OK;
@@ -1389,9 +1726,16 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_residual: null argument");
     hipStream_t s = (hipStream_t)stream;
     float* g = g_out ? g_out : c->g_tmp;
-    hipLaunchKernelGGL(k_solve_color, dim3(c->model.batch_count), dim3(kBlock), 0, s, c->kp, (float*)q, 0,
-                       c->model.batch_count, dt, 1, g);
-    VBD_LAUNCH_CHECK("k_solve_color(residual)");
+    int nb = c->fast_color_off.back();
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_solve_fast, dim3(nb), dim3(kBlock), 0, s, c->kp, (float*)q, 0, nb, dt, 1, g);
+        VBD_LAUNCH_CHECK("k_solve_fast(residual)");
+    }
+    int ns = c->slow_color_off.back();
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_solve_generic, dim3(ns), dim3(kBlock), 0, s, c->kp, (float*)q, 0, dt, 1, g);
+        VBD_LAUNCH_CHECK("k_solve_generic(residual)");
+    }
     if (norm_sq_out) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, s, norm_sq_out, 0.0, 1);
         VBD_LAUNCH_CHECK("k_fill_f64");

@@ -9,8 +9,6 @@
 //   (newton/_src/sim/graph_coloring.py:243-311); the exact color of each
 //   vertex is this library's own and is fed to every consumer (GPU solver and
 //   CPU oracle alike).
-// * Color-pass tile schedule: splits each color group into batches whose
-//   element counts fit one workgroup's item capacity.
 //
 // Plain C ABI; called through ctypes.
 #include <algorithm>
@@ -196,44 +194,6 @@ int32_t vbd_host_color_graph(int32_t n, int64_t m, const int32_t* edges, int32_t
     std::memcpy(colors_out, color.data(), sizeof(int32_t) * (size_t)n);
     if (ratio_out) *ratio_out = ratio;
     return nc;
-}
-
-// Splits each color group into batches for the color-pass kernel.  A batch is
-// a run of consecutive vertices of one color whose summed item counts stay
-// within `cap` and whose vertex count stays within `max_vertices`; a vertex
-// whose own count exceeds `cap` gets a batch to itself (the kernel walks its
-// items in chunks).  batch_start receives, per batch, the index of its first
-// vertex in color_particles; batch_color_offsets[c] .. [c+1] delimit color c's
-// batches, and batch_start[n_batches] = color_offsets[C].  Returns n_batches
-// or a negative code (capacity gives the room in batch_start, excluding the
-// trailing sentinel).
-int64_t vbd_host_build_batches(int32_t n_colors, const int32_t* color_offsets, const int32_t* color_particles,
-                               const int32_t* item_count, int32_t cap, int32_t max_vertices, int64_t capacity,
-                               int32_t* batch_start, int32_t* batch_color_offsets) {
-    if (n_colors < 0 || cap <= 0 || max_vertices <= 0) return -1;
-    int64_t nb = 0;
-    for (int32_t c = 0; c < n_colors; ++c) {
-        batch_color_offsets[c] = (int32_t)nb;
-        int32_t b = color_offsets[c], e = color_offsets[c + 1];
-        int32_t k = b;
-        while (k < e) {
-            if (nb >= capacity) return -2;
-            batch_start[nb++] = k;
-            int64_t items = 0;
-            int32_t nv = 0;
-            while (k < e && nv < max_vertices) {
-                int64_t ic = item_count[color_particles[k]];
-                if (nv > 0 && items + ic > cap) break;
-                items += ic;
-                ++nv;
-                ++k;
-                if (items > cap) break;  // oversized single vertex
-            }
-        }
-    }
-    batch_color_offsets[n_colors] = (int32_t)nb;
-    batch_start[nb] = n_colors > 0 ? color_offsets[n_colors] : 0;
-    return nb;
 }
 
 }  // extern "C"

@@ -20,56 +20,13 @@ from .. import _native
 from ..sim.collide import current_stream_handle
 from ..sim.model import Contacts, Control, Model, State
 
-# must match kBlock / kMaxV of csrc/vbd_hip.hip
-TILE_ITEM_CAPACITY = 256
-TILE_MAX_VERTICES = 21
-
-
-def build_tile_schedule(model: Model, item_cap: int = TILE_ITEM_CAPACITY, max_vertices: int = TILE_MAX_VERTICES):
-    """Per-color batches of vertices for the color-pass kernel.
-
-    Returns ``(color_offsets [C+1], color_particles [n], batch_start [B+1],
-    batch_color_offsets [C+1], item_count [N])`` as int32 numpy arrays; a
-    batch's vertices are consecutive in ``color_particles`` and their summed
-    element counts fit ``item_cap`` (a vertex with more elements than that has
-    a batch of its own).
-    """
-    adj = model.soft_mesh_adjacency
-    n = model.particle_count
-
-    def counts(offsets):
-        if offsets is None or len(offsets) == 0:
-            return np.zeros(n, dtype=np.int64)
-        o = np.asarray(offsets, dtype=np.int64)
-        return (o[1:] - o[:-1]) >> 1
-
-    flags = model.particle_flags.cpu().numpy()
-    mass = model.particle_mass.cpu().numpy()
-    active = ((flags & 1) != 0) & (mass != 0.0)
-    items = counts(adj.v_adj_tris_offsets) + counts(adj.v_adj_edges_offsets) + counts(adj.v_adj_tets_offsets)
-    items = np.where(active, items, 0).astype(np.int32)
+def color_layout(model: Model):
+    """``(color_offsets [C+1], color_particles [N])`` int32 arrays: the concatenated color groups."""
     groups = [g.cpu().numpy().astype(np.int32) for g in model.particle_color_groups]
     color_offsets = np.zeros(len(groups) + 1, dtype=np.int32)
     color_offsets[1:] = np.cumsum([len(g) for g in groups])
     color_particles = np.ascontiguousarray(np.concatenate(groups) if groups else np.empty(0, np.int32), dtype=np.int32)
-    cap = int(color_offsets[-1]) + len(groups) + 1
-    batch_start = np.zeros(cap + 1, dtype=np.int32)
-    batch_color_offsets = np.zeros(len(groups) + 1, dtype=np.int32)
-    i32p = ctypes.POINTER(ctypes.c_int32)
-    nb = _native.host_lib().vbd_host_build_batches(
-        len(groups),
-        color_offsets.ctypes.data_as(i32p),
-        color_particles.ctypes.data_as(i32p),
-        items.ctypes.data_as(i32p),
-        int(item_cap),
-        int(max_vertices),
-        cap,
-        batch_start.ctypes.data_as(i32p),
-        batch_color_offsets.ctypes.data_as(i32p),
-    )
-    if nb < 0:
-        raise RuntimeError(f"tile schedule construction failed ({nb})")
-    return color_offsets, color_particles, batch_start[: nb + 1].copy(), batch_color_offsets, items
+    return color_offsets, color_particles
 
 
 class SolverVBD:
@@ -127,15 +84,11 @@ class SolverVBD:
             )
         if model.device.type != "cuda":
             raise RuntimeError("SolverVBD runs on the GPU through libvbd_hip; finalize the model on a cuda device")
-        co, cp, bs, bco, items = build_tile_schedule(model)
+        co, cp = color_layout(model)
         if int(co[-1]) != model.particle_count or np.unique(cp).size != model.particle_count:
             raise ValueError("particle_color_groups must contain every particle exactly once")
         self.color_offsets = co
-        self.batch_color_offsets = bco
-        self.item_count = items
         self.color_particles = torch.as_tensor(cp, device=model.device)
-        self.batch_start = torch.as_tensor(bs, device=model.device)
-        self.batch_count = int(len(bs) - 1)
         if soft_contact_capacity is None:
             soft_contact_capacity = model.particle_count * max(model.shape_count, 1)
         self._capacity = int(soft_contact_capacity)
@@ -175,9 +128,6 @@ class SolverVBD:
         d.color_count = len(self.color_offsets) - 1
         d.color_offsets_host = self.color_offsets.ctypes.data
         d.color_particles = p(self.color_particles)
-        d.batch_count = self.batch_count
-        d.batch_color_offsets_host = self.batch_color_offsets.ctypes.data
-        d.batch_start = p(self.batch_start)
         d.gravity = p(m.gravity)
         d.shape_count = m.shape_count
         d.shape_margin = p(m.shape_margin)
@@ -338,8 +288,21 @@ class SolverVBD:
         return out
 
     def notify_model_changed(self, flags: int = 0) -> None:
-        """Re-read model scalars (contact material) on the next step."""
+        """Re-read the model after the caller edited it (``solver.py:375``).
+
+        Element arrays (materials, rest data), masses and flags are re-read into
+        the color-pass layout now; contact-material scalars on the next step.
+        """
+        if self._ctx is not None:
+            torch.cuda.synchronize(self.model.device)
+            _native.check(_native.hip_lib().vbd_rebuild(self._ctx), "vbd_rebuild")
         self._material_key = None
+
+    def layout_stats(self) -> dict:
+        """Color-pass layout: packed batches, vertices on the generic path, packed element items, colors."""
+        out = (ctypes.c_int64 * 4)()
+        _native.check(_native.hip_lib().vbd_layout_stats(self._ctx, out), "vbd_layout_stats")
+        return {"batches": out[0], "generic_vertices": out[1], "items": out[2], "colors": out[3]}
 
     def reset(self, state: State, world_mask=None, flags=None) -> None:
         """The particle path keeps no state across steps (``CHANGELOG.md:23``): nothing to reset."""
@@ -353,4 +316,4 @@ class SolverVBD:
         return None
 
 
-__all__ = ["SolverVBD", "build_tile_schedule"]
+__all__ = ["SolverVBD", "color_layout"]

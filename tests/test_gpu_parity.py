@@ -26,6 +26,94 @@ def _scene(name, dev):
     }[name]()
 
 
+def _fan_scene(dev, segments=160):
+    """A triangle fan: the centre vertex has `segments` triangles and ~2x that many hinges (> 256 items)."""
+    import math
+
+    from newton_amd import ModelBuilder
+
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    verts = [(0.0, 0.0, 0.0)] + [(0.5 * math.cos(2 * math.pi * i / segments), 0.5 * math.sin(2 * math.pi * i / segments),
+                                  0.0) for i in range(segments)]
+    tris = []
+    for i in range(segments):
+        tris += [0, 1 + i, 1 + (i + 1) % segments]
+    b.add_cloth_mesh(pos=(0.0, 0.3, 0.0), rot=(0.7071068, 0.0, 0.0, 0.7071068), scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e3, tri_ka=1.0e3, tri_kd=1.0e-3,
+                     edge_ke=0.1, edge_kd=1.0e-4, particle_radius=0.02)
+    b.color(include_bending=True)
+    model = b.finalize(dev)
+    model.soft_contact_ke, model.soft_contact_kd, model.soft_contact_mu = 1.0e4, 0.0, 0.5
+    state = model.state()
+    info = {"dt": 1.0 / 600.0, "substeps": 10, "iterations": 4}
+    return model, state, info
+
+
+def _many_planes_scene(dev, planes=24):
+    """A 16x16 cloth resting within the margin of `planes` coincident ground planes: > 512 rows per batch."""
+    import math
+
+    from newton_amd import ModelBuilder
+    from newton_amd.sim.builder import cloth_grid_mesh, quat_from_axis_angle
+
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    for k in range(planes):
+        b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e3 + 10.0 * k, kd=0.5, mu=0.3 + 0.01 * k))
+    verts, tris = cloth_grid_mesh(15, 15, 0.02, 0.02)
+    b.add_cloth_mesh(pos=(0.0, 0.05, 0.0), rot=quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2), scale=1.0,
+                     vel=(0.0, -0.2, 0.0), vertices=verts, indices=tris, density=0.1, tri_ke=1.0e3, tri_ka=1.0e3,
+                     tri_kd=1.0e-4, edge_ke=0.1, edge_kd=0.0, particle_radius=0.06)
+    b.color(include_bending=True)
+    model = b.finalize(dev)
+    model.soft_contact_ke, model.soft_contact_kd, model.soft_contact_mu = 1.0e3, 0.5, 0.5
+    state = model.state()
+    info = {"dt": 1.0 / 600.0, "substeps": 10, "iterations": 3}
+    return model, state, info
+
+
+def test_generic_path_for_high_valence_vertex_bitexact(cuda_device):
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _fan_scene(cuda_device)
+    stats = SolverVBD(model, iterations=1).layout_stats()
+    assert stats["generic_vertices"] == 1, stats
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 8)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 8)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+
+
+def test_serial_contact_mode_bitexact(cuda_device):
+    model, state, info = _many_planes_scene(cuda_device)
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 4)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+
+
+def test_notify_model_changed_rebuilds_layout(cuda_device):
+    """Editing element materials after construction takes effect after notify_model_changed()."""
+    import torch
+
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _scene("strip", cuda_device)
+    q0, qd0 = initial(state)
+    solver = SolverVBD(model, iterations=info["iterations"])
+    model.tri_materials[:, 0] *= 3.0
+    model.edge_bending_properties[:, 0] *= 0.5
+    solver.notify_model_changed()
+    s1 = model.state()
+    solver.step(state, s1, None, None, info["dt"])
+    torch.cuda.synchronize()
+    om = oracle.OracleModel(model)
+    _, q_ref, qd_ref = oracle.step(om, oracle.params(info["iterations"]), q0, qd0, None, None, info["dt"])
+    np.testing.assert_array_equal(s1.particle_q.cpu().numpy(), q_ref)
+
+
 def test_library_is_native(cuda_device):
     from newton_amd import _native
 

@@ -55,5 +55,5 @@ def test_hip_library_loads_without_gpu_calls():
     from newton_amd import _native
 
     lib = _native.hip_lib()
-    assert lib.vbd_version() == 1
+    assert lib.vbd_version() >= 1
     assert lib.vbd_last_error() is not None
