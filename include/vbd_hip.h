@@ -5,9 +5,9 @@
  * Every pointer named in the descriptors is a DEVICE pointer unless its name
  * ends in _host.  Arrays use the reference's layouts: vec3 = 3 contiguous
  * float32 (AoS), mat22/mat33 row-major, indices int32.  No call allocates
- * device memory after vbd_create / vbd_collide_create, none synchronises the
- * stream, and every kernel is enqueued on the stream passed in, so a whole
- * substep can be captured into a HIP graph.  Functions return 0 on success or
+ * device memory after vbd_create / vbd_collide_create, none but
+ * vbd_iterate_to_residual synchronises the stream, and every kernel is enqueued
+ * on the stream passed in, so a whole substep can be captured into a HIP graph.  Functions return 0 on success or
  * a negative VBD_ERR_* code; vbd_last_error() returns the message of the
  * calling thread's last failure.  Nothing throws across this ABI.
  *
@@ -167,8 +167,26 @@ int32_t vbd_finalize(VbdContext* ctx, void* stream, const float* q, float* q_out
  * elastic + damping + contact forces (the force the block solve assembles),
  * zero for inactive particles.  g_out (optional) receives [N,3]; norm_sq_out
  * (device, 1 double) receives sum |g_i|^2.  Uses the inertia and contact
- * state of the current substep (call after vbd_forward / vbd_contact_prepare). */
+ * state of the current substep (call after vbd_forward / vbd_contact_prepare).
+ * The norm is reduced in a fixed order (bit-reproducible).  No counterpart in
+ * the reference, which has no residual; used by iterate-to-tolerance drivers. */
 int32_t vbd_residual(VbdContext* ctx, void* stream, const float* q, float dt, float* g_out, double* norm_sq_out);
+
+/* Iterate from a guess (device [N,3]): for solved particles (ACTIVE,
+ * inv_mass != 0) d = guess - x0, else d = 0, and q_dst = x0 + d, the form
+ * forward_step writes its inertial guess in (particle_vbd_kernels.py:1350-1375).
+ * commit != 0 also stores d as the solver's displacement, making q_dst the
+ * iterate; commit == 0 fills a trial buffer for vbd_residual.  Warm-start hook
+ * with no counterpart in the reference; call after vbd_forward. */
+int32_t vbd_set_iterate(VbdContext* ctx, void* stream, float* q_dst, const float* guess, int32_t commit);
+
+/* Runs VBD iterations on q until the squared residual norm is <= target_norm_sq
+ * or max_iterations were run, evaluating the residual before the first
+ * iteration and after every check_every iterations (host-synchronous: not for
+ * graph capture).  Reports the iterations run and the last squared norm. */
+int32_t vbd_iterate_to_residual(VbdContext* ctx, void* stream, float* q, float dt, double target_norm_sq,
+                                int32_t max_iterations, int32_t check_every, int32_t* iterations_out,
+                                double* norm_sq_out);
 
 /* Copies the solver's scratch state of the current substep ([N,3] floats)
  * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */

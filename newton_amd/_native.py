@@ -137,6 +137,12 @@ HIP_EXPORTS = {
     "vbd_color_pass": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_float]),
     "vbd_finalize": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float]),
     "vbd_residual": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+    "vbd_set_iterate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32]),
+    "vbd_iterate_to_residual": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_float, ctypes.c_double, c_int32, c_int32, ctypes.POINTER(c_int32),
+         ctypes.POINTER(ctypes.c_double)],
+    ),
     "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
     "vbd_collide_create": (c_int32, [ctypes.POINTER(VbdCollideDesc), ctypes.POINTER(c_void_p)]),
     "vbd_collide_destroy": (c_int32, [c_void_p]),
@@ -144,6 +150,11 @@ HIP_EXPORTS = {
 }
 
 HOST_EXPORTS = {
+    "vbd_host_edge_tri_maps": (c_int32, [ctypes.c_int64, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
+    "vbd_host_vertex_csr": (
+        c_int32,
+        [ctypes.c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int32, c_void_p, c_void_p, ctypes.c_int64],
+    ),
     "vbd_host_color_graph": (
         c_int32,
         [

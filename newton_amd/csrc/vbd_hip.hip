@@ -1011,11 +1011,6 @@ __global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
     if (i < n) p[i] = value;
 }
 
-__global__ void k_fill_f64(double* __restrict__ p, double value, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = value;
-}
-
 // init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + per-particle counts
 __global__ void k_contact_init(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
                                const int* __restrict__ c_shape, float soft_ke, float soft_kd, float soft_mu,
@@ -1147,23 +1142,61 @@ __global__ void k_dual_update(const int* __restrict__ count_ptr, int cmax, const
     if (slot >= 0) cpack[16 * (size_t)slot + 2] = k;
 }
 
-__global__ void k_residual_norm(int n, const float* __restrict__ g, double* __restrict__ out) {
-    __shared__ double s_part[kBlock / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v = 0.0;
-    if (i < n) {
-        double a = g[3 * i], b = g[3 * i + 1], c = g[3 * i + 2];
-        v = a * a + b * b + c * c;
-    }
+// Squared residual norm in two deterministic stages: per-workgroup partial
+// sums (fixed shuffle tree), then one workgroup folds the partials in a fixed
+// order.  Same input => same bits, so iterate-to-tolerance loops stop at the
+// same iteration on every run.
+constexpr int kNormBlock = 256;
+constexpr int kNormItems = 4;  // particles per thread in the first stage
+
+__device__ inline double block_sum_f64(double v, double* s_part) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) s_part[wave] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
+    double s = 0.0;
+    if (threadIdx.x == 0)
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += s_part[w];
-        atomicAdd(out, s);
+    return s;
+}
+
+__global__ __launch_bounds__(kNormBlock) void k_residual_partial(int n, const float* __restrict__ g,
+                                                                double* __restrict__ partial) {
+    __shared__ double s_part[kNormBlock / 64];
+    double v = 0.0;
+    long base = (long)blockIdx.x * kNormBlock * kNormItems + threadIdx.x;
+    for (int k = 0; k < kNormItems; ++k) {
+        long i = base + (long)k * kNormBlock;
+        if (i < n) {
+            double a = g[3 * i], b = g[3 * i + 1], c = g[3 * i + 2];
+            v += a * a + b * b + c * c;
+        }
     }
+    double s = block_sum_f64(v, s_part);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kNormBlock) void k_residual_final(int nparts, const double* __restrict__ partial,
+                                                              double* __restrict__ out) {
+    __shared__ double s_part[kNormBlock / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kNormBlock) v += partial[i];
+    double s = block_sum_f64(v, s_part);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// Iterate from a caller's guess: for solved particles (ACTIVE, inv_mass != 0)
+// d = guess - x0, else d = 0; q_dst = x0 + d (the truncation form forward_step
+// uses).  commit != 0 also stores d as the solver's displacement (q_dst is then
+// the solver's iterate); commit == 0 only fills a trial buffer.
+__global__ void k_set_iterate(KParams P, float* __restrict__ q_dst, const float* __restrict__ guess, int commit) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    bool solved = (P.flags[i] & 1) && P.inv_mass[i] != 0.0f;
+    V3 x0 = ld3(P.q_prev, i);
+    V3 d = solved ? sub(ld3(guess, i), x0) : v3(0.0f, 0.0f, 0.0f);
+    if (commit && solved) st3(P.disp, i, d);
+    st3(q_dst, i, add(x0, scl(d, 1.0f)));
 }
 
 // ---- soft-contact generation (geometry/kernels.py:1020-1184) -------------
@@ -1270,6 +1303,7 @@ __global__ void k_collide_write(ColParams C, const float* __restrict__ q, float 
 }
 
 inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+inline int norm_parts(long n) { return n > 0 ? blocks_for(n, kNormBlock * kNormItems) : 1; }
 
 }  // namespace
 
@@ -1292,6 +1326,9 @@ struct VbdContext {
     float* cpack = nullptr;     // [cap * 16]
     int* cid_to_pack = nullptr; // [cap]
     float* g_tmp = nullptr;     // [N,3] residual scratch
+    double* r_part = nullptr;   // residual partial sums, one per first-stage workgroup
+    double* r_dev = nullptr;    // [1] squared residual norm (iterate-to-tolerance driver)
+    double* r_host = nullptr;   // [1] pinned host copy of r_dev
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int capacity = 0;
@@ -1473,6 +1510,10 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->g_tmp, sizeof(float) * 3 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->r_part, sizeof(double) * (size_t)norm_parts(N))) != hipSuccess)
+        return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->r_dev, sizeof(double))) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipHostMalloc(&c->r_host, sizeof(double))) != hipSuccess) return cleanup("hipHostMalloc", e);
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->c_off, N + 1)) != hipSuccess)
         return cleanup("scan sizing", e);
     if ((e = hipMalloc(&c->cub_tmp, c->cub_bytes > 0 ? c->cub_bytes : 16)) != hipSuccess)
@@ -1554,6 +1595,9 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->c_state);
     hipFree(c->cpack);
     hipFree(c->g_tmp);
+    hipFree(c->r_part);
+    hipFree(c->r_dev);
+    if (c->r_host) hipHostFree(c->r_host);
     hipFree(c->cub_tmp);
     free_layout(c);
     delete c;
@@ -1737,12 +1781,49 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
         VBD_LAUNCH_CHECK("k_solve_generic(residual)");
     }
     if (norm_sq_out) {
-        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, s, norm_sq_out, 0.0, 1);
-        VBD_LAUNCH_CHECK("k_fill_f64");
-        hipLaunchKernelGGL(k_residual_norm, dim3(blocks_for(c->model.particle_count, kBlock)), dim3(kBlock), 0, s,
-                           c->model.particle_count, g, norm_sq_out);
-        VBD_LAUNCH_CHECK("k_residual_norm");
+        const int N = c->model.particle_count;
+        const int parts = norm_parts(N);
+        hipLaunchKernelGGL(k_residual_partial, dim3(parts), dim3(kNormBlock), 0, s, N, g, c->r_part);
+        VBD_LAUNCH_CHECK("k_residual_partial");
+        hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kNormBlock), 0, s, parts, c->r_part, norm_sq_out);
+        VBD_LAUNCH_CHECK("k_residual_final");
     }
+    return VBD_OK;
+}
+
+int32_t vbd_set_iterate(VbdContext* c, void* stream, float* q_dst, const float* guess, int32_t commit) {
+    if (!c || !q_dst || !guess) return fail(VBD_ERR_ARG, "vbd_set_iterate: null argument");
+    hipLaunchKernelGGL(k_set_iterate, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0,
+                       (hipStream_t)stream, c->kp, q_dst, guess, commit ? 1 : 0);
+    VBD_LAUNCH_CHECK("k_set_iterate");
+    return VBD_OK;
+}
+
+int32_t vbd_iterate_to_residual(VbdContext* c, void* stream, float* q, float dt, double target_norm_sq,
+                                int32_t max_iterations, int32_t check_every, int32_t* iterations_out,
+                                double* norm_sq_out) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: null argument");
+    if (max_iterations < 0 || check_every < 1)
+        return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: max_iterations must be >= 0 and check_every >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    int32_t done = 0;
+    double r = 0.0;
+    for (;;) {
+        int32_t rc = vbd_residual(c, stream, q, dt, nullptr, c->r_dev);
+        if (rc) return rc;
+        VBD_HIP_TRY(hipMemcpyAsync(c->r_host, c->r_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+        VBD_HIP_TRY(hipStreamSynchronize(s));
+        r = *c->r_host;
+        if (r <= target_norm_sq || done >= max_iterations) break;
+        int32_t n = std::min(check_every, max_iterations - done);
+        for (int32_t k = 0; k < n; ++k) {
+            rc = vbd_iteration(c, stream, q, dt);
+            if (rc) return rc;
+        }
+        done += n;
+    }
+    if (iterations_out) *iterations_out = done;
+    if (norm_sq_out) *norm_sq_out = r;
     return VBD_OK;
 }
 

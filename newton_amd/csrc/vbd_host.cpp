@@ -196,4 +196,112 @@ int32_t vbd_host_color_graph(int32_t n, int64_t m, const int32_t* edges, int32_t
     return nc;
 }
 
+// Edge->triangle and triangle->edge maps for pre-numbered edges
+// (MeshAdjacency._build_maps, newton/_src/utils/mesh.py:531-576).
+// Triangle slot k spans (v_k, v_{k+1}) with opposite vertex v_{k+2}.  A slot is
+// matched to the lowest-numbered edge with the same unordered vertex pair and
+// linked only if its opposite vertex equals the edge's o0 (side 0) or, failing
+// that, o1 (side 1).  When several slots link the same (edge, side), the last
+// in (triangle, slot) order wins.  Edge rows are (o0, o1, v0, v1).
+// Vertex ids may be any int32 >= 0 (edges with negative v0/v1 never match).
+// Linear time: edges are bucketed by their smaller vertex in id order.
+int32_t vbd_host_edge_tri_maps(int64_t edge_count, const int32_t* edges, int64_t tri_count, const int32_t* tris,
+                               int32_t* edge_tri_out, int32_t* tri_edge_out) {
+    if (edge_count < 0 || tri_count < 0 || (edge_count > 0 && !edges) || (tri_count > 0 && !tris) ||
+        (edge_count > 0 && !edge_tri_out) || (tri_count > 0 && !tri_edge_out))
+        return -1;
+    for (int64_t i = 0; i < 2 * edge_count; ++i) edge_tri_out[i] = -1;
+    for (int64_t i = 0; i < 3 * tri_count; ++i) tri_edge_out[i] = -1;
+    if (edge_count == 0 || tri_count == 0) return 0;
+    int64_t vmax = -1;
+    for (int64_t e = 0; e < edge_count; ++e)
+        vmax = std::max<int64_t>(vmax, std::max(edges[4 * e + 2], edges[4 * e + 3]));
+    if (vmax < 0) return 0;
+    const int64_t nb = vmax + 1;
+    std::vector<int64_t> off((size_t)nb + 1, 0);
+    for (int64_t e = 0; e < edge_count; ++e) {
+        int32_t a = edges[4 * e + 2], b = edges[4 * e + 3];
+        if (a < 0 || b < 0) continue;
+        off[(size_t)std::min(a, b) + 1]++;
+    }
+    for (int64_t i = 0; i < nb; ++i) off[i + 1] += off[i];
+    std::vector<int32_t> hi((size_t)off[nb]), id((size_t)off[nb]);
+    {
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (int64_t e = 0; e < edge_count; ++e) {
+            int32_t a = edges[4 * e + 2], b = edges[4 * e + 3];
+            if (a < 0 || b < 0) continue;
+            int64_t k = fill[std::min(a, b)]++;
+            hi[k] = std::max(a, b);
+            id[k] = (int32_t)e;
+        }
+    }
+    for (int64_t t = 0; t < tri_count; ++t) {
+        const int32_t* v = tris + 3 * t;
+        for (int k = 0; k < 3; ++k) {
+            int32_t a = v[k], b = v[(k + 1) % 3], opp = v[(k + 2) % 3];
+            int32_t lo = std::min(a, b), h = std::max(a, b);
+            if (lo < 0 || lo > vmax) continue;
+            int32_t cand = -1;
+            for (int64_t j = off[lo]; j < off[lo + 1]; ++j)
+                if (hi[j] == h) {
+                    cand = id[j];  // bucket is in edge-id order: first hit is the lowest id
+                    break;
+                }
+            if (cand < 0) continue;
+            if (opp == edges[4 * (int64_t)cand + 0]) {
+                tri_edge_out[3 * t + k] = cand;
+                edge_tri_out[2 * (int64_t)cand + 0] = (int32_t)t;
+            } else if (opp == edges[4 * (int64_t)cand + 1]) {
+                tri_edge_out[3 * t + k] = cand;
+                edge_tri_out[2 * (int64_t)cand + 1] = (int32_t)t;
+            }
+        }
+    }
+    return 0;
+}
+
+// Per-vertex CSR of (element, slot) pairs (values_per_entry 2) or element ids
+// (values_per_entry 1) in the serial count/fill order of the reference's
+// adjacency kernels (newton/_src/utils/mesh.py:674-857): elements in increasing
+// id, within an element the columns in fill_cols order; entries equal to -1 are
+// skipped.  rows is [n_elem, width]; offsets_out [n+1] counts ints.  Returns -2
+// if an entry is outside [0, n) (and not -1), -3 if the table exceeds int32.
+int32_t vbd_host_vertex_csr(int64_t n_elem, int32_t width, const int32_t* rows, int32_t ncols,
+                            const int32_t* fill_cols, int32_t n, int32_t values_per_entry, int32_t* offsets_out,
+                            int32_t* values_out, int64_t values_capacity) {
+    if (n_elem < 0 || width <= 0 || ncols <= 0 || n < 0 || !fill_cols || !offsets_out ||
+        (values_per_entry != 1 && values_per_entry != 2) || (n_elem > 0 && !rows))
+        return -1;
+    for (int32_t c = 0; c < ncols; ++c)
+        if (fill_cols[c] < 0 || fill_cols[c] >= width) return -1;
+    std::vector<int64_t> cnt((size_t)n + 1, 0);
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int32_t c = 0; c < ncols; ++c) {
+            int32_t v = rows[e * width + fill_cols[c]];
+            if (v == -1) continue;
+            if (v < 0 || v >= n) return -2;
+            cnt[(size_t)v + 1]++;
+        }
+    for (int32_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    const int64_t total = cnt[n] * values_per_entry;
+    if (total > INT32_MAX) return -3;
+    for (int32_t i = 0; i <= n; ++i) offsets_out[i] = (int32_t)(cnt[i] * values_per_entry);
+    if (total > values_capacity || (total > 0 && !values_out)) return -1;
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int32_t c = 0; c < ncols; ++c) {
+            int32_t v = rows[e * width + fill_cols[c]];
+            if (v == -1) continue;
+            int64_t k = fill[v]++;
+            if (values_per_entry == 2) {
+                values_out[2 * k] = (int32_t)e;
+                values_out[2 * k + 1] = fill_cols[c];
+            } else {
+                values_out[k] = (int32_t)e;
+            }
+        }
+    return 0;
+}
+
 }  // extern "C"

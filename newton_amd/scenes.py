@@ -116,6 +116,72 @@ def cloth_strip_pinned(device="cuda:0", nx: int = 6, ny: int = 4, seed: int = 0,
     return model, state, info
 
 
+def cloth_batched(device="cuda:0", worlds: int = 4096, side_min: int = 16, side_max: int = 64, seed: int = 2,
+                  iterations: int = 5, cell: float = 1.0 / 31.0, height: float = 1.0, max_tilt_deg: float = 30.0):
+    """Many independent cloth worlds of uneven size solved together (4096 worlds ~ 7.4M vertices by default).
+
+    World w is a square grid of ``s_w ~ U{side_min..side_max}`` vertices per
+    side with the cell size of :func:`cloth_hanging` (1/31 m), the same material
+    except a per-world ``tri_ke ~ log-uniform[5e3, 2e4]``, 0, 1 or 2 of its two
+    top corners pinned (count uniform; a single pinned corner is either one),
+    and a tilt uniform in +/-``max_tilt_deg`` about x, centred at ``height``.
+    One ground plane (world -1) is shared by every world; soft-contact pairs
+    are world-compatible, so each particle pairs with it once.  Worlds are
+    colored independently and merged with the reference's plan
+    (``builder.py:2924-2939``).  Seeded from ``seed`` in this order: sides,
+    ke, corner counts, corner choice, tilts.
+    """
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(side_min, side_max + 1, size=worlds)
+    kes = np.exp(rng.uniform(math.log(5.0e3), math.log(2.0e4), size=worlds))
+    ncorner = rng.integers(0, 3, size=worlds)
+    which = rng.integers(0, 2, size=worlds)
+    tilts = np.radians(rng.uniform(-max_tilt_deg, max_tilt_deg, size=worlds))
+
+    templates = {}
+    for s in np.unique(sides):
+        n = int(s) - 1
+        t = ModelBuilder(up_axis="Y", gravity=-9.81)
+        verts, tris = cloth_grid_mesh(n, n, cell, cell)
+        half = 0.5 * n * cell
+        # grid in the XZ plane centred on the origin; each world's xform tilts and lifts it
+        t.add_cloth_mesh(pos=(-half, 0.0, -half), rot=quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2), scale=1.0,
+                         vel=(0.0, 0.0, 0.0), vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4,
+                         tri_kd=1.0e-5, edge_ke=1.0, edge_kd=0.0)
+        t.color(include_bending=True)
+        templates[int(s)] = t
+
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    p_start = np.zeros(worlds + 1, dtype=np.int64)
+    t_start = np.zeros(worlds + 1, dtype=np.int64)
+    for w in range(worlds):
+        t = templates[int(sides[w])]
+        p_start[w] = b.particle_count
+        t_start[w] = b.tri_count
+        q = quat_from_axis_angle((1.0, 0.0, 0.0), float(tilts[w]))
+        b.add_world(t, xform=(np.array([0.0, height, 0.0], np.float32), q))
+    p_start[worlds] = b.particle_count
+    t_start[worlds] = b.tri_count
+
+    tri_mat = b.tri_materials
+    tri_mat[:, 0] = np.repeat(kes, np.diff(t_start))
+    mass = b.particle_mass
+    flags = b.particle_flags
+    for w in range(worlds):
+        n = int(sides[w]) - 1
+        top = [p_start[w] + n * (n + 1), p_start[w] + n * (n + 1) + n]
+        pinned = top if ncorner[w] == 2 else ([top[which[w]]] if ncorner[w] == 1 else [])
+        for v in pinned:
+            mass[v] = 0.0
+            flags[v] = int(ParticleFlags.ACTIVE) & ~int(ParticleFlags.ACTIVE)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+    info = {"workload": f"cloth_batched_{worlds}w_{side_min}-{side_max}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations, "world_particle_start": p_start,
+            "world_tri_start": t_start, "sides": sides}
+    return model, state, info
+
+
 def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 4, k_damp: float = 1e-3):
     """Small tetrahedral cube (5 tets per hex cell) above the ground, for tet-path parity."""
     b = ModelBuilder(up_axis="Y", gravity=-9.81)
@@ -134,4 +200,5 @@ def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 
     return model, state, info
 
 
-__all__ = ["FRAME_DT", "SUBSTEPS", "cloth_drop_on_sphere", "cloth_hanging", "cloth_strip_pinned", "soft_cube"]
+__all__ = ["FRAME_DT", "SUBSTEPS", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging", "cloth_strip_pinned",
+           "soft_cube"]

@@ -198,11 +198,31 @@ class ModelBuilder:
         self.shape_material_kd: list[float] = []
         self.shape_material_mu: list[float] = []
         self.shape_label: list[str] = []
-        self.particle_color_groups: list[np.ndarray] = []
+        # color groups as a lazy plan of (size, [(group, offset|None), ...]) items,
+        # materialised on first read (merging W worlds one by one stays O(total))
+        self._color_plan: list = []
+        self._color_groups_cache: list[np.ndarray] | None = []
         self.body_count = 0
         self.joint_count = 0
 
     # ------------------------------------------------------------------ counts
+    @property
+    def particle_color_groups(self) -> list[np.ndarray]:
+        if self._color_groups_cache is None:
+            merged = []
+            for _, chunks in self._color_plan:
+                parts = [g if o is None else g + o for g, o in chunks]
+                merged.append(parts[0] if len(parts) == 1 else np.concatenate(parts))
+            self._color_groups_cache = merged
+            self._color_plan = [(len(g), [(g, None)]) for g in merged]
+        return self._color_groups_cache
+
+    @particle_color_groups.setter
+    def particle_color_groups(self, groups) -> None:
+        groups = [np.asarray(g, dtype=np.int64) for g in groups]
+        self._color_plan = [(len(g), [(g, None)]) for g in groups]
+        self._color_groups_cache = groups
+
     @property
     def particle_count(self) -> int:
         return len(self._pq)
@@ -763,16 +783,12 @@ class ModelBuilder:
             self.shape_material_kd.append(builder.shape_material_kd[s])
             self.shape_material_mu.append(builder.shape_material_mu[s])
             self.shape_label.append(builder.shape_label[s])
-        if builder.particle_color_groups:
-            plan = [(len(g), [(g, None)]) for g in self.particle_color_groups]
-            plan = gc.combine_independent_coloring_plan(
-                plan, [(len(g), [(np.asarray(g, dtype=np.int64), off)]) for g in builder.particle_color_groups]
+        src_groups = builder.particle_color_groups
+        if src_groups:
+            self._color_plan = gc.combine_independent_coloring_plan(
+                self._color_plan, [(len(g), [(np.asarray(g, dtype=np.int64), off)]) for g in src_groups]
             )
-            merged = []
-            for _, chunks in plan:
-                parts = [g if o is None else g + o for g, o in chunks]
-                merged.append(parts[0] if len(parts) == 1 else np.concatenate(parts))
-            self.particle_color_groups = merged
+            self._color_groups_cache = None
 
     def add_world(self, builder: "ModelBuilder", xform=None) -> int:
         """Add ``builder`` as a new world (``builder.py:4016-4065``); returns the world index."""

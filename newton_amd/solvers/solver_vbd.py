@@ -12,6 +12,7 @@ Python or CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -19,6 +20,14 @@ import torch
 from .. import _native
 from ..sim.collide import current_stream_handle
 from ..sim.model import Contacts, Control, Model, State
+
+
+class _TrialState:
+    """Positions-only stand-in for a State (residual of a trial iterate)."""
+
+    def __init__(self, q: torch.Tensor):
+        self.particle_q = q
+
 
 def color_layout(model: Model):
     """``(color_offsets [C+1], color_particles [N])`` int32 arrays: the concatenated color groups."""
@@ -275,6 +284,74 @@ class SolverVBD:
             "vbd_residual",
         )
         return out
+
+    def set_iterate(self, state_in: State, guess: torch.Tensor, commit: bool = True,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+        """Iterate from ``guess`` ([N,3] positions) after :meth:`begin_step`.
+
+        Solved particles get ``x0 + (guess - x0)``, the others keep ``x0``.
+        ``commit`` writes the solver's iterate (``state_in.particle_q`` and its
+        displacement); otherwise ``out`` (or a new tensor) receives the trial
+        positions, e.g. to judge the guess with :meth:`residual` first.
+        """
+        guess = guess.contiguous()
+        if guess.shape != (self.model.particle_count, 3) or guess.dtype != torch.float32:
+            raise ValueError("guess must be a float32 tensor of shape [particle_count, 3]")
+        dst = state_in.particle_q if commit else (out if out is not None else torch.empty_like(guess))
+        _native.check(
+            _native.hip_lib().vbd_set_iterate(self._ctx, current_stream_handle(self.model.device), _native.ptr(dst),
+                                              _native.ptr(guess), 1 if commit else 0),
+            "vbd_set_iterate",
+        )
+        return dst
+
+    def iterate_to_residual(self, state_in: State, dt: float, target_norm_sq: float, max_iterations: int = 200,
+                            check_every: int = 1) -> tuple[int, float]:
+        """Iterate until ``|g|^2 <= target_norm_sq`` (checked every ``check_every`` iterations).
+
+        Host-synchronous.  Returns ``(iterations run, final |g|^2)``.
+        """
+        its = ctypes.c_int32(0)
+        r = ctypes.c_double(0.0)
+        _native.check(
+            _native.hip_lib().vbd_iterate_to_residual(self._ctx, current_stream_handle(self.model.device),
+                                                      _native.ptr(state_in.particle_q), float(dt),
+                                                      float(target_norm_sq), int(max_iterations), int(check_every),
+                                                      ctypes.byref(its), ctypes.byref(r)),
+            "vbd_iterate_to_residual",
+        )
+        return int(its.value), float(r.value)
+
+    def step_to_residual(self, state_in: State, state_out: State, control: Control | None,
+                         contacts: Contacts | None, dt: float, rel_tol: float = 1.0e-3, max_iterations: int = 200,
+                         check_every: int = 1, warm_start=None) -> dict:
+        """One substep iterated to a relative force residual instead of a fixed count.
+
+        The residual is normalised by its value at the inertial guess ``r0``;
+        iterations stop once ``r <= rel_tol * r0`` (or after ``max_iterations``).
+        ``warm_start(solver, state_in, dt)`` may return candidate positions
+        ([N,3] float32); they replace the inertial guess only if their residual
+        is strictly lower, otherwise the inertial guess is kept.
+        """
+        self.begin_step(state_in, contacts, dt)
+        r0_dev = self.residual(state_in, dt)
+        accepted = False
+        r_ws = None
+        if warm_start is not None:
+            guess = warm_start(self, state_in, dt)
+            if guess is not None:
+                trial = self.set_iterate(state_in, guess, commit=False)
+                r_ws_dev = self.residual(_TrialState(trial), dt)
+                r0_sq, r_ws_sq = float(r0_dev.item()), float(r_ws_dev.item())
+                r_ws = math.sqrt(r_ws_sq)
+                if r_ws_sq < r0_sq:
+                    self.set_iterate(state_in, guess, commit=True)
+                    accepted = True
+        r0_sq = float(r0_dev.item())
+        its, r_sq = self.iterate_to_residual(state_in, dt, (rel_tol * rel_tol) * r0_sq, max_iterations, check_every)
+        self.end_step(state_in, state_out, dt)
+        return {"iterations": its, "r0": math.sqrt(r0_sq), "r": math.sqrt(r_sq), "warm_start_accepted": accepted,
+                "r_warm_start": r_ws}
 
     def scratch(self, which: str) -> torch.Tensor:
         """Copy of the current substep's ``inertia`` / ``q_prev`` / ``displacement`` array ([N, 3])."""

@@ -91,8 +91,12 @@ def compute_edge_adjacency(tri_indices, tri_start: int = 0):
     return edge_indices, edge_tri_indices, tri_edge_indices
 
 
-def build_edge_tri_maps(edge_indices, tri_indices):
-    """Edge/triangle maps for pre-numbered edges (``mesh.py:531-576``)."""
+def build_edge_tri_maps_np(edge_indices, tri_indices):
+    """Edge/triangle maps for pre-numbered edges (``mesh.py:531-576``), numpy restatement.
+
+    Kept as the readable statement of the rule; :func:`build_edge_tri_maps`
+    (C++, linear time) is what :class:`MeshAdjacency` uses.
+    """
     edges = _int_rows(edge_indices, 4)
     tris = _int_rows(tri_indices, 3)
     edge_count, tri_count = edges.shape[0], tris.shape[0]
@@ -120,8 +124,8 @@ def build_edge_tri_maps(edge_indices, tri_indices):
     return edge_tri, tri_edge
 
 
-def _vertex_csr(rows: np.ndarray, slots_in_fill_order, particle_count: int, values_per_entry: int = 2):
-    """CSR of (element, slot) pairs in the serial count/fill order (``mesh.py:674-926``).
+def _vertex_csr_np(rows: np.ndarray, slots_in_fill_order, particle_count: int, values_per_entry: int = 2):
+    """CSR of (element, slot) pairs in the serial count/fill order (``mesh.py:674-926``), numpy restatement.
 
     ``slots_in_fill_order`` lists the element columns in the order the fill
     kernel visits them; the stored slot is the column index.
@@ -148,6 +152,42 @@ def _vertex_csr(rows: np.ndarray, slots_in_fill_order, particle_count: int, valu
     values[0::2] = elems[order]
     values[1::2] = slots[order]
     return values, offsets
+
+
+def build_edge_tri_maps(edge_indices, tri_indices):
+    """Edge/triangle maps for pre-numbered edges (``mesh.py:531-576``), in C++ (``vbd_host_edge_tri_maps``)."""
+    from .. import _native  # noqa: PLC0415
+
+    edges = np.ascontiguousarray(_int_rows(edge_indices, 4))
+    tris = np.ascontiguousarray(_int_rows(tri_indices, 3))
+    edge_tri = np.empty((edges.shape[0], 2), dtype=np.int32)
+    tri_edge = np.empty((tris.shape[0], 3), dtype=np.int32)
+    rc = _native.host_lib().vbd_host_edge_tri_maps(edges.shape[0], edges.ctypes.data, tris.shape[0], tris.ctypes.data,
+                                                   edge_tri.ctypes.data, tri_edge.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"vbd_host_edge_tri_maps failed ({rc})")
+    return edge_tri, tri_edge
+
+
+def _vertex_csr(rows: np.ndarray, slots_in_fill_order, particle_count: int, values_per_entry: int = 2):
+    """CSR of (element, slot) pairs in the serial count/fill order (``mesh.py:674-926``), in C++."""
+    from .. import _native  # noqa: PLC0415
+
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    if rows.shape[0] == 0:
+        return np.empty(0, dtype=np.int32), np.empty(0, dtype=np.int32)
+    cols = np.ascontiguousarray(slots_in_fill_order, dtype=np.int32)
+    n_valid = int(np.count_nonzero(rows[:, cols] != -1))
+    offsets = np.empty(particle_count + 1, dtype=np.int32)
+    values = np.empty(max(values_per_entry * n_valid, 1), dtype=np.int32)
+    rc = _native.host_lib().vbd_host_vertex_csr(rows.shape[0], rows.shape[1], rows.ctypes.data, len(cols),
+                                                cols.ctypes.data, particle_count, values_per_entry,
+                                                offsets.ctypes.data, values.ctypes.data, values.size)
+    if rc == -2:
+        raise ValueError("element references a particle index out of range")
+    if rc != 0:
+        raise ValueError(f"vbd_host_vertex_csr failed ({rc})")
+    return values[: values_per_entry * n_valid], offsets
 
 
 @dataclass
@@ -250,4 +290,5 @@ def _csr_offsets_or_zeros(offsets: np.ndarray, particle_count: int) -> np.ndarra
     return offsets
 
 
-__all__ = ["MeshAdjacency", "MeshAdjacencyData", "build_edge_tri_maps", "compute_edge_adjacency"]
+__all__ = ["MeshAdjacency", "MeshAdjacencyData", "build_edge_tri_maps", "build_edge_tri_maps_np",
+           "compute_edge_adjacency"]

@@ -242,3 +242,50 @@ def test_grid_edge_counts():
     v = 16 * 32 + 16
     assert (adj.v_adj_tris_offsets[v + 1] - adj.v_adj_tris_offsets[v]) // 2 == 6
     assert (adj.v_adj_edges_offsets[v + 1] - adj.v_adj_edges_offsets[v]) // 2 == 12
+
+
+def _random_soup(rng, n_verts, n_tris):
+    tris = np.stack([rng.choice(n_verts, 3, replace=False) for _ in range(n_tris)]).astype(np.int32)
+    return tris
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_native_edge_tri_maps_equal_numpy_restatement(seed):
+    """C++ vbd_host_edge_tri_maps == the numpy statement of mesh.py:531-576, incl. duplicates and placeholders."""
+    from newton_amd.utils.mesh import build_edge_tri_maps, build_edge_tri_maps_np
+
+    rng = np.random.default_rng(seed)
+    tris = _random_soup(rng, 12, 40)  # dense soup: non-manifold edges, repeated pairs
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        edges, _, _ = compute_edge_adjacency(tris)
+    edges = edges.copy()
+    # duplicate some edges (same vertex pair, other opposites) and add placeholder rows
+    dup = edges[rng.choice(len(edges), 6, replace=False)].copy()
+    dup[:, :2] = rng.integers(-1, 12, size=(6, 2))
+    ph = edges[rng.choice(len(edges), 3, replace=False)].copy()
+    ph[:, :2] = -1
+    edges = np.concatenate([edges, dup, ph])
+    perm = rng.permutation(len(edges))
+    edges = np.ascontiguousarray(edges[perm])
+    a = build_edge_tri_maps(edges, tris)
+    b = build_edge_tri_maps_np(edges, tris)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("cols,vpe", [([0, 1, 2], 2), ([2, 3, 0, 1], 2), ([0, 1], 1)])
+def test_native_vertex_csr_equals_numpy_restatement(cols, vpe):
+    from newton_amd.utils.mesh import _vertex_csr, _vertex_csr_np
+
+    rng = np.random.default_rng(len(cols) * 10 + vpe)
+    width = max(cols) + 1
+    rows = rng.integers(-1, 50, size=(300, width)).astype(np.int32)
+    v_a, o_a = _vertex_csr(rows, cols, 50, values_per_entry=vpe)
+    v_b, o_b = _vertex_csr_np(rows, cols, 50, values_per_entry=vpe)
+    np.testing.assert_array_equal(o_a, o_b)
+    np.testing.assert_array_equal(v_a, v_b)
+    with pytest.raises(ValueError):
+        bad = rows.copy()
+        bad[7, cols[0]] = 50
+        _vertex_csr(bad, cols, 50, values_per_entry=vpe)
