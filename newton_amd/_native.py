@@ -16,6 +16,7 @@ LIB_DIR = Path(__file__).resolve().parent / "lib"
 _lock = threading.Lock()
 _hip = None
 _host = None
+_ws = None
 
 c_int32 = ctypes.c_int32
 c_float = ctypes.c_float
@@ -171,6 +172,29 @@ HOST_EXPORTS = {
 }
 
 
+c_u16p = c_void_p  # BF16 arrays are passed as device addresses
+
+WS_EXPORTS = {
+    "vbd_ws_last_error": (ctypes.c_char_p, []),
+    "vbd_ws_sizes": (c_int32, [ctypes.POINTER(ctypes.c_int64)]),
+    "vbd_ws_features": (
+        c_int32,
+        [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+         c_void_p, c_u16p],
+    ),
+    "vbd_ws_pack": (c_int32, [c_void_p, c_void_p, c_u16p]),
+    "vbd_ws_forward": (c_int32, [c_void_p, c_int32, c_u16p, c_u16p, c_void_p, c_void_p]),
+    "vbd_ws_forward_train": (c_int32, [c_void_p, c_int32, c_u16p, c_u16p, c_void_p, c_void_p, c_u16p, c_void_p]),
+    "vbd_ws_backward": (
+        c_int32,
+        [c_void_p, c_int32, c_u16p, c_void_p, c_void_p, c_u16p, c_void_p, c_u16p, c_void_p, c_void_p, c_void_p],
+    ),
+    "vbd_ws_reduce": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vbd_ws_adam": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_float,
+                              c_int32]),
+}
+
+
 def _load(path: Path, exports: dict):
     if not path.exists():
         raise RuntimeError(
@@ -201,6 +225,25 @@ def host_lib():
         if _host is None:
             _host = _load(LIB_DIR / "libvbd_host.so", HOST_EXPORTS)
         return _host
+
+
+def ws_lib():
+    """The learned warm-start library (raises if it is not built)."""
+    global _ws
+    with _lock:
+        if _ws is None:
+            _ws = _load(LIB_DIR / "libvbd_ws.so", WS_EXPORTS)
+        return _ws
+
+
+def ws_check(rc: int, what: str) -> None:
+    if rc == 0:
+        return
+    msg = ws_lib().vbd_ws_last_error()
+    text = msg.decode() if msg else ""
+    if rc == -1:
+        raise ValueError(f"{what}: {text}")
+    raise VbdError(f"{what} failed ({rc}): {text}")
 
 
 class VbdError(RuntimeError):

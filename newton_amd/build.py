@@ -1,7 +1,9 @@
 """Builds the native libraries of newton_amd in-tree.
 
 * ``newton_amd/lib/libvbd_hip.so``  -- gfx950 HIP kernels + C ABI (``include/vbd_hip.h``)
-* ``newton_amd/lib/libvbd_host.so`` -- host-side coloring / tile schedule (C++)
+* ``newton_amd/lib/libvbd_ws.so``   -- gfx950 learned warm start (features, BF16 MFMA MLP, Adam;
+  ``include/vbd_warmstart.h``)
+* ``newton_amd/lib/libvbd_host.so`` -- host-side model construction: coloring, adjacency (C++)
 
 Run ``python -m newton_amd.build`` (no GPU needed: hipcc cross-compiles).
 """
@@ -73,8 +75,17 @@ def build_hip() -> Path:
     return out
 
 
+def build_ws() -> Path:
+    LIB_DIR.mkdir(exist_ok=True)
+    out = LIB_DIR / "libvbd_ws.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run([_hipcc(), *HIP_FLAGS, "-I", str(INCLUDE), "-o", str(tmp), str(CSRC / "vbd_warmstart.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> list[Path]:
-    return [build_host(), build_hip()]
+    return [build_host(), build_hip(), build_ws()]
 
 
 if __name__ == "__main__":

@@ -182,6 +182,74 @@ def cloth_batched(device="cuda:0", worlds: int = 4096, side_min: int = 16, side_
     return model, state, info
 
 
+class FourierWind:
+    """Spatially smooth random wind: an acceleration field that is a sum of Fourier modes.
+
+    ``a(x) = sum_m A_m sin(k_m . x + phi_m)`` with ``A_m ~ N(0, sigma)`` per
+    component (N/kg), wave vectors with wavelengths in [0.5, 2] x ``length`` and
+    random directions, phases uniform.  ``redraw(frame)`` draws a new set every
+    ``period`` frames from one generator seeded with ``seed``.  ``force(q, mass)``
+    returns ``mass * a(q)`` ([N,3], the ``particle_f`` the solver reads).
+    """
+
+    def __init__(self, device, modes: int = 8, sigma: float = 2.0, length: float = 1.0, period: int = 60,
+                 seed: int = 3):
+        self.device = torch.device(device)
+        self.modes, self.sigma, self.length, self.period = modes, sigma, length, period
+        self.rng = np.random.default_rng(seed)
+        self.epoch = -1
+        self.redraw(0)
+
+    def redraw(self, frame: int) -> None:
+        epoch = frame // self.period
+        while self.epoch < epoch:
+            m = self.modes
+            amp = self.rng.normal(0.0, self.sigma, size=(m, 3))
+            direction = self.rng.normal(size=(m, 3))
+            direction /= np.linalg.norm(direction, axis=1, keepdims=True)
+            wavelength = self.length * self.rng.uniform(0.5, 2.0, size=(m, 1))
+            k = direction * (2.0 * math.pi / wavelength)
+            phase = self.rng.uniform(0.0, 2.0 * math.pi, size=m)
+            self.amp = torch.as_tensor(amp, dtype=torch.float32, device=self.device)
+            self.k = torch.as_tensor(k, dtype=torch.float32, device=self.device)
+            self.phase = torch.as_tensor(phase, dtype=torch.float32, device=self.device)
+            self.epoch += 1
+
+    def acceleration(self, q: torch.Tensor) -> torch.Tensor:
+        s = torch.sin(q @ self.k.T + self.phase)  # [N, modes]
+        return s @ self.amp
+
+    def force(self, q: torch.Tensor, mass: torch.Tensor) -> torch.Tensor:
+        return self.acceleration(q) * mass[:, None]
+
+
+def cloth_wind(device="cuda:0", verts_per_side: int = 128, size: float = 1.0, top: float = 1.5, seed: int = 3):
+    """128x128 cloth hanging in the XY plane, pinned along its top edge, driven by :class:`FourierWind`.
+
+    Material of :func:`cloth_hanging` (areal density 0.1, tri_ke = tri_ka = 1e4,
+    tri_kd = 1e-5, edge_ke = 1); no shapes.  The wind (8 modes, N(0, 2 N/kg)
+    amplitudes, redrawn every 60 frames from ``seed``) is returned in
+    ``info["wind"]``; drivers set ``state.particle_f = wind.force(q, mass)``
+    before each substep.
+    """
+    n = verts_per_side - 1
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    # grid y runs upward in the XY plane; the row y = n is the top edge
+    b.add_cloth_mesh(pos=(-0.5 * size, top - size, 0.0), rot=(0.0, 0.0, 0.0, 1.0), scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
+                     edge_ke=1.0, edge_kd=0.0)
+    top_row = n * (n + 1) + np.arange(n + 1)
+    b.particle_mass[top_row] = 0.0
+    b.particle_flags[top_row] = int(ParticleFlags.ACTIVE) & ~int(ParticleFlags.ACTIVE)
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+    wind = FourierWind(device, length=size, seed=seed)
+    info = {"workload": f"cloth_wind_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": 10, "wind": wind}
+    return model, state, info
+
+
 def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 4, k_damp: float = 1e-3):
     """Small tetrahedral cube (5 tets per hex cell) above the ground, for tet-path parity."""
     b = ModelBuilder(up_axis="Y", gravity=-9.81)
@@ -200,5 +268,5 @@ def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 
     return model, state, info
 
 
-__all__ = ["FRAME_DT", "SUBSTEPS", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging", "cloth_strip_pinned",
-           "soft_cube"]
+__all__ = ["FRAME_DT", "SUBSTEPS", "FourierWind", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging",
+           "cloth_strip_pinned", "cloth_wind", "soft_cube"]
