@@ -626,7 +626,9 @@ struct KParams {
     float* inertia;
     float* disp;
     // pass layout (vbd_layout.h)
-    const int* batch_rec;
+    const int* batch_blk;
+    const int* bat_cinfo;  // per batch {first packed contact, contact count} (k_batch_contacts)
+    const int* bat_coff;   // per batch, kMaxV + 1 contact offsets relative to the first
     const int* vert_rec;
     const int* tri_rec;
     const int* edge_rec;
@@ -693,14 +695,15 @@ __device__ __forceinline__ void eval_packed_contact(const KParams& P, const floa
 
 // Finish one vertex: contacts (when not already summed), then the block solve
 // and position update (mode 0) or the force output (mode 1).
-__device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 f, M3 h,
-                                              bool add_contacts, int crec0, int crec1, float dt, int mode,
-                                              float* __restrict__ g_out) {
+// x0 and d_old are the vertex's q_prev and displacement (read by the caller).
+__device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 x0,
+                                              V3 d_old, V3 f, M3 h, bool add_contacts, int crec0, int crec1, float dt,
+                                              int mode, float* __restrict__ g_out) {
     if (!active) {
         if (mode == 0) {
             V3 z = v3(0.0f, 0.0f, 0.0f);
             st3(P.disp, v, z);
-            st3(q, v, add(ld3(P.q_prev, v), scl(z, 1.0f)));
+            st3(q, v, add(x0, scl(z, 1.0f)));
         } else {
             st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
         }
@@ -725,111 +728,155 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
     }
     V3 dx;
     if (block_solve(h, f, P.fp64_ill, dx)) {
-        V3 d = add(ld3(P.disp, v), dx);
+        V3 d = add(d_old, dx);
         st3(P.disp, v, d);
-        st3(q, v, add(ld3(P.q_prev, v), scl(d, 1.0f)));
+        st3(q, v, add(x0, scl(d, 1.0f)));
     }
 }
 
 // One color pass over packed batches (mode 0: solve and update; mode 1: write the assembled force g).
-//   phase 0  batch record (uniform) and vertex records
-//   phase 1  every lane evaluates one triangle / edge / tet / contact item into its canonical LDS row
-//   phase 2  lane (vertex s, component c) sums component c of s's rows serially, in canonical order,
-//            starting from the inertia term; contacts are summed from zero and added afterwards
-//   phase 3  the vertex's lane solves the 3x3 block and writes displacement and position
+// The work-group's global reads form three dependent round trips:
+//   1  the batch block (header + vertex records) and the batch's contact ranges (indexed by batch)
+//   2  each lane's item record (triangle / edge / tet / packed contact) and, for lanes t < nv, the
+//      vertex's mass, inertia target, position, q_prev and displacement
+//   3  the positions (and q_prev when damped) of the items' vertices
+// then, in LDS:
+//   a  every lane evaluates its item into the item's canonical row
+//   b  lane (vertex s, component c) sums component c of s's rows serially, in canonical order,
+//      starting from the inertia term; contacts are summed from zero and added afterwards
+//   c  the vertex's lane solves the 3x3 block and writes displacement and position
+constexpr int kBlkWords = vbd_layout::kBlockWords;
+constexpr int kBatchHdr = vbd_layout::kBatchWords;
+
 __global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
                                                        float dt, int mode, float* __restrict__ g_out) {
-    __shared__ int s_v[kMaxV];
     __shared__ int s_row[kMaxV];
     __shared__ int s_nrow[kMaxV];
-    __shared__ int s_act[kMaxV];
     __shared__ int s_cb[kMaxV + 1];
     __shared__ __align__(16) float s_rows[kRowsMax * kComp];
     __shared__ float s_acc[kMaxV * kComp];
 
     const int t = threadIdx.x;
     const int b = batch_base + xcd_remap(blockIdx.x, nb);
-    const int4* br = reinterpret_cast<const int4*>(P.batch_rec) + 2 * (size_t)b;
-    const int4 b0 = br[0], b1 = br[1];
-    const int vb = b0.x, nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
+    // ---- round trip 1
+    const int* blk = P.batch_blk + (size_t)b * kBlkWords;
+    const int4 b0 = reinterpret_cast<const int4*>(blk)[0];
+    const int4 b1 = reinterpret_cast<const int4*>(blk)[1];
+    const int2 ci = reinterpret_cast<const int2*>(P.bat_cinfo)[b];
+    int4 vr = make_int4(0, 0, 0, 0);
+    if (t < kMaxV) vr = reinterpret_cast<const int4*>(blk + kBatchHdr)[t];
+    int cbt = 0;
+    if (t <= kMaxV) cbt = P.bat_coff[(size_t)b * (kMaxV + 1) + t];
+    const int nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
     const int nstatic = ntri + nedge + ntet;
-    const int cbeg = P.pos_coff[vb];
-    const int ncont = P.pos_coff[vb + nv] - cbeg;
+    const int cbeg = ci.x, ncont = ci.y;
     const bool par_contacts = nstatic + ncont <= kRowsMax;
     const float dtr = 1.0f / (dt * dt);
 
+    // ---- round trip 2: item record of this lane (items 0..255), own-vertex state
+    int kind = 0;  // 1 triangle, 2 edge, 3 tet, 4 contact
+    int4 r0 = make_int4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
+    int meta = 0;
+    if (t < ntri) {
+        kind = 1;
+        const int4* r4 = reinterpret_cast<const int4*>(P.tri_rec) + 3 * (size_t)(tri_b + t);
+        r0 = r4[0];
+        r1 = r4[1];
+        r2 = r4[2];
+    } else if (t < ntri + nedge) {
+        kind = 2;
+        const int k = t - ntri;
+        const int4* r4 = reinterpret_cast<const int4*>(P.edge_rec) + 2 * (size_t)(edge_b + k);
+        r0 = r4[0];
+        r1 = r4[1];
+        meta = P.edge_meta[edge_b + k];
+    } else if (t < nstatic) {
+        kind = 3;
+        const int k = t - ntri - nedge;
+        const int4* r4 = reinterpret_cast<const int4*>(P.tet_rec) + 4 * (size_t)(tet_b + k);
+        r0 = r4[0];
+        r1 = r4[1];
+        r2 = r4[2];
+        r3 = r4[3];
+        meta = P.tet_meta[tet_b + k];
+    } else if (par_contacts && t - nstatic < ncont) {
+        kind = 4;
+        const int4* c4 = reinterpret_cast<const int4*>(P.cpack) + 4 * (size_t)(cbeg + t - nstatic);
+        r0 = c4[0];
+        r1 = c4[1];
+        r2 = c4[2];
+        r3 = c4[3];
+    }
+    V3 own_x0 = v3(0.0f, 0.0f, 0.0f), own_d = own_x0;
+    const int own_v = vr.x;
+    const bool own_active = (vr.w >> 16) != 0;
     if (t < nv) {
-        int4 vr = reinterpret_cast<const int4*>(P.vert_rec)[vb + t];
-        s_v[t] = vr.x;
+        const float m = P.mass[own_v];
+        float* a = s_acc + t * kComp;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] = (m * (P.inertia[3 * own_v + c] - q[3 * own_v + c])) * dtr;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a[3 + 3 * r + c] = (m * dtr) * (r == c ? 1.0f : 0.0f);
+        own_x0 = ld3(P.q_prev, own_v);
+        own_d = ld3(P.disp, own_v);
         s_row[t] = vr.y;
         s_nrow[t] = vr.z;
-        s_act[t] = vr.w >> 16;
-        s_cb[t] = P.pos_coff[vb + t] - cbeg;
     }
-    if (t == kBlock - 1) s_cb[nv] = ncont;
-    __syncthreads();
+    if (t <= nv) s_cb[t] = cbt;
 
-    const int cs = t / kComp;
-    const int cc = t - cs * kComp;
-    const bool comp_lane = cs < nv;
-    float acc = 0.0f;
-    if (comp_lane) {
-        int v = s_v[cs];
-        float m = P.mass[v];
-        if (cc < 3) {
-            acc = (m * (P.inertia[3 * v + cc] - q[3 * v + cc])) * dtr;
-        } else {
-            int r = (cc - 3) / 3, c = (cc - 3) % 3;
-            acc = (m * dtr) * (r == c ? 1.0f : 0.0f);
-        }
-    }
-
-    const int nitems = nstatic + (par_contacts ? ncont : 0);
-    for (int it = t; it < nitems; it += kBlock) {
+    // ---- round trip 3 + phase a: evaluate the lane's item
+    if (kind != 0) {
         V3 fe = v3(0.0f, 0.0f, 0.0f);
         M3 he = mzero();
         int row;
-        if (it < ntri) {
-            const int4* r4 = reinterpret_cast<const int4*>(P.tri_rec) + 3 * (size_t)(tri_b + it);
-            int4 a = r4[0];
-            float4 pz = __builtin_bit_cast(float4, r4[1]);
-            float4 mt = __builtin_bit_cast(float4, r4[2]);
-            row = a.w & 0xffff;
+        if (kind == 1) {
+            float4 pz = __builtin_bit_cast(float4, r1);
+            float4 mt = __builtin_bit_cast(float4, r2);
+            row = r0.w & 0xffff;
             float pose[4] = {pz.x, pz.y, pz.z, pz.w};
-            eval_membrane(ld3(q, a.x), ld3(q, a.y), ld3(q, a.z), P.q_prev, a.x, a.y, a.z, pose, mt.x, mt.y, mt.z, mt.w,
-                          dt, a.w >> 16, fe, he);
-        } else if (it < ntri + nedge) {
-            const int k = it - ntri;
-            const int4* r4 = reinterpret_cast<const int4*>(P.edge_rec) + 2 * (size_t)(edge_b + k);
-            int4 a = r4[0];
-            float4 pr = __builtin_bit_cast(float4, r4[1]);
-            int mt = P.edge_meta[edge_b + k];
-            row = mt & 0xffff;
-            eval_bending(q, P.q_prev, a.x, a.y, a.z, a.w, pr.x, pr.y, pr.z, pr.w, dt, mt >> 16, fe, he);
-        } else if (it < nstatic) {
-            const int k = it - ntri - nedge;
-            const int4* r4 = reinterpret_cast<const int4*>(P.tet_rec) + 4 * (size_t)(tet_b + k);
-            int4 a = r4[0];
-            float4 d0 = __builtin_bit_cast(float4, r4[1]);
-            float4 d1 = __builtin_bit_cast(float4, r4[2]);
-            float4 d2 = __builtin_bit_cast(float4, r4[3]);
-            int ti[4] = {a.x, a.y, a.z, a.w};
+            eval_membrane(ld3(q, r0.x), ld3(q, r0.y), ld3(q, r0.z), P.q_prev, r0.x, r0.y, r0.z, pose, mt.x, mt.y,
+                          mt.z, mt.w, dt, r0.w >> 16, fe, he);
+        } else if (kind == 2) {
+            float4 pr = __builtin_bit_cast(float4, r1);
+            row = meta & 0xffff;
+            eval_bending(q, P.q_prev, r0.x, r0.y, r0.z, r0.w, pr.x, pr.y, pr.z, pr.w, dt, meta >> 16, fe, he);
+        } else if (kind == 3) {
+            float4 d0 = __builtin_bit_cast(float4, r1);
+            float4 d1 = __builtin_bit_cast(float4, r2);
+            float4 d2 = __builtin_bit_cast(float4, r3);
+            int ti[4] = {r0.x, r0.y, r0.z, r0.w};
             float dm[9] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, d2.x};
-            int mt = P.tet_meta[tet_b + k];
-            row = mt & 0xffff;
-            eval_tet(q, P.q_prev, ti, dm, d2.y, d2.z, d2.w, dt, mt >> 16, fe, he);
+            row = meta & 0xffff;
+            eval_tet(q, P.q_prev, ti, dm, d2.y, d2.z, d2.w, dt, meta >> 16, fe, he);
         } else {
-            const int j = it - nstatic;
-            eval_packed_contact(P, q, cbeg + j, dt, fe, he);
-            row = nstatic + j;
+            float4 A = __builtin_bit_cast(float4, r0), B = __builtin_bit_cast(float4, r1);
+            float4 C = __builtin_bit_cast(float4, r2), D = __builtin_bit_cast(float4, r3);
+            int p = r0.x;
+            eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z,
+                         B.y, A.z, A.w, B.x, P.friction_epsilon, dt, fe, he);
+            row = t;  // = nstatic + contact index
         }
         store_row(s_rows + row * kComp, fe, he);
     }
+    // contacts beyond the first kBlock items (only when nstatic + ncont > kBlock)
+    const int nitems = nstatic + (par_contacts ? ncont : 0);
+    for (int it = t + kBlock; it < nitems; it += kBlock) {
+        V3 fe;
+        M3 he;
+        eval_packed_contact(P, q, cbeg + it - nstatic, dt, fe, he);
+        store_row(s_rows + it * kComp, fe, he);
+    }
     __syncthreads();
 
-    if (comp_lane) {
-        const int r0 = s_row[cs], r1 = r0 + s_nrow[cs];
-        for (int r = r0; r < r1; ++r) acc = acc + s_rows[r * kComp + cc];
+    // ---- phase b
+    const int cs = t / kComp;
+    const int cc = t - cs * kComp;
+    if (cs < nv) {
+        float acc = s_acc[cs * kComp + cc];
+        const int q0 = s_row[cs], q1 = q0 + s_nrow[cs];
+        for (int r = q0; r < q1; ++r) acc = acc + s_rows[r * kComp + cc];
         if (par_contacts) {
             const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
             float ca = 0.0f;
@@ -840,6 +887,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restr
     }
     __syncthreads();
 
+    // ---- phase c
     if (t < nv) {
         const float* a = s_acc + t * kComp;
         V3 f = v3(a[0], a[1], a[2]);
@@ -848,8 +896,8 @@ __global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restr
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
-        finish_vertex(P, q, s_v[t], s_act[t] != 0, f, h, !par_contacts, cbeg + s_cb[t], cbeg + s_cb[t + 1], dt, mode,
-                      g_out);
+        finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, !par_contacts, cbeg + s_cb[t],
+                      cbeg + s_cb[t + 1], dt, mode, g_out);
     }
 }
 
@@ -969,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
             h = madd(h, ch);
             f = add(f, cf);
         }
-        finish_vertex(P, q, v, active, f, h, false, 0, 0, dt, mode, g_out);
+        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, dt, mode, g_out);
     }
 }
 
@@ -1115,6 +1163,21 @@ __global__ void k_contact_pack(int n, const int* __restrict__ color_particles, c
         o[3] = make_float4(body_vel[3 * cid + 2], normal[3 * cid], normal[3 * cid + 1], normal[3 * cid + 2]);
         cid_to_pack[cid] = base + j;
     }
+}
+
+// Contact range of every batch from the color-position offsets: the first packed
+// contact and count (cinfo) and the kMaxV + 1 vertex offsets relative to the first
+// (coff), so the color pass reads them indexed by batch in its first round trip.
+__global__ void k_batch_contacts(int nbatch, const int* __restrict__ blk, const int* __restrict__ pos_coff,
+                                 int* __restrict__ cinfo, int* __restrict__ coff) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatch) return;
+    const int* h = blk + (size_t)b * vbd_layout::kBlockWords;
+    const int vb = h[0], nv = h[1];
+    const int c0 = pos_coff[vb];
+    for (int k = 0; k <= nv; ++k) coff[(size_t)b * (kMaxV + 1) + k] = pos_coff[vb + k] - c0;
+    cinfo[2 * b] = c0;
+    cinfo[2 * b + 1] = pos_coff[vb + nv] - c0;
 }
 
 // update_duals_body_particle_contacts, rigid_vbd_kernels.py:4861-4926 (particle records)
@@ -1341,6 +1404,8 @@ struct VbdContext {
     int* lay_tet = nullptr;
     int* lay_tet_meta = nullptr;
     int* lay_slow = nullptr;
+    int* bat_cinfo = nullptr;  // [B, 2] per-substep contact range of each batch
+    int* bat_coff = nullptr;   // [B, kMaxV + 1] per-substep contact offsets within the batch
     int64_t fast_items = 0;
     VbdContactsDesc contacts{};
     bool have_contacts = false;
@@ -1379,7 +1444,7 @@ std::string h2d(T** dst, const std::vector<T>& src) {
 
 void free_layout(VbdContext* c) {
     for (int** p : {&c->lay_batch, &c->lay_vert, &c->lay_tri, &c->lay_edge, &c->lay_edge_meta, &c->lay_tet,
-                    &c->lay_tet_meta, &c->lay_slow}) {
+                    &c->lay_tet_meta, &c->lay_slow, &c->bat_cinfo, &c->bat_coff}) {
         hipFree(*p);
         *p = nullptr;
     }
@@ -1434,7 +1499,7 @@ int32_t build_layout(VbdContext* c) {
     err = vbd_layout::build_pass_layout(hm, L);
     if (!err.empty()) return fail(VBD_ERR_ARG, "vbd_create: " + err);
     free_layout(c);
-    for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_rec), std::make_pair(&c->lay_vert, &L.vert_rec),
+    for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_blk), std::make_pair(&c->lay_vert, &L.vert_rec),
                     std::make_pair(&c->lay_tri, &L.tri_rec), std::make_pair(&c->lay_edge, &L.edge_rec),
                     std::make_pair(&c->lay_edge_meta, &L.edge_meta), std::make_pair(&c->lay_tet, &L.tet_rec),
                     std::make_pair(&c->lay_tet_meta, &L.tet_meta), std::make_pair(&c->lay_slow, &L.slow_pos)}) {
@@ -1444,8 +1509,21 @@ int32_t build_layout(VbdContext* c) {
     c->fast_color_off = L.fast_color_off;
     c->slow_color_off = L.slow_color_off;
     c->fast_items = L.fast_items;
+    {
+        // contact ranges per batch: zero (no contacts) until vbd_contact_prepare fills them
+        const size_t nbat = (size_t)std::max(L.fast_color_off.back(), 1);
+        hipError_t e;
+        if ((e = hipMalloc(&c->bat_cinfo, sizeof(int) * 2 * nbat)) != hipSuccess ||
+            (e = hipMalloc(&c->bat_coff, sizeof(int) * (kMaxV + 1) * nbat)) != hipSuccess ||
+            (e = hipMemset(c->bat_cinfo, 0, sizeof(int) * 2 * nbat)) != hipSuccess ||
+            (e = hipMemset(c->bat_coff, 0, sizeof(int) * (kMaxV + 1) * nbat)) != hipSuccess ||
+            (e = hipDeviceSynchronize()) != hipSuccess)
+            return fail(VBD_ERR_HIP, std::string("vbd_create: batch contact ranges: ") + hipGetErrorString(e));
+    }
     KParams& k = c->kp;
-    k.batch_rec = c->lay_batch;
+    k.batch_blk = c->lay_batch;
+    k.bat_cinfo = c->bat_cinfo;
+    k.bat_coff = c->bat_coff;
     k.vert_rec = c->lay_vert;
     k.tri_rec = c->lay_tri;
     k.edge_rec = c->lay_edge;
@@ -1622,6 +1700,15 @@ int32_t vbd_layout_stats(VbdContext* c, int64_t* out4) {
     return VBD_OK;
 }
 
+static int32_t launch_batch_contacts(VbdContext* c, hipStream_t s) {
+    const int nbat = c->fast_color_off.back();
+    if (nbat <= 0) return VBD_OK;
+    hipLaunchKernelGGL(k_batch_contacts, dim3(blocks_for(nbat, 256)), dim3(256), 0, s, nbat, c->lay_batch, c->pos_coff,
+                       c->bat_cinfo, c->bat_coff);
+    VBD_LAUNCH_CHECK("k_batch_contacts");
+    return VBD_OK;
+}
+
 int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* cd) {
     if (!c) return fail(VBD_ERR_ARG, "vbd_contact_prepare: null context");
     hipStream_t s = (hipStream_t)stream;
@@ -1632,7 +1719,7 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
         VBD_LAUNCH_CHECK("k_fill_i32");
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->pos_coff, 0, N + 1);
         VBD_LAUNCH_CHECK("k_fill_i32");
-        return VBD_OK;
+        return launch_batch_contacts(c, s);
     }
     if (cd->soft_contact_max > c->capacity)
         return fail(VBD_ERR_STATE, "vbd_contact_prepare: soft_contact_max " + std::to_string(cd->soft_contact_max) +
@@ -1685,6 +1772,8 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
                        c->model.shape_margin, c->model.shape_count, c->model.particle_radius, c->cpack,
                        c->cid_to_pack);
     VBD_LAUNCH_CHECK("k_contact_pack");
+    int32_t rc = launch_batch_contacts(c, s);
+    if (rc) return rc;
     KParams& k = c->kp;
     k.c_shape = cd->soft_contact_shape;
     k.c_body_pos = cd->soft_contact_body_pos;

@@ -32,6 +32,9 @@ constexpr int kEdgeWords = 8;   // o0 o1 v0 v1 | rest_angle rest_length ke kd   
 constexpr int kTetWords = 16;   // i0 i1 i2 i3 | Dm^-1 (9) | mu lambda kd         (+ tet_meta)
 constexpr int kBatchWords = 8;  // vbegin nv tri_begin ntri edge_begin nedge tet_begin ntet
 constexpr int kVertWords = 4;   // v row_start nrows (local | active << 16)
+// batch block: header (kBatchWords) then kLayoutMaxV vertex records (unused ones zero),
+// fixed size so a workgroup reads its header and vertex records in one round trip
+constexpr int kBlockWords = kBatchWords + kLayoutMaxV * kVertWords;  // 92 words = 368 B (16-B multiple)
 
 inline int32_t meta(int row, int slot) { return (int32_t)(row | (slot << 16)); }
 
@@ -50,7 +53,7 @@ struct HostModel {
 struct PassLayout {
     std::vector<int32_t> fast_color_off;  // [C+1] into batch records
     std::vector<int32_t> slow_color_off;  // [C+1] into slow_pos
-    std::vector<int32_t> batch_rec;       // [B * kBatchWords]
+    std::vector<int32_t> batch_blk;       // [B * kBlockWords]: header + vertex records
     std::vector<int32_t> vert_rec;        // [N * kVertWords], indexed by color position
     std::vector<int32_t> tri_rec, edge_rec, edge_meta, tet_rec, tet_meta;
     std::vector<int32_t> slow_pos;        // color positions of oversized vertices
@@ -95,7 +98,7 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L) {
     L.slow_color_off.assign((size_t)C + 1, 0);
     std::vector<int> rows;
     for (int c = 0; c < C; ++c) {
-        L.fast_color_off[c] = (int32_t)(L.batch_rec.size() / kBatchWords);
+        L.fast_color_off[c] = (int32_t)(L.batch_blk.size() / kBlockWords);
         L.slow_color_off[c] = (int32_t)L.slow_pos.size();
         int k = m.color_off[c];
         const int kend = m.color_off[c + 1];
@@ -198,10 +201,14 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L) {
             }
             rec[7] = (int32_t)L.tet_meta.size() - rec[6];
             L.fast_items += rec[3] + rec[5] + rec[7];
-            L.batch_rec.insert(L.batch_rec.end(), rec, rec + kBatchWords);
+            const size_t at = L.batch_blk.size();
+            L.batch_blk.resize(at + kBlockWords, 0);
+            std::memcpy(&L.batch_blk[at], rec, sizeof(rec));
+            std::memcpy(&L.batch_blk[at + kBatchWords], &L.vert_rec[(size_t)vb * kVertWords],
+                        sizeof(int32_t) * kVertWords * (size_t)nv);
         }
     }
-    L.fast_color_off[C] = (int32_t)(L.batch_rec.size() / kBatchWords);
+    L.fast_color_off[C] = (int32_t)(L.batch_blk.size() / kBlockWords);
     L.slow_color_off[C] = (int32_t)L.slow_pos.size();
     return std::string();
 }
