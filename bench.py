@@ -1,17 +1,28 @@
-"""Benchmark: vertex-steps/s of the MI355X VBD solver on the 256x256 cloth-on-sphere workload.
+"""Benchmark of the MI355X VBD solver: vertex-steps/s, HBM roofline, residual-driven comparison.
 
-One step = one frame of the reference's usage loop: 10 substeps, each
-clear_forces -> CollisionPipeline.collide -> SolverVBD.step (10 iterations),
-captured once in a HIP graph and replayed.  Prints ONE JSON line with the
-throughput, a roofline record for the dominant kernel (the color pass) and a
-CPU baseline (the FP32 CPU oracle, OpenMP over each color's vertices, on a
-bounded sample of the same workload).
+Prints ONE JSON line.  Its ``value`` is the throughput of the reference's
+configured loop on the 256x256 cloth-on-sphere workload (65,536 vertices):
+one step = one frame = 10 substeps, each clear_forces ->
+CollisionPipeline.collide -> SolverVBD.step (10 iterations), captured once in a
+HIP graph and replayed; vertex-steps/s = vertices x substeps x frames / GPU time.
+The same line carries
+* ``roofline``: the color-pass kernel's algorithmic bytes / its live launch time
+  (HIP events on its stream) against 8 TB/s, plus HBM traffic from two child
+  ``rocprofv3 --pmc`` runs (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy;
+* ``batched``: the 4096-world cloth (~7.3M vertices, 5 iterations) in one
+  graph-captured frame per step, with its own color-pass roofline;
+* ``residual``: the 128x128 wind cloth driven to a relative force residual,
+  plain VBD against the learned warm start (trained on the GPU in the same run
+  from plain-VBD data), iterations and vertex-steps/s of each;
+* ``cpu_baseline``: the FP32 CPU oracle (OpenMP) on a bounded sample of the
+  headline workload.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -21,37 +32,153 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+MLP_FLOPS_PER_ROW = 2 * (48 * 128 + 2 * 128 * 128 + 128 * 3)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed frames")
+    ap.add_argument("--steps", type=int, default=30, help="timed frames of the headline workload")
     ap.add_argument("--warmup", type=int, default=3, help="untimed frames")
-    ap.add_argument("--verts", type=int, default=256, help="cloth vertices per side")
+    ap.add_argument("--verts", type=int, default=256, help="cloth vertices per side (headline workload)")
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time budget (0 disables)")
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per color pass")
+    ap.add_argument("--worlds", type=int, default=4096, help="worlds of the batched workload (0 disables the leg)")
+    ap.add_argument("--batched-steps", type=int, default=4)
+    ap.add_argument("--residual-frames", type=int, default=3, help="evaluation frames of the residual leg (0 disables)")
+    ap.add_argument("--rel-tol", type=float, default=0.3, help="relative force residual of the residual leg")
+    ap.add_argument("--max-iterations", type=int, default=400)
+    ap.add_argument("--train-frames", type=int, default=40)
+    ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
+    ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
+    ap.add_argument("--pmc-workload", choices=("drop", "batched"), default="drop")
     return ap.parse_args()
+
+
+# ----------------------------------------------------------------------------- helpers
+def _scene(name, args, dev):
+    from newton_amd import scenes
+
+    if name == "drop":
+        return scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
+    return scenes.cloth_batched(dev, worlds=args.worlds)
+
+
+def _contact_counts(model, contacts):
+    import numpy as np
+
+    if not contacts.soft_contact_max:
+        return None
+    c = int(contacts.soft_contact_count.item())
+    return np.bincount(contacts.soft_contact_particle.cpu().numpy()[:c], minlength=model.particle_count)
+
+
+def time_frames(model, state0, info, solver, pipe, contacts, steps, warmup):
+    """Graph-captured frames (10 x collide + step); returns (gpu_ms total, wall s, finite, final state)."""
+    import torch
+
+    sub, dt = info["substeps"], info["dt"]
+    states = [state0, model.state()]
+
+    def frame():
+        a, b = states
+        for _ in range(sub):
+            a.clear_forces()
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, dt)
+            a, b = b, a
+        # an even substep count leaves the result in states[0]
+
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        for _ in range(max(warmup, 1)):
+            frame()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            frame()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            graph.replay()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(steps):
+            graph.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gpu_ms = ev0.elapsed_time(ev1)
+    finite = bool(torch.isfinite(states[0].particle_q).all().item())
+    del graph
+    return gpu_ms, wall, finite, states[0]
+
+
+def color_roofline(model, state, info, solver, pipe, contacts, reps):
+    """Algorithmic bytes of each color pass over its live launch time (back-to-back launches, HIP events)."""
+    import numpy as np
+    import torch
+
+    from newton_amd.roofline import color_pass_bytes
+
+    dt = info["dt"]
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        state.clear_forces()
+        pipe.collide(state, contacts)
+        solver.begin_step(state, contacts, dt)
+        torch.cuda.synchronize()
+        per_color_bytes = color_pass_bytes(model, contacts_per_particle=_contact_counts(model, contacts))
+        ncol = len(model.particle_color_groups)
+        color_ms = np.zeros(ncol)
+        for c in range(ncol):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            solver.color_pass(state, c, dt)  # warm
+            e0.record(stream)
+            for _ in range(reps):
+                solver.color_pass(state, c, dt)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            color_ms[c] = e0.elapsed_time(e1) / reps
+    total_bytes = float(per_color_bytes.sum())
+    achieved = total_bytes / (float(color_ms.sum()) / 1e3) / 1e9
+    return {
+        "kernel": "k_solve_fast",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "traffic_detail": None,
+        "bytes_per_launch": round(total_bytes / ncol, 1),
+        "avg_launch_us": round(float(color_ms.mean()) * 1e3, 3),
+    }
 
 
 def pmc_mode(args):
     """Minimal workload for rocprofv3 --pmc: `reps` launches of each color pass and one known-size copy."""
-    import numpy as np
     import torch
 
-    from newton_amd import CollisionPipeline, scenes
+    from newton_amd import CollisionPipeline
     from newton_amd.roofline import color_pass_bytes
     from newton_amd.solvers import SolverVBD
 
     dev = "cuda:0"
-    model, state, info = scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
-    solver = SolverVBD(model, iterations=args.iterations)
+    model, state, info = _scene(args.pmc_workload, args, dev)
+    solver = SolverVBD(model, iterations=info["iterations"])
     pipe = CollisionPipeline(model, soft_contact_margin=0.01)
     contacts = pipe.contacts()
     state.clear_forces()
@@ -66,14 +193,12 @@ def pmc_mode(args):
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    counts = np.bincount(contacts.soft_contact_particle.cpu().numpy()[: int(contacts.soft_contact_count.item())],
-                         minlength=model.particle_count)
-    b = color_pass_bytes(model, contacts_per_particle=counts)
-    print(json.dumps({"pmc_mode": True, "color_pass_launches": args.roofline_reps * ncol,
+    b = color_pass_bytes(model, contacts_per_particle=_contact_counts(model, contacts))
+    print(json.dumps({"pmc_mode": True, "workload": info["workload"], "color_pass_launches": args.roofline_reps * ncol,
                       "algorithmic_bytes_per_color_pass": [int(x) for x in b], "copy_bytes": nbytes}))
 
 
-def measure_traffic(args, timeout_s: float = 240.0):
+def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
     """HBM bytes per color-pass launch from two rocprofv3 --pmc child runs (FETCH_SIZE, then WRITE_SIZE).
 
     Each counter is scaled by what it reads for the 256 MiB device copy made in
@@ -93,8 +218,8 @@ def measure_traffic(args, timeout_s: float = 240.0):
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             d = os.path.join(tmp, counter)
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
-                   str(ROOT / "bench.py"), "--pmc-mode", "--verts", str(args.verts), "--iterations",
-                   str(args.iterations), "--roofline-reps", "5"]
+                   str(ROOT / "bench.py"), "--pmc-mode", "--pmc-workload", workload, "--verts", str(args.verts),
+                   "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--roofline-reps", "5"]
             try:
                 proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
                                       env={**os.environ, "TMPDIR": "/tmp"})
@@ -119,6 +244,201 @@ def measure_traffic(args, timeout_s: float = 240.0):
                    "write_scale": round(ws, 4)}
 
 
+# ----------------------------------------------------------------------------- legs
+def batched_leg(args):
+    """4096 uneven cloth worlds in one launch per color pass; graph-captured frames; color-pass roofline."""
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.solvers import SolverVBD
+
+    dev = "cuda:0"
+    t0 = time.perf_counter()
+    model, state, info = scenes.cloth_batched(dev, worlds=args.worlds)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    solver = SolverVBD(model, iterations=info["iterations"])
+    t_solver = time.perf_counter() - t0
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, args.batched_steps, 1)
+    n = model.particle_count
+    roof = color_roofline(model, last, info, solver, pipe, contacts, max(3, args.roofline_reps // 4))
+    return {
+        "workload": info["workload"],
+        "vertices": n,
+        "worlds": args.worlds,
+        "colors": len(model.particle_color_groups),
+        "iterations": info["iterations"],
+        "substeps_per_step": info["substeps"],
+        "steps": args.batched_steps,
+        "ms_per_step": round(gpu_ms / args.batched_steps, 3),
+        "vertex_steps_per_s": round(n * info["substeps"] * args.batched_steps / (gpu_ms / 1e3), 1),
+        "hip_graph": True,
+        "finite": finite,
+        "host_build_s": round(t_build, 2),
+        "solver_create_s": round(t_solver, 2),
+        "layout": solver.layout_stats(),
+        "roofline": roof,
+    }
+
+
+def residual_leg(args):
+    """Wind cloth driven to a relative force residual: plain VBD vs the learned warm start.
+
+    The warm-start network is trained in this run, on the GPU, from plain-VBD
+    substeps of the same scene (``train_frames`` frames, ``train_iterations``
+    iterations each), then both methods solve the next ``residual_frames``
+    frames from identical states; each substep stops at ``rel_tol`` (or
+    ``max_iterations``).  Times are wall-clock per substep with the stream
+    synchronised (the residual checks are host round trips).
+    """
+    import numpy as np
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+    from newton_amd.warmstart import LearnedWarmStart
+
+    dev = "cuda:0"
+    model, state, info = scenes.cloth_wind(dev)
+    wind, dt, sub = info["wind"], info["dt"], info["substeps"]
+    n = model.particle_count
+    solver = SolverVBD(model, iterations=10)
+    ws = LearnedWarmStart(model, dt, seed=0)
+    mass = model.particle_mass
+    s0, s1 = state, model.state()
+    feats, targs = [], []
+    t0 = time.perf_counter()
+    for frame in range(args.train_frames):
+        wind.redraw(frame)
+        for _ in range(sub):
+            s0.particle_f.copy_(wind.force(s0.particle_q, mass))
+            solver.begin_step(s0, None, dt)
+            feats.append(ws.features(solver, s0).clone())
+            solver.iterate(s0, dt, args.train_iterations)
+            targs.append(ws.target(solver, s0))
+            ws.observe(solver, s0)
+            solver.end_step(s0, s1, dt)
+            s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    X = torch.cat(feats)
+    T = torch.cat(targs).contiguous()
+    W = ws.active.float().repeat(len(feats))
+    del feats, targs
+    rows = X.shape[0]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    batch = 32768
+    t0 = time.perf_counter()
+    first = None
+    loss = None
+    for step in range(args.train_steps):
+        idx = torch.randint(0, rows, (batch,), device=dev, generator=gen)
+        lr = max(1e-3 * 0.5 * (1.0 + math.cos(math.pi * step / args.train_steps)), 1e-5)
+        loss = ws.mlp.train_step(X[idx].contiguous(), T[idx].contiguous(), W[idx].contiguous(), lr=lr)
+        if step == 0:
+            first = float(loss.item())
+    last = float(loss.item()) if loss is not None else None
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    target_ms = float((T[W > 0] ** 2).mean().item())
+    train_flops = 3 * MLP_FLOPS_PER_ROW * batch * args.train_steps
+    del X, T, W
+
+    # MFMA roofline of the inference kernel on one substep's rows (the last features gathered)
+    pred = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    ws.mlp.forward(ws.feat, pred)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        ws.mlp.forward(ws.feat, pred)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 50
+    fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
+
+    res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in ("plain", "learned")}
+    accepted = 0
+    r_ws_rel = []
+    for frame in range(args.residual_frames):
+        wind.redraw(args.train_frames + frame)
+        for _ in range(sub):
+            q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), ws.corr.clone()
+            f_ext = wind.force(s0.particle_q, mass)
+            for name in ("plain", "learned"):
+                s0.particle_q.copy_(q0)
+                s0.particle_qd.copy_(qd0)
+                s0.particle_f.copy_(f_ext)
+                ws.corr.copy_(corr0)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = solver.step_to_residual(s0, s1, None, None, dt, rel_tol=args.rel_tol,
+                                            max_iterations=args.max_iterations,
+                                            warm_start=ws if name == "learned" else None)
+                torch.cuda.synchronize()
+                res[name]["s"] += time.perf_counter() - t0
+                res[name]["iters"].append(r["iterations"])
+                res[name]["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
+                if name == "learned":
+                    accepted += int(r["warm_start_accepted"])
+                    if r["r_warm_start"] is not None and r["r0"] > 0:
+                        r_ws_rel.append(r["r_warm_start"] / r["r0"])
+            ws.observe(solver, s0)  # the learned run's iterate carries on
+            s0, s1 = s1, s0
+    substeps = args.residual_frames * sub
+    out = {"workload": info["workload"], "vertices": n, "rel_tol": args.rel_tol, "max_iterations": args.max_iterations,
+           "substeps": substeps, "check_every": 1}
+    for name in ("plain", "learned"):
+        d = res[name]
+        out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
+                     "reached_tol": f"{d['reached']}/{substeps}",
+                     "vertex_steps_per_s": round(n * substeps / d["s"], 1)}
+    out["speedup_vertex_steps"] = round(res["plain"]["s"] / res["learned"]["s"], 3)
+    out["iteration_ratio"] = round(float(np.mean(res["plain"]["iters"]) /
+                                         max(float(np.mean(res["learned"]["iters"])), 1e-9)), 3)
+    out["warm_start_accepted"] = f"{accepted}/{substeps}"
+    out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
+    out["training"] = {"frames": args.train_frames, "substeps": args.train_frames * sub,
+                       "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
+                       "batch": batch, "loss_first": None if first is None else round(first, 5),
+                       "loss_last": None if last is None else round(last, 6),
+                       "target_mean_square": round(target_ms, 5), "data_s": round(t_data, 2),
+                       "train_s": round(t_train, 2), "train_tflops": round(train_flops / t_train / 1e12, 2)}
+    out["mlp_roofline"] = {"kernel": "k_forward (libvbd_ws)", "bound": "mfma", "achieved": round(fwd_tflops, 2),
+                           "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(fwd_tflops / BF16_PEAK_TFLOPS, 5), "rows": n,
+                           "avg_launch_us": round(fwd_ms * 1e3, 2)}
+    return out
+
+
+def cpu_baseline_leg(args, model, q_init, qd_init, dt):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # noqa: PLC0415  (the checker, timed as the CPU baseline)
+
+    n = model.particle_count
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    om = oracle.OracleModel(model)
+    prm = oracle.params(args.iterations, num_threads=cores)
+    q, qd = q_init, qd_init
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        q, qd, _ = oracle.simulate(om, prm, q, qd, 1, dt, 0.01, len(om.pairs) // 2)
+        done += 1
+        if time.perf_counter() - t0 >= args.cpu_seconds or done >= 200:
+            break
+    cpu_t = time.perf_counter() - t0
+    return {
+        "value": round(n * done / cpu_t, 1),
+        "unit": "vertex-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{done} substeps (collide + step, {args.iterations} iterations) of the same "
+                  f"{args.verts}x{args.verts} cloth from its initial state, FP32 C oracle, OpenMP {cores} threads",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,135 +448,36 @@ def main():
     if args.pmc_mode:
         pmc_mode(args)
         return
-    import numpy as np
     import torch
 
-    from newton_amd import CollisionPipeline, scenes
-    from newton_amd.roofline import color_pass_bytes
+    from newton_amd import CollisionPipeline
     from newton_amd.solvers import SolverVBD
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    model, state0, info = scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
+    model, state0, info = _scene("drop", args, dev)
     n = model.particle_count
-    dt = info["dt"]
     sub = info["substeps"]
     solver = SolverVBD(model, iterations=args.iterations)
     pipe = CollisionPipeline(model, soft_contact_margin=0.01)
     contacts = pipe.contacts()
-    state1 = model.state()
     q_init = state0.particle_q.cpu().numpy().copy()
     qd_init = state0.particle_qd.cpu().numpy().copy()
-    states = [state0, state1]
-
-    def frame():
-        a, b = states
-        for _ in range(sub):
-            a.clear_forces()
-            pipe.collide(a, contacts)
-            solver.step(a, b, None, contacts, dt)
-            a, b = b, a
-        # an even substep count leaves the result in states[0]
-
-    stream = torch.cuda.Stream()
-    stream.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(stream):
-        for _ in range(max(args.warmup, 1)):
-            frame()
-        torch.cuda.synchronize()
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph, stream=stream):
-            frame()
-        torch.cuda.synchronize()
-        for _ in range(2):
-            graph.replay()
-        torch.cuda.synchronize()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        ev0.record(stream)
-        for _ in range(args.steps):
-            graph.replay()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        gpu_ms = ev0.elapsed_time(ev1)
-    q_now = states[0].particle_q
-    finite = bool(torch.isfinite(q_now).all().item())
-    elapsed = gpu_ms / 1e3
-    value = n * sub * args.steps / elapsed
-
-    # ---- roofline of the color-pass kernel: back-to-back launches timed with events on its stream
-    with torch.cuda.stream(stream):
-        s = states[0]
-        s.clear_forces()
-        pipe.collide(s, contacts)
-        solver.begin_step(s, contacts, dt)
-        torch.cuda.synchronize()
-        counts = np.bincount(contacts.soft_contact_particle.cpu().numpy()[: int(contacts.soft_contact_count.item())],
-                             minlength=n) if contacts.soft_contact_max else None
-        per_color_bytes = color_pass_bytes(model, contacts_per_particle=counts)
-        ncol = len(model.particle_color_groups)
-        color_ms = np.zeros(ncol)
-        for c in range(ncol):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            solver.color_pass(s, c, dt)  # warm
-            e0.record(stream)
-            for _ in range(args.roofline_reps):
-                solver.color_pass(s, c, dt)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            color_ms[c] = e0.elapsed_time(e1) / args.roofline_reps
-    total_bytes = float(per_color_bytes.sum())
-    total_s = float(color_ms.sum()) / 1e3
-    achieved = total_bytes / total_s / 1e9
-    roofline = {
-        "kernel": "k_solve_fast",
-        "bound": "hbm",
-        "achieved": round(achieved, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "traffic_detail": None,
-        "bytes_per_launch": round(total_bytes / ncol, 1),
-        "avg_launch_us": round(float(color_ms.mean()) * 1e3, 3),
-        "launches_per_substep": ncol * args.iterations,
-    }
-
+    gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
+    value = n * sub * args.steps / (gpu_ms / 1e3)
+    roofline = color_roofline(model, last, info, solver, pipe, contacts, args.roofline_reps)
+    roofline["launches_per_substep"] = len(model.particle_color_groups) * args.iterations
     if args.traffic:
-        traffic, detail = measure_traffic(args)
+        traffic, detail = measure_traffic(args, "drop")
         roofline["traffic"] = None if traffic is None else round(traffic, 1)
         roofline["traffic_detail"] = detail
+    del solver, pipe, contacts
+    torch.cuda.empty_cache()
 
-    # ---- CPU baseline: the FP32 CPU oracle on a bounded sample of the same workload
-    cpu = None
-    if args.cpu_seconds > 0:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle  # noqa: PLC0415  (the checker, timed as the CPU baseline)
-
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        om = oracle.OracleModel(model)
-        prm = oracle.params(args.iterations, num_threads=cores)
-        q = q_init
-        qd = qd_init
-        done = 0
-        t0 = time.perf_counter()
-        while True:
-            q, qd, _ = oracle.simulate(om, prm, q, qd, 1, dt, 0.01, len(om.pairs) // 2)
-            done += 1
-            if time.perf_counter() - t0 >= args.cpu_seconds or done >= 200:
-                break
-        cpu_t = time.perf_counter() - t0
-        cpu = {
-            "value": round(n * done / cpu_t, 1),
-            "unit": "vertex-steps/s",
-            "cores": cores,
-            "kind": "port",
-            "sample": f"{done} substeps (collide + step, {args.iterations} iterations) of the same "
-                      f"{args.verts}x{args.verts} cloth from its initial state, FP32 C oracle, OpenMP {cores} threads",
-        }
+    batched = batched_leg(args) if args.worlds > 0 else None
+    torch.cuda.empty_cache()
+    residual = residual_leg(args) if args.residual_frames > 0 else None
+    cpu = cpu_baseline_leg(args, model, q_init, qd_init, info["dt"]) if args.cpu_seconds > 0 else None
 
     rec = {
         "metric": "vertex-steps/s at fixed force residual; plain VBD vs batched+learned warm-start",
@@ -279,12 +500,14 @@ def main():
             "colors": len(model.particle_color_groups),
             "substeps_per_step": sub,
             "iterations": args.iterations,
-            "solver": "plain VBD, fixed iterations",
+            "solver": "plain VBD, the reference's fixed iteration count",
             "hip_graph": True,
             "wall_s": round(wall, 4),
             "finite": finite,
         },
         "roofline": roofline,
+        "batched": batched,
+        "residual": residual,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec))

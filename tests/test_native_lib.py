@@ -57,3 +57,38 @@ def test_hip_library_loads_without_gpu_calls():
     lib = _native.hip_lib()
     assert lib.vbd_version() >= 1
     assert lib.vbd_last_error() is not None
+
+
+def _declared_prefix(header: Path, prefix: str) -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(" + prefix + r"\w+)\s*\(", text, flags=re.M)))
+
+
+def test_warmstart_library_exports_header_symbols():
+    from newton_amd import _native
+
+    names = _declared_prefix(ROOT / "include" / "vbd_warmstart.h", "vbd_ws_")
+    assert "vbd_ws_forward" in names and "vbd_ws_backward" in names and "vbd_ws_adam" in names
+    lib = ctypes.CDLL(str(ROOT / "newton_amd" / "lib" / "libvbd_ws.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"libvbd_ws.so lacks {missing}"
+    assert set(names) == set(_native.WS_EXPORTS)
+
+
+def test_warmstart_sizes_without_gpu():
+    from newton_amd import _native
+
+    import mlp_oracle
+
+    out = (ctypes.c_int64 * 3)()
+    assert _native.ws_lib().vbd_ws_sizes(out) == 0
+    assert out[0] == mlp_oracle.N_PARAMS == 39683
+    assert out[1] == 81920 and out[2] == 64
+
+
+def test_host_adjacency_exports():
+    from newton_amd import _native
+
+    lib = _native.host_lib()
+    for name in ("vbd_host_edge_tri_maps", "vbd_host_vertex_csr", "vbd_host_color_graph"):
+        assert hasattr(lib, name)
