@@ -63,6 +63,11 @@ def parse():
 
 
 # ----------------------------------------------------------------------------- helpers
+def _progress(msg: str) -> None:
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def _scene(name, args, dev):
     from newton_amd import scenes
 
@@ -310,6 +315,8 @@ def residual_leg(args):
     t0 = time.perf_counter()
     for frame in range(args.train_frames):
         wind.redraw(frame)
+        if frame % 10 == 0:
+            _progress(f"residual leg: training data frame {frame}/{args.train_frames}")
         for _ in range(sub):
             s0.particle_f.copy_(wind.force(s0.particle_q, mass))
             solver.begin_step(s0, None, dt)
@@ -338,6 +345,8 @@ def residual_leg(args):
         loss = ws.mlp.train_step(X[idx].contiguous(), T[idx].contiguous(), W[idx].contiguous(), lr=lr)
         if step == 0:
             first = float(loss.item())
+        if step % 500 == 0:
+            _progress(f"residual leg: training step {step}/{args.train_steps}")
     last = float(loss.item()) if loss is not None else None
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -463,21 +472,35 @@ def main():
     contacts = pipe.contacts()
     q_init = state0.particle_q.cpu().numpy().copy()
     qd_init = state0.particle_qd.cpu().numpy().copy()
+    _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames")
     gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
     value = n * sub * args.steps / (gpu_ms / 1e3)
     roofline = color_roofline(model, last, info, solver, pipe, contacts, args.roofline_reps)
     roofline["launches_per_substep"] = len(model.particle_color_groups) * args.iterations
+    _progress(f"headline: {value:.4g} vertex-steps/s, color pass {roofline['avg_launch_us']} us")
     if args.traffic:
+        _progress("HBM traffic: two rocprofv3 --pmc child runs")
         traffic, detail = measure_traffic(args, "drop")
         roofline["traffic"] = None if traffic is None else round(traffic, 1)
         roofline["traffic_detail"] = detail
     del solver, pipe, contacts
     torch.cuda.empty_cache()
 
-    batched = batched_leg(args) if args.worlds > 0 else None
+    batched = None
+    if args.worlds > 0:
+        _progress(f"batched leg: {args.worlds} worlds")
+        batched = batched_leg(args)
+        _progress(f"batched: {batched['vertex_steps_per_s']:.4g} vertex-steps/s")
     torch.cuda.empty_cache()
-    residual = residual_leg(args) if args.residual_frames > 0 else None
-    cpu = cpu_baseline_leg(args, model, q_init, qd_init, info["dt"]) if args.cpu_seconds > 0 else None
+    residual = None
+    if args.residual_frames > 0:
+        _progress("residual leg: data generation, training, evaluation")
+        residual = residual_leg(args)
+        _progress(f"residual: speedup {residual['speedup_vertex_steps']}")
+    cpu = None
+    if args.cpu_seconds > 0:
+        _progress(f"CPU baseline: {args.cpu_seconds} s")
+        cpu = cpu_baseline_leg(args, model, q_init, qd_init, info["dt"])
 
     rec = {
         "metric": "vertex-steps/s at fixed force residual; plain VBD vs batched+learned warm-start",

@@ -250,6 +250,63 @@ def cloth_wind(device="cuda:0", verts_per_side: int = 128, size: float = 1.0, to
     return model, state, info
 
 
+def soft_cubes_batched(device="cuda:0", worlds: int = 512, cells_min: int = 6, cells_max: int = 12,
+                       cell: float = 0.05, seed: int = 4, iterations: int = 8, particle_radius: float = 0.01):
+    """Many independent tetrahedral cubes (5 tets per hex cell) dropped onto one ground plane.
+
+    World w is a cube of ``n_w ~ U{cells_min..cells_max}`` cells per side
+    (``cell`` m each, so (n_w + 1)^3 particles and 5 n_w^3 tets), stable
+    Neo-Hookean with Young's modulus log-uniform[1e4, 1e5] Pa and Poisson's
+    ratio uniform[0.3, 0.45] (mu = E / (2 (1 + nu)), lambda = E nu / ((1 + nu)
+    (1 - 2 nu))), density 1000 kg/m^3, dropped with its centre at a height
+    uniform in [0.5, 1.5] m and a uniformly random rotation.  Ground contact
+    ke 1e5, mu 0.5.  Seeded from ``seed`` in this order: cells, E, nu, heights,
+    rotations.  The surface triangles carry zero material (skipped by the solve).
+    """
+    rng = np.random.default_rng(seed)
+    cells = rng.integers(cells_min, cells_max + 1, size=worlds)
+    young = np.exp(rng.uniform(math.log(1.0e4), math.log(1.0e5), size=worlds))
+    nu = rng.uniform(0.3, 0.45, size=worlds)
+    heights = rng.uniform(0.5, 1.5, size=worlds)
+    quats = rng.normal(size=(worlds, 4))
+    quats /= np.linalg.norm(quats, axis=1, keepdims=True)
+
+    templates = {}
+    for n in np.unique(cells):
+        n = int(n)
+        t = ModelBuilder(up_axis="Y", gravity=-9.81)
+        half = 0.5 * n * cell
+        t.add_soft_grid(pos=(-half, -half, -half), rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0), dim_x=n, dim_y=n,
+                        dim_z=n, cell_x=cell, cell_y=cell, cell_z=cell, density=1000.0, k_mu=1.0, k_lambda=1.0,
+                        k_damp=0.0, particle_radius=particle_radius)
+        t.color(include_bending=False)
+        templates[n] = t
+
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e5, kd=0.0, mu=0.5))
+    tet_start = np.zeros(worlds + 1, dtype=np.int64)
+    p_start = np.zeros(worlds + 1, dtype=np.int64)
+    for w in range(worlds):
+        p_start[w] = b.particle_count
+        tet_start[w] = b.tet_count
+        b.add_world(templates[int(cells[w])],
+                    xform=(np.array([0.0, heights[w], 0.0], np.float32), quats[w].astype(np.float32)))
+    p_start[worlds] = b.particle_count
+    tet_start[worlds] = b.tet_count
+    mu = young / (2.0 * (1.0 + nu))
+    lam = young * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    tet_mat = b._tet_mat.array()
+    counts = np.diff(tet_start)
+    tet_mat[:, 0] = np.repeat(mu, counts)
+    tet_mat[:, 1] = np.repeat(lam, counts)
+    tet_mat[:, 2] = 0.0
+    model, state = _finish(b, device, 0.0, seed, 1.0e5, 0.0, 0.5)
+    info = {"workload": f"soft_cubes_batched_{worlds}w_{cells_min}-{cells_max}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations, "world_particle_start": p_start,
+            "world_tet_start": tet_start, "cells": cells}
+    return model, state, info
+
+
 def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 4, k_damp: float = 1e-3):
     """Small tetrahedral cube (5 tets per hex cell) above the ground, for tet-path parity."""
     b = ModelBuilder(up_axis="Y", gravity=-9.81)
@@ -269,4 +326,4 @@ def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 
 
 
 __all__ = ["FRAME_DT", "SUBSTEPS", "FourierWind", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging",
-           "cloth_strip_pinned", "cloth_wind", "soft_cube"]
+           "cloth_strip_pinned", "cloth_wind", "soft_cube", "soft_cubes_batched"]

@@ -185,3 +185,28 @@ def test_cloth_batched_scene_structure():
     tri = model.tri_indices.numpy()
     for a, c in ((0, 1), (1, 2), (0, 2)):
         assert (colors[tri[:, a]] != colors[tri[:, c]]).all()
+
+
+def test_soft_cubes_batched_scene_structure():
+    from newton_amd import scenes
+
+    model, state, info = scenes.soft_cubes_batched("cpu", worlds=6)
+    cells = info["cells"]
+    assert model.particle_count == int(((cells + 1) ** 3).sum())
+    assert model.tet_count == int((5 * cells ** 3).sum())
+    ts = info["world_tet_start"]
+    mat = model.tet_materials.numpy()
+    for w in range(6):
+        mu, lam = mat[ts[w]:ts[w + 1], 0], mat[ts[w]:ts[w + 1], 1]
+        assert np.unique(mu).size == 1 and np.unique(lam).size == 1
+        nu = lam[0] / (2.0 * (lam[0] + mu[0]))  # Poisson ratio from the Lame parameters
+        e = mu[0] * 2.0 * (1.0 + nu)
+        assert 0.3 - 1e-5 <= nu <= 0.45 + 1e-5 and 1e4 * (1 - 1e-5) <= e <= 1e5 * (1 + 1e-5)
+    # surface triangles carry zero material: the solve skips them
+    assert model.tri_count > 0 and float(np.abs(model.tri_materials.numpy()[:, :2]).max()) == 0.0
+    # a valid coloring: no tet joins two particles of one color
+    colors = model.particle_colors.numpy()
+    tet = model.tet_indices.numpy()
+    for a in range(4):
+        for b in range(a + 1, 4):
+            assert (colors[tet[:, a]] != colors[tet[:, b]]).all()
