@@ -150,6 +150,20 @@ int32_t vbd_rebuild(VbdContext* ctx);
 /* Layout statistics: out4 = {fast batches, generic-path vertices, packed element items, colors}. */
 int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
 
+/* Autotuner of the color-pass work-group shape: builds the layout for 256-lane
+ * work-groups (batches of <= 21 vertices / 256 element items) and for 128-lane
+ * ones (<= 10 vertices / 128 items), times three force-assembly passes over
+ * every color with each (q is only read; call after vbd_contact_prepare and
+ * vbd_forward), and keeps the faster.  block_out receives 256 or 128, ms_out[2]
+ * the two per-pass-set times.  Results do not depend on the shape.  No
+ * counterpart in the reference; synchronises and allocates (not for capture). */
+int32_t vbd_autotune(VbdContext* ctx, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out);
+
+/* Selects the work-group shape (256 or 128 lanes) directly, rebuilding the
+ * layout; the contact ranges of the current substep are carried over.  Not for
+ * capture. */
+int32_t vbd_set_block_shape(VbdContext* ctx, void* stream, int32_t block);
+
 /* Full substep: contact prepare, forward, iterations x colors, finalize.
  * q_in is overwritten with the iterate (as the reference does); q_out/qd_out
  * receive the result.  contacts may be NULL (no soft contacts). */

@@ -57,7 +57,6 @@ int32_t fail(int32_t code, const std::string& msg) {
 
 constexpr int kBlock = 256;     // threads per color-pass workgroup (4 wavefronts)
 constexpr int kComp = 12;       // f (3) + H (9) per element contribution
-constexpr int kMaxV = kBlock / kComp;  // vertices per batch: one lane per (vertex, component)
 
 // ---------------------------------------------------------------------------
 // small vector algebra; each helper evaluates in the reference's order
@@ -628,7 +627,7 @@ struct KParams {
     // pass layout (vbd_layout.h)
     const int* batch_blk;
     const int* bat_cinfo;  // per batch {first packed contact, contact count} (k_batch_contacts)
-    const int* bat_coff;   // per batch, kMaxV + 1 contact offsets relative to the first
+    const int* bat_coff;   // per batch, max_v + 1 contact offsets relative to the first
     const int* vert_rec;
     const int* tri_rec;
     const int* edge_rec;
@@ -658,7 +657,6 @@ struct KParams {
 // packed contact record: 4 x float4
 //   {particle, local slot, ke, kd} {mu, margin, radius, bx} {by, bz, vx, vy} {vz, nx, ny, nz}
 constexpr int kContactWords = 16;
-constexpr int kRowsMax = 512;  // LDS rows of one batch: static items (<= 256) + contacts
 
 __device__ __forceinline__ bool particle_active(const KParams& P, int v) {
     return (P.flags[v] & 1) && P.mass[v] != 0.0f;
@@ -745,32 +743,37 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
 //   b  lane (vertex s, component c) sums component c of s's rows serially, in canonical order,
 //      starting from the inertia term; contacts are summed from zero and added afterwards
 //   c  the vertex's lane solves the 3x3 block and writes displacement and position
-constexpr int kBlkWords = vbd_layout::kBlockWords;
 constexpr int kBatchHdr = vbd_layout::kBatchWords;
 
-__global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
-                                                       float dt, int mode, float* __restrict__ g_out) {
-    __shared__ int s_row[kMaxV];
-    __shared__ int s_nrow[kMaxV];
-    __shared__ int s_cb[kMaxV + 1];
-    __shared__ __align__(16) float s_rows[kRowsMax * kComp];
-    __shared__ float s_acc[kMaxV * kComp];
+// BLOCK lanes per work-group: 256 (batches of <= 21 vertices / 256 items) or
+// 128 (<= 10 vertices / 128 items); the autotuner picks one per model.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
+                                                      float dt, int mode, float* __restrict__ g_out) {
+    constexpr int MAXV = BLOCK / kComp;
+    constexpr int ROWS = 2 * BLOCK;  // static items (<= BLOCK) + contacts evaluated in parallel
+    constexpr int BLK = kBatchHdr + 4 * MAXV;
+    __shared__ int s_row[MAXV];
+    __shared__ int s_nrow[MAXV];
+    __shared__ int s_cb[MAXV + 1];
+    __shared__ __align__(16) float s_rows[ROWS * kComp];
+    __shared__ float s_acc[MAXV * kComp];
 
     const int t = threadIdx.x;
     const int b = batch_base + xcd_remap(blockIdx.x, nb);
     // ---- round trip 1
-    const int* blk = P.batch_blk + (size_t)b * kBlkWords;
+    const int* blk = P.batch_blk + (size_t)b * BLK;
     const int4 b0 = reinterpret_cast<const int4*>(blk)[0];
     const int4 b1 = reinterpret_cast<const int4*>(blk)[1];
     const int2 ci = reinterpret_cast<const int2*>(P.bat_cinfo)[b];
     int4 vr = make_int4(0, 0, 0, 0);
-    if (t < kMaxV) vr = reinterpret_cast<const int4*>(blk + kBatchHdr)[t];
+    if (t < MAXV) vr = reinterpret_cast<const int4*>(blk + kBatchHdr)[t];
     int cbt = 0;
-    if (t <= kMaxV) cbt = P.bat_coff[(size_t)b * (kMaxV + 1) + t];
+    if (t <= MAXV) cbt = P.bat_coff[(size_t)b * (MAXV + 1) + t];
     const int nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
     const int nstatic = ntri + nedge + ntet;
     const int cbeg = ci.x, ncont = ci.y;
-    const bool par_contacts = nstatic + ncont <= kRowsMax;
+    const bool par_contacts = nstatic + ncont <= ROWS;
     const float dtr = 1.0f / (dt * dt);
 
     // ---- round trip 2: item record of this lane (items 0..255), own-vertex state
@@ -860,9 +863,9 @@ __global__ __launch_bounds__(kBlock) void k_solve_fast(KParams P, float* __restr
         }
         store_row(s_rows + row * kComp, fe, he);
     }
-    // contacts beyond the first kBlock items (only when nstatic + ncont > kBlock)
+    // contacts beyond the first BLOCK items (only when nstatic + ncont > BLOCK)
     const int nitems = nstatic + (par_contacts ? ncont : 0);
-    for (int it = t + kBlock; it < nitems; it += kBlock) {
+    for (int it = t + BLOCK; it < nitems; it += BLOCK) {
         V3 fe;
         M3 he;
         eval_packed_contact(P, q, cbeg + it - nstatic, dt, fe, he);
@@ -1166,16 +1169,16 @@ __global__ void k_contact_pack(int n, const int* __restrict__ color_particles, c
 }
 
 // Contact range of every batch from the color-position offsets: the first packed
-// contact and count (cinfo) and the kMaxV + 1 vertex offsets relative to the first
+// contact and count (cinfo) and the max_v + 1 vertex offsets relative to the first
 // (coff), so the color pass reads them indexed by batch in its first round trip.
-__global__ void k_batch_contacts(int nbatch, const int* __restrict__ blk, const int* __restrict__ pos_coff,
-                                 int* __restrict__ cinfo, int* __restrict__ coff) {
+__global__ void k_batch_contacts(int nbatch, const int* __restrict__ blk, int blk_words, int max_v,
+                                 const int* __restrict__ pos_coff, int* __restrict__ cinfo, int* __restrict__ coff) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbatch) return;
-    const int* h = blk + (size_t)b * vbd_layout::kBlockWords;
+    const int* h = blk + (size_t)b * blk_words;
     const int vb = h[0], nv = h[1];
     const int c0 = pos_coff[vb];
-    for (int k = 0; k <= nv; ++k) coff[(size_t)b * (kMaxV + 1) + k] = pos_coff[vb + k] - c0;
+    for (int k = 0; k <= nv; ++k) coff[(size_t)b * (max_v + 1) + k] = pos_coff[vb + k] - c0;
     cinfo[2 * b] = c0;
     cinfo[2 * b + 1] = pos_coff[vb + nv] - c0;
 }
@@ -1405,7 +1408,10 @@ struct VbdContext {
     int* lay_tet_meta = nullptr;
     int* lay_slow = nullptr;
     int* bat_cinfo = nullptr;  // [B, 2] per-substep contact range of each batch
-    int* bat_coff = nullptr;   // [B, kMaxV + 1] per-substep contact offsets within the batch
+    int* bat_coff = nullptr;   // [B, max_v + 1] per-substep contact offsets within the batch
+    int block = 256;           // color-pass work-group shape (256 or 128 lanes)
+    int max_v = vbd_layout::kLayoutMaxV;
+    int blk_words = vbd_layout::block_words(vbd_layout::kLayoutMaxV);
     int64_t fast_items = 0;
     VbdContactsDesc contacts{};
     bool have_contacts = false;
@@ -1496,7 +1502,7 @@ int32_t build_layout(VbdContext* c) {
         }
     }
     vbd_layout::PassLayout L;
-    err = vbd_layout::build_pass_layout(hm, L);
+    err = vbd_layout::build_pass_layout(hm, L, c->block, c->block / kComp);
     if (!err.empty()) return fail(VBD_ERR_ARG, "vbd_create: " + err);
     free_layout(c);
     for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_blk), std::make_pair(&c->lay_vert, &L.vert_rec),
@@ -1509,14 +1515,16 @@ int32_t build_layout(VbdContext* c) {
     c->fast_color_off = L.fast_color_off;
     c->slow_color_off = L.slow_color_off;
     c->fast_items = L.fast_items;
+    c->max_v = L.max_v;
+    c->blk_words = L.blk_words;
     {
         // contact ranges per batch: zero (no contacts) until vbd_contact_prepare fills them
         const size_t nbat = (size_t)std::max(L.fast_color_off.back(), 1);
         hipError_t e;
         if ((e = hipMalloc(&c->bat_cinfo, sizeof(int) * 2 * nbat)) != hipSuccess ||
-            (e = hipMalloc(&c->bat_coff, sizeof(int) * (kMaxV + 1) * nbat)) != hipSuccess ||
+            (e = hipMalloc(&c->bat_coff, sizeof(int) * (L.max_v + 1) * nbat)) != hipSuccess ||
             (e = hipMemset(c->bat_cinfo, 0, sizeof(int) * 2 * nbat)) != hipSuccess ||
-            (e = hipMemset(c->bat_coff, 0, sizeof(int) * (kMaxV + 1) * nbat)) != hipSuccess ||
+            (e = hipMemset(c->bat_coff, 0, sizeof(int) * (L.max_v + 1) * nbat)) != hipSuccess ||
             (e = hipDeviceSynchronize()) != hipSuccess)
             return fail(VBD_ERR_HIP, std::string("vbd_create: batch contact ranges: ") + hipGetErrorString(e));
     }
@@ -1703,8 +1711,8 @@ int32_t vbd_layout_stats(VbdContext* c, int64_t* out4) {
 static int32_t launch_batch_contacts(VbdContext* c, hipStream_t s) {
     const int nbat = c->fast_color_off.back();
     if (nbat <= 0) return VBD_OK;
-    hipLaunchKernelGGL(k_batch_contacts, dim3(blocks_for(nbat, 256)), dim3(256), 0, s, nbat, c->lay_batch, c->pos_coff,
-                       c->bat_cinfo, c->bat_coff);
+    hipLaunchKernelGGL(k_batch_contacts, dim3(blocks_for(nbat, 256)), dim3(256), 0, s, nbat, c->lay_batch,
+                       c->blk_words, c->max_v, c->pos_coff, c->bat_cinfo, c->bat_coff);
     VBD_LAUNCH_CHECK("k_batch_contacts");
     return VBD_OK;
 }
@@ -1792,11 +1800,21 @@ int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, cons
 }
 
 namespace {
+int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, float dt, int mode, float* g) {
+    if (c->block == 128) {
+        hipLaunchKernelGGL(k_solve_fast<128>, dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
+    } else {
+        hipLaunchKernelGGL(k_solve_fast<256>, dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
+    }
+    VBD_LAUNCH_CHECK("k_solve_fast");
+    return VBD_OK;
+}
+
 int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt, int mode, float* g) {
     int b0 = c->fast_color_off[color], b1 = c->fast_color_off[color + 1];
     if (b1 > b0) {
-        hipLaunchKernelGGL(k_solve_fast, dim3(b1 - b0), dim3(kBlock), 0, s, c->kp, q, b0, b1 - b0, dt, mode, g);
-        VBD_LAUNCH_CHECK("k_solve_fast");
+        int32_t rc = launch_fast(c, s, b0, b1 - b0, q, dt, mode, g);
+        if (rc) return rc;
     }
     int s0 = c->slow_color_off[color], s1 = c->slow_color_off[color + 1];
     if (s1 > s0) {
@@ -1855,14 +1873,73 @@ int32_t vbd_step(VbdContext* c, void* stream, float* q_in, const float* qd_in, c
     return vbd_finalize(c, stream, q_in, q_out, qd_out, dt);
 }
 
+int32_t vbd_set_block_shape(VbdContext* c, void* stream, int32_t block) {
+    if (!c || (block != 256 && block != 128)) return fail(VBD_ERR_ARG, "vbd_set_block_shape: block must be 256 or 128");
+    if (c->block == block) return VBD_OK;
+    c->block = block;
+    int32_t rc = build_layout(c);
+    if (rc) return rc;
+    rc = launch_batch_contacts(c, (hipStream_t)stream);
+    if (rc) return rc;
+    VBD_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return VBD_OK;
+}
+
+int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out) {
+    if (!c || !q || !(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_autotune: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int shapes[2] = {256, 128};
+    float ms[2] = {0.0f, 0.0f};
+    hipEvent_t e0, e1;
+    VBD_HIP_TRY(hipEventCreate(&e0));
+    VBD_HIP_TRY(hipEventCreate(&e1));
+    int32_t rc = VBD_OK;
+    for (int k = 0; k < 2 && rc == VBD_OK; ++k) {
+        c->block = shapes[k];
+        rc = build_layout(c);
+        if (rc) break;
+        rc = launch_batch_contacts(c, s);
+        if (rc) break;
+        // warm, then time three force-assembly passes over every color (q is only read)
+        for (int rep = 0; rep < 4 && rc == VBD_OK; ++rep) {
+            if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = fail(VBD_ERR_HIP, "vbd_autotune: event");
+            for (int color = 0; color < c->model.color_count && rc == VBD_OK; ++color)
+                rc = launch_color(c, s, color, (float*)q, dt, 1, c->g_tmp);
+        }
+        if (rc) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms[k], e0, e1) != hipSuccess)
+            rc = fail(VBD_ERR_HIP, "vbd_autotune: timing");
+        ms[k] /= 3.0f;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    const int best = ms[1] < ms[0] ? 128 : 256;
+    if (c->block != best) {
+        c->block = best;
+        rc = build_layout(c);
+        if (rc) return rc;
+        rc = launch_batch_contacts(c, s);
+        if (rc) return rc;
+        VBD_HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (block_out) *block_out = best;
+    if (ms_out) {
+        ms_out[0] = ms[0];
+        ms_out[1] = ms[1];
+    }
+    return VBD_OK;
+}
+
 int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, float* g_out, double* norm_sq_out) {
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_residual: null argument");
     hipStream_t s = (hipStream_t)stream;
     float* g = g_out ? g_out : c->g_tmp;
     int nb = c->fast_color_off.back();
     if (nb > 0) {
-        hipLaunchKernelGGL(k_solve_fast, dim3(nb), dim3(kBlock), 0, s, c->kp, (float*)q, 0, nb, dt, 1, g);
-        VBD_LAUNCH_CHECK("k_solve_fast(residual)");
+        int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g);
+        if (rc) return rc;
     }
     int ns = c->slow_color_off.back();
     if (ns > 0) {

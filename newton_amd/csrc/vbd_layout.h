@@ -1,8 +1,9 @@
 // Host-side "pass layout" for the color-pass kernel (built once per model in vbd_create).
 //
 // For every color, the color's vertices (in color-group order) are cut into
-// batches: runs of at most kLayoutMaxV consecutive vertices whose element
-// counts sum to at most kLayoutCap.  A batch is one workgroup.  Its element
+// batches: runs of at most max_v consecutive vertices whose element counts sum
+// to at most cap (the work-group shape: 256 lanes -> cap 256, max_v 21; 128
+// lanes -> cap 128, max_v 10).  A batch is one workgroup.  Its element
 // contributions are stored as packed records in evaluation order -- all
 // triangle items of the batch, then all bending-edge items, then all tet
 // items -- each carrying the canonical row (the position of that contribution
@@ -13,7 +14,7 @@
 // mu <= 0 and lambda <= 0, edges with ke <= 0 (particle_vbd_kernels.py:2636,
 // :2658, :2679), and boundary hinges (o0 or o1 == -1), whose contribution is
 // an exact zero (:682-683).  Inactive particles (flag bit 0 clear or mass 0)
-// get no items.  A vertex with more than kLayoutCap items goes to the "slow"
+// get no items.  A vertex with more than cap items goes to the "slow"
 // list, handled by the generic kernel that walks the CSR in chunks.
 #pragma once
 
@@ -23,8 +24,8 @@
 
 namespace vbd_layout {
 
-constexpr int kLayoutCap = 256;  // static items (rows) per batch
-constexpr int kLayoutMaxV = 21;  // vertices per batch (12 component lanes each in 256)
+constexpr int kLayoutCap = 256;  // static items (rows) per batch of the 256-lane shape
+constexpr int kLayoutMaxV = 21;  // vertices per batch of the 256-lane shape (12 component lanes each)
 
 // record widths in 32-bit words
 constexpr int kTriWords = 12;   // i0 i1 i2 meta | D00 D01 D10 D11 | area mu lambda kd
@@ -32,9 +33,9 @@ constexpr int kEdgeWords = 8;   // o0 o1 v0 v1 | rest_angle rest_length ke kd   
 constexpr int kTetWords = 16;   // i0 i1 i2 i3 | Dm^-1 (9) | mu lambda kd         (+ tet_meta)
 constexpr int kBatchWords = 8;  // vbegin nv tri_begin ntri edge_begin nedge tet_begin ntet
 constexpr int kVertWords = 4;   // v row_start nrows (local | active << 16)
-// batch block: header (kBatchWords) then kLayoutMaxV vertex records (unused ones zero),
+// batch block: header (kBatchWords) then max_v vertex records (unused ones zero),
 // fixed size so a workgroup reads its header and vertex records in one round trip
-constexpr int kBlockWords = kBatchWords + kLayoutMaxV * kVertWords;  // 92 words = 368 B (16-B multiple)
+inline int block_words(int max_v) { return kBatchWords + max_v * kVertWords; }  // a multiple of 4 words
 
 inline int32_t meta(int row, int slot) { return (int32_t)(row | (slot << 16)); }
 
@@ -53,7 +54,8 @@ struct HostModel {
 struct PassLayout {
     std::vector<int32_t> fast_color_off;  // [C+1] into batch records
     std::vector<int32_t> slow_color_off;  // [C+1] into slow_pos
-    std::vector<int32_t> batch_blk;       // [B * kBlockWords]: header + vertex records
+    std::vector<int32_t> batch_blk;       // [B * block_words(max_v)]: header + vertex records
+    int cap = kLayoutCap, max_v = kLayoutMaxV, blk_words = block_words(kLayoutMaxV);
     std::vector<int32_t> vert_rec;        // [N * kVertWords], indexed by color position
     std::vector<int32_t> tri_rec, edge_rec, edge_meta, tet_rec, tet_meta;
     std::vector<int32_t> slow_pos;        // color positions of oversized vertices
@@ -67,7 +69,8 @@ inline int32_t fbits(float f) {
 }
 
 // Builds the layout; returns an empty string or an error message.
-inline std::string build_pass_layout(const HostModel& m, PassLayout& L) {
+inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap = kLayoutCap,
+                                     int max_v = kLayoutMaxV) {
     const int N = m.N;
     const int C = (int)m.color_off.size() - 1;
     std::vector<char> active((size_t)N), tri_ok((size_t)m.T), edge_ok((size_t)m.E), tet_ok((size_t)m.K);
@@ -93,29 +96,32 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L) {
     }
 
     L = PassLayout();
+    L.cap = cap;
+    L.max_v = max_v;
+    L.blk_words = block_words(max_v);
     L.vert_rec.assign((size_t)N * kVertWords, 0);
     L.fast_color_off.assign((size_t)C + 1, 0);
     L.slow_color_off.assign((size_t)C + 1, 0);
     std::vector<int> rows;
     for (int c = 0; c < C; ++c) {
-        L.fast_color_off[c] = (int32_t)(L.batch_blk.size() / kBlockWords);
+        L.fast_color_off[c] = (int32_t)(L.batch_blk.size() / L.blk_words);
         L.slow_color_off[c] = (int32_t)L.slow_pos.size();
         int k = m.color_off[c];
         const int kend = m.color_off[c + 1];
         while (k < kend) {
             int v0 = m.color_particles[k];
             int items0 = ntri[v0] + nedge[v0] + ntet[v0];
-            if (items0 > kLayoutCap) {  // oversized vertex: generic kernel
+            if (items0 > cap) {  // oversized vertex: generic kernel
                 L.slow_pos.push_back(k);
                 ++k;
                 continue;
             }
             // greedy run of vertices that fit one workgroup
             int vb = k, nv = 0, items = 0;
-            while (k < kend && nv < kLayoutMaxV) {
+            while (k < kend && nv < max_v) {
                 int v = m.color_particles[k];
                 int it = ntri[v] + nedge[v] + ntet[v];
-                if (it > kLayoutCap || items + it > kLayoutCap) break;
+                if (it > cap || items + it > cap) break;
                 items += it;
                 ++nv;
                 ++k;
@@ -202,13 +208,13 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L) {
             rec[7] = (int32_t)L.tet_meta.size() - rec[6];
             L.fast_items += rec[3] + rec[5] + rec[7];
             const size_t at = L.batch_blk.size();
-            L.batch_blk.resize(at + kBlockWords, 0);
+            L.batch_blk.resize(at + L.blk_words, 0);
             std::memcpy(&L.batch_blk[at], rec, sizeof(rec));
             std::memcpy(&L.batch_blk[at + kBatchWords], &L.vert_rec[(size_t)vb * kVertWords],
                         sizeof(int32_t) * kVertWords * (size_t)nv);
         }
     }
-    L.fast_color_off[C] = (int32_t)(L.batch_blk.size() / kBlockWords);
+    L.fast_color_off[C] = (int32_t)(L.batch_blk.size() / L.blk_words);
     L.slow_color_off[C] = (int32_t)L.slow_pos.size();
     return std::string();
 }

@@ -375,6 +375,30 @@ class SolverVBD:
             _native.check(_native.hip_lib().vbd_rebuild(self._ctx), "vbd_rebuild")
         self._material_key = None
 
+    def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
+        """Pick the color-pass work-group shape (256 or 128 lanes) by timing both on this model.
+
+        Runs on a copy of ``state_in`` (prepared with contacts and the forward
+        step); ``state_in`` is left untouched.  Results never depend on the shape.
+        """
+        self._ensure_context(contacts)
+        trial = State()
+        trial.particle_q = state_in.particle_q.clone()
+        trial.particle_qd = state_in.particle_qd.clone()
+        trial.particle_f = state_in.particle_f.clone()
+        self.begin_step(trial, contacts, dt)
+        block = ctypes.c_int32(0)
+        ms = (ctypes.c_float * 2)()
+        _native.check(_native.hip_lib().vbd_autotune(self._ctx, current_stream_handle(self.model.device),
+                                                     _native.ptr(trial.particle_q), float(dt), ctypes.byref(block),
+                                                     ms), "vbd_autotune")
+        return {"block": int(block.value), "ms_256": float(ms[0]), "ms_128": float(ms[1])}
+
+    def set_block_shape(self, block: int) -> None:
+        """Use 256- or 128-lane color-pass work-groups (what :meth:`autotune` chooses between)."""
+        _native.check(_native.hip_lib().vbd_set_block_shape(self._ctx, current_stream_handle(self.model.device),
+                                                            int(block)), "vbd_set_block_shape")
+
     def layout_stats(self) -> dict:
         """Color-pass layout: packed batches, vertices on the generic path, packed element items, colors."""
         out = (ctypes.c_int64 * 4)()

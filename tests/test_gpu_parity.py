@@ -310,3 +310,38 @@ def test_drift_600_frames(cuda_device):
     print(f"error vs FP64 oracle: GPU max {gpu_vs_f64.max():.3e}, FP32 oracle max {f32_vs_f64.max():.3e}")
     assert np.all(np.isfinite(rec_gpu))
     assert per_frame.max() <= 1e-5
+
+
+@pytest.mark.parametrize("scene", ["strip", "cube", "hanging"])
+def test_128_lane_block_shape_bitexact(cuda_device, scene):
+    """The 128-lane work-group shape (batches of <= 10 vertices) gives the oracle's bits too."""
+    model, state, info = _scene(scene, cuda_device)
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 6, block_shape=128)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 6)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+
+
+def test_autotune_keeps_results(cuda_device):
+    """autotune() picks a shape and leaves the state alone; the trajectory stays bit-exact."""
+    import torch
+
+    from newton_amd import CollisionPipeline
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _scene("hanging", cuda_device)
+    q0, qd0 = initial(state)
+    solver = SolverVBD(model, iterations=info["iterations"])
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    state.clear_forces()
+    pipe.collide(state, contacts)
+    before = state.particle_q.clone()
+    choice = solver.autotune(state, contacts, info["dt"])
+    torch.cuda.synchronize()
+    assert choice["block"] in (128, 256) and choice["ms_256"] > 0 and choice["ms_128"] > 0
+    assert torch.equal(state.particle_q, before)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4, solver=solver)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 4)
+    np.testing.assert_array_equal(q_gpu, q_ref)
