@@ -9,11 +9,13 @@ The same line carries
 * ``roofline``: the color-pass kernel's algorithmic bytes / its live launch time
   (HIP events on its stream) against 8 TB/s, plus HBM traffic from two child
   ``rocprofv3 --pmc`` runs (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy;
-* ``batched``: the 4096-world cloth (~7.3M vertices, 5 iterations) in one
-  graph-captured frame per step, with its own color-pass roofline;
+* ``batched`` / ``batched_tets``: the 4096-world cloth (~7.3M vertices, 5
+  iterations) and the 512 tetrahedral cubes (~0.59M vertices, 8 iterations),
+  each one graph-captured frame per step with its own color-pass roofline;
 * ``residual``: the 128x128 wind cloth driven to a relative force residual,
-  plain VBD against the learned warm start (trained on the GPU in the same run
-  from plain-VBD data), iterations and vertex-steps/s of each;
+  plain VBD (inertial guess), the previous substep's correction carried over
+  ("reuse", no network) and the learned warm start (trained on the GPU in the
+  same run from plain-VBD data): iterations and vertex-steps/s of each;
 * ``cpu_baseline``: the FP32 CPU oracle (OpenMP) on a bounded sample of the
   headline workload.
 """
@@ -58,7 +60,8 @@ def parse():
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
-    ap.add_argument("--pmc-workload", choices=("drop", "batched"), default="drop")
+    ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes"), default="drop")
+    ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
     return ap.parse_args()
 
 
@@ -73,6 +76,8 @@ def _scene(name, args, dev):
 
     if name == "drop":
         return scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
+    if name == "cubes":
+        return scenes.soft_cubes_batched(dev, worlds=args.cube_worlds)
     return scenes.cloth_batched(dev, worlds=args.worlds)
 
 
@@ -224,7 +229,8 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
             d = os.path.join(tmp, counter)
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
                    str(ROOT / "bench.py"), "--pmc-mode", "--pmc-workload", workload, "--verts", str(args.verts),
-                   "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--roofline-reps", "5"]
+                   "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--cube-worlds",
+                   str(args.cube_worlds), "--roofline-reps", "5"]
             try:
                 proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
                                       env={**os.environ, "TMPDIR": "/tmp"})
@@ -250,27 +256,34 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
 
 
 # ----------------------------------------------------------------------------- legs
-def batched_leg(args):
-    """4096 uneven cloth worlds in one launch per color pass; graph-captured frames; color-pass roofline."""
-    from newton_amd import CollisionPipeline, scenes
+def batched_leg(args, name="batched"):
+    """Many worlds in one launch per color pass (4096 cloths or 512 tet cubes); graph-captured frames;
+    color-pass roofline; the work-group shape picked by the autotuner."""
+    from newton_amd import CollisionPipeline
     from newton_amd.solvers import SolverVBD
 
     dev = "cuda:0"
     t0 = time.perf_counter()
-    model, state, info = scenes.cloth_batched(dev, worlds=args.worlds)
+    model, state, info = _scene(name, args, dev)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
     solver = SolverVBD(model, iterations=info["iterations"])
     t_solver = time.perf_counter() - t0
     pipe = CollisionPipeline(model, soft_contact_margin=0.01)
     contacts = pipe.contacts()
+    state.clear_forces()
+    pipe.collide(state, contacts)
+    tune = solver.autotune(state, contacts, info["dt"])
     gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, args.batched_steps, 1)
     n = model.particle_count
     roof = color_roofline(model, last, info, solver, pipe, contacts, max(3, args.roofline_reps // 4))
     return {
         "workload": info["workload"],
         "vertices": n,
-        "worlds": args.worlds,
+        "worlds": model.world_count,
+        "triangles": model.tri_count,
+        "tets": model.tet_count,
+        "autotune": tune,
         "colors": len(model.particle_color_groups),
         "iterations": info["iterations"],
         "substeps_per_step": info["substeps"],
@@ -367,7 +380,7 @@ def residual_leg(args):
     fwd_ms = e0.elapsed_time(e1) / 50
     fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
 
-    res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in ("plain", "learned")}
+    res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in ("plain", "reuse", "learned")}
     accepted = 0
     r_ws_rel = []
     for frame in range(args.residual_frames):
@@ -375,16 +388,16 @@ def residual_leg(args):
         for _ in range(sub):
             q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), ws.corr.clone()
             f_ext = wind.force(s0.particle_q, mass)
-            for name in ("plain", "learned"):
+            for name in ("plain", "reuse", "learned"):
                 s0.particle_q.copy_(q0)
                 s0.particle_qd.copy_(qd0)
                 s0.particle_f.copy_(f_ext)
                 ws.corr.copy_(corr0)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
+                start = {"plain": None, "reuse": ws.reuse, "learned": ws}[name]
                 r = solver.step_to_residual(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                            max_iterations=args.max_iterations,
-                                            warm_start=ws if name == "learned" else None)
+                                            max_iterations=args.max_iterations, warm_start=start)
                 torch.cuda.synchronize()
                 res[name]["s"] += time.perf_counter() - t0
                 res[name]["iters"].append(r["iterations"])
@@ -398,12 +411,13 @@ def residual_leg(args):
     substeps = args.residual_frames * sub
     out = {"workload": info["workload"], "vertices": n, "rel_tol": args.rel_tol, "max_iterations": args.max_iterations,
            "substeps": substeps, "check_every": 1}
-    for name in ("plain", "learned"):
+    for name in ("plain", "reuse", "learned"):
         d = res[name]
         out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
                      "reached_tol": f"{d['reached']}/{substeps}",
                      "vertex_steps_per_s": round(n * substeps / d["s"], 1)}
     out["speedup_vertex_steps"] = round(res["plain"]["s"] / res["learned"]["s"], 3)
+    out["speedup_vs_reuse"] = round(res["reuse"]["s"] / res["learned"]["s"], 3)
     out["iteration_ratio"] = round(float(np.mean(res["plain"]["iters"]) /
                                          max(float(np.mean(res["learned"]["iters"])), 1e-9)), 3)
     out["warm_start_accepted"] = f"{accepted}/{substeps}"
@@ -488,9 +502,15 @@ def main():
 
     batched = None
     if args.worlds > 0:
-        _progress(f"batched leg: {args.worlds} worlds")
-        batched = batched_leg(args)
+        _progress(f"batched leg: {args.worlds} cloth worlds")
+        batched = batched_leg(args, "batched")
         _progress(f"batched: {batched['vertex_steps_per_s']:.4g} vertex-steps/s")
+    torch.cuda.empty_cache()
+    cubes = None
+    if args.cube_worlds > 0:
+        _progress(f"batched leg: {args.cube_worlds} tet cubes")
+        cubes = batched_leg(args, "cubes")
+        _progress(f"cubes: {cubes['vertex_steps_per_s']:.4g} vertex-steps/s")
     torch.cuda.empty_cache()
     residual = None
     if args.residual_frames > 0:
@@ -530,6 +550,7 @@ def main():
         },
         "roofline": roofline,
         "batched": batched,
+        "batched_tets": cubes,
         "residual": residual,
         "cpu_baseline": cpu,
     }

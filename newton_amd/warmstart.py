@@ -3,7 +3,9 @@
 Newton's SolverVBD starts every substep from the inertial guess ``y``
 (``forward_step``, ``particle_vbd_kernels.py:1350-1375``).  Here a small
 per-particle MLP predicts the correction ``x* - y`` of the converged iterate
-from 1-ring features; :meth:`SolverVBD.step_to_residual` evaluates the force
+from 1-ring features, as a change of the previous substep's correction
+``c_prev`` (guess = ``y + c_prev + MLP``, so an untrained network starts from
+carrying ``c_prev`` over); :meth:`SolverVBD.step_to_residual` evaluates the force
 residual of the predicted iterate and keeps it only if that residual is lower
 than the inertial guess's, so plain VBD iterations always follow and the
 result never gets worse than starting from ``y``.
@@ -220,13 +222,20 @@ class LearnedWarmStart:
         return self.feat
 
     def __call__(self, solver, state_in, dt) -> torch.Tensor:
+        """Candidate iterate y + c_prev + scale * MLP(features): the network predicts how this
+        substep's correction differs from the previous substep's (zero = reuse it)."""
         feat = self.features(solver, state_in)
         self.mlp.forward(feat, self.pred)
-        return self._y + self.pred * self.scale_corr
+        return self._y + self.corr + self.pred * self.scale_corr
+
+    def reuse(self, solver, state_in, dt) -> torch.Tensor:
+        """The no-network baseline: y + c_prev (the previous substep's correction carried over)."""
+        return solver.scratch("inertia") + self.corr
 
     def target(self, solver, state_in) -> torch.Tensor:
-        """Normalised correction (x - y) / scale of the current iterate (call after iterating)."""
-        return (state_in.particle_q - solver.scratch("inertia")) / self.scale_corr
+        """Training target (x - y - c_prev) / scale of the current iterate (call after iterating,
+        before :meth:`observe`)."""
+        return (state_in.particle_q - solver.scratch("inertia") - self.corr) / self.scale_corr
 
     def observe(self, solver, state_in) -> None:
         """Remember this substep's correction x* - y for the next substep's features."""
