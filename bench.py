@@ -324,6 +324,10 @@ def residual_leg(args):
     ws = LearnedWarmStart(model, dt, seed=0)
     mass = model.particle_mass
     s0, s1 = state, model.state()
+    # Training data along the trajectory the deployed solver follows: each substep starts from the
+    # network-free warm start, iterates to the tolerance (that iterate continues the trajectory and
+    # its correction is the next substep's c_prev), then keeps iterating to train_iterations in total;
+    # the converged correction is the target.
     feats, targs = [], []
     t0 = time.perf_counter()
     for frame in range(args.train_frames):
@@ -334,8 +338,16 @@ def residual_leg(args):
             s0.particle_f.copy_(wind.force(s0.particle_q, mass))
             solver.begin_step(s0, None, dt)
             feats.append(ws.features(solver, s0).clone())
-            solver.iterate(s0, dt, args.train_iterations)
+            r0 = float(solver.residual(s0, dt).item())
+            guess = ws.reuse(solver, s0, dt)
+            trial = solver.set_iterate(s0, guess, commit=False)
+            if float(solver.residual(_trial(trial), dt).item()) < r0:
+                solver.set_iterate(s0, guess, commit=True)
+            its, _ = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0, args.max_iterations)
+            x_tol = s0.particle_q.clone()
+            solver.iterate(s0, dt, max(args.train_iterations - its, 0))
             targs.append(ws.target(solver, s0))
+            solver.set_iterate(s0, x_tol, commit=True)
             ws.observe(solver, s0)
             solver.end_step(s0, s1, dt)
             s0, s1 = s1, s0
@@ -403,7 +415,7 @@ def residual_leg(args):
                 res[name]["iters"].append(r["iterations"])
                 res[name]["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
                 if name == "learned":
-                    accepted += int(r["warm_start_accepted"])
+                    accepted += int(r["warm_start_index"] == 1)
                     if r["r_warm_start"] is not None and r["r0"] > 0:
                         r_ws_rel.append(r["r_warm_start"] / r["r0"])
             ws.observe(solver, s0)  # the learned run's iterate carries on
@@ -420,7 +432,7 @@ def residual_leg(args):
     out["speedup_vs_reuse"] = round(res["reuse"]["s"] / res["learned"]["s"], 3)
     out["iteration_ratio"] = round(float(np.mean(res["plain"]["iters"]) /
                                          max(float(np.mean(res["learned"]["iters"])), 1e-9)), 3)
-    out["warm_start_accepted"] = f"{accepted}/{substeps}"
+    out["learned_candidate_chosen"] = f"{accepted}/{substeps}"
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
     out["training"] = {"frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
@@ -433,6 +445,11 @@ def residual_leg(args):
                            "frac": round(fwd_tflops / BF16_PEAK_TFLOPS, 5), "rows": n,
                            "avg_launch_us": round(fwd_ms * 1e3, 2)}
     return out
+
+
+class _trial:  # noqa: N801  (positions-only state for a residual evaluation)
+    def __init__(self, q):
+        self.particle_q = q
 
 
 def cpu_baseline_leg(args, model, q_init, qd_init, dt):

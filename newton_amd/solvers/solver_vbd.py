@@ -330,28 +330,40 @@ class SolverVBD:
         The residual is normalised by its value at the inertial guess ``r0``;
         iterations stop once ``r <= rel_tol * r0`` (or after ``max_iterations``).
         ``warm_start(solver, state_in, dt)`` may return candidate positions
-        ([N,3] float32); they replace the inertial guess only if their residual
-        is strictly lower, otherwise the inertial guess is kept.
+        ([N,3] float32) or a list of them; the candidate with the lowest
+        residual replaces the inertial guess only if that residual is strictly
+        lower than ``r0`` (ties keep the earlier candidate), otherwise the
+        inertial guess is kept.  ``warm_start_accepted`` reports the index of the
+        chosen candidate (None: the inertial guess).
         """
         self.begin_step(state_in, contacts, dt)
         r0_dev = self.residual(state_in, dt)
-        accepted = False
+        chosen = None
         r_ws = None
+        r_candidates = []
         if warm_start is not None:
-            guess = warm_start(self, state_in, dt)
-            if guess is not None:
-                trial = self.set_iterate(state_in, guess, commit=False)
-                r_ws_dev = self.residual(_TrialState(trial), dt)
-                r0_sq, r_ws_sq = float(r0_dev.item()), float(r_ws_dev.item())
-                r_ws = math.sqrt(r_ws_sq)
-                if r_ws_sq < r0_sq:
-                    self.set_iterate(state_in, guess, commit=True)
-                    accepted = True
+            guesses = warm_start(self, state_in, dt)
+            if guesses is not None and not isinstance(guesses, (list, tuple)):
+                guesses = [guesses]
+            if guesses:
+                trial = None
+                norms = []
+                for g in guesses:
+                    trial = self.set_iterate(state_in, g, commit=False, out=trial)
+                    norms.append(self.residual(_TrialState(trial), dt))
+                r0_sq = float(r0_dev.item())
+                r_candidates = [math.sqrt(float(n.item())) for n in norms]
+                best = min(range(len(guesses)), key=lambda k: (r_candidates[k], k))
+                r_ws = r_candidates[best]
+                if r_ws * r_ws < r0_sq:
+                    self.set_iterate(state_in, guesses[best], commit=True)
+                    chosen = best
         r0_sq = float(r0_dev.item())
         its, r_sq = self.iterate_to_residual(state_in, dt, (rel_tol * rel_tol) * r0_sq, max_iterations, check_every)
         self.end_step(state_in, state_out, dt)
-        return {"iterations": its, "r0": math.sqrt(r0_sq), "r": math.sqrt(r_sq), "warm_start_accepted": accepted,
-                "r_warm_start": r_ws}
+        return {"iterations": its, "r0": math.sqrt(r0_sq), "r": math.sqrt(r_sq),
+                "warm_start_accepted": chosen is not None, "warm_start_index": chosen, "r_warm_start": r_ws,
+                "r_candidates": r_candidates}
 
     def scratch(self, which: str) -> torch.Tensor:
         """Copy of the current substep's ``inertia`` / ``q_prev`` / ``displacement`` array ([N, 3])."""

@@ -174,8 +174,10 @@ class LearnedWarmStart:
     correction the solver made; :meth:`reset` forgets it.
     """
 
-    def __init__(self, model: Model, dt: float, mlp: WarmStartMLP | None = None, seed: int = 0):
+    def __init__(self, model: Model, dt: float, mlp: WarmStartMLP | None = None, seed: int = 0,
+                 offer_reuse: bool = True):
         self.model = model
+        self.offer_reuse = offer_reuse
         self.device = model.device
         self.dt = float(dt)
         n = model.particle_count
@@ -221,12 +223,16 @@ class LearnedWarmStart:
         self._y = y
         return self.feat
 
-    def __call__(self, solver, state_in, dt) -> torch.Tensor:
-        """Candidate iterate y + c_prev + scale * MLP(features): the network predicts how this
-        substep's correction differs from the previous substep's (zero = reuse it)."""
+    def __call__(self, solver, state_in, dt):
+        """Candidate iterates for ``step_to_residual``: the learned one, y + c_prev + scale * MLP(features)
+        (the network predicts how this substep's correction differs from the previous substep's), and,
+        when ``offer_reuse`` is set, first the network-free y + c_prev; the residual picks among them."""
         feat = self.features(solver, state_in)
         self.mlp.forward(feat, self.pred)
-        return self._y + self.corr + self.pred * self.scale_corr
+        learned = self._y + self.corr + self.pred * self.scale_corr
+        if self.offer_reuse:
+            return [self._y + self.corr, learned]
+        return learned
 
     def reuse(self, solver, state_in, dt) -> torch.Tensor:
         """The no-network baseline: y + c_prev (the previous substep's correction carried over)."""
