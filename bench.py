@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--train-frames", type=int, default=40)
     ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-protocol", choices=("deployed", "converged"), default="deployed",
+                    help="trajectory of the training data: the deployed tolerance solve, or converged solves")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
@@ -324,10 +326,12 @@ def residual_leg(args):
     ws = LearnedWarmStart(model, dt, seed=0)
     mass = model.particle_mass
     s0, s1 = state, model.state()
-    # Training data along the trajectory the deployed solver follows: each substep starts from the
-    # network-free warm start, iterates to the tolerance (that iterate continues the trajectory and
-    # its correction is the next substep's c_prev), then keeps iterating to train_iterations in total;
-    # the converged correction is the target.
+    # Training data.  "deployed": along the trajectory the deployed solver follows -- each substep
+    # starts from the network-free warm start, iterates to the tolerance (that iterate continues the
+    # trajectory and its correction is the next substep's c_prev), then keeps iterating to
+    # train_iterations in total; the converged correction is the target.  "converged": the
+    # trajectory itself follows the train_iterations solves (c_prev is a converged correction).
+    deployed = args.train_protocol == "deployed"
     feats, targs = [], []
     t0 = time.perf_counter()
     for frame in range(args.train_frames):
@@ -343,11 +347,14 @@ def residual_leg(args):
             trial = solver.set_iterate(s0, guess, commit=False)
             if float(solver.residual(_trial(trial), dt).item()) < r0:
                 solver.set_iterate(s0, guess, commit=True)
-            its, _ = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0, args.max_iterations)
-            x_tol = s0.particle_q.clone()
+            its = 0
+            if deployed:
+                its, _ = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0, args.max_iterations)
+                x_tol = s0.particle_q.clone()
             solver.iterate(s0, dt, max(args.train_iterations - its, 0))
             targs.append(ws.target(solver, s0))
-            solver.set_iterate(s0, x_tol, commit=True)
+            if deployed:
+                solver.set_iterate(s0, x_tol, commit=True)
             ws.observe(solver, s0)
             solver.end_step(s0, s1, dt)
             s0, s1 = s1, s0
@@ -434,7 +441,7 @@ def residual_leg(args):
                                          max(float(np.mean(res["learned"]["iters"])), 1e-9)), 3)
     out["learned_candidate_chosen"] = f"{accepted}/{substeps}"
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
-    out["training"] = {"frames": args.train_frames, "substeps": args.train_frames * sub,
+    out["training"] = {"protocol": args.train_protocol, "frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
                        "batch": batch, "loss_first": None if first is None else round(first, 5),
                        "loss_last": None if last is None else round(last, 6),
