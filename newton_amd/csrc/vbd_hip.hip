@@ -879,7 +879,17 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
     if (cs < nv) {
         float acc = s_acc[cs * kComp + cc];
         const int q0 = s_row[cs], q1 = q0 + s_nrow[cs];
-        for (int r = q0; r < q1; ++r) acc = acc + s_rows[r * kComp + cc];
+        // the LDS reads of four rows are issued together; the additions stay serial, in row order
+        int r = q0;
+        for (; r + 4 <= q1; r += 4) {
+            const float a0 = s_rows[r * kComp + cc], a1 = s_rows[(r + 1) * kComp + cc];
+            const float a2 = s_rows[(r + 2) * kComp + cc], a3 = s_rows[(r + 3) * kComp + cc];
+            acc = acc + a0;
+            acc = acc + a1;
+            acc = acc + a2;
+            acc = acc + a3;
+        }
+        for (; r < q1; ++r) acc = acc + s_rows[r * kComp + cc];
         if (par_contacts) {
             const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
             float ca = 0.0f;
