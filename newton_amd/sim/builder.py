@@ -162,6 +162,8 @@ class ModelBuilder:
         self.default_tri_lift = 0.0
         self.default_edge_ke = 100.0
         self.default_edge_kd = 0.0
+        self.default_spring_ke = 100.0
+        self.default_spring_kd = 0.0
         self.default_tet_k_mu = 1.0e3
         self.default_tet_k_lambda = 1.0e3
         self.default_tet_k_damp = 0.0
@@ -182,6 +184,12 @@ class ModelBuilder:
         self._edge_rest = _Col(np.float64)
         self._edge_len = _Col(np.float64)
         self._edge_bend = _Col(np.float64, 2)
+        # springs
+        self._spring = _Col(np.int32, 2)
+        self._spring_rest = _Col(np.float64)
+        self._spring_ke = _Col(np.float64)
+        self._spring_kd = _Col(np.float64)
+        self._spring_control = _Col(np.float64)
         # tets
         self._tet = _Col(np.int32, 4)
         self._tet_pose = _Col(np.float64, (3, 3))
@@ -234,6 +242,14 @@ class ModelBuilder:
     @property
     def edge_count(self) -> int:
         return len(self._edge)
+
+    @property
+    def spring_count(self) -> int:
+        return len(self._spring)
+
+    @property
+    def spring_indices(self) -> np.ndarray:
+        return self._spring.array()
 
     @property
     def tet_count(self) -> int:
@@ -457,6 +473,30 @@ class ModelBuilder:
         return range(start, self.edge_count)
 
     # ------------------------------------------------------------------- tets
+    def add_spring(self, i: int, j: int, ke: float, kd: float, control: float = 0.0) -> None:
+        """A spring between particles i and j, rest length from their current positions (``builder.py:8409-8446``)."""
+        self.add_springs([i], [j], [ke], [kd], [control])
+
+    def add_springs(self, i, j, ke, kd, control=None) -> None:
+        """Vectorised :meth:`add_spring`: rest length sqrt(dot(p_i - p_j, p_i - p_j)) in float32."""
+        i = np.asarray(i, dtype=np.int64).reshape(-1)
+        j = np.asarray(j, dtype=np.int64).reshape(-1)
+        n = len(i)
+        if n == 0:
+            return
+        pos = self.particle_q
+        p = pos[i].astype(np.float32)
+        q = pos[j].astype(np.float32)
+        delta = np.subtract(p, q)
+        # np.dot(delta, delta) of the reference, summed in index order
+        rest = np.sqrt((delta[:, 0] * delta[:, 0] + delta[:, 1] * delta[:, 1]) + delta[:, 2] * delta[:, 2])
+        self._spring.extend(np.stack([i, j], axis=1))
+        self._spring_rest.extend(rest)
+        self._spring_ke.extend(np.broadcast_to(np.asarray(ke, dtype=np.float64), (n,)))
+        self._spring_kd.extend(np.broadcast_to(np.asarray(kd, dtype=np.float64), (n,)))
+        self._spring_control.extend(np.broadcast_to(np.asarray(0.0 if control is None else control,
+                                                               dtype=np.float64), (n,)))
+
     def add_tetrahedron(self, i: int, j: int, k: int, l: int, k_mu: float = 1.0e3, k_lambda: float = 1.0e3,
                         k_damp: float = 0.0) -> float:
         """``builder.py:8657-8722``."""
@@ -485,10 +525,10 @@ class ModelBuilder:
     # ------------------------------------------------------------ cloth helpers
     def add_cloth_mesh(self, *, pos, rot, scale: float, vel, vertices, indices, density: float, tri_ke=None,
                        tri_ka=None, tri_kd=None, tri_drag=None, tri_lift=None, edge_ke=None, edge_kd=None,
-                       particle_radius=None, add_springs: bool = False) -> None:
-        """``builder.py:9071-9218``: particles, triangles, lumped mass density*area/3, bending edges."""
-        if add_springs:
-            raise NotImplementedError("springs are not part of this VBD path")
+                       particle_radius=None, add_springs: bool = False, spring_ke: float | None = None,
+                       spring_kd: float | None = None) -> None:
+        """``builder.py:9071-9218``: particles, triangles, lumped mass density*area/3, bending edges
+        and, with ``add_springs``, a spring on every mesh edge and across every hinge."""
         tri_ke = self.default_tri_ke if tri_ke is None else tri_ke
         tri_ka = self.default_tri_ka if tri_ka is None else tri_ka
         tri_kd = self.default_tri_kd if tri_kd is None else tri_kd
@@ -518,12 +558,37 @@ class ModelBuilder:
         np.add.at(mass, inds.reshape(-1), np.repeat(contrib, 3))
         self._pmass._cache = mass
         end_tri = self.tri_count
-        self._add_soft_mesh_edges_from_triangles(start_tri, end_tri, edge_ke=edge_ke, edge_kd=edge_kd)
+        edge_range = self._add_soft_mesh_edges_from_triangles(start_tri, end_tri, edge_ke=edge_ke, edge_kd=edge_kd)
+        if add_springs:
+            self._add_cloth_springs(edge_range, self.default_spring_ke if spring_ke is None else spring_ke,
+                                    self.default_spring_kd if spring_kd is None else spring_kd)
+
+    def _add_cloth_springs(self, edge_range: range, spring_ke: float, spring_kd: float) -> None:
+        """Springs of ``builder.py:9204-9218``: every hinge adds its edge (k, l), the four opposite-to-edge
+        pairs and (i, j), as sorted pairs in a Python set, then the set is walked in its own order
+        (CPython's iteration order for the same insertions, so the spring numbering matches)."""
+        rows = self.edge_indices[edge_range.start:edge_range.stop]
+        pairs = set()
+        for i, j, k, l in rows.tolist():
+            pairs.add((min(k, l), max(k, l)))
+            if i != -1:
+                pairs.add((min(i, k), max(i, k)))
+                pairs.add((min(i, l), max(i, l)))
+            if j != -1:
+                pairs.add((min(j, k), max(j, k)))
+                pairs.add((min(j, l), max(j, l)))
+            if i != -1 and j != -1:
+                pairs.add((min(i, j), max(i, j)))
+        if pairs:
+            ij = np.asarray(list(pairs), dtype=np.int64)
+            self.add_springs(ij[:, 0], ij[:, 1], spring_ke, spring_kd, 0.0)
 
     def add_cloth_grid(self, *, pos, rot, vel, dim_x: int, dim_y: int, cell_x: float, cell_y: float, mass: float,
                        reverse_winding: bool = False, fix_left: bool = False, fix_right: bool = False,
                        fix_top: bool = False, fix_bottom: bool = False, tri_ke=None, tri_ka=None, tri_kd=None,
-                       tri_drag=None, tri_lift=None, edge_ke=None, edge_kd=None, particle_radius=None) -> None:
+                       tri_drag=None, tri_lift=None, edge_ke=None, edge_kd=None, particle_radius=None,
+                       add_springs: bool = False, spring_ke: float | None = None,
+                       spring_kd: float | None = None) -> None:
         """``builder.py:8944-9068``; per-particle mass is then overwritten with ``mass``."""
         verts, indices = cloth_grid_mesh(dim_x, dim_y, cell_x, cell_y, reverse_winding)
         start_vertex = self.particle_count
@@ -532,7 +597,8 @@ class ModelBuilder:
         density = total_mass / total_area
         self.add_cloth_mesh(pos=pos, rot=rot, scale=1.0, vel=vel, vertices=verts, indices=indices, density=density,
                             tri_ke=tri_ke, tri_ka=tri_ka, tri_kd=tri_kd, tri_drag=tri_drag, tri_lift=tri_lift,
-                            edge_ke=edge_ke, edge_kd=edge_kd, particle_radius=particle_radius)
+                            edge_ke=edge_ke, edge_kd=edge_kd, particle_radius=particle_radius,
+                            add_springs=add_springs, spring_ke=spring_ke, spring_kd=spring_kd)
         ys, xs = np.meshgrid(np.arange(dim_y + 1), np.arange(dim_x + 1), indexing="ij")
         xs, ys = xs.reshape(-1), ys.reshape(-1)
         fixed = (xs == 0) & fix_left | (xs == dim_x) & fix_right | (ys == 0) & fix_bottom | (ys == dim_y) & fix_top
@@ -761,6 +827,12 @@ class ModelBuilder:
             self._edge_rest.extend(builder._edge_rest.array())
             self._edge_len.extend(builder._edge_len.array())
             self._edge_bend.extend(builder.edge_bending_properties)
+        if builder.spring_count:
+            self._spring.extend(builder.spring_indices + off)
+            self._spring_rest.extend(builder._spring_rest.array())
+            self._spring_ke.extend(builder._spring_ke.array())
+            self._spring_kd.extend(builder._spring_kd.array())
+            self._spring_control.extend(builder._spring_control.array())
         if builder.tet_count:
             self._tet.extend(builder.tet_indices + off)
             self._tet_pose.extend(builder._tet_pose.array())
@@ -851,11 +923,16 @@ class ModelBuilder:
         m.tet_poses = f32(self._tet_pose.array(), (-1, 3, 3))
         m.tet_activations = torch.zeros(self.tet_count, dtype=torch.float32, device=dev)
         m.tet_materials = f32(self._tet_mat.array(), (-1, 3))
-        m.spring_count = 0
-        m.spring_indices = torch.zeros(0, dtype=torch.int32, device=dev)
+        m.spring_count = self.spring_count
+        m.spring_indices = i32(self.spring_indices.reshape(-1))
+        m.spring_rest_length = f32(self._spring_rest.array())
+        m.spring_stiffness = f32(self._spring_ke.array())
+        m.spring_damping = f32(self._spring_kd.array())
+        m.spring_control = f32(self._spring_control.array())
         adj = MeshAdjacency(
             tri_indices=self.tri_indices,
             edge_indices=self.edge_indices if self.edge_count else np.empty((0, 4), dtype=np.int32),
+            spring_indices=self.spring_indices.reshape(-1) if self.spring_count else None,
             tet_indices=self.tet_indices if self.tet_count else None,
         )
         adj.init_vertex_adjacency(n)

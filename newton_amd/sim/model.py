@@ -105,6 +105,10 @@ class Model:
         self.tet_materials = None
         self.spring_count = 0
         self.spring_indices = None
+        self.spring_rest_length = None
+        self.spring_stiffness = None
+        self.spring_damping = None
+        self.spring_control = None
         self.soft_mesh_adjacency = None
         self.soft_mesh_adjacency_device = None
         self.gravity = None

@@ -71,6 +71,11 @@ class _OracleModel(ctypes.Structure):
         ("soft_contact_ke", f32),
         ("soft_contact_kd", f32),
         ("soft_contact_mu", f32),
+        ("spring_count", i32),
+        ("spring_indices", vp),
+        ("spring_rest_length", vp),
+        ("spring_stiffness", vp),
+        ("spring_damping", vp),
     ]
 
 
@@ -108,7 +113,7 @@ def lib():
         for sfx in ("f32", "f64"):
             for name in ("step", "create_soft_contacts", "simulate", "residual", "block_solve"):
                 getattr(L, f"vbd_oracle_{name}_{sfx}").restype = i32
-            for name in ("membrane", "bending", "tet", "body_contact"):
+            for name in ("membrane", "bending", "tet", "body_contact", "spring"):
                 getattr(L, f"vbd_oracle_{name}_{sfx}").restype = None
         L.vbd_oracle_atan2_f32.restype = f32
         L.vbd_oracle_atan2_f32.argtypes = [f32, f32]
@@ -187,6 +192,10 @@ class OracleModel:
         a["shape_ke"] = _np(model.shape_material_ke, np.float32)
         a["shape_kd"] = _np(model.shape_material_kd, np.float32)
         a["shape_mu"] = _np(model.shape_material_mu, np.float32)
+        ns = int(getattr(model, "spring_count", 0) or 0)
+        a["spring_indices"] = _np(model.spring_indices, np.int32) if ns else np.zeros(2, np.int32)
+        for key in ("spring_rest_length", "spring_stiffness", "spring_damping"):
+            a[key] = _np(getattr(model, key), np.float32) if ns else np.zeros(1, np.float32)
         self.arrays = a
         s = _OracleModel()
         s.particle_count = self.N
@@ -236,6 +245,11 @@ class OracleModel:
         s.soft_contact_ke = float(model.soft_contact_ke)
         s.soft_contact_kd = float(model.soft_contact_kd)
         s.soft_contact_mu = float(model.soft_contact_mu)
+        s.spring_count = ns
+        s.spring_indices = _addr(a["spring_indices"])
+        s.spring_rest_length = _addr(a["spring_rest_length"])
+        s.spring_stiffness = _addr(a["spring_stiffness"])
+        s.spring_damping = _addr(a["spring_damping"])
         self.struct = s
         self.world_count = model.world_count
         from newton_amd.sim.collide import world_compatible_pairs  # noqa: PLC0415
@@ -398,6 +412,16 @@ def body_contact(x, x_prev, body_pos, body_vel, normal, radius, margin, ke, kd, 
     getattr(lib(), f"vbd_oracle_body_contact_{precision}")(*[_p(a) for a in arrs], ct(radius), ct(margin), ct(ke),
                                                            ct(kd), ct(mu), ct(friction_epsilon), ct(dt), _p(f), _p(h))
     return f, h.reshape(3, 3)
+
+
+def spring(x, x_anchor, rest, ke, kd, dt, precision="f64"):
+    """evaluate_spring_force_and_hessian_both_vertices: (force on x0, force on x1, shared Hessian)."""
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(2, 3)
+    xa = np.ascontiguousarray(x_anchor, dtype=rt).reshape(2, 3)
+    f, h = np.zeros(6, dtype=rt), np.zeros(9, dtype=rt)
+    getattr(lib(), f"vbd_oracle_spring_{precision}")(_p(x), _p(xa), ct(rest), ct(ke), ct(kd), ct(dt), _p(f), _p(h))
+    return f[:3], f[3:], h.reshape(3, 3)
 
 
 def block_solve(h, f, fp64_ill_conditioned=False, precision="f32"):

@@ -68,6 +68,11 @@ typedef struct OracleModel {
     float soft_contact_ke;
     float soft_contact_kd;
     float soft_contact_mu;
+    int32_t spring_count;
+    const int32_t* spring_indices;   /* [S*2] */
+    const float* spring_rest_length; /* [S] */
+    const float* spring_stiffness;   /* [S] */
+    const float* spring_damping;     /* [S] */
 } OracleModel;
 
 typedef struct OracleParams {
@@ -120,6 +125,9 @@ typedef struct OraclePairs {
     int32_t vbd_oracle_block_solve_##SFX(const REAL* h /*9*/, const REAL* f /*3*/,            \
                                          int32_t fp64_ill_conditioned, REAL* dx /*3*/);        \
     REAL vbd_oracle_atan2_##SFX(REAL y, REAL x);                                               \
+    void vbd_oracle_spring_##SFX(const REAL* x /*6: x0, x1*/, const REAL* anchor /*6*/,        \
+                                 REAL rest, REAL ke, REAL kd, REAL dt, REAL* f /*6: f0, f1*/,    \
+                                 REAL* h /*9*/);                                                  \
     int32_t vbd_oracle_residual_##SFX(const OracleModel* m, const OracleParams* p,             \
                                       int32_t contact_count, int32_t contact_max,              \
                                       const int32_t* c_indices, const int32_t* c_shape,        \

@@ -714,6 +714,81 @@ static void accumulate_contacts(const OracleModel* m, const OracleParams* p, int
     }
 }
 
+/* evaluate_spring_force_and_hessian_both_vertices (K:1685-1735): the forces on v0 and v1 and the
+ * shared Hessian, in the reference's operation order. */
+static void spring_eval_both(const OracleModel* m, int s, const REAL* q, const REAL* q_anchor, REAL dt, V3* f0,
+                             V3* f1, M3* h) {
+    int v0 = m->spring_indices[2 * s], v1 = m->spring_indices[2 * s + 1];
+    V3 diff = vsub(vld(q, v0), vld(q, v1));
+    REAL len = rmax(vlength(diff), R(1e-8));
+    REAL l0 = R(m->spring_rest_length[s]);
+    REAL ke = R(m->spring_stiffness[s]), kd = R(m->spring_damping[s]);
+    V3 base = vscale(diff, ke * (l0 - len) / len);
+    M3 nd = mouter(diff, diff);
+    M3 inner;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) inner.m[i][j] = (i == j ? R(1.0) : R(0.0)) - nd.m[i][j] / (len * len);
+    M3 structural;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) structural.m[i][j] = (i == j ? R(1.0) : R(0.0)) - (l0 / len) * inner.m[i][j];
+    M3 hs = mscale(structural, ke);
+    V3 dir = vdivs(diff, len);
+    V3 diff_a = vsub(vld(q_anchor, v0), vld(q_anchor, v1));
+    REAL len_a = rmax(vlength(diff_a), R(1e-8));
+    REAL dlen = len - len_a;
+    REAL kdt = kd / dt;
+    M3 hd;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) hd.m[i][j] = (vcomp(dir, i) * vcomp(dir, j)) * kdt;
+    V3 fd0 = vscale(vneg(dir), kdt * dlen);
+    V3 fd1 = vscale(dir, kdt * dlen);
+    *f0 = vadd(base, fd0);
+    *f1 = vadd(vneg(base), fd1);
+    *h = madd(hs, hd);
+}
+
+void API(vbd_oracle_spring)(const REAL* x, const REAL* anchor, REAL rest, REAL ke, REAL kd, REAL dt, REAL* f,
+                            REAL* h) {
+    int32_t idx[2] = {0, 1};
+    float rl = (float)rest, k = (float)ke, d = (float)kd;
+    OracleModel m;
+    memset(&m, 0, sizeof(m));
+    m.spring_count = 1;
+    m.spring_indices = idx;
+    m.spring_rest_length = &rl;
+    m.spring_stiffness = &k;
+    m.spring_damping = &d;
+    V3 f0, f1;
+    M3 hm;
+    spring_eval_both(&m, 0, x, anchor, dt, &f0, &f1, &hm);
+    vst(f, 0, f0);
+    vst(f, 1, f1);
+    mstore(h, hm);
+}
+
+/* accumulate_spring_force_and_hessian (K:1738-1783): after the contacts, in spring index order, for
+ * the endpoints of the current color (color < 0: every endpoint), positions of the pass start. */
+static void accumulate_springs(const OracleModel* m, const REAL* q, const REAL* q_prev, REAL dt, int color,
+                               REAL* pf, REAL* ph) {
+    for (int s = 0; s < m->spring_count; ++s) {
+        int v0 = m->spring_indices[2 * s], v1 = m->spring_indices[2 * s + 1];
+        int c0 = m->particle_colors[v0] == color || color < 0;
+        int c1 = m->particle_colors[v1] == color || color < 0;
+        if (!c0 && !c1) continue;
+        V3 f0, f1;
+        M3 h;
+        spring_eval_both(m, s, q, q_prev, dt, &f0, &f1, &h);
+        if (c0) {
+            vst(pf, v0, vadd(vld(pf, v0), f0));
+            mstore(ph + 9 * v0, madd(mload(ph + 9 * v0), h));
+        }
+        if (c1) {
+            vst(pf, v1, vadd(vld(pf, v1), f1));
+            mstore(ph + 9 * v1, madd(mload(ph + 9 * v1), h));
+        }
+    }
+}
+
 int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, int32_t contact_count,
                                  int32_t contact_max, const int32_t* c_indices, const int32_t* c_shape,
                                  const REAL* c_body_pos, const REAL* c_body_vel, const REAL* c_normal,
@@ -731,6 +806,7 @@ int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, in
     init_contacts(m, p, nc, c_shape, pk, mke, mkd, mmu);
     accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q, q_prev,
                         dt, -1, pf, ph);
+    accumulate_springs(m, q, q_prev, dt, -1, pf, ph);
     Ctx c = {m, q, q_prev, dt};
     double acc = 0.0;
     for (int i = 0; i < N; ++i) {
@@ -812,6 +888,7 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
         for (int color = 0; color < m->color_count; ++color) {
             accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q,
                                 q_prev, dt, color, pf, ph);
+            accumulate_springs(m, q, q_prev, dt, color, pf, ph);
             int cb = m->color_offsets[color], ce = m->color_offsets[color + 1];
 #ifdef _OPENMP
             int nt = p->num_threads > 1 ? p->num_threads : 1;

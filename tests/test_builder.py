@@ -210,3 +210,31 @@ def test_soft_cubes_batched_scene_structure():
     for a in range(4):
         for b in range(a + 1, 4):
             assert (colors[tet[:, a]] != colors[tet[:, b]]).all()
+
+
+def test_cloth_springs_cover_edges_and_hinges():
+    """add_springs: one spring per mesh edge plus one across every interior hinge, rest = distance."""
+    b = ModelBuilder(up_axis="Y")
+    verts, tris = cloth_grid_mesh(3, 2, 0.1, 0.2)
+    b.add_cloth_mesh(pos=(0.0, 1.0, 0.0), rot=(0.0, 0.0, 0.0, 1.0), scale=1.0, vel=(0.0, 0.0, 0.0), vertices=verts,
+                     indices=tris, density=0.1, add_springs=True, spring_ke=55.0, spring_kd=0.5)
+    edges = b.edge_indices
+    interior = int(((edges[:, 0] >= 0) & (edges[:, 1] >= 0)).sum())
+    assert b.spring_count == len(edges) + interior
+    pairs = {tuple(sorted(p)) for p in b.spring_indices.tolist()}
+    assert len(pairs) == b.spring_count
+    for i, j, k, l in edges.tolist():
+        assert (min(k, l), max(k, l)) in pairs
+        if i >= 0 and j >= 0:
+            assert (min(i, j), max(i, j)) in pairs
+    model = b.finalize("cpu")
+    q = model.particle_q.numpy()
+    si = model.spring_indices.numpy().reshape(-1, 2)
+    rest = np.linalg.norm(q[si[:, 0]] - q[si[:, 1]], axis=1)
+    np.testing.assert_allclose(model.spring_rest_length.numpy(), rest, rtol=1e-6)
+    assert (model.spring_stiffness.numpy() == 55.0).all() and (model.spring_damping.numpy() == 0.5).all()
+    # per-vertex spring CSR in spring index order
+    adj = model.soft_mesh_adjacency
+    v = 0
+    ids = adj.v_adj_springs[adj.v_adj_springs_offsets[v]:adj.v_adj_springs_offsets[v + 1]]
+    np.testing.assert_array_equal(ids, np.nonzero((si == v).any(axis=1))[0])

@@ -358,3 +358,54 @@ def test_portable_atan2_accuracy_and_special_cases():
                          (0.0, 0.0, 0.0), (0.0, -0.0, pi)):
         got = oracle.atan2(y, x)
         assert got == expect and math.copysign(1, got) == math.copysign(1, expect), (y, x, got)
+
+
+def test_spring_force_is_minus_gradient_and_hessian_is_exact():
+    """Spring (particle_vbd_kernels.py:1685-1735): f = -dE/dx of E = k/2 (|x0 - x1| - L0)^2; H = d^2E/dx0^2."""
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        x = rng.normal(0, 0.3, (2, 3))
+        rest = float(np.float32(rng.uniform(0.05, 0.6)))
+        ke = float(np.float32(rng.uniform(10.0, 1e3)))
+        f0, f1, h = oracle.spring(x, x, rest, ke, 0.0, 1.0 / 600.0, precision="f64")
+
+        def energy(xx):
+            d = xx[0] - xx[1]
+            return 0.5 * ke * (np.sqrt(d @ d) - rest) ** 2
+
+        eps = 1e-6
+        for v, f in ((0, f0), (1, f1)):
+            g = np.zeros(3)
+            for k in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[v, k] += eps
+                xm[v, k] -= eps
+                g[k] = (energy(xp) - energy(xm)) / (2 * eps)
+            np.testing.assert_allclose(f, -g, rtol=1e-6, atol=1e-6 * ke)
+        jac = np.zeros((3, 3))
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[0, k] += eps
+            xm[0, k] -= eps
+            jac[:, k] = -(oracle.spring(xp, xp, rest, ke, 0.0, 1.0 / 600.0)[0] -
+                          oracle.spring(xm, xm, rest, ke, 0.0, 1.0 / 600.0)[0]) / (2 * eps)
+        np.testing.assert_allclose(h, jac, rtol=1e-5, atol=1e-5 * ke)
+
+
+def test_spring_damping_known_answer():
+    """Stretching by dL adds -kd/dt * dL along the spring direction on x0 (the opposite on x1) and kd/dt d d^T to H;
+    at the rest length the elastic Hessian is k d d^T (its transverse part l0/L - 1 vanishes)."""
+    dt = 0.01
+    x = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 2.0]])
+    xa = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.5]])
+    f0, f1, h = oracle.spring(x, xa, 2.0, 50.0, 3.0, dt, precision="f64")
+    # at rest length: no elastic force; damping: dir = (x0 - x1)/L = (0, 0, -1), dL = 0.5
+    np.testing.assert_allclose(f0, [0.0, 0.0, 3.0 / dt * 0.5], atol=1e-9)
+    np.testing.assert_allclose(f1, -f0, atol=1e-9)
+    np.testing.assert_allclose(h, np.diag([0.0, 0.0, 50.0 + 3.0 / dt]), atol=1e-9)
+
+
+def test_zero_length_spring_is_finite():
+    x = np.array([[0.1, 0.1, 0.1], [0.1, 0.1, 0.1]])
+    f0, f1, h = oracle.spring(x, x, 0.2, 10.0, 0.0, 1.0 / 600.0, precision="f32")
+    assert np.isfinite(f0).all() and np.isfinite(h).all()
