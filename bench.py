@@ -12,6 +12,7 @@ The same line carries
 * ``batched`` / ``batched_tets``: the 4096-world cloth (~7.3M vertices, 5
   iterations) and the 512 tetrahedral cubes (~0.59M vertices, 8 iterations),
   each one graph-captured frame per step with its own color-pass roofline;
+* ``springs``: the headline cloth with a spring on every mesh edge and hinge;
 * ``residual``: the 128x128 wind cloth driven to a relative force residual,
   plain VBD (inertial guess), the previous substep's correction carried over
   ("reuse", no network) and the learned warm start (trained on the GPU in the
@@ -62,8 +63,10 @@ def parse():
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
-    ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes"), default="drop")
+    ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
+    ap.add_argument("--spring-steps", type=int, default=10,
+                    help="timed frames of the headline cloth with springs on every edge and hinge (0 disables)")
     return ap.parse_args()
 
 
@@ -80,6 +83,8 @@ def _scene(name, args, dev):
         return scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
     if name == "cubes":
         return scenes.soft_cubes_batched(dev, worlds=args.cube_worlds)
+    if name == "springs":
+        return scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations, springs=True)
     return scenes.cloth_batched(dev, worlds=args.worlds)
 
 
@@ -258,9 +263,11 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
 
 
 # ----------------------------------------------------------------------------- legs
-def batched_leg(args, name="batched"):
-    """Many worlds in one launch per color pass (4096 cloths or 512 tet cubes); graph-captured frames;
-    color-pass roofline; the work-group shape picked by the autotuner."""
+def batched_leg(args, name="batched", steps=None):
+    """Many worlds in one launch per color pass (4096 cloths or 512 tet cubes), or the headline cloth
+    with a spring on every edge and hinge; graph-captured frames; color-pass roofline; the work-group
+    shape picked by the autotuner."""
+    steps = args.batched_steps if steps is None else steps
     from newton_amd import CollisionPipeline
     from newton_amd.solvers import SolverVBD
 
@@ -276,7 +283,7 @@ def batched_leg(args, name="batched"):
     state.clear_forces()
     pipe.collide(state, contacts)
     tune = solver.autotune(state, contacts, info["dt"])
-    gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, args.batched_steps, 1)
+    gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, steps, 1)
     n = model.particle_count
     roof = color_roofline(model, last, info, solver, pipe, contacts, max(3, args.roofline_reps // 4))
     return {
@@ -285,13 +292,14 @@ def batched_leg(args, name="batched"):
         "worlds": model.world_count,
         "triangles": model.tri_count,
         "tets": model.tet_count,
+        "springs": model.spring_count,
         "autotune": tune,
         "colors": len(model.particle_color_groups),
         "iterations": info["iterations"],
         "substeps_per_step": info["substeps"],
-        "steps": args.batched_steps,
-        "ms_per_step": round(gpu_ms / args.batched_steps, 3),
-        "vertex_steps_per_s": round(n * info["substeps"] * args.batched_steps / (gpu_ms / 1e3), 1),
+        "steps": steps,
+        "ms_per_step": round(gpu_ms / steps, 3),
+        "vertex_steps_per_s": round(n * info["substeps"] * steps / (gpu_ms / 1e3), 1),
         "hip_graph": True,
         "finite": finite,
         "host_build_s": round(t_build, 2),
@@ -536,6 +544,12 @@ def main():
         cubes = batched_leg(args, "cubes")
         _progress(f"cubes: {cubes['vertex_steps_per_s']:.4g} vertex-steps/s")
     torch.cuda.empty_cache()
+    springs = None
+    if args.spring_steps > 0:
+        _progress("springs leg: the headline cloth with a spring on every edge and hinge")
+        springs = batched_leg(args, "springs", args.spring_steps)
+        _progress(f"springs: {springs['vertex_steps_per_s']:.4g} vertex-steps/s")
+    torch.cuda.empty_cache()
     residual = None
     if args.residual_frames > 0:
         _progress("residual leg: data generation, training, evaluation")
@@ -575,6 +589,7 @@ def main():
         "roofline": roofline,
         "batched": batched,
         "batched_tets": cubes,
+        "springs": springs,
         "residual": residual,
         "cpu_baseline": cpu,
     }

@@ -80,6 +80,13 @@ typedef struct VbdModelDesc {
     float soft_contact_ke;
     float soft_contact_kd;
     float soft_contact_mu;
+    /* Model.spring_* (newton/_src/sim/model.py); springs are not part of the coloring graph, so both
+     * endpoints of one spring may share a color */
+    int32_t spring_count;
+    const int32_t* spring_indices;         /* [S*2] */
+    const float* spring_rest_length;       /* [S] */
+    const float* spring_stiffness;         /* [S] */
+    const float* spring_damping;           /* [S] */
 } VbdModelDesc;
 
 typedef struct VbdSolverParams {

@@ -62,6 +62,11 @@ class VbdModelDesc(ctypes.Structure):
         ("soft_contact_ke", c_float),
         ("soft_contact_kd", c_float),
         ("soft_contact_mu", c_float),
+        ("spring_count", c_int32),
+        ("spring_indices", c_void_p),
+        ("spring_rest_length", c_void_p),
+        ("spring_stiffness", c_void_p),
+        ("spring_damping", c_void_p),
     ]
 
 

@@ -635,6 +635,17 @@ struct KParams {
     const int* tet_rec;
     const int* tet_meta;
     const int* slow_pos;
+    // springs: model arrays, batch items, same-color pre items and their rows, slow-vertex lists
+    int has_spring;
+    const int* spring_idx;
+    const float* spring_rest;
+    const float* spring_ke;
+    const float* spring_kd;
+    const int* spr_rec;
+    const int* pre_item;
+    float* spr_rows;
+    const int* slow_spr_off;
+    const int* slow_spr;
     // per-particle contact lists (contact-index order) and per-contact state
     const int* c_off;
     const int* c_ids;
@@ -691,12 +702,78 @@ __device__ __forceinline__ void eval_packed_contact(const KParams& P, const floa
                  A.z, A.w, B.x, P.friction_epsilon, dt, f, h);
 }
 
+// evaluate_spring_force_and_hessian_both_vertices (particle_vbd_kernels.py:1685-1730), the
+// contribution to one endpoint (0: v0, 1: v1); x* are the pass-start positions, a* the anchors.
+__device__ __forceinline__ void eval_spring(V3 x0, V3 x1, V3 a0, V3 a1, float rest, float ke, float kd, float dt,
+                                            int endpoint, V3& f, M3& h) {
+    const V3 diff = sub(x0, x1);
+    const float len = fmaxf(length(diff), 1.0e-8f);
+    const V3 base = scl(diff, ke * (rest - len) / len);
+    const M3 nd = outer(diff, diff);
+    const float lsq = len * len, rl = rest / len;
+    M3 hs;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float id = i == j ? 1.0f : 0.0f;
+            const float inner = id - nd.m[i][j] / lsq;
+            hs.m[i][j] = ke * (id - rl * inner);
+        }
+    const V3 dir = divs(diff, len);
+    const float len_a = fmaxf(length(sub(a0, a1)), 1.0e-8f);
+    const float dlen = len - len_a;
+    const float kdt = kd / dt;
+    M3 hd;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) hd.m[i][j] = (comp(dir, i) * comp(dir, j)) * kdt;
+    const float fdm = kdt * dlen;
+    f = endpoint == 0 ? add(base, scl(neg(dir), fdm)) : add(neg(base), scl(dir, fdm));
+    h = madd(hs, hd);
+}
+
+// Spring item: in-pass evaluation (pre_row < 0) or the pre-pass row.
+__device__ __forceinline__ void spring_item(const KParams& P, const float* __restrict__ q, int v0, int v1,
+                                            int endpoint, int pre_row, float rest, float ke, float kd, float dt,
+                                            V3& f, M3& h) {
+    if (pre_row >= 0) {
+        const float4* r4 = reinterpret_cast<const float4*>(P.spr_rows) + 3 * (size_t)pre_row;
+        const float4 A = r4[0], B = r4[1], C = r4[2];
+        f = v3(A.x, A.y, A.z);
+        h.m[0][0] = A.w; h.m[0][1] = B.x; h.m[0][2] = B.y;
+        h.m[1][0] = B.z; h.m[1][1] = B.w; h.m[1][2] = C.x;
+        h.m[2][0] = C.y; h.m[2][1] = C.z; h.m[2][2] = C.w;
+    } else {
+        eval_spring(ld3(q, v0), ld3(q, v1), ld3(P.q_prev, v0), ld3(P.q_prev, v1), rest, ke, kd, dt, endpoint, f, h);
+    }
+}
+
+// Pre-pass of one color: rows of the spring items whose endpoints share the color, evaluated
+// at the positions of the color pass start (accumulate_spring_force_and_hessian,
+// particle_vbd_kernels.py:1733-1783, runs before the color's solve).
+__global__ void k_spring_rows(KParams P, const float* __restrict__ q, int j0, int j1, float dt) {
+    const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= j1) return;
+    const int2 it = reinterpret_cast<const int2*>(P.pre_item)[j];
+    const int s = it.x;
+    const int v0 = P.spring_idx[2 * s], v1 = P.spring_idx[2 * s + 1];
+    V3 f;
+    M3 h;
+    eval_spring(ld3(q, v0), ld3(q, v1), ld3(P.q_prev, v0), ld3(P.q_prev, v1), P.spring_rest[s], P.spring_ke[s],
+                P.spring_kd[s], dt, it.y, f, h);
+    store_row(P.spr_rows + 12 * (size_t)j, f, h);
+}
+
 // Finish one vertex: contacts (when not already summed), then the block solve
 // and position update (mode 0) or the force output (mode 1).
 // x0 and d_old are the vertex's q_prev and displacement (read by the caller).
+// With add_contacts the vertex's spring rows (srows, nspr rows in LDS) continue the contact sum.
 __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 x0,
-                                              V3 d_old, V3 f, M3 h, bool add_contacts, int crec0, int crec1, float dt,
-                                              int mode, float* __restrict__ g_out) {
+                                              V3 d_old, V3 f, M3 h, bool add_contacts, int crec0, int crec1,
+                                              const float* srows, int nspr, float dt, int mode,
+                                              float* __restrict__ g_out) {
     if (!active) {
         if (mode == 0) {
             V3 z = v3(0.0f, 0.0f, 0.0f);
@@ -716,6 +793,14 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
             eval_packed_contact(P, q, j, dt, fc, hc);
             cf = add(cf, fc);
             ch = madd(ch, hc);
+        }
+        for (int j = 0; j < nspr; ++j) {
+            const float* r = srows + j * kComp;
+            cf = add(cf, v3(r[0], r[1], r[2]));
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
         }
         h = madd(h, ch);
         f = add(f, cf);
@@ -755,6 +840,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
     constexpr int BLK = kBatchHdr + 4 * MAXV;
     __shared__ int s_row[MAXV];
     __shared__ int s_nrow[MAXV];
+    __shared__ int s_nspr[MAXV];
     __shared__ int s_cb[MAXV + 1];
     __shared__ __align__(16) float s_rows[ROWS * kComp];
     __shared__ float s_acc[MAXV * kComp];
@@ -765,19 +851,21 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
     const int* blk = P.batch_blk + (size_t)b * BLK;
     const int4 b0 = reinterpret_cast<const int4*>(blk)[0];
     const int4 b1 = reinterpret_cast<const int4*>(blk)[1];
+    const int4 b2 = reinterpret_cast<const int4*>(blk)[2];
     const int2 ci = reinterpret_cast<const int2*>(P.bat_cinfo)[b];
     int4 vr = make_int4(0, 0, 0, 0);
     if (t < MAXV) vr = reinterpret_cast<const int4*>(blk + kBatchHdr)[t];
     int cbt = 0;
     if (t <= MAXV) cbt = P.bat_coff[(size_t)b * (MAXV + 1) + t];
     const int nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
-    const int nstatic = ntri + nedge + ntet;
+    const int spr_b = b2.x, nspr = b2.y;
+    const int nstatic = ntri + nedge + ntet + nspr;
     const int cbeg = ci.x, ncont = ci.y;
     const bool par_contacts = nstatic + ncont <= ROWS;
     const float dtr = 1.0f / (dt * dt);
 
     // ---- round trip 2: item record of this lane (items 0..255), own-vertex state
-    int kind = 0;  // 1 triangle, 2 edge, 3 tet, 4 contact
+    int kind = 0;  // 1 triangle, 2 edge, 3 tet, 4 contact, 5 spring
     int4 r0 = make_int4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     int meta = 0;
     if (t < ntri) {
@@ -793,7 +881,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         r0 = r4[0];
         r1 = r4[1];
         meta = P.edge_meta[edge_b + k];
-    } else if (t < nstatic) {
+    } else if (t < ntri + nedge + ntet) {
         kind = 3;
         const int k = t - ntri - nedge;
         const int4* r4 = reinterpret_cast<const int4*>(P.tet_rec) + 4 * (size_t)(tet_b + k);
@@ -802,6 +890,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         r2 = r4[2];
         r3 = r4[3];
         meta = P.tet_meta[tet_b + k];
+    } else if (t < nstatic) {
+        kind = 5;
+        const int4* r4 = reinterpret_cast<const int4*>(P.spr_rec) + 2 * (size_t)(spr_b + t - ntri - nedge - ntet);
+        r0 = r4[0];
+        r1 = r4[1];
     } else if (par_contacts && t - nstatic < ncont) {
         kind = 4;
         const int4* c4 = reinterpret_cast<const int4*>(P.cpack) + 4 * (size_t)(cbeg + t - nstatic);
@@ -812,7 +905,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
     }
     V3 own_x0 = v3(0.0f, 0.0f, 0.0f), own_d = own_x0;
     const int own_v = vr.x;
-    const bool own_active = (vr.w >> 16) != 0;
+    const bool own_active = ((vr.w >> 16) & 1) != 0;
     if (t < nv) {
         const float m = P.mass[own_v];
         float* a = s_acc + t * kComp;
@@ -826,6 +919,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         own_d = ld3(P.disp, own_v);
         s_row[t] = vr.y;
         s_nrow[t] = vr.z;
+        s_nspr[t] = vr.w >> 17;
     }
     if (t <= nv) s_cb[t] = cbt;
 
@@ -853,6 +947,10 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
             float dm[9] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, d2.x};
             row = meta & 0xffff;
             eval_tet(q, P.q_prev, ti, dm, d2.y, d2.z, d2.w, dt, meta >> 16, fe, he);
+        } else if (kind == 5) {
+            const float4 pr = __builtin_bit_cast(float4, r1);
+            row = r0.z & 0xffff;
+            spring_item(P, q, r0.x, r0.y, r0.z >> 16, r0.w, pr.x, pr.y, pr.z, dt, fe, he);
         } else {
             float4 A = __builtin_bit_cast(float4, r0), B = __builtin_bit_cast(float4, r1);
             float4 C = __builtin_bit_cast(float4, r2), D = __builtin_bit_cast(float4, r3);
@@ -891,9 +989,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         }
         for (; r < q1; ++r) acc = acc + s_rows[r * kComp + cc];
         if (par_contacts) {
+            // contacts from zero, then the vertex's spring rows, then added to the element sum
             const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
             float ca = 0.0f;
             for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
+            for (int r = q1, re = q1 + s_nspr[cs]; r < re; ++r) ca = ca + s_rows[r * kComp + cc];
             acc = acc + ca;
         }
         s_acc[cs * kComp + cc] = acc;
@@ -909,8 +1009,9 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
+        const int q1 = s_row[t] + s_nrow[t];
         finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, !par_contacts, cbeg + s_cb[t],
-                      cbeg + s_cb[t + 1], dt, mode, g_out);
+                      cbeg + s_cb[t + 1], s_rows + q1 * kComp, s_nspr[t], dt, mode, g_out);
     }
 }
 
@@ -1027,10 +1128,24 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
                     ch = madd(ch, hc);
                 }
             }
+            if (P.has_spring) {
+                const int si = slow_base + blockIdx.x;
+                for (int j = P.slow_spr_off[si]; j < P.slow_spr_off[si + 1]; ++j) {
+                    const int4 it = reinterpret_cast<const int4*>(P.slow_spr)[j];
+                    const int sp = it.x;
+                    V3 fs;
+                    M3 hs;
+                    spring_item(P, q, P.spring_idx[2 * sp], P.spring_idx[2 * sp + 1], it.y, it.z, P.spring_rest[sp],
+                                P.spring_ke[sp], P.spring_kd[sp], dt, fs, hs);
+                    cf = add(cf, fs);
+                    ch = madd(ch, hs);
+                }
+            }
             h = madd(h, ch);
             f = add(f, cf);
         }
-        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, dt, mode, g_out);
+        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, nullptr, 0, dt, mode,
+                      g_out);
     }
 }
 
@@ -1417,6 +1532,12 @@ struct VbdContext {
     int* lay_tet = nullptr;
     int* lay_tet_meta = nullptr;
     int* lay_slow = nullptr;
+    int* lay_spr = nullptr;
+    int* lay_pre = nullptr;
+    float* spr_rows = nullptr;
+    int* lay_slow_spr_off = nullptr;
+    int* lay_slow_spr = nullptr;
+    std::vector<int32_t> pre_color_off;
     int* bat_cinfo = nullptr;  // [B, 2] per-substep contact range of each batch
     int* bat_coff = nullptr;   // [B, max_v + 1] per-substep contact offsets within the batch
     int block = 256;           // color-pass work-group shape (256 or 128 lanes)
@@ -1460,10 +1581,13 @@ std::string h2d(T** dst, const std::vector<T>& src) {
 
 void free_layout(VbdContext* c) {
     for (int** p : {&c->lay_batch, &c->lay_vert, &c->lay_tri, &c->lay_edge, &c->lay_edge_meta, &c->lay_tet,
-                    &c->lay_tet_meta, &c->lay_slow, &c->bat_cinfo, &c->bat_coff}) {
+                    &c->lay_tet_meta, &c->lay_slow, &c->bat_cinfo, &c->bat_coff, &c->lay_spr, &c->lay_pre,
+                    &c->lay_slow_spr_off, &c->lay_slow_spr}) {
         hipFree(*p);
         *p = nullptr;
     }
+    hipFree(c->spr_rows);
+    c->spr_rows = nullptr;
 }
 
 // Reads the model arrays back once and builds the color-pass layout on the host (vbd_layout.h).
@@ -1497,6 +1621,11 @@ int32_t build_layout(VbdContext* c) {
     if (err.empty() && hm.T > 0) rd(hm.adj_tri, m->v_adj_tris, (size_t)hm.adj_tri_off[hm.N]);
     if (err.empty() && hm.E > 0) rd(hm.adj_edge, m->v_adj_edges, (size_t)hm.adj_edge_off[hm.N]);
     if (err.empty() && hm.K > 0) rd(hm.adj_tet, m->v_adj_tets, (size_t)hm.adj_tet_off[hm.N]);
+    hm.S = m->spring_count;
+    rd(hm.spring_idx, m->spring_indices, 2 * (size_t)hm.S);
+    rd(hm.spring_rest, m->spring_rest_length, (size_t)hm.S);
+    rd(hm.spring_ke, m->spring_stiffness, (size_t)hm.S);
+    rd(hm.spring_kd, m->spring_damping, (size_t)hm.S);
     hm.color_off = c->color_offsets;
     rd(hm.color_particles, m->color_particles, (size_t)c->color_offsets.back());
     if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: reading the model: " + err);
@@ -1518,12 +1647,21 @@ int32_t build_layout(VbdContext* c) {
     for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_blk), std::make_pair(&c->lay_vert, &L.vert_rec),
                     std::make_pair(&c->lay_tri, &L.tri_rec), std::make_pair(&c->lay_edge, &L.edge_rec),
                     std::make_pair(&c->lay_edge_meta, &L.edge_meta), std::make_pair(&c->lay_tet, &L.tet_rec),
-                    std::make_pair(&c->lay_tet_meta, &L.tet_meta), std::make_pair(&c->lay_slow, &L.slow_pos)}) {
+                    std::make_pair(&c->lay_tet_meta, &L.tet_meta), std::make_pair(&c->lay_slow, &L.slow_pos),
+                    std::make_pair(&c->lay_spr, &L.spr_rec), std::make_pair(&c->lay_pre, &L.pre_item),
+                    std::make_pair(&c->lay_slow_spr_off, &L.slow_spr_off),
+                    std::make_pair(&c->lay_slow_spr, &L.slow_spr)}) {
         err = h2d(pr.first, *pr.second);
         if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: uploading the layout: " + err);
     }
     c->fast_color_off = L.fast_color_off;
     c->slow_color_off = L.slow_color_off;
+    c->pre_color_off = L.pre_color_off;
+    {
+        const size_t npre = (size_t)std::max(L.pre_color_off.back(), 1);
+        hipError_t e = hipMalloc(&c->spr_rows, sizeof(float) * 12 * npre);
+        if (e != hipSuccess) return fail(VBD_ERR_HIP, std::string("vbd_create: spring rows: ") + hipGetErrorString(e));
+    }
     c->fast_items = L.fast_items;
     c->max_v = L.max_v;
     c->blk_words = L.blk_words;
@@ -1549,6 +1687,16 @@ int32_t build_layout(VbdContext* c) {
     k.tet_rec = c->lay_tet;
     k.tet_meta = c->lay_tet_meta;
     k.slow_pos = c->lay_slow;
+    k.has_spring = m->spring_count > 0;
+    k.spring_idx = m->spring_indices;
+    k.spring_rest = m->spring_rest_length;
+    k.spring_ke = m->spring_stiffness;
+    k.spring_kd = m->spring_damping;
+    k.spr_rec = c->lay_spr;
+    k.pre_item = c->lay_pre;
+    k.spr_rows = c->spr_rows;
+    k.slow_spr_off = c->lay_slow_spr_off;
+    k.slow_spr = c->lay_slow_spr;
     return VBD_OK;
 }
 
@@ -1578,6 +1726,9 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if (m->tet_count > 0 && (!m->v_adj_tets || !m->v_adj_tets_offsets || !m->tet_indices || !m->tet_poses ||
                              !m->tet_materials))
         return fail(VBD_ERR_ARG, "vbd_create: tet arrays or adjacency missing");
+    if (m->spring_count < 0 || (m->spring_count > 0 && (!m->spring_indices || !m->spring_rest_length ||
+                                                         !m->spring_stiffness || !m->spring_damping)))
+        return fail(VBD_ERR_ARG, "vbd_create: spring arrays missing");
     if (prm->iterations < 0) return fail(VBD_ERR_ARG, "vbd_create: iterations must be >= 0");
     if (prm->soft_contact_capacity < 0) return fail(VBD_ERR_ARG, "vbd_create: negative soft_contact_capacity");
 
@@ -1820,7 +1971,16 @@ int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, floa
     return VBD_OK;
 }
 
+int32_t launch_spring_rows(VbdContext* c, hipStream_t s, int j0, int j1, const float* q, float dt) {
+    if (j1 <= j0) return VBD_OK;
+    hipLaunchKernelGGL(k_spring_rows, dim3(blocks_for(j1 - j0, 256)), dim3(256), 0, s, c->kp, q, j0, j1, dt);
+    VBD_LAUNCH_CHECK("k_spring_rows");
+    return VBD_OK;
+}
+
 int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt, int mode, float* g) {
+    int32_t rc0 = launch_spring_rows(c, s, c->pre_color_off[color], c->pre_color_off[color + 1], q, dt);
+    if (rc0) return rc0;
     int b0 = c->fast_color_off[color], b1 = c->fast_color_off[color + 1];
     if (b1 > b0) {
         int32_t rc = launch_fast(c, s, b0, b1 - b0, q, dt, mode, g);
@@ -1946,6 +2106,8 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_residual: null argument");
     hipStream_t s = (hipStream_t)stream;
     float* g = g_out ? g_out : c->g_tmp;
+    int32_t rc0 = launch_spring_rows(c, s, 0, c->pre_color_off.back(), q, dt);
+    if (rc0) return rc0;
     int nb = c->fast_color_off.back();
     if (nb > 0) {
         int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g);

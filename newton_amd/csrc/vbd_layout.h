@@ -16,6 +16,14 @@
 // an exact zero (:682-683).  Inactive particles (flag bit 0 clear or mass 0)
 // get no items.  A vertex with more than cap items goes to the "slow"
 // list, handled by the generic kernel that walks the CSR in chunks.
+//
+// Springs (particle_vbd_kernels.py:1733-1783) join the batch as items of their
+// own: each active endpoint of a spring gets one item, in spring-index order
+// per vertex, whose row follows the vertex's element rows (the solve adds them
+// to the contact sum, after the contacts).  A spring whose endpoints share a
+// color cannot be evaluated inside the pass (the other endpoint moves during
+// it); its items are evaluated by a pre-pass over the color into global rows
+// ("pre items") and the pass item only points at its row.
 #pragma once
 
 #include <cstdint>
@@ -31,8 +39,9 @@ constexpr int kLayoutMaxV = 21;  // vertices per batch of the 256-lane shape (12
 constexpr int kTriWords = 12;   // i0 i1 i2 meta | D00 D01 D10 D11 | area mu lambda kd
 constexpr int kEdgeWords = 8;   // o0 o1 v0 v1 | rest_angle rest_length ke kd      (+ edge_meta)
 constexpr int kTetWords = 16;   // i0 i1 i2 i3 | Dm^-1 (9) | mu lambda kd         (+ tet_meta)
-constexpr int kBatchWords = 8;  // vbegin nv tri_begin ntri edge_begin nedge tet_begin ntet
-constexpr int kVertWords = 4;   // v row_start nrows (local | active << 16)
+constexpr int kSpringWords = 8; // v0 v1 (row | endpoint << 16) pre_row | rest ke kd 0
+constexpr int kBatchWords = 12; // vbegin nv tri_begin ntri edge_begin nedge tet_begin ntet spr_begin nspr 0 0
+constexpr int kVertWords = 4;   // v row_start nrows (local | active << 16 | nsprings << 17)
 // batch block: header (kBatchWords) then max_v vertex records (unused ones zero),
 // fixed size so a workgroup reads its header and vertex records in one round trip
 inline int block_words(int max_v) { return kBatchWords + max_v * kVertWords; }  // a multiple of 4 words
@@ -49,6 +58,9 @@ struct HostModel {
     std::vector<float> tet_pose, tet_mat;
     std::vector<int32_t> adj_tri, adj_tri_off, adj_edge, adj_edge_off, adj_tet, adj_tet_off;
     std::vector<int32_t> color_off, color_particles;
+    int S = 0;
+    std::vector<int32_t> spring_idx;
+    std::vector<float> spring_rest, spring_ke, spring_kd;
 };
 
 struct PassLayout {
@@ -59,6 +71,11 @@ struct PassLayout {
     std::vector<int32_t> vert_rec;        // [N * kVertWords], indexed by color position
     std::vector<int32_t> tri_rec, edge_rec, edge_meta, tet_rec, tet_meta;
     std::vector<int32_t> slow_pos;        // color positions of oversized vertices
+    std::vector<int32_t> spr_rec;         // [*, kSpringWords] spring items of the batches
+    std::vector<int32_t> pre_item;        // [P, 2] (spring, endpoint) of same-color spring items
+    std::vector<int32_t> pre_color_off;   // [C+1] into pre_item
+    std::vector<int32_t> slow_spr_off;    // [slow + 1] into slow_spr
+    std::vector<int32_t> slow_spr;        // [*, 4] (spring, endpoint, pre_row or -1, 0) of slow vertices
     int64_t fast_items = 0;
 };
 
@@ -79,6 +96,25 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
     for (int e = 0; e < m.E; ++e)
         edge_ok[e] = m.edge_bend[2 * e] > 0.0f && m.edge_idx[4 * e] != -1 && m.edge_idx[4 * e + 1] != -1;
     for (int k = 0; k < m.K; ++k) tet_ok[k] = m.tet_mat[3 * k] > 0.0f || m.tet_mat[3 * k + 1] > 0.0f;
+    std::vector<int32_t> color_of((size_t)N, -1);
+    for (int c = 0; c < C; ++c)
+        for (int k = m.color_off[c]; k < m.color_off[c + 1]; ++k) color_of[m.color_particles[k]] = c;
+    // per-vertex spring items in spring-index order: 2 * spring + endpoint
+    std::vector<int32_t> vs_off((size_t)N + 1, 0), vs_item;
+    for (int s = 0; s < m.S; ++s)
+        for (int e = 0; e < 2; ++e) {
+            int v = m.spring_idx[2 * s + e];
+            if (v < 0 || v >= N) return "spring_indices out of range";
+            ++vs_off[v + 1];
+        }
+    for (int v = 0; v < N; ++v) vs_off[v + 1] += vs_off[v];
+    vs_item.assign((size_t)vs_off[N], 0);
+    {
+        std::vector<int32_t> fill(vs_off.begin(), vs_off.end() - 1);
+        for (int s = 0; s < m.S; ++s)
+            for (int e = 0; e < 2; ++e) vs_item[fill[m.spring_idx[2 * s + e]]++] = 2 * s + e;
+    }
+    auto same_color = [&](int s) { return color_of[m.spring_idx[2 * s]] == color_of[m.spring_idx[2 * s + 1]]; };
 
     auto count_ok = [&](const std::vector<int32_t>& vals, const std::vector<int32_t>& off, const std::vector<char>& ok,
                         int v) {
@@ -87,13 +123,15 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
         for (int p = off[v]; p < off[v + 1]; p += 2) c += ok[vals[p]] ? 1 : 0;
         return c;
     };
-    std::vector<int32_t> ntri((size_t)N, 0), nedge((size_t)N, 0), ntet((size_t)N, 0);
+    std::vector<int32_t> ntri((size_t)N, 0), nedge((size_t)N, 0), ntet((size_t)N, 0), nspr((size_t)N, 0);
     for (int v = 0; v < N; ++v) {
         if (!active[v]) continue;
         if (m.T > 0) ntri[v] = count_ok(m.adj_tri, m.adj_tri_off, tri_ok, v);
         if (m.E > 0) nedge[v] = count_ok(m.adj_edge, m.adj_edge_off, edge_ok, v);
         if (m.K > 0) ntet[v] = count_ok(m.adj_tet, m.adj_tet_off, tet_ok, v);
+        nspr[v] = vs_off[v + 1] - vs_off[v];
     }
+    auto nitems = [&](int v) { return ntri[v] + nedge[v] + ntet[v] + nspr[v]; };
 
     L = PassLayout();
     L.cap = cap;
@@ -102,6 +140,25 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
     L.vert_rec.assign((size_t)N * kVertWords, 0);
     L.fast_color_off.assign((size_t)C + 1, 0);
     L.slow_color_off.assign((size_t)C + 1, 0);
+    L.pre_color_off.assign((size_t)C + 1, 0);
+    L.slow_spr_off.push_back(0);
+    // pre rows of same-color spring items, per color in color-position order
+    std::vector<int32_t> pre_row((size_t)vs_off[N], -1);
+    for (int c = 0; c < C; ++c) {
+        L.pre_color_off[c] = (int32_t)(L.pre_item.size() / 2);
+        for (int k = m.color_off[c]; k < m.color_off[c + 1]; ++k) {
+            int v = m.color_particles[k];
+            if (!active[v]) continue;
+            for (int p = vs_off[v]; p < vs_off[v + 1]; ++p) {
+                int s = vs_item[p] >> 1;
+                if (!same_color(s)) continue;
+                pre_row[p] = (int32_t)(L.pre_item.size() / 2);
+                L.pre_item.push_back(s);
+                L.pre_item.push_back(vs_item[p] & 1);
+            }
+        }
+    }
+    L.pre_color_off[C] = (int32_t)(L.pre_item.size() / 2);
     std::vector<int> rows;
     for (int c = 0; c < C; ++c) {
         L.fast_color_off[c] = (int32_t)(L.batch_blk.size() / L.blk_words);
@@ -110,9 +167,15 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
         const int kend = m.color_off[c + 1];
         while (k < kend) {
             int v0 = m.color_particles[k];
-            int items0 = ntri[v0] + nedge[v0] + ntet[v0];
+            int items0 = nitems(v0);
             if (items0 > cap) {  // oversized vertex: generic kernel
                 L.slow_pos.push_back(k);
+                if (active[v0])
+                    for (int p = vs_off[v0]; p < vs_off[v0 + 1]; ++p) {
+                        const int32_t r[4] = {vs_item[p] >> 1, vs_item[p] & 1, pre_row[p], 0};
+                        L.slow_spr.insert(L.slow_spr.end(), r, r + 4);
+                    }
+                L.slow_spr_off.push_back((int32_t)(L.slow_spr.size() / 4));
                 ++k;
                 continue;
             }
@@ -120,7 +183,7 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
             int vb = k, nv = 0, items = 0;
             while (k < kend && nv < max_v) {
                 int v = m.color_particles[k];
-                int it = ntri[v] + nedge[v] + ntet[v];
+                int it = nitems(v);
                 if (it > cap || items + it > cap) break;
                 items += it;
                 ++nv;
@@ -137,8 +200,8 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
                 vr[0] = v;
                 vr[1] = acc;
                 vr[2] = nr;
-                vr[3] = s | ((active[v] ? 1 : 0) << 16);
-                acc += nr;
+                vr[3] = s | ((active[v] ? 1 : 0) << 16) | (nspr[v] << 17);
+                acc += nr + nspr[v];
             }
             int32_t rec[kBatchWords];
             rec[0] = vb;
@@ -206,7 +269,29 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
                 }
             }
             rec[7] = (int32_t)L.tet_meta.size() - rec[6];
-            L.fast_items += rec[3] + rec[5] + rec[7];
+            rec[8] = (int32_t)(L.spr_rec.size() / kSpringWords);
+            for (int s = 0; s < nv; ++s) {
+                int v = m.color_particles[vb + s];
+                if (!active[v]) continue;
+                const int row0 = rows[s] + ntri[v] + nedge[v] + ntet[v];
+                for (int p = vs_off[v]; p < vs_off[v + 1]; ++p) {
+                    const int sp = vs_item[p] >> 1, end = vs_item[p] & 1;
+                    int32_t r[kSpringWords];
+                    r[0] = m.spring_idx[2 * sp];
+                    r[1] = m.spring_idx[2 * sp + 1];
+                    r[2] = meta(row0 + (p - vs_off[v]), end);
+                    r[3] = pre_row[p];
+                    r[4] = fbits(m.spring_rest[sp]);
+                    r[5] = fbits(m.spring_ke[sp]);
+                    r[6] = fbits(m.spring_kd[sp]);
+                    r[7] = 0;
+                    L.spr_rec.insert(L.spr_rec.end(), r, r + kSpringWords);
+                }
+            }
+            rec[9] = (int32_t)(L.spr_rec.size() / kSpringWords) - rec[8];
+            rec[10] = 0;
+            rec[11] = 0;
+            L.fast_items += rec[3] + rec[5] + rec[7] + rec[9];
             const size_t at = L.batch_blk.size();
             L.batch_blk.resize(at + L.blk_words, 0);
             std::memcpy(&L.batch_blk[at], rec, sizeof(rec));

@@ -13,7 +13,10 @@ FP32 / INT32 words):
 * per distinct vertex outside the pass whose position an element reads: 12,
   plus 12 for its previous position when that element is damped;
 * per contact record of the pass: 64 (penalty, kd, mu, shape, body position,
-  normal, indices) plus 8 of per-particle list offsets.
+  normal, indices) plus 8 of per-particle list offsets;
+* per spring with an active endpoint in the color: 20 (indices 8, rest length 4,
+  ke/kd 8); its other endpoint counts as a neighbour read (12, plus 12 for both
+  endpoints' previous positions when kd > 0).
 """
 
 from __future__ import annotations
@@ -50,6 +53,10 @@ def color_pass_bytes(model, color_groups=None, contacts_per_particle: np.ndarray
     ):
         off, val = _csr(getattr(adj, f"v_adj_{key}_offsets"), getattr(adj, f"v_adj_{key}"), n)
         kinds.append((off, val, elem_bytes, idx, damped, len(idx) > 0))
+    S = int(getattr(model, "spring_count", 0) or 0)
+    if S:
+        sp_idx = model.spring_indices.cpu().numpy().reshape(-1, 2).astype(np.int64)
+        sp_damped = model.spring_damping.cpu().numpy() > 0
     out = np.zeros(len(groups), dtype=np.int64)
     for c, g in enumerate(groups):
         g = np.asarray(g, dtype=np.int64)
@@ -81,6 +88,15 @@ def color_pass_bytes(model, color_groups=None, contacts_per_particle: np.ndarray
                 dv = dv[dv >= 0]
                 neighbor_prev[dv] = True
                 needs_prev[dv] = True
+        if S:
+            act_in = in_color & active
+            touched = act_in[sp_idx[:, 0]] | act_in[sp_idx[:, 1]]
+            b += 20 * int(touched.sum())
+            ends = sp_idx[touched].reshape(-1)
+            neighbor[ends] = True
+            dends = sp_idx[touched & sp_damped].reshape(-1)
+            neighbor_prev[dends] = True
+            needs_prev[dends] = True
         own_prev = needs_prev & in_color
         own_prev[act[cpp[act] > 0]] = True
         b += 12 * int(own_prev.sum())

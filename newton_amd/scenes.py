@@ -67,11 +67,14 @@ def cloth_hanging(device="cuda:0", verts_per_side: int = 32, size: float = 1.0, 
 
 
 def cloth_drop_on_sphere(device="cuda:0", verts_per_side: int = 256, size: float = 2.0, height: float = 1.5,
-                         seed: int = 1, iterations: int = 10):
+                         seed: int = 1, iterations: int = 10, springs: bool = False, spring_ke: float = 100.0,
+                         spring_kd: float = 1.0e-3):
     """256x256 cloth over 2 m falling onto a static sphere (r 0.4 m) and the ground (65,536 vertices).
 
     Same material as :func:`cloth_hanging` but tri_ke per triangle from
     log-uniform[5e3, 2e4] (``seed``); sphere contact ke 1e4, effective mu 0.3.
+    With ``springs`` the cloth also gets a spring on every mesh edge and across
+    every hinge (``add_cloth_mesh(add_springs=True)``, ke ``spring_ke``, kd ``spring_kd``).
     """
     n = verts_per_side - 1
     b = ModelBuilder(up_axis="Y", gravity=-9.81)
@@ -81,13 +84,14 @@ def cloth_drop_on_sphere(device="cuda:0", verts_per_side: int = 256, size: float
     rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
     b.add_cloth_mesh(pos=(-0.5 * size, height, -0.5 * size), rot=rot, scale=1.0, vel=(0.0, 0.0, 0.0),
                      vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
-                     edge_ke=1.0, edge_kd=0.0)
+                     edge_ke=1.0, edge_kd=0.0, add_springs=springs, spring_ke=spring_ke, spring_kd=spring_kd)
     rng = np.random.default_rng(seed)
     ke = np.exp(rng.uniform(math.log(5.0e3), math.log(2.0e4), size=b.tri_count))
     b.tri_materials[:, 0] = ke
     b.color(include_bending=True)
     model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
-    info = {"workload": f"cloth_drop_on_sphere_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+    tag = "_springs" if springs else ""
+    info = {"workload": f"cloth_drop_on_sphere_{verts_per_side}x{verts_per_side}{tag}", "dt": FRAME_DT / SUBSTEPS,
             "substeps": SUBSTEPS, "iterations": iterations}
     return model, state, info
 

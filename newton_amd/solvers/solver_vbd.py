@@ -146,6 +146,12 @@ class SolverVBD:
         d.soft_contact_ke = float(m.soft_contact_ke)
         d.soft_contact_kd = float(m.soft_contact_kd)
         d.soft_contact_mu = float(m.soft_contact_mu)
+        d.spring_count = m.spring_count
+        if m.spring_count:
+            d.spring_indices = p(m.spring_indices)
+            d.spring_rest_length = p(m.spring_rest_length)
+            d.spring_stiffness = p(m.spring_stiffness)
+            d.spring_damping = p(m.spring_damping)
         prm = _native.VbdSolverParams()
         prm.iterations = self.iterations
         prm.friction_epsilon = self.friction_epsilon
