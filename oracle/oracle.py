@@ -88,6 +88,33 @@ class _OracleParams(ctypes.Structure):
         ("use_libm_atan2", i32),
         ("fp64_ill_conditioned", i32),
         ("num_threads", i32),
+        ("self_contact", vp),
+    ]
+
+
+class _OracleSelf(ctypes.Structure):
+    _fields_ = [
+        ("radius", f32),
+        ("margin", f32),
+        ("relaxation", f32),
+        ("max_displacement", f32),
+        ("vt_capacity", i32),
+        ("ee_capacity", i32),
+        ("detection_interval", i32),
+        ("edge_parallel_eps", f32),
+        ("rest_exclusion_radius", f32),
+        ("q_rest", vp),
+        ("vt_filter", vp),
+        ("vt_filter_offsets", vp),
+        ("ee_filter", vp),
+        ("ee_filter_offsets", vp),
+        ("vt_list", vp),
+        ("vt_count", vp),
+        ("vt_min_dist", vp),
+        ("tri_min_dist", vp),
+        ("ee_list", vp),
+        ("ee_count", vp),
+        ("ee_min_dist", vp),
     ]
 
 
@@ -111,9 +138,11 @@ def lib():
             build()
         L = ctypes.CDLL(str(LIB_PATH))
         for sfx in ("f32", "f64"):
-            for name in ("step", "create_soft_contacts", "simulate", "residual", "block_solve"):
+            for name in ("step", "create_soft_contacts", "simulate", "residual", "block_solve", "triangle_closest",
+                         "vt_contact", "ee_contact", "self_detect"):
                 getattr(L, f"vbd_oracle_{name}_{sfx}").restype = i32
-            for name in ("membrane", "bending", "tet", "body_contact", "spring"):
+            for name in ("membrane", "bending", "tet", "body_contact", "spring", "closest_edge_edge",
+                         "self_force_norm", "truncation_ts"):
                 getattr(L, f"vbd_oracle_{name}_{sfx}").restype = None
         L.vbd_oracle_atan2_f32.restype = f32
         L.vbd_oracle_atan2_f32.argtypes = [f32, f32]
@@ -260,9 +289,64 @@ class OracleModel:
 
 
 def params(iterations: int, friction_epsilon: float = 1e-2, avbd_beta: float = 0.0, k_start: float = -1.0,
-           use_libm_atan2: bool = False, fp64_ill_conditioned: bool = False, num_threads: int = 1) -> _OracleParams:
-    return _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
-                         int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads))
+           use_libm_atan2: bool = False, fp64_ill_conditioned: bool = False, num_threads: int = 1,
+           self_contact: "SelfContact | None" = None) -> _OracleParams:
+    prm = _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
+                        int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads),
+                        None if self_contact is None else ctypes.addressof(self_contact.struct))
+    prm._self_contact = self_contact  # keeps the settings and their arrays alive
+    return prm
+
+
+class SelfContact:
+    """Cloth self-contact settings (SolverVBD ``particle_*`` arguments) and the outputs of the
+    oracle's last collision detection (triangle / edge lists in ascending primitive order)."""
+
+    def __init__(self, particle_count: int, tri_count: int, edge_count: int, radius: float = 0.2,
+                 margin: float = 0.2, relaxation: float = 0.85, vt_capacity: int = 32, ee_capacity: int = 64,
+                 detection_interval: int = 0, edge_parallel_eps: float = 1e-5, rest_exclusion_radius: float = 0.0,
+                 q_rest=None, vt_filter=None, vt_filter_offsets=None, ee_filter=None, ee_filter_offsets=None):
+        n, t, e = int(particle_count), int(tri_count), int(edge_count)
+        self._keep = []
+
+        def arr(x, dtype):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x), dtype=dtype)
+            self._keep.append(a)
+            return a
+
+        self.q_rest = arr(None if q_rest is None else np.asarray(q_rest).reshape(-1, 3), np.float32)
+        self.vt_filter = arr(vt_filter, np.int32)
+        self.vt_filter_offsets = arr(vt_filter_offsets, np.int32)
+        self.ee_filter = arr(ee_filter, np.int32)
+        self.ee_filter_offsets = arr(ee_filter_offsets, np.int32)
+        self.vt_list = np.full((max(n, 1), vt_capacity), -1, dtype=np.int32)
+        self.vt_count = np.zeros(max(n, 1), dtype=np.int32)
+        self.vt_min_dist = np.zeros(max(n, 1), dtype=np.float64)
+        self.tri_min_dist = np.zeros(max(t, 1), dtype=np.float64)
+        self.ee_list = np.full((max(e, 1), ee_capacity), -1, dtype=np.int32)
+        self.ee_count = np.zeros(max(e, 1), dtype=np.int32)
+        self.ee_min_dist = np.zeros(max(e, 1), dtype=np.float64)
+        # the isotropic clamp is computed by the reference in Python double, then passed as float32
+        max_disp = float(np.float32(float(margin) * float(relaxation) * 0.5))
+        self.struct = _OracleSelf(
+            float(radius), float(margin), float(relaxation), max_disp, int(vt_capacity), int(ee_capacity),
+            int(detection_interval), float(edge_parallel_eps), float(rest_exclusion_radius),
+            *(None if x is None or x.size == 0 else x.ctypes.data
+              for x in (self.q_rest, self.vt_filter, self.vt_filter_offsets, self.ee_filter, self.ee_filter_offsets,
+                        self.vt_list, self.vt_count, self.vt_min_dist, self.tri_min_dist, self.ee_list,
+                        self.ee_count, self.ee_min_dist)))
+
+
+def self_detect(om: "OracleModel", sc: SelfContact, q, precision: str = "f32") -> None:
+    """Vertex-triangle and edge-edge detection at q into sc's output arrays."""
+    rt, _ = _real(precision)
+    q = np.ascontiguousarray(q, dtype=rt).reshape(-1, 3)
+    rc = getattr(lib(), f"vbd_oracle_self_detect_{precision}")(ctypes.byref(om.struct), ctypes.byref(sc.struct),
+                                                                 _p(q))
+    if rc != 0:
+        raise RuntimeError(f"oracle self_detect failed ({rc})")
 
 
 class Contacts:
@@ -435,3 +519,63 @@ def block_solve(h, f, fp64_ill_conditioned=False, precision="f32"):
 
 def atan2(y, x, precision="f32"):
     return getattr(lib(), f"vbd_oracle_atan2_{precision}")(y, x)
+
+
+# ---- self-contact element-level entry points ---------------------------------
+def triangle_closest(a, b, c, p, precision="f64"):
+    """triangle_closest_point: (closest point, barycentrics, feature 0..6)."""
+    rt, _ = _real(precision)
+    x = np.ascontiguousarray(np.stack([a, b, c, p]), dtype=rt)
+    cp, bary = np.zeros(3, dtype=rt), np.zeros(3, dtype=rt)
+    feat = getattr(lib(), f"vbd_oracle_triangle_closest_{precision}")(_p(x), _p(cp), _p(bary))
+    return cp, bary, int(feat)
+
+
+def closest_edge_edge(p1, q1, p2, q2, eps=1e-5, precision="f64"):
+    """(s, t, distance) of the closest points of segments p1q1 and p2q2."""
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(np.stack([p1, q1, p2, q2]), dtype=rt)
+    out = np.zeros(3, dtype=rt)
+    getattr(lib(), f"vbd_oracle_closest_edge_edge_{precision}")(_p(x), ct(eps), _p(out))
+    return out
+
+
+def self_force_norm(dis, radius, k, precision="f64"):
+    rt, ct = _real(precision)
+    out = np.zeros(2, dtype=rt)
+    getattr(lib(), f"vbd_oracle_self_force_norm_{precision}")(ct(dis), ct(radius), ct(k), _p(out))
+    return out
+
+
+def vt_contact(abcp, anchor, radius, ke, kd, mu, friction_epsilon, dt, precision="f64"):
+    """Vertex-triangle contact: (has_contact, forces [4,3] on a, b, c, p, Hessians [4,3,3])."""
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(abcp, dtype=rt).reshape(4, 3)
+    xa = np.ascontiguousarray(anchor, dtype=rt).reshape(4, 3)
+    f, h = np.zeros(12, dtype=rt), np.zeros(36, dtype=rt)
+    has = getattr(lib(), f"vbd_oracle_vt_contact_{precision}")(_p(x), _p(xa), ct(radius), ct(ke), ct(kd), ct(mu),
+                                                               ct(friction_epsilon), ct(dt), _p(f), _p(h))
+    return bool(has), f.reshape(4, 3), h.reshape(4, 3, 3)
+
+
+def ee_contact(x, anchor, radius, ke, kd, mu, friction_epsilon, dt, parallel_eps=1e-5, precision="f64"):
+    """Edge-edge contact on e1's vertices: (has_contact, forces [2,3], Hessians [2,3,3])."""
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(4, 3)
+    xa = np.ascontiguousarray(anchor, dtype=rt).reshape(4, 3)
+    f, h = np.zeros(6, dtype=rt), np.zeros(18, dtype=rt)
+    has = getattr(lib(), f"vbd_oracle_ee_contact_{precision}")(_p(x), _p(xa), ct(radius), ct(ke), ct(kd), ct(mu),
+                                                               ct(friction_epsilon), ct(dt), ct(parallel_eps),
+                                                               _p(f), _p(h))
+    return bool(has), f.reshape(2, 3), h.reshape(2, 3, 3)
+
+
+def truncation_ts(kind, x, delta, parallel_eps=1e-5, gamma=0.85, precision="f64"):
+    """Planar truncation factors of one collision's four vertices (kind 0 edge-edge, 1 vertex-triangle)."""
+    rt, ct = _real(precision)
+    x = np.ascontiguousarray(x, dtype=rt).reshape(4, 3)
+    d = np.ascontiguousarray(delta, dtype=rt).reshape(4, 3)
+    t = np.zeros(4, dtype=rt)
+    getattr(lib(), f"vbd_oracle_truncation_ts_{precision}")(i32(kind), _p(x), _p(d), ct(parallel_eps), ct(gamma),
+                                                            _p(t))
+    return t

@@ -75,6 +75,8 @@ typedef struct OracleModel {
     const float* spring_damping;     /* [S] */
 } OracleModel;
 
+struct OracleSelf;
+
 typedef struct OracleParams {
     int32_t iterations;
     float friction_epsilon;
@@ -83,7 +85,37 @@ typedef struct OracleParams {
     int32_t use_libm_atan2;   /* 1: C library atan2f, 0: portable atan2 (matches the HIP kernels) */
     int32_t fp64_ill_conditioned; /* 1: solve blocks with cond > 1e6 in double */
     int32_t num_threads;      /* OpenMP threads for the per-color vertex loop (<=1: serial) */
+    const struct OracleSelf* self_contact; /* NULL: particle_enable_self_contact = False */
 } OracleParams;
+
+/* Cloth self-contact settings (SolverVBD particle_* arguments, solver_vbd.py:231-242) and the
+ * optional outputs of the last collision detection.  Triangle / edge lists are stored in
+ * ascending primitive order (the reference records them in BVH traversal order; the two agree
+ * whenever a buffer does not overflow). */
+typedef struct OracleSelf {
+    float radius;                    /* particle_self_contact_radius */
+    float margin;                    /* particle_self_contact_margin (detection query radius) */
+    float relaxation;                /* particle_conservative_bound_relaxation (gamma) */
+    float max_displacement;          /* float32(margin * relaxation * 0.5), computed in double */
+    int32_t vt_capacity;             /* particle_vertex_contact_buffer_size */
+    int32_t ee_capacity;             /* particle_edge_contact_buffer_size */
+    int32_t detection_interval;      /* particle_collision_detection_interval */
+    float edge_parallel_eps;         /* particle_edge_parallel_epsilon */
+    float rest_exclusion_radius;     /* particle_rest_shape_contact_exclusion_radius */
+    const float* q_rest;             /* [N*3] model.particle_q (rest-shape exclusion) */
+    const int32_t* vt_filter;        /* per-vertex sorted triangle ids to skip (NULL: none) */
+    const int32_t* vt_filter_offsets;/* [N+1] */
+    const int32_t* ee_filter;        /* per-edge sorted edge ids to skip (NULL: none) */
+    const int32_t* ee_filter_offsets;/* [E+1] */
+    /* outputs of the last detection (each may be NULL) */
+    int32_t* vt_list;                /* [N*vt_capacity], -1 padded */
+    int32_t* vt_count;               /* [N] (may exceed the capacity) */
+    double* vt_min_dist;             /* [N] */
+    double* tri_min_dist;            /* [T] */
+    int32_t* ee_list;                /* [E*ee_capacity], -1 padded */
+    int32_t* ee_count;               /* [E] */
+    double* ee_min_dist;             /* [E] */
+} OracleSelf;
 
 /* Soft-contact pair list for create_soft_contacts: [P*2] (particle, shape). */
 typedef struct OraclePairs {
@@ -128,6 +160,23 @@ typedef struct OraclePairs {
     void vbd_oracle_spring_##SFX(const REAL* x /*6: x0, x1*/, const REAL* anchor /*6*/,        \
                                  REAL rest, REAL ke, REAL kd, REAL dt, REAL* f /*6: f0, f1*/,    \
                                  REAL* h /*9*/);                                                  \
+    int32_t vbd_oracle_triangle_closest_##SFX(const REAL* abcp /*12*/, REAL* closest /*3*/,   \
+                                              REAL* bary /*3*/);                              \
+    void vbd_oracle_closest_edge_edge_##SFX(const REAL* x /*12*/, REAL eps, REAL* std /*3*/);  \
+    void vbd_oracle_self_force_norm_##SFX(REAL dis, REAL radius, REAL k, REAL* out /*2*/);      \
+    int32_t vbd_oracle_vt_contact_##SFX(const REAL* abcp /*12*/, const REAL* anchor /*12*/,     \
+                                        REAL radius, REAL ke, REAL kd, REAL mu,                \
+                                        REAL friction_epsilon, REAL dt, REAL* f /*12*/,         \
+                                        REAL* h /*36*/);                                       \
+    int32_t vbd_oracle_ee_contact_##SFX(const REAL* x /*12*/, const REAL* anchor /*12*/,        \
+                                        REAL radius, REAL ke, REAL kd, REAL mu,                \
+                                        REAL friction_epsilon, REAL dt, REAL parallel_eps,     \
+                                        REAL* f /*6*/, REAL* h /*18*/);                         \
+    void vbd_oracle_truncation_ts_##SFX(int32_t kind, const REAL* x /*12*/,                   \
+                                        const REAL* delta /*12*/, REAL parallel_eps,           \
+                                        REAL gamma, REAL* t /*4*/);                              \
+    int32_t vbd_oracle_self_detect_##SFX(const OracleModel* m, const struct OracleSelf* sc,    \
+                                         const REAL* q);                                       \
     int32_t vbd_oracle_residual_##SFX(const OracleModel* m, const OracleParams* p,             \
                                       int32_t contact_count, int32_t contact_max,              \
                                       const int32_t* c_indices, const int32_t* c_shape,        \

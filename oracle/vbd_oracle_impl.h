@@ -789,6 +789,8 @@ static void accumulate_springs(const OracleModel* m, const REAL* q, const REAL* 
     }
 }
 
+#include "vbd_oracle_self.h"
+
 int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, int32_t contact_count,
                                  int32_t contact_max, const int32_t* c_indices, const int32_t* c_shape,
                                  const REAL* c_body_pos, const REAL* c_body_vel, const REAL* c_normal,
@@ -807,6 +809,14 @@ int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, in
     accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q, q_prev,
                         dt, -1, pf, ph);
     accumulate_springs(m, q, q_prev, dt, -1, pf, ph);
+    if (p->self_contact) {
+        /* self-contact terms of the contact set detected at q */
+        SelfLists L;
+        self_lists_alloc(m, p->self_contact, &L);
+        self_detect(m, p->self_contact, q, &L);
+        accumulate_self_contacts(m, p, &L, q, q_prev, dt, -1, pf, ph);
+        self_lists_free(&L);
+    }
     Ctx c = {m, q, q_prev, dt};
     double acc = 0.0;
     for (int i = 0; i < N; ++i) {
@@ -847,6 +857,20 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
     /* _initialize_rigid_bodies -> init_body_particle_contacts (S:2493-2514) */
     init_contacts(m, p, nc, c_shape, pk, mke, mkd, mmu);
 
+    /* self-contact: detection before initialization (S:2074-2075), positions of the last
+     * detection (pos_cd, the base of the displacements) and truncation factors */
+    const OracleSelf* sc = p->self_contact;
+    SelfLists SL;
+    REAL *pos_cd = NULL, *ts = NULL;
+    if (sc) {
+        self_lists_alloc(m, sc, &SL);
+        pos_cd = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)N);
+        ts = (REAL*)malloc(sizeof(REAL) * (size_t)N);
+        self_detect(m, sc, q, &SL);
+        self_publish(m, sc, &SL);
+        memcpy(pos_cd, q, sizeof(REAL) * 3 * (size_t)N);
+    }
+
     /* _initialize_particles (S:2065-2100): x0 <- q, d <- 0, forward_step (K:1350-1375) */
     memcpy(x0, q, sizeof(REAL) * 3 * (size_t)N);
     for (int i = 0; i < N; ++i) {
@@ -863,8 +887,12 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
         vst(inertia, i, vadd(x, vscale(vnew, dt)));
         vst(disp, i, vscale(vnew, dt));
     }
-    /* apply_truncation_ts with t = 1, max = inf (K:2266-2286): q = x0 + d */
-    for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
+    /* apply_truncation_ts with t = 1, max = inf (K:2266-2286): q = x0 + d; with self-contact the
+     * planar truncation of S:2025-2063 */
+    if (sc)
+        self_truncation(m, sc, &SL, pos_cd, disp, q, ts);
+    else
+        for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
 
     Ctx c = {m, q, q_prev, dt};
     if (residual_hist) {
@@ -872,6 +900,14 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
                                  c_normal, q, q_prev, inertia, dt, NULL, &residual_hist[0]);
     }
     for (int it = 0; it < p->iterations; ++it) {
+        /* self-contact re-detection (S:2540-2545): positions and displacement base reset */
+        if (sc && ((sc->detection_interval == 0 && it == 0) ||
+                   (sc->detection_interval >= 1 && it % sc->detection_interval == 0))) {
+            self_detect(m, sc, q, &SL);
+            self_publish(m, sc, &SL);
+            memcpy(pos_cd, q, sizeof(REAL) * 3 * (size_t)N);
+            memset(disp, 0, sizeof(REAL) * 3 * (size_t)N);
+        }
         /* update_duals_body_particle_contacts (R:4861-4926) */
         for (int t = 0; t < nc; ++t) {
             const int32_t* cr = c_indices + 3 * t;
@@ -889,6 +925,7 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
             accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q,
                                 q_prev, dt, color, pf, ph);
             accumulate_springs(m, q, q_prev, dt, color, pf, ph);
+            if (sc) accumulate_self_contacts(m, p, &SL, q, q_prev, dt, color, pf, ph);
             int cb = m->color_offsets[color], ce = m->color_offsets[color + 1];
 #ifdef _OPENMP
             int nt = p->num_threads > 1 ? p->num_threads : 1;
@@ -907,7 +944,10 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
                 V3 dx;
                 if (block_solve(h, f, p->fp64_ill_conditioned, &dx)) vst(disp, i, vadd(vld(disp, i), dx));
             }
-            for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
+            if (sc)
+                self_truncation(m, sc, &SL, pos_cd, disp, q, ts);
+            else
+                for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
         }
         memcpy(q_out, q, sizeof(REAL) * 3 * (size_t)N);
         if (residual_hist) {
@@ -918,6 +958,11 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
     if (p->iterations <= 0) memcpy(q_out, q, sizeof(REAL) * 3 * (size_t)N);
     /* _finalize_particles -> update_velocity (K:1468-1471) */
     for (int i = 0; i < N; ++i) vst(qd_out, i, vdivs(vsub(vld(q_out, i), vld(q_prev, i)), dt));
+    if (sc) {
+        self_lists_free(&SL);
+        free(pos_cd);
+        free(ts);
+    }
     free(buf);
     free(cbuf);
     return 0;
