@@ -96,6 +96,24 @@ typedef struct VbdSolverParams {
     float k_start;              /* < 0: penalty seeded at the averaged ke */
     int32_t fp64_ill_conditioned; /* 1: blocks with Frobenius condition > 1e6 are re-solved in FP64 */
     int32_t soft_contact_capacity; /* largest soft_contact_max the solver will be given */
+    /* cloth self-contact: SolverVBD(particle_enable_self_contact=True, particle_*) (solver_vbd.py:231-242,
+     * :556-606); all pointers are device arrays, filters may be null */
+    int32_t self_contact;                   /* 0: off */
+    float self_contact_radius;              /* particle_self_contact_radius */
+    float self_contact_margin;              /* particle_self_contact_margin (detection query radius) */
+    float conservative_bound_relaxation;    /* particle_conservative_bound_relaxation */
+    float max_displacement;                 /* float32(margin * relaxation * 0.5) evaluated in double */
+    int32_t vertex_contact_buffer_size;     /* triangles kept per vertex (<= 64) */
+    int32_t edge_contact_buffer_size;       /* edges kept per edge (<= 64) */
+    int32_t collision_detection_interval;   /* < 0: before initialization only; 0: also at iteration 0; n: every n */
+    float edge_parallel_epsilon;
+    float rest_shape_contact_exclusion_radius;
+    const float* particle_q_rest;           /* [N,3] rest positions (model.particle_q at creation) */
+    const int32_t* particle_world;          /* [N], null: one world */
+    const int32_t* vt_filter;               /* sorted triangle ids per vertex */
+    const int32_t* vt_filter_offsets;       /* [N+1] */
+    const int32_t* ee_filter;               /* sorted edge ids per edge */
+    const int32_t* ee_filter_offsets;       /* [E+1] */
 } VbdSolverParams;
 
 /* Contacts.soft_contact_* (newton/_src/sim/contacts.py:327-371). */
@@ -212,6 +230,26 @@ int32_t vbd_iterate_to_residual(VbdContext* ctx, void* stream, float* q, float d
 /* Copies the solver's scratch state of the current substep ([N,3] floats)
  * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */
 int32_t vbd_copy_scratch(VbdContext* ctx, void* stream, int32_t which, float* dst);
+
+/* Cloth self-contact (contexts created with params.self_contact = 1).  With it on, vbd_forward
+ * detects before the forward step and truncates after it, vbd_iteration re-detects on the
+ * collision-detection interval and truncates after every color pass (solver_vbd.py:2003-2063,
+ * :2540-2545, :3221-3237), and the color pass adds each vertex's self-contact terms after its
+ * springs (particle_vbd_kernels.py:1498-1641).
+ *
+ * TriMeshCollisionDetector.refit + vertex_triangle_collision_detection + edge_edge_collision_detection
+ * (tri_mesh_collision.py:604-734) at positions q with the solver's margin; also makes q the base of
+ * the displacements. */
+int32_t vbd_self_detect(VbdContext* ctx, void* stream, const float* q);
+/* Copies the last detection's results into caller buffers (each may be null): per vertex the
+ * colliding triangles [N, vertex buffer size] (-1 padded, ascending), their count and the minimum
+ * distance; per triangle the minimum distance to any colliding vertex; per edge the colliding
+ * edges [E, edge buffer size], count and minimum distance. */
+int32_t vbd_self_copy_lists(VbdContext* ctx, void* stream, int32_t* vt_list, int32_t* vt_count, float* vt_min,
+                            float* tri_min, int32_t* ee_list, int32_t* ee_count, float* ee_min);
+/* Synchronises the device; *grid_overflow = 1 when a primitive spanned more detection-grid cells
+ * than a detection can list (its contacts were not detected). */
+int32_t vbd_self_status(VbdContext* ctx, int32_t* grid_overflow);
 
 /* Soft-contact generation (CollisionPipeline.collide, particle part). */
 int32_t vbd_collide_create(const VbdCollideDesc* desc, VbdCollide** out);

@@ -78,6 +78,22 @@ class VbdSolverParams(ctypes.Structure):
         ("k_start", c_float),
         ("fp64_ill_conditioned", c_int32),
         ("soft_contact_capacity", c_int32),
+        ("self_contact", c_int32),
+        ("self_contact_radius", c_float),
+        ("self_contact_margin", c_float),
+        ("conservative_bound_relaxation", c_float),
+        ("max_displacement", c_float),
+        ("vertex_contact_buffer_size", c_int32),
+        ("edge_contact_buffer_size", c_int32),
+        ("collision_detection_interval", c_int32),
+        ("edge_parallel_epsilon", c_float),
+        ("rest_shape_contact_exclusion_radius", c_float),
+        ("particle_q_rest", c_void_p),
+        ("particle_world", c_void_p),
+        ("vt_filter", c_void_p),
+        ("vt_filter_offsets", c_void_p),
+        ("ee_filter", c_void_p),
+        ("ee_filter_offsets", c_void_p),
     ]
 
 
@@ -152,6 +168,9 @@ HIP_EXPORTS = {
          ctypes.POINTER(ctypes.c_double)],
     ),
     "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
+    "vbd_self_detect": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "vbd_self_copy_lists": (c_int32, [c_void_p, c_void_p] + [c_void_p] * 7),
+    "vbd_self_status": (c_int32, [c_void_p, ctypes.POINTER(c_int32)]),
     "vbd_collide_create": (c_int32, [ctypes.POINTER(VbdCollideDesc), ctypes.POINTER(c_void_p)]),
     "vbd_collide_destroy": (c_int32, [c_void_p]),
     "vbd_collide_generate": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),

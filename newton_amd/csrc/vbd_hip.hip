@@ -646,6 +646,11 @@ struct KParams {
     float* spr_rows;
     const int* slow_spr_off;
     const int* slow_spr;
+    // self-contact rows per color position (vbd_self.h) and whether the pass writes q
+    int has_self;
+    const int* so_off;
+    const float* so_rows;
+    int write_q;
     // per-particle contact lists (contact-index order) and per-contact state
     const int* c_off;
     const int* c_ids;
@@ -766,19 +771,23 @@ __global__ void k_spring_rows(KParams P, const float* __restrict__ q, int j0, in
     store_row(P.spr_rows + 12 * (size_t)j, f, h);
 }
 
+#include "vbd_self.h"
+
 // Finish one vertex: contacts (when not already summed), then the block solve
 // and position update (mode 0) or the force output (mode 1).
 // x0 and d_old are the vertex's q_prev and displacement (read by the caller).
-// With add_contacts the vertex's spring rows (srows, nspr rows in LDS) continue the contact sum.
+// With add_contacts the vertex's spring rows (srows, nspr rows in LDS) and then its self-contact
+// rows (color position self_pos) continue the contact sum.  With self-contact on, the pass writes
+// only the displacement; the truncation pass that follows writes q.
 __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 x0,
                                               V3 d_old, V3 f, M3 h, bool add_contacts, int crec0, int crec1,
-                                              const float* srows, int nspr, float dt, int mode,
+                                              const float* srows, int nspr, int self_pos, float dt, int mode,
                                               float* __restrict__ g_out) {
     if (!active) {
         if (mode == 0) {
             V3 z = v3(0.0f, 0.0f, 0.0f);
             st3(P.disp, v, z);
-            st3(q, v, add(x0, scl(z, 1.0f)));
+            if (P.write_q) st3(q, v, add(x0, scl(z, 1.0f)));
         } else {
             st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
         }
@@ -802,6 +811,16 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
 #pragma unroll
                 for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
         }
+        if (P.has_self && self_pos >= 0) {
+            for (int j = P.so_off[self_pos], je = P.so_off[self_pos + 1]; j < je; ++j) {
+                const float* r = P.so_rows + (size_t)kComp * j;
+                cf = add(cf, v3(r[0], r[1], r[2]));
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
+            }
+        }
         h = madd(h, ch);
         f = add(f, cf);
     }
@@ -813,7 +832,7 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
     if (block_solve(h, f, P.fp64_ill, dx)) {
         V3 d = add(d_old, dx);
         st3(P.disp, v, d);
-        st3(q, v, add(x0, scl(d, 1.0f)));
+        if (P.write_q) st3(q, v, add(x0, scl(d, 1.0f)));
     }
 }
 
@@ -994,6 +1013,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
             float ca = 0.0f;
             for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
             for (int r = q1, re = q1 + s_nspr[cs]; r < re; ++r) ca = ca + s_rows[r * kComp + cc];
+            if (P.has_self) {
+                const int pos = b0.x + cs;
+                for (int r = P.so_off[pos], re = P.so_off[pos + 1]; r < re; ++r)
+                    ca = ca + P.so_rows[(size_t)kComp * r + cc];
+            }
             acc = acc + ca;
         }
         s_acc[cs * kComp + cc] = acc;
@@ -1011,7 +1035,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
         const int q1 = s_row[t] + s_nrow[t];
         finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, !par_contacts, cbeg + s_cb[t],
-                      cbeg + s_cb[t + 1], s_rows + q1 * kComp, s_nspr[t], dt, mode, g_out);
+                      cbeg + s_cb[t + 1], s_rows + q1 * kComp, s_nspr[t], b0.x + t, dt, mode, g_out);
     }
 }
 
@@ -1141,10 +1165,21 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
                     ch = madd(ch, hs);
                 }
             }
+            if (P.has_self) {
+                const int pos = P.slow_pos[slow_base + blockIdx.x];
+                for (int j = P.so_off[pos], je = P.so_off[pos + 1]; j < je; ++j) {
+                    const float* r = P.so_rows + (size_t)kComp * j;
+                    cf = add(cf, v3(r[0], r[1], r[2]));
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
+                }
+            }
             h = madd(h, ch);
             f = add(f, cf);
         }
-        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, nullptr, 0, dt, mode,
+        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, nullptr, 0, -1, dt, mode,
                       g_out);
     }
 }
@@ -1541,6 +1576,17 @@ struct VbdContext {
     int* bat_cinfo = nullptr;  // [B, 2] per-substep contact range of each batch
     int* bat_coff = nullptr;   // [B, max_v + 1] per-substep contact offsets within the batch
     int block = 256;           // color-pass work-group shape (256 or 128 lanes)
+    // cloth self-contact (vbd_self.h)
+    bool self_on = false;
+    SelfParams sp{};
+    int iter = 0;              // iteration index within the current step (collision detection interval)
+    int interval = 0;
+    int* pos_of = nullptr;     // [N] color position of each particle
+    int* sc_tri_world = nullptr;
+    int* sc_edge_world = nullptr;
+    void* sc_cub = nullptr;
+    size_t sc_cub_bytes = 0;
+    std::vector<void*> sc_alloc;  // every self-contact buffer (freed by vbd_destroy)
     int max_v = vbd_layout::kLayoutMaxV;
     int blk_words = vbd_layout::block_words(vbd_layout::kLayoutMaxV);
     int64_t fast_items = 0;
@@ -1658,6 +1704,13 @@ int32_t build_layout(VbdContext* c) {
     c->slow_color_off = L.slow_color_off;
     c->pre_color_off = L.pre_color_off;
     {
+        std::vector<int32_t> pos_of((size_t)hm.N);
+        for (int k = 0; k < hm.N; ++k) pos_of[hm.color_particles[k]] = k;
+        err = h2d(&c->pos_of, pos_of);
+        if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: color positions: " + err);
+        c->sp.pos_of = c->pos_of;
+    }
+    {
         const size_t npre = (size_t)std::max(L.pre_color_off.back(), 1);
         hipError_t e = hipMalloc(&c->spr_rows, sizeof(float) * 12 * npre);
         if (e != hipSuccess) return fail(VBD_ERR_HIP, std::string("vbd_create: spring rows: ") + hipGetErrorString(e));
@@ -1697,6 +1750,218 @@ int32_t build_layout(VbdContext* c) {
     k.spr_rows = c->spr_rows;
     k.slow_spr_off = c->lay_slow_spr_off;
     k.slow_spr = c->lay_slow_spr;
+    return VBD_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// cloth self-contact: buffers, detection pipeline, truncation (kernels in vbd_self.h)
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t sc_malloc(VbdContext* c, T** p, size_t count) {
+    hipError_t e = hipMalloc(p, sizeof(T) * (count > 0 ? count : 1));
+    if (e == hipSuccess) c->sc_alloc.push_back((void*)*p);
+    return e;
+}
+
+inline int next_pow2(long n) {
+    int p = 1;
+    while (p < n && p < (1 << 30)) p <<= 1;
+    return p;
+}
+
+int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* prm) {
+    const int N = m->particle_count, T = m->tri_count, E = m->edge_count;
+    if (prm->vertex_contact_buffer_size < 1 || prm->vertex_contact_buffer_size > 64 ||
+        prm->edge_contact_buffer_size < 1 || prm->edge_contact_buffer_size > 64)
+        return fail(VBD_ERR_ARG, "vbd_create: vertex/edge contact buffer sizes must be in [1, 64]");
+    if (!(prm->self_contact_radius > 0.0f) || prm->self_contact_margin < prm->self_contact_radius)
+        return fail(VBD_ERR_ARG, "vbd_create: self-contact margin must be >= radius > 0");
+    if (T > 0 && (!m->tri_indices)) return fail(VBD_ERR_ARG, "vbd_create: self-contact needs tri_indices");
+    if (E > 0 && (!m->edge_indices)) return fail(VBD_ERR_ARG, "vbd_create: self-contact needs edge_indices");
+    if ((prm->vt_filter == nullptr) != (prm->vt_filter_offsets == nullptr) ||
+        (prm->ee_filter == nullptr) != (prm->ee_filter_offsets == nullptr))
+        return fail(VBD_ERR_ARG, "vbd_create: a contact filter needs both its list and its offsets");
+    const long vt_cap = prm->vertex_contact_buffer_size, ee_cap = prm->edge_contact_buffer_size;
+    const long nslots = (long)E * ee_cap + (long)N * vt_cap;
+    const long it_cap = 2L * E * ee_cap + 4L * N * vt_cap;
+    if (it_cap >= (1L << 31) - 1)
+        return fail(VBD_ERR_ARG, "vbd_create: self-contact buffers exceed 2^31 items; lower the buffer sizes");
+    // per-primitive worlds and whether the grid key can include the world
+    std::vector<int32_t> tri, edge, pw;
+    std::string err = d2h(tri, m->tri_indices, 3 * (size_t)T);
+    if (err.empty()) err = d2h(edge, m->edge_indices, 4 * (size_t)E);
+    if (err.empty() && prm->particle_world) err = d2h(pw, prm->particle_world, (size_t)N);
+    if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: reading the mesh for self-contact: " + err);
+    std::vector<int32_t> tw((size_t)T, 0), ew((size_t)E, 0);
+    bool any_global = false;
+    int W = 1;
+    for (int i = 0; i < (int)pw.size(); ++i) {
+        any_global |= pw[i] < 0;
+        W = std::max(W, pw[i] + 1);
+    }
+    for (int t = 0; t < T; ++t) tw[t] = pw.empty() ? 0 : pw[tri[3 * t]];
+    for (int e = 0; e < E; ++e) ew[e] = pw.empty() ? 0 : pw[edge[4 * e + 2]];
+    if ((err = h2d(&c->sc_tri_world, tw)).size() || (err = h2d(&c->sc_edge_world, ew)).size())
+        return fail(VBD_ERR_HIP, "vbd_create: " + err);
+    c->sc_alloc.push_back(c->sc_tri_world);
+    c->sc_alloc.push_back(c->sc_edge_world);
+
+    SelfParams& S = c->sp;
+    S.N = N;
+    S.T = T;
+    S.E = E;
+    S.W = W;
+    S.world_hash = (!pw.empty() && !any_global && W > 1) ? 1 : 0;
+    S.tri_idx = m->tri_indices;
+    S.edge_idx = m->edge_indices;
+    S.tri_world = c->sc_tri_world;
+    S.edge_world = c->sc_edge_world;
+    S.pworld = prm->particle_world;
+    S.color_particles = m->color_particles;
+    S.radius = prm->self_contact_radius;
+    S.margin = prm->self_contact_margin;
+    S.relax = prm->conservative_bound_relaxation;
+    S.max_disp = prm->max_displacement;
+    S.par_eps = prm->edge_parallel_epsilon;
+    S.rest_excl = prm->rest_shape_contact_exclusion_radius;
+    S.ke = m->soft_contact_ke;
+    S.kd = m->soft_contact_kd;
+    S.mu = m->soft_contact_mu;
+    S.friction_eps = prm->friction_epsilon;
+    S.vt_cap = (int)vt_cap;
+    S.ee_cap = (int)ee_cap;
+    S.q_rest = prm->particle_q_rest;
+    S.vt_filter = prm->vt_filter;
+    S.vt_filter_off = prm->vt_filter_offsets;
+    S.ee_filter = prm->ee_filter;
+    S.ee_filter_off = prm->ee_filter_offsets;
+    const long prims = std::max(std::max(T, E), 1);
+    S.nbucket = next_pow2(std::max(16L * prims, 1024L));
+    S.g_cap = (int)(27L * prims);
+    hipError_t e;
+    if ((e = sc_malloc(c, &S.vt_list, (size_t)N * vt_cap)) != hipSuccess ||
+        (e = sc_malloc(c, &S.vt_count, (size_t)N)) != hipSuccess ||
+        (e = sc_malloc(c, &S.vt_min, (size_t)N)) != hipSuccess || (e = sc_malloc(c, &S.tri_min, (size_t)T)) != hipSuccess ||
+        (e = sc_malloc(c, &S.ee_list, (size_t)E * ee_cap)) != hipSuccess ||
+        (e = sc_malloc(c, &S.ee_count, (size_t)E)) != hipSuccess || (e = sc_malloc(c, &S.ee_min, (size_t)E)) != hipSuccess ||
+        (e = sc_malloc(c, &S.ext, 2)) != hipSuccess || (e = sc_malloc(c, &S.g_count, (size_t)S.nbucket + 1)) != hipSuccess ||
+        (e = sc_malloc(c, &S.g_off, (size_t)S.nbucket + 1)) != hipSuccess ||
+        (e = sc_malloc(c, &S.g_cur, (size_t)S.nbucket)) != hipSuccess ||
+        (e = sc_malloc(c, &S.g_ent, (size_t)S.g_cap)) != hipSuccess || (e = sc_malloc(c, &S.g_overflow, 1)) != hipSuccess ||
+        (e = sc_malloc(c, &S.it_cnt, (size_t)N + 1)) != hipSuccess ||
+        (e = sc_malloc(c, &S.it_off, (size_t)N + 1)) != hipSuccess || (e = sc_malloc(c, &S.it_cur, (size_t)N)) != hipSuccess ||
+        (e = sc_malloc(c, &S.items, (size_t)it_cap)) != hipSuccess ||
+        (e = sc_malloc(c, &S.rows, (size_t)it_cap * 12)) != hipSuccess ||
+        (e = sc_malloc(c, &S.pos_cd, 3 * (size_t)N)) != hipSuccess || (e = sc_malloc(c, &S.trunc, (size_t)N)) != hipSuccess ||
+        (e = sc_malloc(c, &S.coll, (size_t)nslots)) != hipSuccess || (e = sc_malloc(c, &S.ncoll, 1)) != hipSuccess)
+        return fail(VBD_ERR_HIP, std::string("vbd_create: self-contact buffers: ") + hipGetErrorString(e));
+    size_t b1 = 0, b2 = 0;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, b1, S.g_count, S.g_off, S.nbucket + 1)) != hipSuccess ||
+        (e = hipcub::DeviceScan::ExclusiveSum(nullptr, b2, S.it_cnt, S.it_off, N + 1)) != hipSuccess)
+        return fail(VBD_ERR_HIP, std::string("vbd_create: scan sizing: ") + hipGetErrorString(e));
+    c->sc_cub_bytes = std::max(std::max(b1, b2), (size_t)16);
+    if ((e = hipMalloc(&c->sc_cub, c->sc_cub_bytes)) != hipSuccess)
+        return fail(VBD_ERR_HIP, std::string("vbd_create: scan scratch: ") + hipGetErrorString(e));
+    c->sc_alloc.push_back(c->sc_cub);
+    // empty lists until the first detection; truncation factors 1; overflow flag clear
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, 0, S.it_off, 0, N + 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N, 256)), dim3(256), 0, 0, S.vt_count, 0, N);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(std::max(E, 1), 256)), dim3(256), 0, 0, S.ee_count, 0, std::max(E, 1));
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N, 256)), dim3(256), 0, 0, S.trunc, vbd_layout::fbits(1.0f), N);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, S.ncoll, 0, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, S.g_overflow, 0, 1);
+    if ((e = hipDeviceSynchronize()) != hipSuccess)
+        return fail(VBD_ERR_HIP, std::string("vbd_create: self-contact init: ") + hipGetErrorString(e));
+    c->self_on = true;
+    c->interval = prm->collision_detection_interval;
+    KParams& k = c->kp;
+    k.has_self = 1;
+    k.so_off = S.it_off;
+    k.so_rows = S.rows;
+    k.write_q = 0;
+    return VBD_OK;
+}
+
+int32_t sc_scan(VbdContext* c, hipStream_t s, const int* in, int* out, int n) {
+    size_t bytes = c->sc_cub_bytes;
+    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->sc_cub, bytes, in, out, n, s));
+    return VBD_OK;
+}
+
+// TriMeshCollisionDetector.refit + vertex_triangle / edge_edge_collision_detection
+// (solver_vbd.py:3221-3237) at positions q, then the per-color-position item lists
+int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
+    SelfParams& S = c->sp;
+    const int N = S.N, T = S.T, E = S.E;
+    const long nslots = (long)E * S.ee_cap + (long)N * S.vt_cap;
+    const int nb = S.nbucket;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, S.ext, 0, 2);
+    VBD_LAUNCH_CHECK("k_fill_i32");
+    const int np = std::max(std::max(T, E), 1);
+    hipLaunchKernelGGL(k_sc_extents, dim3(blocks_for(np, 256)), dim3(256), 0, s, S, q);
+    VBD_LAUNCH_CHECK("k_sc_extents");
+    for (int mode = 0; mode < 2; ++mode) {
+        const int n = mode == 0 ? T : E;
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, s, S.g_count, 0, nb + 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        if (n > 0) {
+            hipLaunchKernelGGL(k_sc_grid_insert, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, q, mode, 0);
+            VBD_LAUNCH_CHECK("k_sc_grid_insert");
+        }
+        int32_t rc = sc_scan(c, s, S.g_count, S.g_off, nb + 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sc_copy_i32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, S.g_cur, S.g_off, nb);
+        VBD_LAUNCH_CHECK("k_sc_copy_i32");
+        if (n > 0) {
+            hipLaunchKernelGGL(k_sc_grid_insert, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, q, mode, 1);
+            VBD_LAUNCH_CHECK("k_sc_grid_insert");
+        }
+        if (mode == 0) {
+            if (T > 0) {
+                hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(T, 256)), dim3(256), 0, s, S.tri_min,
+                                   vbd_layout::fbits(S.margin), T);
+                VBD_LAUNCH_CHECK("k_fill_i32");
+            }
+            hipLaunchKernelGGL(k_sc_vt_query, dim3(blocks_for(N, 128)), dim3(128), 0, s, S, q);
+            VBD_LAUNCH_CHECK("k_sc_vt_query");
+        } else if (E > 0) {
+            hipLaunchKernelGGL(k_sc_ee_query, dim3(blocks_for(E, 128)), dim3(128), 0, s, S, q);
+            VBD_LAUNCH_CHECK("k_sc_ee_query");
+        }
+    }
+    // item lists per color position
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, S.it_cnt, 0, N + 1);
+    VBD_LAUNCH_CHECK("k_fill_i32");
+    const int gs = (int)std::max(1L, (nslots + 255) / 256);
+    hipLaunchKernelGGL(k_sc_items<false>, dim3(gs), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items");
+    int32_t rc = sc_scan(c, s, S.it_cnt, S.it_off, N + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sc_copy_i32, dim3(blocks_for(N, 256)), dim3(256), 0, s, S.it_cur, S.it_off, N);
+    VBD_LAUNCH_CHECK("k_sc_copy_i32");
+    hipLaunchKernelGGL(k_sc_items<true>, dim3(gs), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items");
+    hipLaunchKernelGGL(k_sc_items_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items_sort");
+    // compact collision list for the truncation
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, S.ncoll, 0, 1);
+    VBD_LAUNCH_CHECK("k_fill_i32");
+    hipLaunchKernelGGL(k_sc_collect, dim3(gs), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_collect");
+    // displacements are measured from the positions of the last detection
+    hipLaunchKernelGGL(k_sc_copy3, dim3(blocks_for(3L * N, 256)), dim3(256), 0, s, S.pos_cd, q, N);
+    VBD_LAUNCH_CHECK("k_sc_copy3");
+    return VBD_OK;
+}
+
+// _penetration_free_truncation with self-contact (solver_vbd.py:2025-2063)
+int32_t self_truncate(VbdContext* c, hipStream_t s, float* q) {
+    SelfParams& S = c->sp;
+    hipLaunchKernelGGL(k_sc_trunc_collisions, dim3(kSelfGrid), dim3(256), 0, s, S, c->kp.disp);
+    VBD_LAUNCH_CHECK("k_sc_trunc_collisions");
+    hipLaunchKernelGGL(k_sc_trunc_apply, dim3(blocks_for(S.N, 256)), dim3(256), 0, s, S, c->kp.disp, q);
+    VBD_LAUNCH_CHECK("k_sc_trunc_apply");
     return VBD_OK;
 }
 
@@ -1814,7 +2079,9 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.shape_count = m->shape_count;
     k.friction_epsilon = prm->friction_epsilon;
     k.fp64_ill = prm->fp64_ill_conditioned;
+    k.write_q = 1;
     int32_t rc = build_layout(c);
+    if (rc == VBD_OK && prm->self_contact) rc = setup_self(c, m, prm);
     if (rc) {
         std::string msg = g_last_error;
         vbd_destroy(c);
@@ -1847,7 +2114,43 @@ int32_t vbd_destroy(VbdContext* c) {
     if (c->r_host) hipHostFree(c->r_host);
     hipFree(c->cub_tmp);
     free_layout(c);
+    hipFree(c->pos_of);
+    for (void* p : c->sc_alloc) hipFree(p);
     delete c;
+    return VBD_OK;
+}
+
+int32_t vbd_self_detect(VbdContext* c, void* stream, const float* q) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_self_detect: null argument");
+    if (!c->self_on) return fail(VBD_ERR_STATE, "vbd_self_detect: self-contact is off");
+    return self_detect(c, (hipStream_t)stream, q);
+}
+
+int32_t vbd_self_copy_lists(VbdContext* c, void* stream, int32_t* vt_list, int32_t* vt_count, float* vt_min,
+                            float* tri_min, int32_t* ee_list, int32_t* ee_count, float* ee_min) {
+    if (!c) return fail(VBD_ERR_ARG, "vbd_self_copy_lists: null context");
+    if (!c->self_on) return fail(VBD_ERR_STATE, "vbd_self_copy_lists: self-contact is off");
+    hipStream_t s = (hipStream_t)stream;
+    const SelfParams& S = c->sp;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return (dst && bytes) ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) : hipSuccess;
+    };
+    VBD_HIP_TRY(cp(vt_list, S.vt_list, sizeof(int) * (size_t)S.N * S.vt_cap));
+    VBD_HIP_TRY(cp(vt_count, S.vt_count, sizeof(int) * (size_t)S.N));
+    VBD_HIP_TRY(cp(vt_min, S.vt_min, sizeof(float) * (size_t)S.N));
+    VBD_HIP_TRY(cp(tri_min, S.tri_min, sizeof(float) * (size_t)S.T));
+    VBD_HIP_TRY(cp(ee_list, S.ee_list, sizeof(int) * (size_t)S.E * S.ee_cap));
+    VBD_HIP_TRY(cp(ee_count, S.ee_count, sizeof(int) * (size_t)S.E));
+    VBD_HIP_TRY(cp(ee_min, S.ee_min, sizeof(float) * (size_t)S.E));
+    return VBD_OK;
+}
+
+int32_t vbd_self_status(VbdContext* c, int32_t* grid_overflow) {
+    if (!c || !grid_overflow) return fail(VBD_ERR_ARG, "vbd_self_status: null argument");
+    *grid_overflow = 0;
+    if (!c->self_on) return VBD_OK;
+    VBD_HIP_TRY(hipDeviceSynchronize());
+    VBD_HIP_TRY(hipMemcpy(grid_overflow, c->sp.g_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
     return VBD_OK;
 }
 
@@ -1954,9 +2257,16 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
 int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, const float* f_ext, float dt) {
     if (!c || !q || !qd) return fail(VBD_ERR_ARG, "vbd_forward: null argument");
     hipStream_t s = (hipStream_t)stream;
+    // with self-contact: detection before initialization (solver_vbd.py:2074-2075)
+    if (c->self_on) {
+        int32_t rc = self_detect(c, s, q);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_forward, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0, s, c->kp, q, qd, f_ext,
                        dt);
     VBD_LAUNCH_CHECK("k_forward");
+    c->iter = 0;
+    if (c->self_on) return self_truncate(c, s, q);
     return VBD_OK;
 }
 
@@ -1981,6 +2291,11 @@ int32_t launch_spring_rows(VbdContext* c, hipStream_t s, int j0, int j1, const f
 int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt, int mode, float* g) {
     int32_t rc0 = launch_spring_rows(c, s, c->pre_color_off[color], c->pre_color_off[color + 1], q, dt);
     if (rc0) return rc0;
+    if (c->self_on) {
+        hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev,
+                           c->color_offsets[color], c->color_offsets[color + 1], dt);
+        VBD_LAUNCH_CHECK("k_sc_rows");
+    }
     int b0 = c->fast_color_off[color], b1 = c->fast_color_off[color + 1];
     if (b1 > b0) {
         int32_t rc = launch_fast(c, s, b0, b1 - b0, q, dt, mode, g);
@@ -1998,12 +2313,22 @@ int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt
 int32_t vbd_color_pass(VbdContext* c, void* stream, int32_t color, float* q, float dt) {
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_color_pass: null argument");
     if (color < 0 || color >= c->model.color_count) return fail(VBD_ERR_ARG, "vbd_color_pass: color out of range");
-    return launch_color(c, (hipStream_t)stream, color, q, dt, 0, nullptr);
+    int32_t rc = launch_color(c, (hipStream_t)stream, color, q, dt, 0, nullptr);
+    if (rc == VBD_OK && c->self_on) rc = self_truncate(c, (hipStream_t)stream, q);
+    return rc;
 }
 
 int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_iteration: null argument");
     hipStream_t s = (hipStream_t)stream;
+    // self-contact re-detection (solver_vbd.py:2540-2545); displacements restart from the new base
+    if (c->self_on && ((c->interval == 0 && c->iter == 0) || (c->interval >= 1 && c->iter % c->interval == 0))) {
+        int32_t rc = self_detect(c, s, q);
+        if (rc) return rc;
+        const int n3 = 3 * c->model.particle_count;
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n3, 256)), dim3(256), 0, s, (int*)c->kp.disp, 0, n3);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+    }
     if (c->have_contacts && c->params.avbd_beta != 0.0f) {
         const VbdContactsDesc& cd = c->contacts;
         hipLaunchKernelGGL(k_dual_update, dim3(blocks_for(cd.soft_contact_max, 256)), dim3(256), 0, s,
@@ -2016,7 +2341,12 @@ int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
     for (int color = 0; color < c->model.color_count; ++color) {
         int32_t rc = launch_color(c, s, color, q, dt, 0, nullptr);
         if (rc) return rc;
+        if (c->self_on) {
+            rc = self_truncate(c, s, q);
+            if (rc) return rc;
+        }
     }
+    ++c->iter;
     return VBD_OK;
 }
 
@@ -2108,6 +2438,11 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
     float* g = g_out ? g_out : c->g_tmp;
     int32_t rc0 = launch_spring_rows(c, s, 0, c->pre_color_off.back(), q, dt);
     if (rc0) return rc0;
+    if (c->self_on) {
+        hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev, 0,
+                           c->model.particle_count, dt);
+        VBD_LAUNCH_CHECK("k_sc_rows");
+    }
     int nb = c->fast_color_off.back();
     if (nb > 0) {
         int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g);
@@ -2131,6 +2466,7 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
 
 int32_t vbd_set_iterate(VbdContext* c, void* stream, float* q_dst, const float* guess, int32_t commit) {
     if (!c || !q_dst || !guess) return fail(VBD_ERR_ARG, "vbd_set_iterate: null argument");
+    if (c->self_on) return fail(VBD_ERR_STATE, "vbd_set_iterate: not available with self-contact");
     hipLaunchKernelGGL(k_set_iterate, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0,
                        (hipStream_t)stream, c->kp, q_dst, guess, commit ? 1 : 0);
     VBD_LAUNCH_CHECK("k_set_iterate");

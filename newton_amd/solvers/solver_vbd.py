@@ -49,7 +49,18 @@ class SolverVBD:
         friction_epsilon: float = 1e-2,
         integrate_with_external_rigid_solver: bool = False,
         particle_enable_self_contact: bool = False,
+        particle_self_contact_radius: float = 0.2,
+        particle_self_contact_margin: float = 0.2,
+        particle_conservative_bound_relaxation: float = 0.85,
+        particle_vertex_contact_buffer_size: int = 32,
+        particle_edge_contact_buffer_size: int = 64,
+        particle_collision_detection_interval: int = 0,
+        particle_edge_parallel_epsilon: float = 1e-5,
         particle_enable_tile_solve: bool = True,
+        particle_topological_contact_filter_threshold: int = 2,
+        particle_rest_shape_contact_exclusion_radius: float = 0.0,
+        particle_external_vertex_contact_filtering_map: dict | None = None,
+        particle_external_edge_contact_filtering_map: dict | None = None,
         rigid_avbd_beta: float = 0.0,
         rigid_avbd_linear_beta: float | None = None,
         rigid_contact_k_start: float = 1.0e2,
@@ -64,8 +75,12 @@ class SolverVBD:
             raise ValueError(f"rigid_avbd_linear_beta must be >= 0, got {linear_beta}")
         if rigid_contact_k_start < 0:
             raise ValueError(f"rigid_contact_k_start must be >= 0, got {rigid_contact_k_start}")
-        if particle_enable_self_contact:
-            raise NotImplementedError("particle self-contact is not part of this VBD path")
+        if particle_enable_self_contact and particle_self_contact_margin < particle_self_contact_radius:
+            raise ValueError(
+                "particle_self_contact_margin is smaller than particle_self_contact_radius, this will result in "
+                "missing contacts and cause instability.\nIt is advisable to make particle_self_contact_margin "
+                "1.5-2 times larger than particle_self_contact_radius."
+            )
         for key in unsupported:
             if not (key.startswith("particle_") or key.startswith("rigid_") or key == "deterministic"):
                 raise TypeError(f"SolverVBD got an unexpected keyword argument {key!r}")
@@ -75,7 +90,18 @@ class SolverVBD:
         self.iterations = int(iterations)
         self.friction_epsilon = float(friction_epsilon)
         self.integrate_with_external_rigid_solver = integrate_with_external_rigid_solver
-        self.particle_enable_self_contact = False
+        self.particle_enable_self_contact = bool(particle_enable_self_contact)
+        self.particle_self_contact_radius = float(particle_self_contact_radius)
+        self.particle_self_contact_margin = float(particle_self_contact_margin)
+        self.particle_conservative_bound_relaxation = float(particle_conservative_bound_relaxation)
+        self.particle_vertex_contact_buffer_size = int(particle_vertex_contact_buffer_size)
+        self.particle_edge_contact_buffer_size = int(particle_edge_contact_buffer_size)
+        self.particle_collision_detection_interval = int(particle_collision_detection_interval)
+        self.particle_edge_parallel_epsilon = float(particle_edge_parallel_epsilon)
+        self.particle_topological_contact_filter_threshold = int(particle_topological_contact_filter_threshold)
+        self.particle_rest_shape_contact_exclusion_radius = float(particle_rest_shape_contact_exclusion_radius)
+        self._external_vt_map = particle_external_vertex_contact_filtering_map
+        self._external_ee_map = particle_external_edge_contact_filtering_map
         self.particle_enable_tile_solve = particle_enable_tile_solve
         self.rigid_linear_beta = float(linear_beta)
         # fixed penalty (k = averaged ke) when the linear beta is 0 (solver_vbd.py:671)
@@ -101,7 +127,29 @@ class SolverVBD:
         if soft_contact_capacity is None:
             soft_contact_capacity = model.particle_count * max(model.shape_count, 1)
         self._capacity = int(soft_contact_capacity)
+        if self.particle_enable_self_contact:
+            self._prepare_self_contact()
         self._create_context()
+
+    def _prepare_self_contact(self):
+        """Rest positions and topological contact filters (``solver_vbd.py:556-606``,
+        ``tri_mesh_collision.py:289-313``), uploaded once."""
+        from .tri_mesh_collision import build_filters
+
+        m = self.model
+        dev = m.device
+        self._q_rest = m.particle_q.detach().clone().contiguous()
+        vt, ee = build_filters(m, self.particle_topological_contact_filter_threshold, self._external_vt_map,
+                               self._external_ee_map)
+
+        def up(a):
+            return None if a is None else torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+
+        self._vt_filter = None if vt is None else (up(vt[0]), up(vt[1]))
+        self._ee_filter = None if ee is None else (up(ee[0]), up(ee[1]))
+        self._particle_world = None
+        if getattr(m, "particle_world", None) is not None and int(getattr(m, "world_count", 1)) > 1:
+            self._particle_world = m.particle_world.to(torch.int32).contiguous()
 
     # ------------------------------------------------------------- internals
     def _create_context(self):
@@ -159,6 +207,25 @@ class SolverVBD:
         prm.k_start = self.rigid_contact_k_start_value
         prm.fp64_ill_conditioned = int(self.fp64_ill_conditioned)
         prm.soft_contact_capacity = self._capacity
+        if self.particle_enable_self_contact:
+            prm.self_contact = 1
+            prm.self_contact_radius = self.particle_self_contact_radius
+            prm.self_contact_margin = self.particle_self_contact_margin
+            prm.conservative_bound_relaxation = self.particle_conservative_bound_relaxation
+            # the reference multiplies the Python floats in double, then launches with float32
+            prm.max_displacement = float(np.float32(self.particle_self_contact_margin
+                                                    * self.particle_conservative_bound_relaxation * 0.5))
+            prm.vertex_contact_buffer_size = self.particle_vertex_contact_buffer_size
+            prm.edge_contact_buffer_size = self.particle_edge_contact_buffer_size
+            prm.collision_detection_interval = self.particle_collision_detection_interval
+            prm.edge_parallel_epsilon = self.particle_edge_parallel_epsilon
+            prm.rest_shape_contact_exclusion_radius = self.particle_rest_shape_contact_exclusion_radius
+            prm.particle_q_rest = p(self._q_rest)
+            prm.particle_world = p(self._particle_world)
+            if self._vt_filter is not None and self._vt_filter[0].numel():
+                prm.vt_filter, prm.vt_filter_offsets = p(self._vt_filter[0]), p(self._vt_filter[1])
+            if self._ee_filter is not None and self._ee_filter[0].numel():
+                prm.ee_filter, prm.ee_filter_offsets = p(self._ee_filter[0]), p(self._ee_filter[1])
         self._desc, self._params = d, prm
         lib = _native.hip_lib()
         h = ctypes.c_void_p()
@@ -290,6 +357,48 @@ class SolverVBD:
             "vbd_residual",
         )
         return out
+
+    # ------------------------------------------------------------ self-contact
+    def collision_detection(self, state: State) -> None:
+        """Self-contact detection at ``state.particle_q`` (``solver_vbd.py:3221-3237``); the positions
+        become the base of the displacements, as in the reference."""
+        if not self.particle_enable_self_contact:
+            raise RuntimeError("self-contact is off (particle_enable_self_contact=False)")
+        _native.check(_native.hip_lib().vbd_self_detect(self._ctx, current_stream_handle(self.model.device),
+                                                        _native.ptr(state.particle_q)), "vbd_self_detect")
+
+    def collision_info(self) -> dict:
+        """The last detection's lists as device tensors: ``vertex_colliding_triangles`` [N, vertex buffer]
+        (-1 padded, ascending), ``..._count``, ``..._min_dist``, ``triangle_colliding_vertices_min_dist``,
+        ``edge_colliding_edges`` [E, edge buffer], ``..._count``, ``..._min_dist``."""
+        if not self.particle_enable_self_contact:
+            raise RuntimeError("self-contact is off (particle_enable_self_contact=False)")
+        m = self.model
+        dev = m.device
+        i32, f32 = torch.int32, torch.float32
+        out = {
+            "vertex_colliding_triangles": torch.empty((m.particle_count, self.particle_vertex_contact_buffer_size),
+                                                      dtype=i32, device=dev),
+            "vertex_colliding_triangles_count": torch.empty(m.particle_count, dtype=i32, device=dev),
+            "vertex_colliding_triangles_min_dist": torch.empty(m.particle_count, dtype=f32, device=dev),
+            "triangle_colliding_vertices_min_dist": torch.empty(m.tri_count, dtype=f32, device=dev),
+            "edge_colliding_edges": torch.empty((m.edge_count, self.particle_edge_contact_buffer_size), dtype=i32,
+                                                device=dev),
+            "edge_colliding_edges_count": torch.empty(m.edge_count, dtype=i32, device=dev),
+            "edge_colliding_edges_min_dist": torch.empty(m.edge_count, dtype=f32, device=dev),
+        }
+        _native.check(
+            _native.hip_lib().vbd_self_copy_lists(self._ctx, current_stream_handle(dev),
+                                                  *[_native.ptr(t) for t in out.values()]),
+            "vbd_self_copy_lists",
+        )
+        return out
+
+    def self_contact_overflow(self) -> bool:
+        """True when a primitive spanned more detection-grid cells than a detection lists (synchronises)."""
+        flag = ctypes.c_int32(0)
+        _native.check(_native.hip_lib().vbd_self_status(self._ctx, ctypes.byref(flag)), "vbd_self_status")
+        return bool(flag.value)
 
     def set_iterate(self, state_in: State, guess: torch.Tensor, commit: bool = True,
                     out: torch.Tensor | None = None) -> torch.Tensor:
