@@ -13,6 +13,7 @@ The same line carries
   iterations) and the 512 tetrahedral cubes (~0.59M vertices, 8 iterations),
   each one graph-captured frame per step with its own color-pass roofline;
 * ``springs``: the headline cloth with a spring on every mesh edge and hinge;
+* ``self_contact``: two stacked 128x128 sheets draping a sphere with cloth self-contact;
 * ``residual``: the 128x128 wind cloth driven to a relative force residual,
   plain VBD (inertial guess), the previous substep's correction carried over
   ("reuse", no network) and the learned warm start (trained on the GPU in the
@@ -65,6 +66,10 @@ def parse():
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
+    ap.add_argument("--self-contact-steps", type=int, default=10,
+                    help="timed frames of the stacked-sheets self-contact drape (0 disables)")
+    ap.add_argument("--self-contact-warm", type=int, default=20, help="frames before timing (sheets land)")
+    ap.add_argument("--self-contact-verts", type=int, default=128, help="vertices per side of each sheet")
     ap.add_argument("--spring-steps", type=int, default=10,
                     help="timed frames of the headline cloth with springs on every edge and hinge (0 disables)")
     return ap.parse_args()
@@ -309,6 +314,68 @@ def batched_leg(args, name="batched", steps=None):
     }
 
 
+def self_contact_leg(args):
+    """Two stacked cloth sheets draping a sphere with self-contact on: warm-up frames until the sheets
+    touch, then graph-captured frames; detection and color-pass (rows + solve + truncation) times from
+    HIP events, and the live contact counts."""
+    import numpy as np
+    import torch
+
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.solvers import SolverVBD
+
+    dev = "cuda:0"
+    model, state, info = scenes.cloth_layers_on_sphere(dev, verts_per_side=args.self_contact_verts)
+    solver = SolverVBD(model, iterations=info["iterations"], **info["solver_kwargs"])
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    n = model.particle_count
+    gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, args.self_contact_steps,
+                                             args.self_contact_warm)
+    ci = solver.collision_info()
+    vt = int(ci["vertex_colliding_triangles_count"].sum().item())
+    ee = int(ci["edge_colliding_edges_count"].sum().item())
+    dt, reps = info["dt"], 10
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        last.clear_forces()
+        pipe.collide(last, contacts)
+        solver.begin_step(last, contacts, dt)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        solver.collision_detection(last)
+        e[0].record(stream)
+        for _ in range(reps):
+            solver.collision_detection(last)
+        e[1].record(stream)
+        for _ in range(reps):
+            for c in range(len(model.particle_color_groups)):
+                solver.color_pass(last, c, dt)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+    return {
+        "workload": info["workload"],
+        "vertices": n,
+        "triangles": model.tri_count,
+        "edges": model.edge_count,
+        "colors": len(model.particle_color_groups),
+        "iterations": info["iterations"],
+        "substeps_per_step": info["substeps"],
+        "self_contact": {k: v for k, v in info["solver_kwargs"].items() if k != "particle_enable_self_contact"},
+        "warmup_frames": args.self_contact_warm,
+        "steps": args.self_contact_steps,
+        "ms_per_step": round(gpu_ms / args.self_contact_steps, 3),
+        "vertex_steps_per_s": round(n * info["substeps"] * args.self_contact_steps / (gpu_ms / 1e3), 1),
+        "vertex_triangle_contacts": vt,
+        "edge_edge_contacts": ee,
+        "detection_ms": round(e[0].elapsed_time(e[1]) / reps, 4),
+        "iteration_ms": round(e[1].elapsed_time(e[2]) / reps, 4),
+        "grid_overflow": solver.self_contact_overflow(),
+        "hip_graph": True,
+        "finite": finite,
+    }
+
+
 def residual_leg(args):
     """Wind cloth driven to a relative force residual: plain VBD vs the learned warm start.
 
@@ -550,6 +617,12 @@ def main():
         springs = batched_leg(args, "springs", args.spring_steps)
         _progress(f"springs: {springs['vertex_steps_per_s']:.4g} vertex-steps/s")
     torch.cuda.empty_cache()
+    self_contact = None
+    if args.self_contact_steps > 0:
+        _progress("self-contact leg: stacked sheets on a sphere")
+        self_contact = self_contact_leg(args)
+        _progress(f"self-contact: {self_contact['vertex_steps_per_s']:.4g} vertex-steps/s")
+    torch.cuda.empty_cache()
     residual = None
     if args.residual_frames > 0:
         _progress("residual leg: data generation, training, evaluation")
@@ -590,6 +663,7 @@ def main():
         "batched": batched,
         "batched_tets": cubes,
         "springs": springs,
+        "self_contact": self_contact,
         "residual": residual,
         "cpu_baseline": cpu,
     }

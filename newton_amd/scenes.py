@@ -330,4 +330,55 @@ def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 
 
 
 __all__ = ["FRAME_DT", "SUBSTEPS", "FourierWind", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging",
-           "cloth_strip_pinned", "cloth_wind", "soft_cube", "soft_cubes_batched"]
+           "cloth_strip_pinned", "cloth_wind", "soft_cube", "soft_cubes_batched", "cloth_hanging_self_contact",
+           "cloth_layers_on_sphere"]
+
+
+def cloth_hanging_self_contact(device="cuda:0", width: int = 64, height: int = 32, iterations: int = 10):
+    """The reference's hanging-cloth example with VBD self-contact (newton/examples/cloth/example_cloth_hanging.py).
+
+    A width x height cell grid (cell 0.1 m, 0.1 kg per particle, left edge fixed) at z = 4 m (Z up),
+    tri_ke = tri_ka = 1e3, tri_kd = 1e2, edge_ke = 10, particle radius 0.05; ground plane with the
+    default shape material; soft contact ke 1e2, kd 1e2, mu 1.0; self-contact radius 0.02, margin 0.03;
+    60 fps, 10 substeps, 10 iterations.
+    """
+    b = ModelBuilder(up_axis="Z", gravity=-9.81)
+    b.add_ground_plane()
+    b.add_cloth_grid(pos=(0.0, 0.0, 4.0), rot=quat_from_axis_angle((0.0, 0.0, 1.0), math.pi * 0.5),
+                     vel=(0.0, 0.0, 0.0), dim_x=width, dim_y=height, cell_x=0.1, cell_y=0.1, mass=0.1,
+                     fix_left=True, edge_ke=1.0e1, edge_kd=0.0, particle_radius=0.05, tri_ke=1.0e3, tri_ka=1.0e3,
+                     tri_kd=1.0e2)
+    b.color(include_bending=True)
+    model = b.finalize(device)
+    model.soft_contact_ke, model.soft_contact_kd, model.soft_contact_mu = 1.0e2, 1.0e2, 1.0
+    info = {"workload": f"cloth_hanging_self_contact_{width}x{height}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations,
+            "solver_kwargs": {"particle_enable_self_contact": True, "particle_self_contact_radius": 0.02,
+                              "particle_self_contact_margin": 0.03}}
+    return model, model.state(), info
+
+
+def cloth_layers_on_sphere(device="cuda:0", verts_per_side: int = 128, layers: int = 2, gap: float = 0.008,
+                           size: float = 2.0, height: float = 1.0, seed: int = 1, iterations: int = 10):
+    """Stacked cloth sheets (the drop-on-sphere material) falling onto a sphere and the ground with
+    self-contact: the sheets land on each other and fold over the sphere, so vertex-triangle and
+    edge-edge contacts are everywhere.  Self-contact radius 0.4 x the gap, margin 0.75 x the gap."""
+    n = verts_per_side - 1
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    b.add_shape_sphere(-1, radius=0.4, cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.18))
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
+    for k in range(layers):
+        b.add_cloth_mesh(pos=(-0.5 * size + 0.3 * gap * k, height + gap * k, -0.5 * size + 0.2 * gap * k), rot=rot,
+                         scale=1.0, vel=(0.0, 0.0, 0.0), vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4,
+                         tri_ka=1.0e4, tri_kd=1.0e-5, edge_ke=1.0, edge_kd=0.0)
+    rng = np.random.default_rng(seed)
+    b.tri_materials[:, 0] = np.exp(rng.uniform(math.log(5.0e3), math.log(2.0e4), size=b.tri_count))
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+    info = {"workload": f"cloth_layers_on_sphere_{layers}x{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations,
+            "solver_kwargs": {"particle_enable_self_contact": True, "particle_self_contact_radius": 0.4 * gap,
+                              "particle_self_contact_margin": 0.75 * gap}}
+    return model, state, info

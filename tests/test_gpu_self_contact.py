@@ -179,3 +179,49 @@ def test_self_contact_residual_matches_oracle(cuda_device):
                                    dt, precision="f32")
     np.testing.assert_array_equal(g.cpu().numpy(), g_ref)
     assert abs(float(r2.item()) ** 0.5 - n_ref) <= 1e-6 * max(n_ref, 1.0)
+
+
+def test_graph_capture_replay_matches_oracle(cuda_device):
+    """A captured substep with self-contact (detection, scans, rows, truncation) replayed several times,
+    eager work and synchronisations between replays; every replay equals the oracle's substep."""
+    import torch
+
+    from newton_amd.solvers import SolverVBD
+
+    model = _layers(cuda_device)
+    dt = 1.0 / 600.0
+    solver = SolverVBD(model, iterations=3, particle_enable_self_contact=True, **SC)
+    s0, s1 = model.state(), model.state()
+
+    def substep():
+        solver.step(s0, s1, None, None, dt)
+        s0.particle_q.copy_(s1.particle_q)
+        s0.particle_qd.copy_(s1.particle_qd)
+
+    q_init, qd_init = s0.particle_q.clone(), s0.particle_qd.clone()
+    substep()  # warm-up (eager) step
+    torch.cuda.synchronize()
+    s0.particle_q.copy_(q_init)
+    s0.particle_qd.copy_(qd_init)
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(graph, stream=stream):
+            substep()
+    torch.cuda.synchronize()
+    s0.particle_q.copy_(q_init)
+    s0.particle_qd.copy_(qd_init)
+    torch.cuda.synchronize()
+    replays = []
+    for _ in range(6):
+        graph.replay()
+        torch.cuda.synchronize()
+        _ = (s1.particle_q * 2.0).sum().item()  # unrelated eager work between replays
+        replays.append(s0.particle_q.cpu().numpy().copy())
+    om = oracle.OracleModel(model)
+    prm = oracle.params(3, self_contact=_oracle_self(model, SC))
+    q, qd = q_init.cpu().numpy(), qd_init.cpu().numpy()
+    for k in range(6):
+        _, q, qd = oracle.step(om, prm, q, qd, None, None, dt)
+        np.testing.assert_array_equal(replays[k], q)
