@@ -1793,6 +1793,18 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
     if (err.empty()) err = d2h(edge, m->edge_indices, 4 * (size_t)E);
     if (err.empty() && prm->particle_world) err = d2h(pw, prm->particle_world, (size_t)N);
     if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: reading the mesh for self-contact: " + err);
+    if (E > 0) {
+        // edge-edge items are listed by walking each particle's edge CSR, which must be ascending
+        std::vector<int32_t> off, adj;
+        err = d2h(off, m->v_adj_edges_offsets, (size_t)N + 1);
+        if (err.empty()) err = d2h(adj, m->v_adj_edges, (size_t)off[N]);
+        if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: reading the edge adjacency: " + err);
+        for (int v = 0; v < N; ++v)
+            for (int c2 = off[v] + 2; c2 < off[v + 1]; c2 += 2)
+                if (adj[c2] <= adj[c2 - 2])
+                    return fail(VBD_ERR_ARG, "vbd_create: self-contact needs each particle's edge adjacency in "
+                                             "ascending edge order");
+    }
     std::vector<int32_t> tw((size_t)T, 0), ew((size_t)E, 0);
     bool any_global = false;
     int W = 1;
@@ -1832,6 +1844,8 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
     S.vt_cap = (int)vt_cap;
     S.ee_cap = (int)ee_cap;
     S.q_rest = prm->particle_q_rest;
+    S.adj_edge = m->v_adj_edges;
+    S.adj_edge_off = m->v_adj_edges_offsets;
     S.vt_filter = prm->vt_filter;
     S.vt_filter_off = prm->vt_filter_offsets;
     S.ee_filter = prm->ee_filter;
@@ -1851,6 +1865,7 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
         (e = sc_malloc(c, &S.g_ent, (size_t)S.g_cap)) != hipSuccess || (e = sc_malloc(c, &S.g_overflow, 1)) != hipSuccess ||
         (e = sc_malloc(c, &S.it_cnt, (size_t)N + 1)) != hipSuccess ||
         (e = sc_malloc(c, &S.it_off, (size_t)N + 1)) != hipSuccess || (e = sc_malloc(c, &S.it_cur, (size_t)N)) != hipSuccess ||
+        (e = sc_malloc(c, &S.ee_n, (size_t)N)) != hipSuccess ||
         (e = sc_malloc(c, &S.items, (size_t)it_cap)) != hipSuccess ||
         (e = sc_malloc(c, &S.rows, (size_t)it_cap * 12)) != hipSuccess ||
         (e = sc_malloc(c, &S.pos_cd, 3 * (size_t)N)) != hipSuccess || (e = sc_malloc(c, &S.trunc, (size_t)N)) != hipSuccess ||
@@ -1923,25 +1938,28 @@ int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
                                    vbd_layout::fbits(S.margin), T);
                 VBD_LAUNCH_CHECK("k_fill_i32");
             }
-            hipLaunchKernelGGL(k_sc_vt_query, dim3(blocks_for(N, 128)), dim3(128), 0, s, S, q);
+            hipLaunchKernelGGL(k_sc_vt_query, dim3(blocks_for(N, kQPerBlock)), dim3(256), 0, s, S, q);
             VBD_LAUNCH_CHECK("k_sc_vt_query");
         } else if (E > 0) {
-            hipLaunchKernelGGL(k_sc_ee_query, dim3(blocks_for(E, 128)), dim3(128), 0, s, S, q);
+            hipLaunchKernelGGL(k_sc_ee_query, dim3(blocks_for(E, kQPerBlock)), dim3(256), 0, s, S, q);
             VBD_LAUNCH_CHECK("k_sc_ee_query");
         }
     }
     // item lists per color position
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, S.it_cnt, 0, N + 1);
     VBD_LAUNCH_CHECK("k_fill_i32");
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N, 256)), dim3(256), 0, s, S.ee_n, 0, N);
+    VBD_LAUNCH_CHECK("k_fill_i32");
     const int gs = (int)std::max(1L, (nslots + 255) / 256);
-    hipLaunchKernelGGL(k_sc_items<false>, dim3(gs), dim3(256), 0, s, S);
-    VBD_LAUNCH_CHECK("k_sc_items");
+    hipLaunchKernelGGL(k_sc_items_count, dim3(gs), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items_count");
     int32_t rc = sc_scan(c, s, S.it_cnt, S.it_off, N + 1);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sc_copy_i32, dim3(blocks_for(N, 256)), dim3(256), 0, s, S.it_cur, S.it_off, N);
-    VBD_LAUNCH_CHECK("k_sc_copy_i32");
-    hipLaunchKernelGGL(k_sc_items<true>, dim3(gs), dim3(256), 0, s, S);
-    VBD_LAUNCH_CHECK("k_sc_items");
+    hipLaunchKernelGGL(k_sc_items_ee, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items_ee");
+    const int gv = (int)std::max(1L, ((long)N * S.vt_cap + 255) / 256);
+    hipLaunchKernelGGL(k_sc_items_vt, dim3(gv), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items_vt");
     hipLaunchKernelGGL(k_sc_items_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_sort");
     // compact collision list for the truncation

@@ -70,6 +70,9 @@ struct SelfParams {
     int* it_cnt;                 // [N+1]
     int* it_off;                 // [N+1]
     int* it_cur;                 // [N]
+    int* ee_n;                   // [N] edge-edge items of each color position
+    const int* adj_edge;         // per-particle (edge, slot) CSR, ascending edge ids
+    const int* adj_edge_off;     // [N+1]
     int2* items;                 // [it_cap] (key, role)
     float* rows;                 // [it_cap * 12]
     // displacement base and truncation
@@ -479,19 +482,34 @@ __device__ __forceinline__ void sc_prim_box(const SelfParams& S, const float* q,
     }
 }
 
-__global__ void k_sc_extents(SelfParams S, const float* __restrict__ q) {
+// largest half-extents, reduced per work-group (256 lanes) before one atomic each
+__global__ __launch_bounds__(256) void k_sc_extents(SelfParams S, const float* __restrict__ q) {
+    __shared__ float s_t[256], s_e[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float ht = 0.0f, he = 0.0f;
     if (i < S.T) {
         V3 lo, hi;
         sc_prim_box(S, q, 0, i, lo, hi);
-        const float h = 0.5f * fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
-        atomicMax(S.ext + 0, __float_as_int(h));
+        ht = 0.5f * fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
     }
     if (i < S.E) {
         V3 lo, hi;
         sc_prim_box(S, q, 1, i, lo, hi);
-        const float h = 0.5f * fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
-        atomicMax(S.ext + 1, __float_as_int(h));
+        he = 0.5f * fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+    }
+    s_t[threadIdx.x] = ht;
+    s_e[threadIdx.x] = he;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_t[threadIdx.x] = fmaxf(s_t[threadIdx.x], s_t[threadIdx.x + w]);
+            s_e[threadIdx.x] = fmaxf(s_e[threadIdx.x], s_e[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(S.ext + 0, __float_as_int(s_t[0]));
+        atomicMax(S.ext + 1, __float_as_int(s_e[0]));
     }
 }
 
@@ -565,122 +583,225 @@ __device__ __forceinline__ void sc_keep_smallest(int* row, int cap, int have, in
     row[k] = id;
 }
 
-// vertex_triangle_collision_detection_kernel (G:1329-1497), one vertex per thread
-__global__ void k_sc_vt_query(SelfParams S, const float* __restrict__ q) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= S.N) return;
-    const float r = S.margin;
-    const V3 x = ld3(q, v);
-    const V3 lo = v3(x.x - r, x.y - r, x.z - r), hi = v3(x.x + r, x.y + r, x.z + r);
-    const int wv = S.pworld ? S.pworld[v] : 0;
-    int* row = S.vt_list + (size_t)v * S.vt_cap;
-    for (int k = 0; k < S.vt_cap; ++k) row[k] = -1;
-    const float h = sc_cell_size(S, 0);
-    const int ix = sc_cell(x.x, h), iy = sc_cell(x.y, h), iz = sc_cell(x.z, h);
-    const int b = sc_hash(ix, iy, iz, S.world_hash ? wv : 0, S.nbucket);
-    const bool use_rest = S.q_rest && S.rest_excl > 0.0f;
-    int cnt = 0;
-    float mind = r;
-    for (int j = S.g_off[b], je = S.g_off[b + 1]; j < je; ++j) {
-        const int4 en = S.g_ent[j];
-        if (en.y != ix || en.z != iy || en.w != iz) continue;
-        const int t = en.x;
-        if (!sc_world_ok(wv, S.tri_world[t])) continue;
-        const int* ti = S.tri_idx + 3 * t;
-        const V3 A = ld3(q, ti[0]), B = ld3(q, ti[1]), C = ld3(q, ti[2]);
-        if (!sc_box_overlap(lo, hi, sc_vmin(sc_vmin(A, B), C), sc_vmax(sc_vmax(A, B), C))) continue;
-        if (v == ti[0] || v == ti[1] || v == ti[2]) continue;
-        if (sc_filtered(S.vt_filter, S.vt_filter_off, v, t)) continue;
-        V3 bary;
-        const V3 cp = sc_tri_closest(A, B, C, x, bary);
-        const float dist = length(sub(cp, x));
-        if (use_rest) {
-            const V3 rp = ld3(S.q_rest, v);
-            const V3 cr = sc_tri_closest(ld3(S.q_rest, ti[0]), ld3(S.q_rest, ti[1]), ld3(S.q_rest, ti[2]), rp, bary);
-            if (length(sub(cr, rp)) < S.rest_excl) continue;
-        }
-        if (dist < r) {
-            mind = sc_min(mind, dist);
-            sc_keep_smallest(row, S.vt_cap, cnt, t);
-            ++cnt;
-            atomicMin(S.tri_min + t, __float_as_int(dist));
-        }
+// Candidate tests of one query (the BVH leaf test, adjacency, filter, rest-shape exclusion and
+// the distance test of G:1329-1637).  true and the distance when the candidate collides.
+__device__ __forceinline__ bool sc_vt_test(const SelfParams& S, const float* __restrict__ q, int v, V3 x, int wv,
+                                           V3 lo, V3 hi, bool use_rest, int t, float& dist) {
+    if (!sc_world_ok(wv, S.tri_world[t])) return false;
+    const int* ti = S.tri_idx + 3 * t;
+    const int i0 = ti[0], i1 = ti[1], i2 = ti[2];
+    const V3 A = ld3(q, i0), B = ld3(q, i1), C = ld3(q, i2);
+    if (!sc_box_overlap(lo, hi, sc_vmin(sc_vmin(A, B), C), sc_vmax(sc_vmax(A, B), C))) return false;
+    if (v == i0 || v == i1 || v == i2) return false;
+    if (sc_filtered(S.vt_filter, S.vt_filter_off, v, t)) return false;
+    V3 bary;
+    const V3 cp = sc_tri_closest(A, B, C, x, bary);
+    dist = length(sub(cp, x));
+    if (use_rest) {
+        const V3 rp = ld3(S.q_rest, v);
+        const V3 cr = sc_tri_closest(ld3(S.q_rest, i0), ld3(S.q_rest, i1), ld3(S.q_rest, i2), rp, bary);
+        if (length(sub(cr, rp)) < S.rest_excl) return false;
     }
-    S.vt_count[v] = cnt;
-    S.vt_min[v] = mind;
+    return dist < S.margin;
 }
 
-// edge_colliding_edges_detection_kernel (G:1500-1637), one edge per thread
-__global__ void k_sc_ee_query(SelfParams S, const float* __restrict__ q) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= S.E) return;
+__device__ __forceinline__ bool sc_ee_test(const SelfParams& S, const float* __restrict__ q, int e, int a0, int a1,
+                                           V3 P, V3 Q, int we, V3 lo, V3 hi, bool use_rest, int g, float& dist) {
+    if (!sc_world_ok(we, S.edge_world[g])) return false;
+    const int b0 = S.edge_idx[4 * g + 2], b1 = S.edge_idx[4 * g + 3];
+    const V3 P2 = ld3(q, b0), Q2 = ld3(q, b1);
+    if (!sc_box_overlap(lo, hi, sc_vmin(P2, Q2), sc_vmax(P2, Q2))) return false;
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+    if (sc_filtered(S.ee_filter, S.ee_filter_off, e, g)) return false;
+    const V3 st = sc_closest_edge_edge(P, Q, P2, Q2, S.par_eps);
+    dist = st.z;
+    if (use_rest) {
+        const V3 sr = sc_closest_edge_edge(ld3(S.q_rest, a0), ld3(S.q_rest, a1), ld3(S.q_rest, b0), ld3(S.q_rest, b1),
+                                           S.par_eps);
+        if (sr.z < S.rest_excl) return false;
+    }
+    return dist < S.margin;
+}
+
+// A query runs on kQL lanes that stride over its cell's bucket; matches are appended to an LDS
+// list (kQCap per query), counts and minima are reduced across the lanes, and the list's
+// `capacity` smallest ids are placed by rank (ids are distinct: a cell lists a primitive once).
+// A query with more than kQCap matches is redone by its first lane alone (sc_keep_smallest).
+constexpr int kQL = 8;
+constexpr int kQCap = 128;
+constexpr int kQPerBlock = 256 / kQL;
+
+__device__ __forceinline__ void sc_place_by_rank(int* __restrict__ row, int cap, const int* buf, int n, int lane) {
+    for (int i = lane; i < n; i += kQL) {
+        const int x = buf[i];
+        int rank = 0;
+        for (int k = 0; k < n; ++k) rank += buf[k] < x ? 1 : 0;
+        if (rank < cap) row[rank] = x;
+    }
+    for (int k = (n < cap ? n : cap) + lane; k < cap; k += kQL) row[k] = -1;
+}
+
+// vertex_triangle_collision_detection_kernel (G:1329-1497)
+__global__ __launch_bounds__(256) void k_sc_vt_query(SelfParams S, const float* __restrict__ q) {
+    __shared__ int s_buf[kQPerBlock][kQCap];
+    __shared__ int s_n[kQPerBlock];
+    const int lane = threadIdx.x % kQL, qi = threadIdx.x / kQL;
+    const int v = blockIdx.x * kQPerBlock + qi;
+    if (lane == 0) s_n[qi] = 0;
+    __syncthreads();
+    const bool valid = v < S.N;
     const float r = S.margin;
-    const int* ei = S.edge_idx + 4 * e;
-    const V3 P = ld3(q, ei[2]), Q = ld3(q, ei[3]);
-    V3 lo = sc_vmin(P, Q), hi = sc_vmax(P, Q);
-    lo = v3(lo.x - r, lo.y - r, lo.z - r);
-    hi = v3(hi.x + r, hi.y + r, hi.z + r);
-    const int we = S.edge_world[e];
-    int* row = S.ee_list + (size_t)e * S.ee_cap;
-    for (int k = 0; k < S.ee_cap; ++k) row[k] = -1;
-    const float h = sc_cell_size(S, 1);
-    const V3 mid = scl(add(P, Q), 0.5f);
-    const int ix = sc_cell(mid.x, h), iy = sc_cell(mid.y, h), iz = sc_cell(mid.z, h);
-    const int b = sc_hash(ix, iy, iz, S.world_hash ? we : 0, S.nbucket);
     const bool use_rest = S.q_rest && S.rest_excl > 0.0f;
     int cnt = 0;
     float mind = r;
-    for (int j = S.g_off[b], je = S.g_off[b + 1]; j < je; ++j) {
-        const int4 en = S.g_ent[j];
-        if (en.y != ix || en.z != iy || en.w != iz) continue;
-        const int g = en.x;
-        if (!sc_world_ok(we, S.edge_world[g])) continue;
-        const int* gi = S.edge_idx + 4 * g;
-        const V3 P2 = ld3(q, gi[2]), Q2 = ld3(q, gi[3]);
-        if (!sc_box_overlap(lo, hi, sc_vmin(P2, Q2), sc_vmax(P2, Q2))) continue;
-        if (ei[2] == gi[2] || ei[2] == gi[3] || ei[3] == gi[2] || ei[3] == gi[3]) continue;
-        if (sc_filtered(S.ee_filter, S.ee_filter_off, e, g)) continue;
-        const V3 st = sc_closest_edge_edge(P, Q, P2, Q2, S.par_eps);
-        if (use_rest) {
-            const V3 sr = sc_closest_edge_edge(ld3(S.q_rest, ei[2]), ld3(S.q_rest, ei[3]), ld3(S.q_rest, gi[2]),
-                                               ld3(S.q_rest, gi[3]), S.par_eps);
-            if (sr.z < S.rest_excl) continue;
-        }
-        if (st.z < r) {
-            mind = sc_min(mind, st.z);
-            sc_keep_smallest(row, S.ee_cap, cnt, g);
+    V3 x = v3(0.0f, 0.0f, 0.0f), lo = x, hi = x;
+    int wv = 0, jb = 0, je = 0;
+    if (valid) {
+        x = ld3(q, v);
+        lo = v3(x.x - r, x.y - r, x.z - r);
+        hi = v3(x.x + r, x.y + r, x.z + r);
+        wv = S.pworld ? S.pworld[v] : 0;
+        const float h = sc_cell_size(S, 0);
+        const int ix = sc_cell(x.x, h), iy = sc_cell(x.y, h), iz = sc_cell(x.z, h);
+        const int b = sc_hash(ix, iy, iz, S.world_hash ? wv : 0, S.nbucket);
+        jb = S.g_off[b];
+        je = S.g_off[b + 1];
+        for (int j = jb + lane; j < je; j += kQL) {
+            const int4 en = S.g_ent[j];
+            if (en.y != ix || en.z != iy || en.w != iz) continue;
+            float dist;
+            if (!sc_vt_test(S, q, v, x, wv, lo, hi, use_rest, en.x, dist)) continue;
+            mind = sc_min(mind, dist);
             ++cnt;
+            atomicMin(S.tri_min + en.x, __float_as_int(dist));
+            const int k = atomicAdd(&s_n[qi], 1);
+            if (k < kQCap) s_buf[qi][k] = en.x;
         }
     }
-    S.ee_count[e] = cnt;
-    S.ee_min[e] = mind;
+#pragma unroll
+    for (int o = kQL / 2; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, kQL);
+        mind = fminf(mind, __shfl_xor(mind, o, kQL));
+    }
+    __syncthreads();
+    if (!valid) return;
+    int* row = S.vt_list + (size_t)v * S.vt_cap;
+    const int n = s_n[qi];
+    if (n <= kQCap) {
+        sc_place_by_rank(row, S.vt_cap, s_buf[qi], n, lane);
+    } else if (lane == 0) {
+        for (int k = 0; k < S.vt_cap; ++k) row[k] = -1;
+        const float h = sc_cell_size(S, 0);
+        const int ix = sc_cell(x.x, h), iy = sc_cell(x.y, h), iz = sc_cell(x.z, h);
+        int have = 0;
+        for (int j = jb; j < je; ++j) {
+            const int4 en = S.g_ent[j];
+            if (en.y != ix || en.z != iy || en.w != iz) continue;
+            float dist;
+            if (!sc_vt_test(S, q, v, x, wv, lo, hi, use_rest, en.x, dist)) continue;
+            sc_keep_smallest(row, S.vt_cap, have, en.x);
+            ++have;
+        }
+    }
+    if (lane == 0) {
+        S.vt_count[v] = cnt;
+        S.vt_min[v] = mind;
+    }
+}
+
+// edge_colliding_edges_detection_kernel (G:1500-1637)
+__global__ __launch_bounds__(256) void k_sc_ee_query(SelfParams S, const float* __restrict__ q) {
+    __shared__ int s_buf[kQPerBlock][kQCap];
+    __shared__ int s_n[kQPerBlock];
+    const int lane = threadIdx.x % kQL, qi = threadIdx.x / kQL;
+    const int e = blockIdx.x * kQPerBlock + qi;
+    if (lane == 0) s_n[qi] = 0;
+    __syncthreads();
+    const bool valid = e < S.E;
+    const float r = S.margin;
+    const bool use_rest = S.q_rest && S.rest_excl > 0.0f;
+    int cnt = 0;
+    float mind = r;
+    V3 P = v3(0.0f, 0.0f, 0.0f), Q = P, lo = P, hi = P;
+    int a0 = 0, a1 = 0, we = 0, jb = 0, je = 0, ix = 0, iy = 0, iz = 0;
+    if (valid) {
+        a0 = S.edge_idx[4 * e + 2];
+        a1 = S.edge_idx[4 * e + 3];
+        P = ld3(q, a0);
+        Q = ld3(q, a1);
+        lo = sc_vmin(P, Q);
+        hi = sc_vmax(P, Q);
+        lo = v3(lo.x - r, lo.y - r, lo.z - r);
+        hi = v3(hi.x + r, hi.y + r, hi.z + r);
+        we = S.edge_world[e];
+        const float h = sc_cell_size(S, 1);
+        const V3 mid = scl(add(P, Q), 0.5f);
+        ix = sc_cell(mid.x, h);
+        iy = sc_cell(mid.y, h);
+        iz = sc_cell(mid.z, h);
+        const int b = sc_hash(ix, iy, iz, S.world_hash ? we : 0, S.nbucket);
+        jb = S.g_off[b];
+        je = S.g_off[b + 1];
+        for (int j = jb + lane; j < je; j += kQL) {
+            const int4 en = S.g_ent[j];
+            if (en.y != ix || en.z != iy || en.w != iz) continue;
+            float dist;
+            if (!sc_ee_test(S, q, e, a0, a1, P, Q, we, lo, hi, use_rest, en.x, dist)) continue;
+            mind = sc_min(mind, dist);
+            ++cnt;
+            const int k = atomicAdd(&s_n[qi], 1);
+            if (k < kQCap) s_buf[qi][k] = en.x;
+        }
+    }
+#pragma unroll
+    for (int o = kQL / 2; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, kQL);
+        mind = fminf(mind, __shfl_xor(mind, o, kQL));
+    }
+    __syncthreads();
+    if (!valid) return;
+    int* row = S.ee_list + (size_t)e * S.ee_cap;
+    const int n = s_n[qi];
+    if (n <= kQCap) {
+        sc_place_by_rank(row, S.ee_cap, s_buf[qi], n, lane);
+    } else if (lane == 0) {
+        for (int k = 0; k < S.ee_cap; ++k) row[k] = -1;
+        int have = 0;
+        for (int j = jb; j < je; ++j) {
+            const int4 en = S.g_ent[j];
+            if (en.y != ix || en.z != iy || en.w != iz) continue;
+            float dist;
+            if (!sc_ee_test(S, q, e, a0, a1, P, Q, we, lo, hi, use_rest, en.x, dist)) continue;
+            sc_keep_smallest(row, S.ee_cap, have, en.x);
+            ++have;
+        }
+    }
+    if (lane == 0) {
+        S.ee_count[e] = cnt;
+        S.ee_min[e] = mind;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // per color position item lists: key = e1 * ee_cap + slot (edge-edge) or
 // E * ee_cap + p * vt_cap + slot (vertex-triangle); role = vertex's place in the contact
 // ---------------------------------------------------------------------------
-template <bool FILL>
-__device__ __forceinline__ void sc_item_put(const SelfParams& S, int v, int key, int role) {
-    const int pos = S.pos_of[v];
-    if (!FILL) {
-        atomicAdd(S.it_cnt + pos, 1);
-    } else {
-        const int slot = atomicAdd(S.it_cur + pos, 1);
-        S.items[slot] = make_int2(key, role);
-    }
-}
-
-template <bool FILL>
-__global__ void k_sc_items(SelfParams S) {
+// Per color position: the edge-edge items first, in (e1, slot) order -- written by a walk over the
+// particle's edges (its CSR lists them in ascending id) -- then the vertex-triangle items in
+// (p, slot) order, appended by atomics and sorted (short lists: a vertex's own contacts and
+// those of the few triangles around it).
+__global__ void k_sc_items_count(SelfParams S) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nee = (long)S.E * S.ee_cap;
     if (i < nee) {
         const int e1 = (int)(i / S.ee_cap), k = (int)(i - (long)e1 * S.ee_cap);
         if (k >= S.ee_count[e1]) return;
         const int* a = S.edge_idx + 4 * e1;
-        sc_item_put<FILL>(S, a[2], (int)i, 0);
-        sc_item_put<FILL>(S, a[3], (int)i, 1);
+        for (int r = 0; r < 2; ++r) {
+            const int pos = S.pos_of[a[2 + r]];
+            atomicAdd(S.it_cnt + pos, 1);
+            atomicAdd(S.ee_n + pos, 1);
+        }
         return;
     }
     const long j = i - nee;
@@ -688,18 +809,54 @@ __global__ void k_sc_items(SelfParams S) {
     const int p = (int)(j / S.vt_cap), k = (int)(j - (long)p * S.vt_cap);
     if (k >= S.vt_count[p]) return;
     const int* t = S.tri_idx + 3 * S.vt_list[j];
-    const int key = (int)(nee + j);
-    sc_item_put<FILL>(S, t[0], key, 0);
-    sc_item_put<FILL>(S, t[1], key, 1);
-    sc_item_put<FILL>(S, t[2], key, 2);
-    sc_item_put<FILL>(S, p, key, 3);
+    atomicAdd(S.it_cnt + S.pos_of[t[0]], 1);
+    atomicAdd(S.it_cnt + S.pos_of[t[1]], 1);
+    atomicAdd(S.it_cnt + S.pos_of[t[2]], 1);
+    atomicAdd(S.it_cnt + S.pos_of[p], 1);
 }
 
-// insertion sort of each color position's items by key
+// edge-edge items of each color position by walking the particle's edges; sets the cursor of its
+// vertex-triangle items
+__global__ void k_sc_items_ee(SelfParams S) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= S.N) return;
+    const int v = S.color_particles[pos];
+    int j = S.it_off[pos];
+    if (S.E > 0) {
+        for (int c = S.adj_edge_off[v]; c < S.adj_edge_off[v + 1]; c += 2) {
+            const int slot = S.adj_edge[c + 1];
+            if (slot < 2) continue;
+            const int e1 = S.adj_edge[c];
+            const int n = S.ee_count[e1] < S.ee_cap ? S.ee_count[e1] : S.ee_cap;
+            for (int k = 0; k < n; ++k) S.items[j++] = make_int2(e1 * S.ee_cap + k, slot - 2);
+        }
+    }
+    S.it_cur[pos] = j;
+}
+
+__device__ __forceinline__ void sc_vt_put(const SelfParams& S, int v, int key, int role) {
+    const int slot = atomicAdd(S.it_cur + S.pos_of[v], 1);
+    S.items[slot] = make_int2(key, role);
+}
+
+__global__ void k_sc_items_vt(SelfParams S) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (long)S.N * S.vt_cap) return;
+    const int p = (int)(j / S.vt_cap), k = (int)(j - (long)p * S.vt_cap);
+    if (k >= S.vt_count[p]) return;
+    const int* t = S.tri_idx + 3 * S.vt_list[j];
+    const int key = (int)((long)S.E * S.ee_cap + j);
+    sc_vt_put(S, t[0], key, 0);
+    sc_vt_put(S, t[1], key, 1);
+    sc_vt_put(S, t[2], key, 2);
+    sc_vt_put(S, p, key, 3);
+}
+
+// insertion sort of each color position's vertex-triangle items by key
 __global__ void k_sc_items_sort(SelfParams S) {
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= S.N) return;
-    const int b = S.it_off[pos], e = S.it_off[pos + 1];
+    const int b = S.it_off[pos] + S.ee_n[pos], e = S.it_off[pos + 1];
     for (int i = b + 1; i < e; ++i) {
         const int2 x = S.items[i];
         int j = i;
