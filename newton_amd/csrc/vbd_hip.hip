@@ -698,6 +698,16 @@ __device__ __forceinline__ void store_row(float* __restrict__ row, V3 f, const M
         for (int j = 0; j < 3; ++j) row[3 + 3 * i + j] = h.m[i][j];
 }
 
+// component c (0..11) of a row {f, H row-major}, by selects (no dynamic register indexing)
+__device__ __forceinline__ float row_component(V3 f, const M3& h, int c) {
+    const float r[12] = {f.x, f.y, f.z, h.m[0][0], h.m[0][1], h.m[0][2],
+                         h.m[1][0], h.m[1][1], h.m[1][2], h.m[2][0], h.m[2][1], h.m[2][2]};
+    float v = r[0];
+#pragma unroll
+    for (int k = 1; k < 12; ++k) v = c == k ? r[k] : v;
+    return v;
+}
+
 __device__ __forceinline__ void eval_packed_contact(const KParams& P, const float* __restrict__ q, int rec, float dt,
                                                     V3& f, M3& h) {
     const float4* c4 = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)rec;
@@ -773,16 +783,11 @@ __global__ void k_spring_rows(KParams P, const float* __restrict__ q, int j0, in
 
 #include "vbd_self.h"
 
-// Finish one vertex: contacts (when not already summed), then the block solve
-// and position update (mode 0) or the force output (mode 1).
-// x0 and d_old are the vertex's q_prev and displacement (read by the caller).
-// With add_contacts the vertex's spring rows (srows, nspr rows in LDS) and then its self-contact
-// rows (color position self_pos) continue the contact sum.  With self-contact on, the pass writes
-// only the displacement; the truncation pass that follows writes q.
+// Finish one vertex: the block solve and position update (mode 0) or the force output (mode 1).
+// x0 and d_old are the vertex's q_prev and displacement (read by the caller).  With self-contact
+// on, the pass writes only the displacement; the truncation pass that follows writes q.
 __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restrict__ q, int v, bool active, V3 x0,
-                                              V3 d_old, V3 f, M3 h, bool add_contacts, int crec0, int crec1,
-                                              const float* srows, int nspr, int self_pos, float dt, int mode,
-                                              float* __restrict__ g_out) {
+                                              V3 d_old, V3 f, M3 h, float dt, int mode, float* __restrict__ g_out) {
     if (!active) {
         if (mode == 0) {
             V3 z = v3(0.0f, 0.0f, 0.0f);
@@ -792,37 +797,6 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
             st3(g_out, v, v3(0.0f, 0.0f, 0.0f));
         }
         return;
-    }
-    if (add_contacts) {
-        V3 cf = v3(0.0f, 0.0f, 0.0f);
-        M3 ch = mzero();
-        for (int j = crec0; j < crec1; ++j) {
-            V3 fc;
-            M3 hc;
-            eval_packed_contact(P, q, j, dt, fc, hc);
-            cf = add(cf, fc);
-            ch = madd(ch, hc);
-        }
-        for (int j = 0; j < nspr; ++j) {
-            const float* r = srows + j * kComp;
-            cf = add(cf, v3(r[0], r[1], r[2]));
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
-        }
-        if (P.has_self && self_pos >= 0) {
-            for (int j = P.so_off[self_pos], je = P.so_off[self_pos + 1]; j < je; ++j) {
-                const float* r = P.so_rows + (size_t)kComp * j;
-                cf = add(cf, v3(r[0], r[1], r[2]));
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) ch.m[a][b] = ch.m[a][b] + r[3 + 3 * a + b];
-            }
-        }
-        h = madd(h, ch);
-        f = add(f, cf);
     }
     if (mode == 1) {
         st3(g_out, v, f);
@@ -1007,19 +981,28 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
             acc = acc + a3;
         }
         for (; r < q1; ++r) acc = acc + s_rows[r * kComp + cc];
+        // contacts from zero -- their LDS rows, or, when they did not fit, evaluated here by each of
+        // the vertex's 12 lanes (component cc kept) -- then the vertex's spring rows and
+        // self-contact rows continue that sum, which is added to the element sum
+        const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
+        float ca = 0.0f;
         if (par_contacts) {
-            // contacts from zero, then the vertex's spring rows, then added to the element sum
-            const int c0 = s_cb[cs], c1 = s_cb[cs + 1];
-            float ca = 0.0f;
             for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
-            for (int r = q1, re = q1 + s_nspr[cs]; r < re; ++r) ca = ca + s_rows[r * kComp + cc];
-            if (P.has_self) {
-                const int pos = b0.x + cs;
-                for (int r = P.so_off[pos], re = P.so_off[pos + 1]; r < re; ++r)
-                    ca = ca + P.so_rows[(size_t)kComp * r + cc];
+        } else {
+            for (int r = c0; r < c1; ++r) {
+                V3 fc;
+                M3 hc;
+                eval_packed_contact(P, q, cbeg + r, dt, fc, hc);
+                ca = ca + row_component(fc, hc, cc);
             }
-            acc = acc + ca;
         }
+        for (int r = q1, re = q1 + s_nspr[cs]; r < re; ++r) ca = ca + s_rows[r * kComp + cc];
+        if (P.has_self) {
+            const int pos = b0.x + cs;
+            for (int r = P.so_off[pos], re = P.so_off[pos + 1]; r < re; ++r)
+                ca = ca + P.so_rows[(size_t)kComp * r + cc];
+        }
+        acc = acc + ca;
         s_acc[cs * kComp + cc] = acc;
     }
     __syncthreads();
@@ -1033,9 +1016,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
-        const int q1 = s_row[t] + s_nrow[t];
-        finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, !par_contacts, cbeg + s_cb[t],
-                      cbeg + s_cb[t + 1], s_rows + q1 * kComp, s_nspr[t], b0.x + t, dt, mode, g_out);
+        finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, dt, mode, g_out);
     }
 }
 
@@ -1179,8 +1160,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
             h = madd(h, ch);
             f = add(f, cf);
         }
-        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, false, 0, 0, nullptr, 0, -1, dt, mode,
-                      g_out);
+        finish_vertex(P, q, v, active, ld3(P.q_prev, v), ld3(P.disp, v), f, h, dt, mode, g_out);
     }
 }
 
