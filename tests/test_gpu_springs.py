@@ -154,3 +154,31 @@ def test_spring_residual_matches_oracle(cuda_device):
     assert np.any(g_ref != 0.0)
     np.testing.assert_array_equal(g.cpu().numpy(), g_ref)
     assert abs(float(r2.item()) ** 0.5 - n_ref) <= 1e-6 * max(n_ref, 1.0)
+
+
+def test_spring_only_rope_bitexact(cuda_device):
+    """Particles joined only by springs (no triangles, edges or tets): the coloring graph has no edges,
+    so every spring joins two vertices of one color and goes through the pre-pass."""
+    from newton_amd import ModelBuilder
+
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=1.0, mu=0.4))
+    n = 40
+    for k in range(n):
+        b.add_particle((0.02 * k, 0.3 + 0.01 * math.sin(k), 0.0), (0.0, 0.0, 0.0), 0.0 if k == 0 else 0.01,
+                       radius=0.01)
+    b.add_springs(np.arange(n - 1), np.arange(1, n), 200.0, 0.05)
+    b.add_springs(np.arange(n - 2), np.arange(2, n), 50.0, 0.0)  # bending-like second neighbours
+    b.color()
+    model = b.finalize(cuda_device)
+    model.soft_contact_ke, model.soft_contact_kd, model.soft_contact_mu = 1.0e4, 1.0, 0.4
+    assert model.tri_count == 0 and model.edge_count == 0 and model.spring_count == 2 * n - 3
+    assert _same_color_springs(model) > 0
+    state = model.state()
+    info = {"dt": 1.0 / 600.0, "substeps": 10, "iterations": 5}
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 60)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 60)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+    assert np.abs(q_gpu - q0).max() > 0.01
