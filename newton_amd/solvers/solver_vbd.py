@@ -394,6 +394,13 @@ class SolverVBD:
         )
         return out
 
+    def rebuild_bvh(self, state: State) -> None:
+        """``solver_vbd.py:3239-3249``: the reference rebuilds its self-contact BVHs from ``state``.  The
+        detection grid here is rebuilt from the current positions at every detection, so there is
+        nothing to refresh; kept for callers of the reference's API."""
+        if not self.particle_enable_self_contact:
+            return
+
     def self_contact_overflow(self) -> bool:
         """True when a primitive spanned more detection-grid cells than a detection lists (synchronises)."""
         flag = ctypes.c_int32(0)
