@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--train-frames", type=int, default=40)
     ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--trial-iterations", type=int, default=2,
+                    help="iterations each start runs before the residual judges it (learned_trial variant)")
     ap.add_argument("--train-protocol", choices=("deployed", "converged"), default="deployed",
                     help="trajectory of the training data: the deployed tolerance solve, or converged solves")
     ap.add_argument("--pmc-mode", action="store_true",
@@ -474,28 +476,33 @@ def residual_leg(args):
     fwd_ms = e0.elapsed_time(e1) / 50
     fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
 
-    res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in ("plain", "reuse", "learned")}
+    variants = ("plain", "reuse", "learned_trial", "learned")
+    res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in variants}
     accepted = 0
+    accepted_trial = 0
     r_ws_rel = []
     for frame in range(args.residual_frames):
         wind.redraw(args.train_frames + frame)
         for _ in range(sub):
             q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), ws.corr.clone()
             f_ext = wind.force(s0.particle_q, mass)
-            for name in ("plain", "reuse", "learned"):
+            for name in variants:
                 s0.particle_q.copy_(q0)
                 s0.particle_qd.copy_(qd0)
                 s0.particle_f.copy_(f_ext)
                 ws.corr.copy_(corr0)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                start = {"plain": None, "reuse": ws.reuse, "learned": ws}[name]
+                start = {"plain": None, "reuse": ws.reuse, "learned": ws, "learned_trial": ws}[name]
                 r = solver.step_to_residual(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                            max_iterations=args.max_iterations, warm_start=start)
+                                            max_iterations=args.max_iterations, warm_start=start,
+                                            trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
                 torch.cuda.synchronize()
                 res[name]["s"] += time.perf_counter() - t0
                 res[name]["iters"].append(r["iterations"])
                 res[name]["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
+                if name == "learned_trial":
+                    accepted_trial += int(r["warm_start_index"] == 1)
                 if name == "learned":
                     accepted += int(r["warm_start_index"] == 1)
                     if r["r_warm_start"] is not None and r["r0"] > 0:
@@ -505,7 +512,7 @@ def residual_leg(args):
     substeps = args.residual_frames * sub
     out = {"workload": info["workload"], "vertices": n, "rel_tol": args.rel_tol, "max_iterations": args.max_iterations,
            "substeps": substeps, "check_every": 1}
-    for name in ("plain", "reuse", "learned"):
+    for name in variants:
         d = res[name]
         out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
                      "reached_tol": f"{d['reached']}/{substeps}",
@@ -515,6 +522,9 @@ def residual_leg(args):
     out["iteration_ratio"] = round(float(np.mean(res["plain"]["iters"]) /
                                          max(float(np.mean(res["learned"]["iters"])), 1e-9)), 3)
     out["learned_candidate_chosen"] = f"{accepted}/{substeps}"
+    out["speedup_trial_vertex_steps"] = round(res["plain"]["s"] / res["learned_trial"]["s"], 3)
+    out["trial_iterations"] = args.trial_iterations
+    out["learned_candidate_chosen_after_trial"] = f"{accepted_trial}/{substeps}"
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
     out["training"] = {"protocol": args.train_protocol, "frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,

@@ -54,8 +54,8 @@ SC = dict(particle_self_contact_radius=0.004, particle_self_contact_margin=0.006
           particle_vertex_contact_buffer_size=16, particle_edge_contact_buffer_size=32)
 
 
-def _oracle_self(model, solver_kw, threshold=2):
-    vt, ee = build_filters(model, threshold)
+def _oracle_self(model, solver_kw, threshold=2, vt_map=None):
+    vt, ee = build_filters(model, threshold, vt_map)
     return oracle.SelfContact(
         model.particle_count, model.tri_count, model.edge_count,
         radius=solver_kw["particle_self_contact_radius"], margin=solver_kw["particle_self_contact_margin"],
@@ -69,14 +69,23 @@ def _oracle_self(model, solver_kw, threshold=2):
         ee_filter=None if ee is None else ee[0], ee_filter_offsets=None if ee is None else ee[1])
 
 
-@pytest.mark.parametrize("worlds", [False, True])
-def test_detection_matches_oracle(cuda_device, worlds):
+@pytest.mark.parametrize("worlds,variant", [(False, "default"), (True, "default"), (False, "filters")])
+def test_detection_matches_oracle(cuda_device, worlds, variant):
+    """Detection lists against the oracle's brute force; the "filters" variant adds a 3-ring topological
+    filter, an external vertex-triangle map, a rest-shape exclusion radius and small buffers that
+    overflow (both keep the smallest ids)."""
     import torch
 
     from newton_amd.solvers import SolverVBD
 
     model = _layers(cuda_device, worlds=worlds)
     kw = dict(SC, particle_self_contact_margin=0.016, particle_self_contact_radius=0.01)
+    ext = None
+    if variant == "filters":
+        kw.update(particle_rest_shape_contact_exclusion_radius=0.004, particle_vertex_contact_buffer_size=3,
+                  particle_edge_contact_buffer_size=5, particle_topological_contact_filter_threshold=3)
+        ext = {70: [10, 11, 12], 71: list(range(90, 110))}
+        kw["particle_external_vertex_contact_filtering_map"] = ext
     solver = SolverVBD(model, iterations=2, particle_enable_self_contact=True, **kw)
     state = model.state()
     rng = np.random.default_rng(3)
@@ -86,7 +95,8 @@ def test_detection_matches_oracle(cuda_device, worlds):
     info = {k: v.cpu().numpy() for k, v in solver.collision_info().items()}
     assert not solver.self_contact_overflow()
     om = oracle.OracleModel(model)
-    sc = _oracle_self(model, kw)
+    sc = _oracle_self(model, kw, threshold=kw.get("particle_topological_contact_filter_threshold", 2),
+                      vt_map=ext)
     oracle.self_detect(om, sc, q)
     np.testing.assert_array_equal(info["vertex_colliding_triangles_count"], sc.vt_count)
     np.testing.assert_array_equal(info["vertex_colliding_triangles"], sc.vt_list)
@@ -99,6 +109,9 @@ def test_detection_matches_oracle(cuda_device, worlds):
         assert sc.vt_count.sum() == 0  # the layers are in different worlds
     else:
         assert sc.vt_count.sum() > 20 and sc.ee_count.sum() > 20
+    if variant == "filters":
+        assert (sc.vt_count > kw["particle_vertex_contact_buffer_size"]).any()
+        assert (sc.ee_count > kw["particle_edge_contact_buffer_size"]).any()
 
 
 def _run_gpu(model, kw, substeps, dt, iterations=4):
