@@ -59,7 +59,12 @@ def parse():
     ap.add_argument("--train-frames", type=int, default=40)
     ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
     ap.add_argument("--train-steps", type=int, default=2000)
-    ap.add_argument("--trial-iterations", type=int, default=2,
+    ap.add_argument("--residual-worlds", type=int, default=64,
+                    help="worlds of the batched wind cloth in the residual leg (0: skip)")
+    ap.add_argument("--batched-residual-warm-frames", type=int, default=2)
+    ap.add_argument("--single-world-stride", type=int, default=4,
+                    help="batched residual leg: every k-th world is also solved alone (the plain baseline)")
+    ap.add_argument("--trial-iterations", type=int, default=4,
                     help="iterations each start runs before the residual judges it (learned_trial variant)")
     ap.add_argument("--train-protocol", choices=("deployed", "converged"), default="deployed",
                     help="trajectory of the training data: the deployed tolerance solve, or converged solves")
@@ -526,6 +531,8 @@ def residual_leg(args):
     out["trial_iterations"] = args.trial_iterations
     out["learned_candidate_chosen_after_trial"] = f"{accepted_trial}/{substeps}"
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
+    if args.residual_worlds > 0:
+        out["batched"] = batched_residual(args, s0, ws)
     out["training"] = {"protocol": args.train_protocol, "frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
                        "batch": batch, "loss_first": None if first is None else round(first, 5),
@@ -536,6 +543,112 @@ def residual_leg(args):
                            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": round(fwd_tflops / BF16_PEAK_TFLOPS, 5), "rows": n,
                            "avg_launch_us": round(fwd_ms * 1e3, 2)}
+    return out
+
+
+def batched_residual(args, s_single, ws):
+    """The wind cloth batched: ``residual_worlds`` worlds in one model, every world from the single-world
+    leg's last state and its own wind, each judged by its own residual.  From identical states each
+    substep: plain VBD solving the worlds one at a time (a single-world solver, every
+    ``single_world_stride``-th world, time scaled to all worlds), plain VBD on the batch, and the batch
+    with the warm start (``y + c`` and the trained network, chosen per world after
+    ``trial_iterations``)."""
+    import numpy as np
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+    from newton_amd.warmstart import LearnedWarmStart
+
+    dev = "cuda:0"
+    W = args.residual_worlds
+    model, s0, info = scenes.cloth_wind_batched(dev, worlds=W)
+    wind, dt, sub = info["wind"], info["dt"], info["substeps"]
+    n = model.particle_count
+    s1 = model.state()
+    s0.particle_q.copy_(s_single.particle_q.repeat(W, 1))
+    s0.particle_qd.copy_(s_single.particle_qd.repeat(W, 1))
+    solver = SolverVBD(model, iterations=10)
+    wsb = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0)
+    wsb.corr.copy_(ws.corr.repeat(W, 1))
+    mass = model.particle_mass
+    first = args.train_frames + args.residual_frames
+    # the worlds drift apart under their own winds before the measured frames
+    for frame in range(args.batched_residual_warm_frames):
+        wind.redraw(first + frame)
+        for _ in range(sub):
+            s0.particle_f.copy_(wind.force(s0.particle_q, mass))
+            solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=args.rel_tol,
+                                           max_iterations=args.max_iterations, warm_start=wsb.reuse)
+            wsb.observe(solver, s0)
+            s0, s1 = s1, s0
+    variants = ("plain", "learned_trial")
+    res = {k: {"iters": [], "s": 0.0, "reached": 0, "chosen": None} for k in variants}
+    one_model, one_state, _ = scenes.cloth_wind(dev)
+    one_solver = SolverVBD(one_model, iterations=10)
+    one_out = one_model.state()
+    per = one_model.particle_count
+    sample = list(range(0, W, max(args.single_world_stride, 1)))
+    single = {"iters": [], "s": 0.0, "reached": 0}
+    first += args.batched_residual_warm_frames
+    for frame in range(args.residual_frames):
+        wind.redraw(first + frame)
+        _progress(f"batched residual: frame {frame}/{args.residual_frames}")
+        for _ in range(sub):
+            q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), wsb.corr.clone()
+            f_ext = wind.force(s0.particle_q, mass)
+            for w in sample:
+                sl = slice(w * per, (w + 1) * per)
+                one_state.particle_q.copy_(q0[sl])
+                one_state.particle_qd.copy_(qd0[sl])
+                one_state.particle_f.copy_(f_ext[sl])
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = one_solver.step_to_residual(one_state, one_out, None, None, dt, rel_tol=args.rel_tol,
+                                                max_iterations=args.max_iterations)
+                torch.cuda.synchronize()
+                single["s"] += time.perf_counter() - t0
+                single["iters"].append(r["iterations"])
+                single["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
+            for name in variants:
+                s0.particle_q.copy_(q0)
+                s0.particle_qd.copy_(qd0)
+                s0.particle_f.copy_(f_ext)
+                wsb.corr.copy_(corr0)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = solver.step_to_residual_worlds(
+                    s0, s1, None, None, dt, rel_tol=args.rel_tol, max_iterations=args.max_iterations,
+                    warm_start=None if name == "plain" else wsb,
+                    trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
+                torch.cuda.synchronize()
+                d = res[name]
+                d["s"] += time.perf_counter() - t0
+                d["iters"].append(r["iterations"])
+                d["reached"] += r["reached"]
+                if r["candidate_chosen"]:
+                    d["chosen"] = r["candidate_chosen"] if d["chosen"] is None else \
+                        [a + b for a, b in zip(d["chosen"], r["candidate_chosen"])]
+            wsb.observe(solver, s0)
+            s0, s1 = s1, s0
+    substeps = args.residual_frames * sub
+    out = {"workload": info["workload"], "worlds": W, "vertices": n, "stop": "every world at rel_tol",
+           "warm_frames": args.batched_residual_warm_frames, "substeps": substeps}
+    single_vs = len(sample) * per * substeps / single["s"]
+    out["single_world_plain"] = {"worlds_timed": len(sample), "mean_iterations": round(float(np.mean(single["iters"])), 2),
+                                 "reached_tol": f"{single['reached']}/{len(sample) * substeps} world-substeps",
+                                 "vertex_steps_per_s": round(single_vs, 1)}
+    for name in variants:
+        d = res[name]
+        out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
+                     "reached_tol": f"{d['reached']}/{substeps * W} world-substeps",
+                     "vertex_steps_per_s": round(n * substeps / d["s"], 1)}
+        if d["chosen"] is not None:
+            keys = ("reuse", "network") if len(d["chosen"]) == 2 else [str(k) for k in range(len(d["chosen"]))]
+            out[name]["world_substeps_choosing"] = dict(zip(keys, d["chosen"]))
+    out["speedup_vs_single_world_plain"] = round(out["learned_trial"]["vertex_steps_per_s"] / single_vs, 3)
+    out["speedup_batching_only"] = round(out["plain"]["vertex_steps_per_s"] / single_vs, 3)
+    out["speedup_warm_start_in_batch"] = round(res["plain"]["s"] / res["learned_trial"]["s"], 3)
     return out
 
 

@@ -227,6 +227,28 @@ int32_t vbd_iterate_to_residual(VbdContext* ctx, void* stream, float* q, float d
                                 int32_t max_iterations, int32_t check_every, int32_t* iterations_out,
                                 double* norm_sq_out);
 
+/* Per-world residuals for models holding several worlds in contiguous particle ranges:
+ * norm_sq_out[w] (device, world_count doubles) = sum of |g_i|^2 over the particles
+ * world_offsets[w] <= i < world_offsets[w+1] (device int32 [world_count+1]), each reduced
+ * in a fixed order.  Batched iterate-to-tolerance drivers judge every world by its own
+ * residual; no counterpart in the reference. */
+int32_t vbd_residual_worlds(VbdContext* ctx, void* stream, const float* q, float dt, int32_t world_count,
+                            const int32_t* world_offsets, double* norm_sq_out);
+
+/* vbd_iterate_to_residual over worlds: each world is iterated until its squared norm is
+ * <= target_norm_sq[w] (host array), then frozen -- its vertices are neither solved nor
+ * re-evaluated, so its iterate is the one a solve of that world alone stops at -- until all
+ * worlds are frozen or max_iterations were run.  particle_world (device int32 [N]) gives
+ * each particle's world, consistent with world_offsets; norm_sq_dev (world_count doubles)
+ * and world_active_dev (world_count int32) are device scratch.  Reports the iterations run,
+ * optionally each world's count (world_iterations_out, host), and the last norms
+ * (norm_sq_out, host).  Host-synchronous; not available with self-contact. */
+int32_t vbd_iterate_to_residual_worlds(VbdContext* ctx, void* stream, float* q, float dt, int32_t world_count,
+                                       const int32_t* world_offsets, const int32_t* particle_world,
+                                       const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
+                                       double* norm_sq_dev, int32_t* world_active_dev, int32_t* iterations_out,
+                                       int32_t* world_iterations_out, double* norm_sq_out);
+
 /* Copies the solver's scratch state of the current substep ([N,3] floats)
  * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */
 int32_t vbd_copy_scratch(VbdContext* ctx, void* stream, int32_t which, float* dst);

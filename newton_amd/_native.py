@@ -167,6 +167,12 @@ HIP_EXPORTS = {
         [c_void_p, c_void_p, c_void_p, c_float, ctypes.c_double, c_int32, c_int32, ctypes.POINTER(c_int32),
          ctypes.POINTER(ctypes.c_double)],
     ),
+    "vbd_residual_worlds": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p]),
+    "vbd_iterate_to_residual_worlds": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_double), c_int32,
+         c_int32, c_void_p, c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32), ctypes.POINTER(ctypes.c_double)],
+    ),
     "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
     "vbd_self_detect": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "vbd_self_copy_lists": (c_int32, [c_void_p, c_void_p] + [c_void_p] * 7),

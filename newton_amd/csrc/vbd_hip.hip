@@ -651,6 +651,10 @@ struct KParams {
     const int* so_off;
     const float* so_rows;
     int write_q;
+    // per-world freezing (batched iterate-to-tolerance): when w_active is set, vertices of
+    // worlds with w_active[p_world[v]] == 0 are neither solved nor re-evaluated
+    const int* p_world;
+    const int* w_active;
     // per-particle contact lists (contact-index order) and per-contact state
     const int* c_off;
     const int* c_ids;
@@ -824,12 +828,15 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
 constexpr int kBatchHdr = vbd_layout::kBatchWords;
 
 // BLOCK lanes per work-group: 256 (batches of <= 21 vertices / 256 items) or
-// 128 (<= 10 vertices / 128 items); the autotuner picks one per model.
+// 128 (<= 10 vertices / 128 items); the autotuner picks one per model.  LDS holds the static
+// items plus 64 contact rows (16.7 KB at 256 lanes), and the register budget is set for seven
+// waves per SIMD: occupancy is then bound by neither, and the few values the compiler keeps in
+// scratch cost less than the waves gained (the pass is VALU- and latency-bound).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
                                                       float dt, int mode, float* __restrict__ g_out) {
     constexpr int MAXV = BLOCK / kComp;
-    constexpr int ROWS = 2 * BLOCK;  // static items (<= BLOCK) + contacts evaluated in parallel
+    constexpr int ROWS = BLOCK + 64;  // static items (<= BLOCK) + contacts evaluated in parallel
     constexpr int BLK = kBatchHdr + 4 * MAXV;
     __shared__ int s_row[MAXV];
     __shared__ int s_nrow[MAXV];
@@ -851,6 +858,13 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
     int cbt = 0;
     if (t <= MAXV) cbt = P.bat_coff[(size_t)b * (MAXV + 1) + t];
     const int nv = b0.y, tri_b = b0.z, ntri = b0.w, edge_b = b1.x, nedge = b1.y, tet_b = b1.z, ntet = b1.w;
+    // frozen worlds: a batch none of whose vertices is live ends here (uniformly)
+    bool own_live = true;
+    if (P.w_active) {
+        int live = 0;
+        if (t < nv) live = own_live = P.w_active[P.p_world[vr.x]] != 0;
+        if (!__syncthreads_or(live)) return;
+    }
     const int spr_b = b2.x, nspr = b2.y;
     const int nstatic = ntri + nedge + ntet + nspr;
     const int cbeg = ci.x, ncont = ci.y;
@@ -1016,7 +1030,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_fast(KParams P, float* __restri
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
-        finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, dt, mode, g_out);
+        if (own_live) finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, dt, mode, g_out);
     }
 }
 
@@ -1032,6 +1046,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
 
     const int t = threadIdx.x;
     const int v = P.color_particles[P.slow_pos[slow_base + blockIdx.x]];
+    if (P.w_active && P.w_active[P.p_world[v]] == 0) return;  // frozen world (uniform)
     const bool active = particle_active(P, v);
     const float dtr = 1.0f / (dt * dt);
     if (t == 0) {
@@ -1389,6 +1404,24 @@ __global__ __launch_bounds__(kNormBlock) void k_residual_final(int nparts, const
     for (int i = threadIdx.x; i < nparts; i += kNormBlock) v += partial[i];
     double s = block_sum_f64(v, s_part);
     if (threadIdx.x == 0) out[0] = s;
+}
+
+// Per-world norms: one work-group per world sums |g_p|^2 over the world's particle range
+// [off[w], off[w+1]) (thread-strided in double, then the fixed shuffle tree); the range is
+// clipped to [0, n).
+__global__ __launch_bounds__(kNormBlock) void k_residual_worlds(int n, const int* __restrict__ off,
+                                                               const float* __restrict__ g,
+                                                               double* __restrict__ out) {
+    __shared__ double s_part[kNormBlock / 64];
+    const int w = blockIdx.x;
+    const int p0 = max(off[w], 0), p1 = min(off[w + 1], n);
+    double v = 0.0;
+    for (int i = p0 + (int)threadIdx.x; i < p1; i += kNormBlock) {
+        double a = g[3 * (size_t)i], b = g[3 * (size_t)i + 1], c = g[3 * (size_t)i + 2];
+        v += a * a + b * b + c * c;
+    }
+    double sum = block_sum_f64(v, s_part);
+    if (threadIdx.x == 0) out[w] = sum;
 }
 
 // Iterate from a caller's guess: for solved particles (ACTIVE, inv_mass != 0)
@@ -2496,6 +2529,89 @@ int32_t vbd_iterate_to_residual(VbdContext* c, void* stream, float* q, float dt,
     }
     if (iterations_out) *iterations_out = done;
     if (norm_sq_out) *norm_sq_out = r;
+    return VBD_OK;
+}
+
+int32_t vbd_residual_worlds(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
+                            const int32_t* world_offsets, double* norm_sq_out) {
+    if (!c || !q || !world_offsets || !norm_sq_out) return fail(VBD_ERR_ARG, "vbd_residual_worlds: null argument");
+    if (world_count < 1) return fail(VBD_ERR_ARG, "vbd_residual_worlds: world_count must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    int32_t rc = vbd_residual(c, stream, q, dt, nullptr, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual_worlds, dim3(world_count), dim3(kNormBlock), 0, s, c->model.particle_count,
+                       world_offsets, c->g_tmp,
+                       norm_sq_out);
+    VBD_LAUNCH_CHECK("k_residual_worlds");
+    return VBD_OK;
+}
+
+namespace {
+// Installs a world mask in the context's kernel parameters for the lifetime of one driver call.
+struct WorldMask {
+    VbdContext* c;
+    WorldMask(VbdContext* ctx, const int* p_world, const int* w_active) : c(ctx) {
+        c->kp.p_world = p_world;
+        c->kp.w_active = w_active;
+    }
+    ~WorldMask() {
+        c->kp.p_world = nullptr;
+        c->kp.w_active = nullptr;
+    }
+};
+}  // namespace
+
+int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, float dt, int32_t world_count,
+                                       const int32_t* world_offsets, const int32_t* particle_world,
+                                       const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
+                                       double* norm_sq_dev, int32_t* world_active_dev, int32_t* iterations_out,
+                                       int32_t* world_iterations_out, double* norm_sq_out) {
+    if (!c || !q || !world_offsets || !particle_world || !target_norm_sq || !norm_sq_dev || !world_active_dev ||
+        !norm_sq_out)
+        return fail(VBD_ERR_ARG, "vbd_iterate_to_residual_worlds: null argument");
+    if (world_count < 1 || max_iterations < 0 || check_every < 1)
+        return fail(VBD_ERR_ARG,
+                    "vbd_iterate_to_residual_worlds: world_count and check_every must be >= 1, max_iterations >= 0");
+    if (c->self_on) return fail(VBD_ERR_STATE, "vbd_iterate_to_residual_worlds: not available with self-contact");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(world_count, 256)), dim3(256), 0, s, world_active_dev, 1,
+                       world_count);
+    VBD_LAUNCH_CHECK("k_fill_i32(world_active)");
+    WorldMask mask(c, particle_world, world_active_dev);
+    std::vector<int32_t> live((size_t)world_count, 1);
+    int32_t done = 0;
+    for (;;) {
+        int32_t rc = vbd_residual_worlds(c, stream, q, dt, world_count, world_offsets, norm_sq_dev);
+        if (rc) return rc;
+        VBD_HIP_TRY(hipMemcpyAsync(norm_sq_out, norm_sq_dev, sizeof(double) * (size_t)world_count,
+                                   hipMemcpyDeviceToHost, s));
+        VBD_HIP_TRY(hipStreamSynchronize(s));
+        int32_t n_live = 0;
+        bool changed = false;
+        for (int32_t w = 0; w < world_count; ++w) {
+            if (live[w] && norm_sq_out[w] <= target_norm_sq[w]) {
+                live[w] = 0;  // frozen: its iterate stays, its norm (from g at that iterate) stays valid
+                changed = true;
+                if (world_iterations_out) world_iterations_out[w] = done;
+            }
+            n_live += live[w];
+        }
+        if (n_live == 0 || done >= max_iterations) break;
+        if (changed) {
+            VBD_HIP_TRY(hipMemcpyAsync(world_active_dev, live.data(), sizeof(int32_t) * (size_t)world_count,
+                                       hipMemcpyHostToDevice, s));
+        }
+        int32_t n = std::min(check_every, max_iterations - done);
+        for (int32_t k = 0; k < n; ++k) {
+            rc = vbd_iteration(c, stream, q, dt);
+            if (rc) return rc;
+        }
+        done += n;
+    }
+    if (world_iterations_out)
+        for (int32_t w = 0; w < world_count; ++w)
+            if (live[w]) world_iterations_out[w] = done;
+    if (iterations_out) *iterations_out = done;
     return VBD_OK;
 }
 

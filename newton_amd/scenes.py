@@ -254,6 +254,65 @@ def cloth_wind(device="cuda:0", verts_per_side: int = 128, size: float = 1.0, to
     return model, state, info
 
 
+class BatchedFourierWind:
+    """One :class:`FourierWind` per world for models of equal-size worlds in contiguous particle ranges.
+
+    World ``w`` draws its modes from ``seed + w`` (world 0 has the wind of the single-world
+    scene); ``force(q, mass)`` evaluates every world's field at its own particles.
+    """
+
+    def __init__(self, device, worlds: int, per_world: int, seed: int = 3, **kw):
+        self.worlds, self.per_world = worlds, per_world
+        self.winds = [FourierWind(device, seed=seed + w, **kw) for w in range(worlds)]
+        self._stack()
+
+    def _stack(self) -> None:
+        self.k_t = torch.stack([w.k.T.contiguous() for w in self.winds])        # [W, 3, m]
+        self.phase = torch.stack([w.phase for w in self.winds])[:, None, :]     # [W, 1, m]
+        self.amp = torch.stack([w.amp for w in self.winds])                     # [W, m, 3]
+
+    def redraw(self, frame: int) -> None:
+        before = [w.epoch for w in self.winds]
+        for w in self.winds:
+            w.redraw(frame)
+        if [w.epoch for w in self.winds] != before:
+            self._stack()
+
+    def acceleration(self, q: torch.Tensor) -> torch.Tensor:
+        qw = q.reshape(self.worlds, self.per_world, 3).contiguous()
+        s = torch.sin(torch.bmm(qw, self.k_t) + self.phase)                     # [W, n, m]
+        return torch.bmm(s, self.amp).reshape(-1, 3)
+
+    def force(self, q: torch.Tensor, mass: torch.Tensor) -> torch.Tensor:
+        return self.acceleration(q) * mass[:, None]
+
+
+def cloth_wind_batched(device="cuda:0", worlds: int = 64, verts_per_side: int = 128, size: float = 1.0,
+                       top: float = 1.5, seed: int = 3):
+    """``worlds`` copies of :func:`cloth_wind` in one model, each world driven by its own wind
+    (:class:`BatchedFourierWind`, world ``w`` seeded ``seed + w``); worlds colored independently and
+    merged with the reference's plan, one launch per color pass for all of them."""
+    n = verts_per_side - 1
+    t = ModelBuilder(up_axis="Y", gravity=-9.81)
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    t.add_cloth_mesh(pos=(-0.5 * size, top - size, 0.0), rot=(0.0, 0.0, 0.0, 1.0), scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
+                     edge_ke=1.0, edge_kd=0.0)
+    top_row = n * (n + 1) + np.arange(n + 1)
+    t.particle_mass[top_row] = 0.0
+    t.particle_flags[top_row] = int(ParticleFlags.ACTIVE) & ~int(ParticleFlags.ACTIVE)
+    t.color(include_bending=True)
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    for _ in range(worlds):
+        b.add_world(t)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+    per = verts_per_side * verts_per_side
+    wind = BatchedFourierWind(device, worlds, per, seed=seed, length=size)
+    info = {"workload": f"cloth_wind_batched_{worlds}w_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": 10, "wind": wind}
+    return model, state, info
+
+
 def soft_cubes_batched(device="cuda:0", worlds: int = 512, cells_min: int = 6, cells_max: int = 12,
                        cell: float = 0.05, seed: int = 4, iterations: int = 8, particle_radius: float = 0.01):
     """Many independent tetrahedral cubes (5 tets per hex cell) dropped onto one ground plane.

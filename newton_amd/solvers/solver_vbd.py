@@ -444,6 +444,140 @@ class SolverVBD:
         )
         return int(its.value), float(r.value)
 
+    # ------------------------------------------------------------ per-world residuals
+    def world_offsets(self) -> torch.Tensor:
+        """Device int32 [world_count + 1]: world w owns particles ``off[w] <= i < off[w+1]``.
+
+        Requires every particle in a world and the worlds in contiguous, ascending particle
+        ranges (the layout :meth:`ModelBuilder.add_world` produces); raises ``ValueError`` otherwise.
+        """
+        if getattr(self, "_world_off", None) is None:
+            m = self.model
+            pw = m.particle_world.cpu().numpy().astype(np.int64) if m.particle_count else np.zeros(0, np.int64)
+            nw = max(int(m.world_count), 1)
+            if pw.size and (pw.min() < 0 or (np.diff(pw) < 0).any() or pw.max() >= nw):
+                raise ValueError("per-world residuals need every particle in a world, in ascending world order")
+            off = np.searchsorted(pw, np.arange(nw + 1), side="left").astype(np.int32)
+            self._world_off = torch.as_tensor(off, device=m.device)
+            self._world_of_particle = torch.as_tensor(pw, device=m.device)
+            self._world_of_particle_i32 = self._world_of_particle.to(torch.int32).contiguous()
+            self._world_norm_dev = torch.empty(nw, dtype=torch.float64, device=m.device)
+            self._world_active_dev = torch.empty(nw, dtype=torch.int32, device=m.device)
+        return self._world_off
+
+    def residual_worlds(self, state_in: State, dt: float) -> torch.Tensor:
+        """Squared force-residual norm of every world (device double tensor [world_count])."""
+        off = self.world_offsets()
+        out = torch.empty(off.numel() - 1, dtype=torch.float64, device=self.model.device)
+        _native.check(
+            _native.hip_lib().vbd_residual_worlds(self._ctx, current_stream_handle(self.model.device),
+                                                  _native.ptr(state_in.particle_q), float(dt), off.numel() - 1,
+                                                  _native.ptr(off), _native.ptr(out)),
+            "vbd_residual_worlds",
+        )
+        return out
+
+    def iterate_to_residual_worlds(self, state_in: State, dt: float, target_norm_sq, max_iterations: int = 200,
+                                   check_every: int = 1, per_world: bool = False):
+        """Iterate every world until its ``|g_w|^2 <= target_norm_sq[w]``, then freeze it (its vertices
+        are no longer solved, so its iterate is the one a solve of that world alone stops at);
+        host-synchronous.
+
+        Returns ``(iterations run, last squared norms [world_count])``, plus each world's iteration
+        count when ``per_world``.
+        """
+        off = self.world_offsets()
+        nw = off.numel() - 1
+        tgt = np.ascontiguousarray(np.broadcast_to(np.asarray(target_norm_sq, dtype=np.float64), (nw,)))
+        out = np.zeros(nw, dtype=np.float64)
+        wits = np.zeros(nw, dtype=np.int32)
+        its = ctypes.c_int32(0)
+        dptr = ctypes.POINTER(ctypes.c_double)
+        _native.check(
+            _native.hip_lib().vbd_iterate_to_residual_worlds(
+                self._ctx, current_stream_handle(self.model.device), _native.ptr(state_in.particle_q), float(dt), nw,
+                _native.ptr(off), _native.ptr(self._world_of_particle_i32), tgt.ctypes.data_as(dptr),
+                int(max_iterations), int(check_every), _native.ptr(self._world_norm_dev),
+                _native.ptr(self._world_active_dev), ctypes.byref(its), wits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                out.ctypes.data_as(dptr)),
+            "vbd_iterate_to_residual_worlds",
+        )
+        if per_world:
+            return int(its.value), out, wits
+        return int(its.value), out
+
+    def step_to_residual_worlds(self, state_in: State, state_out: State, control: Control | None,
+                                contacts: Contacts | None, dt: float, rel_tol: float = 1.0e-3,
+                                max_iterations: int = 200, check_every: int = 1, warm_start=None,
+                                trial_iterations: int = 0) -> dict:
+        """:meth:`step_to_residual` for a model of several worlds, each judged by its own residual.
+
+        Each world is frozen once ``r_w <= rel_tol * r0_w`` (its vertices are no longer solved, and a
+        batch of frozen vertices costs one header read), so each world's result is the iterate a solve
+        of that world alone stops at; the batch runs until all worlds are frozen (or
+        ``max_iterations``).  ``world_iterations`` reports each world's count.  Warm-start candidates are
+        chosen per world: after ``trial_iterations`` iterations from each start (0: at the start
+        itself), each world keeps the start with its lowest residual (a candidate must beat the
+        inertial start strictly); the mixed iterate then restarts through :meth:`set_iterate`.
+        """
+        self.world_offsets()
+        self.begin_step(state_in, contacts, dt)
+        r0 = self.residual_worlds(state_in, dt).cpu().numpy()
+        nw = r0.size
+        n_cand = 0
+        chosen = np.full(nw, -1, dtype=np.int64)
+        trial_its = 0
+        if warm_start is not None:
+            guesses = warm_start(self, state_in, dt)
+            if guesses is not None and not isinstance(guesses, (list, tuple)):
+                guesses = [guesses]
+            guesses = list(guesses or [])
+            n_cand = len(guesses)
+            if guesses:
+                wp = self._world_of_particle
+                if trial_iterations > 0:
+                    # each start runs k iterations; its iterate is kept for the worlds it wins
+                    self.iterate(state_in, dt, trial_iterations)
+                    best_r = self.residual_worlds(state_in, dt).cpu().numpy()
+                    best_q = state_in.particle_q.clone()
+                    for k, g in enumerate(guesses):
+                        self.set_iterate(state_in, g, commit=True)
+                        self.iterate(state_in, dt, trial_iterations)
+                        r = self.residual_worlds(state_in, dt).cpu().numpy()
+                        better = r < best_r
+                        if better.any():
+                            best_r = np.where(better, r, best_r)
+                            chosen[better] = k
+                            take = torch.as_tensor(better, device=wp.device)[wp]
+                            best_q = torch.where(take[:, None], state_in.particle_q, best_q)
+                    trial_its = trial_iterations * (1 + n_cand)
+                    self.set_iterate(state_in, best_q, commit=True)
+                else:
+                    # judged at the starts themselves; the solver state is untouched unless a
+                    # candidate wins somewhere, and then the mixed guess restarts every world
+                    best_r = r0.copy()
+                    best_g = state_in.particle_q.clone()
+                    trial = None
+                    for k, g in enumerate(guesses):
+                        trial = self.set_iterate(state_in, g, commit=False, out=trial)
+                        r = self.residual_worlds(_TrialState(trial), dt).cpu().numpy()
+                        better = r < best_r
+                        if better.any():
+                            best_r = np.where(better, r, best_r)
+                            chosen[better] = k
+                            take = torch.as_tensor(better, device=wp.device)[wp]
+                            best_g = torch.where(take[:, None], g, best_g)
+                    if (chosen >= 0).any():
+                        self.set_iterate(state_in, best_g, commit=True)
+        its, r_sq, wits = self.iterate_to_residual_worlds(state_in, dt, (rel_tol * rel_tol) * r0,
+                                                          max(max_iterations - trial_iterations, 0) if trial_its else
+                                                          max_iterations, check_every, per_world=True)
+        self.end_step(state_in, state_out, dt)
+        return {"iterations": its + trial_its, "world_iterations": wits + trial_its, "r0": np.sqrt(r0),
+                "r": np.sqrt(r_sq),
+                "reached": int((r_sq <= (rel_tol * rel_tol) * r0).sum()), "worlds": nw,
+                "candidate_chosen": [int((chosen == k).sum()) for k in range(n_cand)]}
+
     def step_to_residual(self, state_in: State, state_out: State, control: Control | None,
                          contacts: Contacts | None, dt: float, rel_tol: float = 1.0e-3, max_iterations: int = 200,
                          check_every: int = 1, warm_start=None, trial_iterations: int = 0) -> dict:

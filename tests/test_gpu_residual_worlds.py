@@ -1,0 +1,147 @@
+"""GPU: per-world force residuals and the batched iterate-to-residual driver.
+
+Per-world squared norms must equal the sums of |g_i|^2 of the FP32 oracle's residual over each
+world's particle range; the batched driver must stop at the first iteration at which every
+world is within its own tolerance; per-world warm-start selection must pick a candidate only in
+the worlds where it lowers the residual.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _wind(dev, worlds=3, side=16):
+    from newton_amd import scenes
+
+    model, state, info = scenes.cloth_wind_batched(dev, worlds=worlds, verts_per_side=side)
+    state.particle_f.copy_(info["wind"].force(state.particle_q, model.particle_mass))
+    return model, state, info
+
+
+def test_residual_worlds_equals_oracle_range_sums(cuda_device):
+    import torch
+
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _wind(cuda_device)
+    dt = info["dt"]
+    solver = SolverVBD(model, iterations=2)
+    solver.begin_step(state, None, dt)
+    solver.iterate(state, dt, 2)
+    rw = solver.residual_worlds(state, dt).cpu().numpy()
+    torch.cuda.synchronize()
+    om = oracle.OracleModel(model)
+    g_ref, _ = oracle.residual(om, oracle.params(2), None, state.particle_q.cpu().numpy(),
+                               solver.scratch("q_prev").cpu().numpy(), solver.scratch("inertia").cpu().numpy(), dt,
+                               precision="f32")
+    off = solver.world_offsets().cpu().numpy()
+    assert off.tolist() == [0, 256, 512, 768]
+    g64 = g_ref.astype(np.float64)
+    ref = np.array([(g64[off[w]:off[w + 1]] ** 2).sum() for w in range(3)])
+    np.testing.assert_allclose(rw, ref, rtol=1e-12)
+    assert len(set(rw.tolist())) == 3  # three different winds
+    total = float(solver.residual(state, dt).item())
+    assert abs(rw.sum() - total) <= 1e-12 * total
+
+
+def test_batched_driver_stops_at_first_iteration_all_worlds_reach(cuda_device):
+    import torch
+
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _wind(cuda_device)
+    dt = info["dt"]
+    solver = SolverVBD(model, iterations=2)
+    q0 = state.particle_q.clone()
+    solver.begin_step(state, None, dt)
+    r0 = solver.residual_worlds(state, dt).cpu().numpy()
+    target = (0.3 ** 2) * r0
+    its, r_sq = solver.iterate_to_residual_worlds(state, dt, target, max_iterations=300)
+    assert 0 < its < 300 and (r_sq <= target).all()
+    # the same substep with one iteration fewer leaves some world above its target
+    state.particle_q.copy_(q0)
+    solver.begin_step(state, None, dt)
+    solver.iterate(state, dt, its - 1)
+    r_prev = solver.residual_worlds(state, dt).cpu().numpy()
+    assert (r_prev > target).any()
+    # per-world counts: each world is within its target after its own count and (unless 0) not before
+    state.particle_q.copy_(q0)
+    solver.begin_step(state, None, dt)
+    its2, _, wits = solver.iterate_to_residual_worlds(state, dt, target, max_iterations=300, per_world=True)
+    torch.cuda.synchronize()
+    assert its2 == its and wits.max() == its
+    for w in range(3):
+        state.particle_q.copy_(q0)
+        solver.begin_step(state, None, dt)
+        solver.iterate(state, dt, int(wits[w]))
+        assert solver.residual_worlds(state, dt).cpu().numpy()[w] <= target[w]
+        if wits[w] > 0:
+            state.particle_q.copy_(q0)
+            solver.begin_step(state, None, dt)
+            solver.iterate(state, dt, int(wits[w]) - 1)
+            assert solver.residual_worlds(state, dt).cpu().numpy()[w] > target[w]
+
+
+def test_per_world_candidate_selection(cuda_device):
+    import torch
+
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _wind(cuda_device)
+    dt = info["dt"]
+    solver = SolverVBD(model, iterations=2)
+    s1 = model.state()
+    q0, qd0 = state.particle_q.clone(), state.particle_qd.clone()
+    # a candidate near each world's solution, spoiled in world 1
+    solver.begin_step(state, None, dt)
+    solver.iterate(state, dt, 60)
+    good = state.particle_q.clone()
+    off = solver.world_offsets().cpu().numpy()
+    gen = torch.Generator(device=cuda_device)
+    gen.manual_seed(0)
+    good[off[1]:off[2]] += 0.05 * torch.randn((off[2] - off[1], 3), device=cuda_device, generator=gen)
+    for trial in (0, 2):
+        state.particle_q.copy_(q0)
+        state.particle_qd.copy_(qd0)
+        r = solver.step_to_residual_worlds(state, s1, None, None, dt, rel_tol=0.3, max_iterations=300,
+                                           warm_start=lambda *_: [good], trial_iterations=trial)
+        assert r["candidate_chosen"] == [2], (trial, r)
+        assert r["reached"] == 3
+        assert np.isfinite(s1.particle_q.cpu().numpy()).all()
+
+
+def test_frozen_worlds_equal_single_world_solves(cuda_device):
+    """Each world of the batched driver ends at the iterate a single-world solve with that world's
+    iteration count reaches, bit for bit (worlds are frozen once they reach their tolerance)."""
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = _wind(cuda_device, worlds=4)
+    dt = info["dt"]
+    solver = SolverVBD(model, iterations=2)
+    q0, qd0, f0 = state.particle_q.clone(), state.particle_qd.clone(), state.particle_f.clone()
+    s1 = model.state()
+    r = solver.step_to_residual_worlds(state, s1, None, None, dt, rel_tol=0.2, max_iterations=400)
+    wits = r["world_iterations"]
+    assert r["reached"] == 4 and len(set(wits.tolist())) > 1, wits  # worlds froze at different iterations
+    one, one_state, _ = scenes.cloth_wind(cuda_device, verts_per_side=16)
+    one_solver = SolverVBD(one, iterations=2)
+    one_out = one.state()
+    per = one.particle_count
+    for w in range(4):
+        sl = slice(w * per, (w + 1) * per)
+        one_state.particle_q.copy_(q0[sl])
+        one_state.particle_qd.copy_(qd0[sl])
+        one_state.particle_f.copy_(f0[sl])
+        one_solver.begin_step(one_state, None, dt)
+        one_solver.iterate(one_state, dt, int(wits[w]))
+        one_solver.end_step(one_state, one_out, dt)
+        torch.cuda.synchronize()
+        assert torch.equal(one_out.particle_q, s1.particle_q[sl]), w
+        assert torch.equal(one_out.particle_qd, s1.particle_qd[sl]), w
