@@ -212,6 +212,7 @@ def pmc_mode(args):
     contacts = pipe.contacts()
     state.clear_forces()
     pipe.collide(state, contacts)
+    tune = solver.autotune(state, contacts, info["dt"])
     solver.begin_step(state, contacts, info["dt"])
     ncol = len(model.particle_color_groups)
     for _ in range(args.roofline_reps):
@@ -223,7 +224,8 @@ def pmc_mode(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     b = color_pass_bytes(model, contacts_per_particle=_contact_counts(model, contacts))
-    print(json.dumps({"pmc_mode": True, "workload": info["workload"], "color_pass_launches": args.roofline_reps * ncol,
+    print(json.dumps({"pmc_mode": True, "workload": info["workload"], "autotune": tune,
+                      "color_pass_launches": args.roofline_reps * ncol,
                       "algorithmic_bytes_per_color_pass": [int(x) for x in b], "copy_bytes": nbytes}))
 
 
@@ -583,7 +585,7 @@ def batched_residual(args, s_single, ws):
             wsb.observe(solver, s0)
             s0, s1 = s1, s0
     variants = ("plain", "learned_trial")
-    res = {k: {"iters": [], "s": 0.0, "reached": 0, "chosen": None} for k in variants}
+    res = {k: {"iters": [], "witers": [], "s": 0.0, "reached": 0, "chosen": None} for k in variants}
     one_model, one_state, _ = scenes.cloth_wind(dev)
     one_solver = SolverVBD(one_model, iterations=10)
     one_out = one_model.state()
@@ -625,6 +627,7 @@ def batched_residual(args, s_single, ws):
                 d = res[name]
                 d["s"] += time.perf_counter() - t0
                 d["iters"].append(r["iterations"])
+                d["witers"].append(r["world_iterations"])
                 d["reached"] += r["reached"]
                 if r["candidate_chosen"]:
                     d["chosen"] = r["candidate_chosen"] if d["chosen"] is None else \
@@ -640,7 +643,10 @@ def batched_residual(args, s_single, ws):
                                  "vertex_steps_per_s": round(single_vs, 1)}
     for name in variants:
         d = res[name]
-        out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
+        wit = np.stack(d["witers"])
+        out[name] = {"mean_iterations_launched": round(float(np.mean(d["iters"])), 2),
+                     "mean_world_iterations": round(float(wit.mean()), 2),
+                     "mean_slowest_world_iterations": round(float(wit.max(axis=1).mean()), 2),
                      "reached_tol": f"{d['reached']}/{substeps * W} world-substeps",
                      "vertex_steps_per_s": round(n * substeps / d["s"], 1)}
         if d["chosen"] is not None:
@@ -708,11 +714,15 @@ def main():
     contacts = pipe.contacts()
     q_init = state0.particle_q.cpu().numpy().copy()
     qd_init = state0.particle_qd.cpu().numpy().copy()
-    _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames")
+    state0.clear_forces()
+    pipe.collide(state0, contacts)
+    tune = solver.autotune(state0, contacts, info["dt"])
+    _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames, shape {tune}")
     gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
     value = n * sub * args.steps / (gpu_ms / 1e3)
     roofline = color_roofline(model, last, info, solver, pipe, contacts, args.roofline_reps)
     roofline["launches_per_substep"] = len(model.particle_color_groups) * args.iterations
+    roofline["pass_shape"] = tune
     _progress(f"headline: {value:.4g} vertex-steps/s, color pass {roofline['avg_launch_us']} us")
     if args.traffic:
         _progress("HBM traffic: two rocprofv3 --pmc child runs")

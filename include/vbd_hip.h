@@ -175,14 +175,20 @@ int32_t vbd_rebuild(VbdContext* ctx);
 /* Layout statistics: out4 = {fast batches, generic-path vertices, packed element items, colors}. */
 int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
 
-/* Autotuner of the color-pass work-group shape: builds the layout for 256-lane
- * work-groups (batches of <= 21 vertices / 256 element items) and for 128-lane
- * ones (<= 10 vertices / 128 items), times three force-assembly passes over
- * every color with each (q is only read; call after vbd_contact_prepare and
- * vbd_forward), and keeps the faster.  block_out receives 256 or 128, ms_out[2]
- * the two per-pass-set times.  Results do not depend on the shape.  No
+/* Autotuner of the color-pass shape: builds the layout for 256-lane work-groups
+ * (batches of <= 21 vertices / 256 element items) and for 128-lane ones (<= 10
+ * vertices / 128 items), times three force-assembly passes over every color with
+ * each candidate -- 256 lanes at the compiler's register budget, 256 lanes at a
+ * seven-waves-per-SIMD budget, 128 lanes (q is only read; call after
+ * vbd_contact_prepare and vbd_forward) -- and keeps the fastest.  block_out[2]
+ * receives the lanes (256 or 128) and the waves target (1: compiler's choice, 7),
+ * ms_out[3] the three per-pass-set times.  Results do not depend on the shape.  No
  * counterpart in the reference; synchronises and allocates (not for capture). */
 int32_t vbd_autotune(VbdContext* ctx, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out);
+
+/* Register-budget occupancy target of the 256-lane color pass: 1 (compiler's
+ * choice) or 7 waves per SIMD; what vbd_autotune chooses between. */
+int32_t vbd_set_pass_waves(VbdContext* ctx, int32_t waves);
 
 /* Selects the work-group shape (256 or 128 lanes) directly, rebuilding the
  * layout; the contact ranges of the current substep are carried over.  Not for
@@ -236,17 +242,19 @@ int32_t vbd_residual_worlds(VbdContext* ctx, void* stream, const float* q, float
                             const int32_t* world_offsets, double* norm_sq_out);
 
 /* vbd_iterate_to_residual over worlds: each world is iterated until its squared norm is
- * <= target_norm_sq[w] (host array), then frozen -- its vertices are neither solved nor
- * re-evaluated, so its iterate is the one a solve of that world alone stops at -- until all
- * worlds are frozen or max_iterations were run.  particle_world (device int32 [N]) gives
- * each particle's world, consistent with world_offsets; norm_sq_dev (world_count doubles)
- * and world_active_dev (world_count int32) are device scratch.  Reports the iterations run,
- * optionally each world's count (world_iterations_out, host), and the last norms
- * (norm_sq_out, host).  Host-synchronous; not available with self-contact. */
+ * <= target_norm_sq[w] (host array; evaluated every check_every iterations), then frozen --
+ * its vertices are neither solved nor re-evaluated, so its iterate is the one a solve of
+ * that world alone stops at -- until all worlds are frozen or max_iterations were run.  The
+ * freezing is decided on the device; the host polls the live-world count every 8
+ * evaluations.  particle_world (device int32 [N]) gives each particle's world, consistent
+ * with world_offsets; world_scratch_dev (2 * world_count doubles) and world_state_dev
+ * (2 * world_count + 1 int32) are device scratch.  Reports the iterations run, optionally
+ * each world's count (world_iterations_out, host), and the last norms (norm_sq_out, host).
+ * Host-synchronous; not available with self-contact. */
 int32_t vbd_iterate_to_residual_worlds(VbdContext* ctx, void* stream, float* q, float dt, int32_t world_count,
                                        const int32_t* world_offsets, const int32_t* particle_world,
                                        const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
-                                       double* norm_sq_dev, int32_t* world_active_dev, int32_t* iterations_out,
+                                       double* world_scratch_dev, int32_t* world_state_dev, int32_t* iterations_out,
                                        int32_t* world_iterations_out, double* norm_sq_out);
 
 /* Copies the solver's scratch state of the current substep ([N,3] floats)

@@ -149,8 +149,9 @@ HIP_EXPORTS = {
     "vbd_destroy": (c_int32, [c_void_p]),
     "vbd_rebuild": (c_int32, [c_void_p]),
     "vbd_layout_stats": (c_int32, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
-    "vbd_autotune": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, ctypes.POINTER(c_int32), ctypes.POINTER(c_float)]),
+    "vbd_autotune": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "vbd_set_block_shape": (c_int32, [c_void_p, c_void_p, c_int32]),
+    "vbd_set_pass_waves": (c_int32, [c_void_p, c_int32]),
     "vbd_step": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc), c_float],

@@ -828,12 +828,13 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
 constexpr int kBatchHdr = vbd_layout::kBatchWords;
 
 // BLOCK lanes per work-group: 256 (batches of <= 21 vertices / 256 items) or
-// 128 (<= 10 vertices / 128 items); the autotuner picks one per model.  LDS holds the static
-// items plus 64 contact rows (16.7 KB at 256 lanes), and the register budget is set for seven
-// waves per SIMD: occupancy is then bound by neither, and the few values the compiler keeps in
-// scratch cost less than the waves gained (the pass is VALU- and latency-bound).
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
+// 128 (<= 10 vertices / 128 items).  LDS holds the static items plus 64 contact rows (17 KB at
+// 256 lanes), so LDS does not bound occupancy; WAVES is the register budget's occupancy target:
+// 1 leaves it to the compiler (77 VGPRs, six waves per SIMD), 7 fits seven waves per SIMD at the
+// price of a few values kept in scratch -- faster on cloth, slower on tets (whose evaluation then
+// spills more).  The autotuner picks the shape per model.
+template <int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
                                                       float dt, int mode, float* __restrict__ g_out) {
     constexpr int MAXV = BLOCK / kComp;
     constexpr int ROWS = BLOCK + 64;  // static items (<= BLOCK) + contacts evaluated in parallel
@@ -1589,6 +1590,7 @@ struct VbdContext {
     int* bat_cinfo = nullptr;  // [B, 2] per-substep contact range of each batch
     int* bat_coff = nullptr;   // [B, max_v + 1] per-substep contact offsets within the batch
     int block = 256;           // color-pass work-group shape (256 or 128 lanes)
+    int waves = 1;             // occupancy target of the 256-lane pass (1: compiler's choice, or 7)
     // cloth self-contact (vbd_self.h)
     bool self_on = false;
     SelfParams sp{};
@@ -2304,9 +2306,11 @@ int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, cons
 namespace {
 int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, float dt, int mode, float* g) {
     if (c->block == 128) {
-        hipLaunchKernelGGL(k_solve_fast<128>, dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
+        hipLaunchKernelGGL((k_solve_fast<128, 1>), dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
+    } else if (c->waves == 7) {
+        hipLaunchKernelGGL((k_solve_fast<256, 7>), dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
     } else {
-        hipLaunchKernelGGL(k_solve_fast<256>, dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
+        hipLaunchKernelGGL((k_solve_fast<256, 1>), dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
     }
     VBD_LAUNCH_CHECK("k_solve_fast");
     return VBD_OK;
@@ -2416,21 +2420,31 @@ int32_t vbd_set_block_shape(VbdContext* c, void* stream, int32_t block) {
     return VBD_OK;
 }
 
+int32_t vbd_set_pass_waves(VbdContext* c, int32_t waves) {
+    if (!c || (waves != 1 && waves != 7)) return fail(VBD_ERR_ARG, "vbd_set_pass_waves: waves must be 1 or 7");
+    c->waves = waves;
+    return VBD_OK;
+}
+
 int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out) {
     if (!c || !q || !(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_autotune: bad argument");
     hipStream_t s = (hipStream_t)stream;
-    const int shapes[2] = {256, 128};
-    float ms[2] = {0.0f, 0.0f};
+    const int shapes[3] = {256, 256, 128};
+    const int waves[3] = {1, 7, 1};
+    float ms[3] = {0.0f, 0.0f, 0.0f};
     hipEvent_t e0, e1;
     VBD_HIP_TRY(hipEventCreate(&e0));
     VBD_HIP_TRY(hipEventCreate(&e1));
     int32_t rc = VBD_OK;
-    for (int k = 0; k < 2 && rc == VBD_OK; ++k) {
-        c->block = shapes[k];
-        rc = build_layout(c);
-        if (rc) break;
-        rc = launch_batch_contacts(c, s);
-        if (rc) break;
+    for (int k = 0; k < 3 && rc == VBD_OK; ++k) {
+        c->waves = waves[k];
+        if (k == 0 || shapes[k] != shapes[k - 1]) {
+            c->block = shapes[k];
+            rc = build_layout(c);
+            if (rc) break;
+            rc = launch_batch_contacts(c, s);
+            if (rc) break;
+        }
         // warm, then time three force-assembly passes over every color (q is only read)
         for (int rep = 0; rep < 4 && rc == VBD_OK; ++rep) {
             if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = fail(VBD_ERR_HIP, "vbd_autotune: event");
@@ -2446,20 +2460,24 @@ int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int3
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
-    const int best = ms[1] < ms[0] ? 128 : 256;
-    if (c->block != best) {
-        c->block = best;
+    int best = 0;
+    for (int k = 1; k < 3; ++k)
+        if (ms[k] < ms[best]) best = k;
+    c->waves = waves[best];
+    if (c->block != shapes[best]) {
+        c->block = shapes[best];
         rc = build_layout(c);
         if (rc) return rc;
         rc = launch_batch_contacts(c, s);
         if (rc) return rc;
         VBD_HIP_TRY(hipStreamSynchronize(s));
     }
-    if (block_out) *block_out = best;
-    if (ms_out) {
-        ms_out[0] = ms[0];
-        ms_out[1] = ms[1];
+    if (block_out) {
+        block_out[0] = shapes[best];
+        block_out[1] = waves[best];
     }
+    if (ms_out)
+        for (int k = 0; k < 3; ++k) ms_out[k] = ms[k];
     return VBD_OK;
 }
 
@@ -2561,12 +2579,51 @@ struct WorldMask {
 };
 }  // namespace
 
+// Freezes the worlds whose squared norm reached their target after `done` iterations (one
+// work-group; st = [active W | iterations W | live count]).  Frozen worlds stay frozen.
+__global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __restrict__ norm,
+                                                      const double* __restrict__ target, int* __restrict__ st,
+                                                      int done) {
+    __shared__ int s_live[1024 / 64];
+    int live = 0;
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+        int a = st[w];
+        if (a && norm[w] <= target[w]) {
+            a = 0;
+            st[w] = 0;
+            st[nw + w] = done;
+        }
+        live += a;
+    }
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_down(live, off, 64);
+    if ((threadIdx.x & 63) == 0) s_live[threadIdx.x >> 6] = live;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) sum += s_live[k];
+        st[2 * nw] = sum;
+    }
+}
+
+__global__ void k_world_init(int nw, int* __restrict__ st) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw) {
+        st[i] = 1;
+        st[nw + i] = 0;
+    }
+    if (i == 0) st[2 * nw] = nw;
+}
+
+// Iterations between the host's polls of the live-world count; the freezing itself is decided
+// on the device after every evaluation, so the poll interval only bounds the idle tail.
+constexpr int kWorldPoll = 8;
+
 int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, float dt, int32_t world_count,
                                        const int32_t* world_offsets, const int32_t* particle_world,
                                        const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
-                                       double* norm_sq_dev, int32_t* world_active_dev, int32_t* iterations_out,
+                                       double* world_scratch_dev, int32_t* world_state_dev, int32_t* iterations_out,
                                        int32_t* world_iterations_out, double* norm_sq_out) {
-    if (!c || !q || !world_offsets || !particle_world || !target_norm_sq || !norm_sq_dev || !world_active_dev ||
+    if (!c || !q || !world_offsets || !particle_world || !target_norm_sq || !world_scratch_dev || !world_state_dev ||
         !norm_sq_out)
         return fail(VBD_ERR_ARG, "vbd_iterate_to_residual_worlds: null argument");
     if (world_count < 1 || max_iterations < 0 || check_every < 1)
@@ -2574,43 +2631,44 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
                     "vbd_iterate_to_residual_worlds: world_count and check_every must be >= 1, max_iterations >= 0");
     if (c->self_on) return fail(VBD_ERR_STATE, "vbd_iterate_to_residual_worlds: not available with self-contact");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(world_count, 256)), dim3(256), 0, s, world_active_dev, 1,
-                       world_count);
-    VBD_LAUNCH_CHECK("k_fill_i32(world_active)");
-    WorldMask mask(c, particle_world, world_active_dev);
-    std::vector<int32_t> live((size_t)world_count, 1);
-    int32_t done = 0;
+    const int W = world_count;
+    double* norm = world_scratch_dev;
+    double* target = world_scratch_dev + W;
+    VBD_HIP_TRY(hipMemcpyAsync(target, target_norm_sq, sizeof(double) * (size_t)W, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_world_init, dim3(blocks_for(W, 256)), dim3(256), 0, s, W, world_state_dev);
+    VBD_LAUNCH_CHECK("k_world_init");
+    WorldMask mask(c, particle_world, world_state_dev);
+    int32_t done = 0, live = W;
+    int32_t rc = vbd_residual_worlds(c, stream, q, dt, W, world_offsets, norm);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0);
+    VBD_LAUNCH_CHECK("k_world_freeze");
+    int32_t since_poll = kWorldPoll;  // poll before the first iteration
     for (;;) {
-        int32_t rc = vbd_residual_worlds(c, stream, q, dt, world_count, world_offsets, norm_sq_dev);
-        if (rc) return rc;
-        VBD_HIP_TRY(hipMemcpyAsync(norm_sq_out, norm_sq_dev, sizeof(double) * (size_t)world_count,
-                                   hipMemcpyDeviceToHost, s));
-        VBD_HIP_TRY(hipStreamSynchronize(s));
-        int32_t n_live = 0;
-        bool changed = false;
-        for (int32_t w = 0; w < world_count; ++w) {
-            if (live[w] && norm_sq_out[w] <= target_norm_sq[w]) {
-                live[w] = 0;  // frozen: its iterate stays, its norm (from g at that iterate) stays valid
-                changed = true;
-                if (world_iterations_out) world_iterations_out[w] = done;
-            }
-            n_live += live[w];
+        if (since_poll >= kWorldPoll || done >= max_iterations) {
+            VBD_HIP_TRY(hipMemcpyAsync(&live, world_state_dev + 2 * W, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            VBD_HIP_TRY(hipStreamSynchronize(s));
+            since_poll = 0;
+            if (live == 0 || done >= max_iterations) break;
         }
-        if (n_live == 0 || done >= max_iterations) break;
-        if (changed) {
-            VBD_HIP_TRY(hipMemcpyAsync(world_active_dev, live.data(), sizeof(int32_t) * (size_t)world_count,
-                                       hipMemcpyHostToDevice, s));
-        }
-        int32_t n = std::min(check_every, max_iterations - done);
+        const int32_t n = std::min(check_every, max_iterations - done);
         for (int32_t k = 0; k < n; ++k) {
             rc = vbd_iteration(c, stream, q, dt);
             if (rc) return rc;
         }
         done += n;
+        rc = vbd_residual_worlds(c, stream, q, dt, W, world_offsets, norm);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, done);
+        VBD_LAUNCH_CHECK("k_world_freeze");
+        ++since_poll;
     }
+    VBD_HIP_TRY(hipMemcpyAsync(norm_sq_out, norm, sizeof(double) * (size_t)W, hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> st((size_t)(2 * W));
+    VBD_HIP_TRY(hipMemcpyAsync(st.data(), world_state_dev, sizeof(int32_t) * (size_t)(2 * W), hipMemcpyDeviceToHost, s));
+    VBD_HIP_TRY(hipStreamSynchronize(s));
     if (world_iterations_out)
-        for (int32_t w = 0; w < world_count; ++w)
-            if (live[w]) world_iterations_out[w] = done;
+        for (int w = 0; w < W; ++w) world_iterations_out[w] = st[w] ? done : st[W + w];
     if (iterations_out) *iterations_out = done;
     return VBD_OK;
 }

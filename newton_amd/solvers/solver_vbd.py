@@ -461,8 +461,8 @@ class SolverVBD:
             self._world_off = torch.as_tensor(off, device=m.device)
             self._world_of_particle = torch.as_tensor(pw, device=m.device)
             self._world_of_particle_i32 = self._world_of_particle.to(torch.int32).contiguous()
-            self._world_norm_dev = torch.empty(nw, dtype=torch.float64, device=m.device)
-            self._world_active_dev = torch.empty(nw, dtype=torch.int32, device=m.device)
+            self._world_scratch_dev = torch.empty(2 * nw, dtype=torch.float64, device=m.device)
+            self._world_state_dev = torch.empty(2 * nw + 1, dtype=torch.int32, device=m.device)
         return self._world_off
 
     def residual_worlds(self, state_in: State, dt: float) -> torch.Tensor:
@@ -497,8 +497,8 @@ class SolverVBD:
             _native.hip_lib().vbd_iterate_to_residual_worlds(
                 self._ctx, current_stream_handle(self.model.device), _native.ptr(state_in.particle_q), float(dt), nw,
                 _native.ptr(off), _native.ptr(self._world_of_particle_i32), tgt.ctypes.data_as(dptr),
-                int(max_iterations), int(check_every), _native.ptr(self._world_norm_dev),
-                _native.ptr(self._world_active_dev), ctypes.byref(its), wits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                int(max_iterations), int(check_every), _native.ptr(self._world_scratch_dev),
+                _native.ptr(self._world_state_dev), ctypes.byref(its), wits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                 out.ctypes.data_as(dptr)),
             "vbd_iterate_to_residual_worlds",
         )
@@ -676,7 +676,8 @@ class SolverVBD:
         self._material_key = None
 
     def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
-        """Pick the color-pass work-group shape (256 or 128 lanes) by timing both on this model.
+        """Pick the color-pass shape by timing the candidates on this model: 256-lane work-groups at the
+        compiler's register budget, 256 lanes at seven waves per SIMD, 128 lanes.
 
         Runs on a copy of ``state_in`` (prepared with contacts and the forward
         step); ``state_in`` is left untouched.  Results never depend on the shape.
@@ -687,12 +688,18 @@ class SolverVBD:
         trial.particle_qd = state_in.particle_qd.clone()
         trial.particle_f = state_in.particle_f.clone()
         self.begin_step(trial, contacts, dt)
-        block = ctypes.c_int32(0)
-        ms = (ctypes.c_float * 2)()
+        shape = (ctypes.c_int32 * 2)()
+        ms = (ctypes.c_float * 3)()
         _native.check(_native.hip_lib().vbd_autotune(self._ctx, current_stream_handle(self.model.device),
-                                                     _native.ptr(trial.particle_q), float(dt), ctypes.byref(block),
-                                                     ms), "vbd_autotune")
-        return {"block": int(block.value), "ms_256": float(ms[0]), "ms_128": float(ms[1])}
+                                                     _native.ptr(trial.particle_q), float(dt), shape, ms),
+                      "vbd_autotune")
+        return {"block": int(shape[0]), "waves": int(shape[1]), "ms_256": float(ms[0]), "ms_256_w7": float(ms[1]),
+                "ms_128": float(ms[2])}
+
+    def set_pass_waves(self, waves: int) -> None:
+        """Occupancy target of the 256-lane color pass: 1 (compiler's register budget) or 7 waves per
+        SIMD (what :meth:`autotune` also chooses between)."""
+        _native.check(_native.hip_lib().vbd_set_pass_waves(self._ctx, int(waves)), "vbd_set_pass_waves")
 
     def set_block_shape(self, block: int) -> None:
         """Use 256- or 128-lane color-pass work-groups (what :meth:`autotune` chooses between)."""

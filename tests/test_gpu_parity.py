@@ -323,6 +323,17 @@ def test_128_lane_block_shape_bitexact(cuda_device, scene):
     np.testing.assert_array_equal(qd_gpu, qd_ref)
 
 
+@pytest.mark.parametrize("scene", ["strip", "cube", "hanging"])
+def test_seven_wave_pass_bitexact(cuda_device, scene):
+    """The 256-lane pass built for seven waves per SIMD (a smaller register budget) gives the oracle's bits."""
+    model, state, info = _scene(scene, cuda_device)
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 6, waves=7)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 6)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+
+
 def test_autotune_keeps_results(cuda_device):
     """autotune() picks a shape and leaves the state alone; the trajectory stays bit-exact."""
     import torch
@@ -340,7 +351,8 @@ def test_autotune_keeps_results(cuda_device):
     before = state.particle_q.clone()
     choice = solver.autotune(state, contacts, info["dt"])
     torch.cuda.synchronize()
-    assert choice["block"] in (128, 256) and choice["ms_256"] > 0 and choice["ms_128"] > 0
+    assert choice["block"] in (128, 256) and choice["waves"] in (1, 7)
+    assert choice["ms_256"] > 0 and choice["ms_256_w7"] > 0 and choice["ms_128"] > 0
     assert torch.equal(state.particle_q, before)
     q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4, solver=solver)
     q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 4)

@@ -60,20 +60,11 @@ def test_batched_driver_stops_at_first_iteration_all_worlds_reach(cuda_device):
     solver.begin_step(state, None, dt)
     r0 = solver.residual_worlds(state, dt).cpu().numpy()
     target = (0.3 ** 2) * r0
-    its, r_sq = solver.iterate_to_residual_worlds(state, dt, target, max_iterations=300)
-    assert 0 < its < 300 and (r_sq <= target).all()
-    # the same substep with one iteration fewer leaves some world above its target
-    state.particle_q.copy_(q0)
-    solver.begin_step(state, None, dt)
-    solver.iterate(state, dt, its - 1)
-    r_prev = solver.residual_worlds(state, dt).cpu().numpy()
-    assert (r_prev > target).any()
-    # per-world counts: each world is within its target after its own count and (unless 0) not before
-    state.particle_q.copy_(q0)
-    solver.begin_step(state, None, dt)
-    its2, _, wits = solver.iterate_to_residual_worlds(state, dt, target, max_iterations=300, per_world=True)
-    torch.cuda.synchronize()
-    assert its2 == its and wits.max() == its
+    its, r_sq, wits = solver.iterate_to_residual_worlds(state, dt, target, max_iterations=300, per_world=True)
+    slowest = int(wits.max())
+    assert 0 < slowest <= its < 300 and (r_sq <= target).all()
+    assert its - slowest < 8  # the host polls the live count every 8 evaluations
+    # each world is within its target after its own count and (unless 0) not before
     for w in range(3):
         state.particle_q.copy_(q0)
         solver.begin_step(state, None, dt)
