@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -626,6 +628,7 @@ struct KParams {
     float* disp;
     // pass layout (vbd_layout.h)
     const int* batch_blk;
+    const int* batch_list;  // when set: launches enumerate these batch indices (live worlds only)
     const int* bat_cinfo;  // per batch {first packed contact, contact count} (k_batch_contacts)
     const int* bat_coff;   // per batch, max_v + 1 contact offsets relative to the first
     const int* vert_rec;
@@ -847,7 +850,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
     __shared__ float s_acc[MAXV * kComp];
 
     const int t = threadIdx.x;
-    const int b = batch_base + xcd_remap(blockIdx.x, nb);
+    const int bi = batch_base + xcd_remap(blockIdx.x, nb);
+    const int b = P.batch_list ? P.batch_list[bi] : bi;
     // ---- round trip 1
     const int* blk = P.batch_blk + (size_t)b * BLK;
     const int4 b0 = reinterpret_cast<const int4*>(blk)[0];
@@ -1556,6 +1560,14 @@ struct VbdContext {
     std::vector<int32_t> color_offsets;
     std::vector<int32_t> fast_color_off;
     std::vector<int32_t> slow_color_off;
+    // batched drivers: batches of live worlds only (see vbd_iterate_to_residual_worlds)
+    std::vector<int32_t> bat_pmin, bat_pmax;  // smallest / largest particle of each packed batch
+    std::vector<int32_t> bat_wlo, bat_whi;    // their worlds, for the offsets in bw_key
+    std::vector<int32_t> bw_key;
+    std::vector<int32_t> live_off;            // per color, into h_batch_list
+    std::vector<int32_t> h_batch_list;
+    int* batch_list = nullptr;                // device copy (capacity: every packed batch)
+    bool use_list = false;
     KParams kp;
     float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
     int* c_cnt = nullptr;       // [N+1]
@@ -1643,10 +1655,11 @@ std::string h2d(T** dst, const std::vector<T>& src) {
 void free_layout(VbdContext* c) {
     for (int** p : {&c->lay_batch, &c->lay_vert, &c->lay_tri, &c->lay_edge, &c->lay_edge_meta, &c->lay_tet,
                     &c->lay_tet_meta, &c->lay_slow, &c->bat_cinfo, &c->bat_coff, &c->lay_spr, &c->lay_pre,
-                    &c->lay_slow_spr_off, &c->lay_slow_spr}) {
+                    &c->lay_slow_spr_off, &c->lay_slow_spr, &c->batch_list}) {
         hipFree(*p);
         *p = nullptr;
     }
+    c->bw_key.clear();
     hipFree(c->spr_rows);
     c->spr_rows = nullptr;
 }
@@ -1733,6 +1746,24 @@ int32_t build_layout(VbdContext* c) {
     c->fast_items = L.fast_items;
     c->max_v = L.max_v;
     c->blk_words = L.blk_words;
+    {
+        const int nb = L.fast_color_off.back();
+        c->bat_pmin.assign((size_t)nb, 0);
+        c->bat_pmax.assign((size_t)nb, 0);
+        for (int b = 0; b < nb; ++b) {
+            const int32_t* blk = &L.batch_blk[(size_t)b * L.blk_words];
+            int lo = INT32_MAX, hi = -1;
+            for (int i = 0; i < blk[1]; ++i) {
+                const int v = blk[kBatchHdr + 4 * i];
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+            c->bat_pmin[b] = hi < 0 ? 0 : lo;
+            c->bat_pmax[b] = hi < 0 ? 0 : hi;
+        }
+        hipError_t e = hipMalloc(&c->batch_list, sizeof(int) * (size_t)std::max(nb, 1));
+        if (e != hipSuccess) return fail(VBD_ERR_HIP, std::string("vbd_create: batch list: ") + hipGetErrorString(e));
+    }
     {
         // contact ranges per batch: zero (no contacts) until vbd_contact_prepare fills them
         const size_t nbat = (size_t)std::max(L.fast_color_off.back(), 1);
@@ -2332,6 +2363,10 @@ int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt
         VBD_LAUNCH_CHECK("k_sc_rows");
     }
     int b0 = c->fast_color_off[color], b1 = c->fast_color_off[color + 1];
+    if (c->use_list) {
+        b0 = c->live_off[color];
+        b1 = c->live_off[color + 1];
+    }
     if (b1 > b0) {
         int32_t rc = launch_fast(c, s, b0, b1 - b0, q, dt, mode, g);
         if (rc) return rc;
@@ -2492,7 +2527,7 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
                            c->model.particle_count, dt);
         VBD_LAUNCH_CHECK("k_sc_rows");
     }
-    int nb = c->fast_color_off.back();
+    int nb = c->use_list ? c->live_off.back() : c->fast_color_off.back();
     if (nb > 0) {
         int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g);
         if (rc) return rc;
@@ -2575,8 +2610,49 @@ struct WorldMask {
     ~WorldMask() {
         c->kp.p_world = nullptr;
         c->kp.w_active = nullptr;
+        c->kp.batch_list = nullptr;
+        c->use_list = false;
     }
 };
+
+// Worlds of each packed batch's smallest and largest particle, for the given world offsets.
+void batch_worlds(VbdContext* c, const std::vector<int32_t>& off) {
+    if (c->bw_key == off) return;
+    const size_t nb = c->bat_pmin.size();
+    c->bat_wlo.resize(nb);
+    c->bat_whi.resize(nb);
+    auto world_of = [&](int p) {
+        // last w with off[w] <= p (clipped to the world range)
+        int w = (int)(std::upper_bound(off.begin(), off.end(), p) - off.begin()) - 1;
+        return std::min(std::max(w, 0), (int)off.size() - 2);
+    };
+    for (size_t b = 0; b < nb; ++b) {
+        c->bat_wlo[b] = world_of(c->bat_pmin[b]);
+        c->bat_whi[b] = world_of(c->bat_pmax[b]);
+    }
+    c->bw_key = off;
+}
+
+// Rebuilds the live-batch list (batches with a vertex in a live world, per color) and uploads it.
+int32_t build_live_list(VbdContext* c, hipStream_t s, const std::vector<int32_t>& live) {
+    const int C = c->model.color_count;
+    c->live_off.assign((size_t)C + 1, 0);
+    c->h_batch_list.clear();
+    std::vector<int32_t> pre((size_t)live.size() + 1, 0);  // prefix counts of live worlds
+    for (size_t w = 0; w < live.size(); ++w) pre[w + 1] = pre[w] + (live[w] ? 1 : 0);
+    for (int col = 0; col < C; ++col) {
+        c->live_off[col] = (int32_t)c->h_batch_list.size();
+        for (int b = c->fast_color_off[col]; b < c->fast_color_off[col + 1]; ++b)
+            if (pre[c->bat_whi[b] + 1] - pre[c->bat_wlo[b]] > 0) c->h_batch_list.push_back(b);
+    }
+    c->live_off[C] = (int32_t)c->h_batch_list.size();
+    if (!c->h_batch_list.empty())
+        VBD_HIP_TRY(hipMemcpyAsync(c->batch_list, c->h_batch_list.data(), sizeof(int32_t) * c->h_batch_list.size(),
+                                   hipMemcpyHostToDevice, s));
+    c->kp.batch_list = c->batch_list;
+    c->use_list = true;
+    return VBD_OK;
+}
 }  // namespace
 
 // Freezes the worlds whose squared norm reached their target after `done` iterations (one
@@ -2638,6 +2714,12 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     hipLaunchKernelGGL(k_world_init, dim3(blocks_for(W, 256)), dim3(256), 0, s, W, world_state_dev);
     VBD_LAUNCH_CHECK("k_world_init");
     WorldMask mask(c, particle_world, world_state_dev);
+    {
+        std::vector<int32_t> off((size_t)W + 1);
+        VBD_HIP_TRY(hipMemcpy(off.data(), world_offsets, sizeof(int32_t) * (size_t)(W + 1), hipMemcpyDeviceToHost));
+        batch_worlds(c, off);
+    }
+    std::vector<int32_t> st_host((size_t)(2 * W + 1), 1), live_prev((size_t)W, 1);
     int32_t done = 0, live = W;
     int32_t rc = vbd_residual_worlds(c, stream, q, dt, W, world_offsets, norm);
     if (rc) return rc;
@@ -2646,10 +2728,19 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     int32_t since_poll = kWorldPoll;  // poll before the first iteration
     for (;;) {
         if (since_poll >= kWorldPoll || done >= max_iterations) {
-            VBD_HIP_TRY(hipMemcpyAsync(&live, world_state_dev + 2 * W, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            VBD_HIP_TRY(hipMemcpyAsync(st_host.data(), world_state_dev, sizeof(int32_t) * (size_t)(2 * W + 1),
+                                       hipMemcpyDeviceToHost, s));
             VBD_HIP_TRY(hipStreamSynchronize(s));
             since_poll = 0;
+            live = st_host[2 * W];
             if (live == 0 || done >= max_iterations) break;
+            // launch only the batches of worlds still live (frozen ones inside them are masked)
+            std::vector<int32_t> now(st_host.begin(), st_host.begin() + W);
+            if (now != live_prev) {
+                live_prev = now;
+                rc = build_live_list(c, s, now);
+                if (rc) return rc;
+            }
         }
         const int32_t n = std::min(check_every, max_iterations - done);
         for (int32_t k = 0; k < n; ++k) {
