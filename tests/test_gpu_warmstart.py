@@ -177,3 +177,73 @@ def test_training_reduces_loss(cuda_device):
         last = mlp.train_step(xd, td, lr=3e-3)
     torch.cuda.synchronize()
     assert float(last.item()) < 0.1 * first, (first, float(last.item()))
+
+
+def test_trained_network_gains_over_reuse(cuda_device):
+    """What the trained network gains over the method without it: on held-out substeps of the wind
+    cloth, the learned start y + c + MLP lies closer to the converged solution x* than the
+    network-free start y + c (the correction carried over from the previous substep).
+
+    Trained on the GPU as bench.py trains it (deployed trajectory: each substep starts from the better
+    of y + c and y, iterates to 0.3 r0, then on to 150 iterations for the target), at 32 x 32."""
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+    from newton_amd.warmstart import LearnedWarmStart
+
+    model, s0, info = scenes.cloth_wind(cuda_device, verts_per_side=32)
+    wind, dt, sub = info["wind"], info["dt"], info["substeps"]
+    s1 = model.state()
+    solver = SolverVBD(model, iterations=10)
+    ws = LearnedWarmStart(model, dt, seed=0)
+    mass = model.particle_mass
+
+    def substep(record):
+        s0.particle_f.copy_(wind.force(s0.particle_q, mass))
+        solver.begin_step(s0, None, dt)
+        feat = ws.features(solver, s0).clone()
+        cands = [c.clone() for c in ws(solver, s0, dt)]
+        y = s0.particle_q.clone()
+        r0 = float(solver.residual(s0, dt).item())
+        trial = solver.set_iterate(s0, cands[0], commit=False)
+        if float(solver.residual(type("S", (), {"particle_q": trial})(), dt).item()) < r0:
+            solver.set_iterate(s0, cands[0], commit=True)
+        its, _ = solver.iterate_to_residual(s0, dt, 0.09 * r0, 150)
+        x_tol = s0.particle_q.clone()
+        solver.iterate(s0, dt, 150 - its)
+        record(feat, ws.target(solver, s0), cands, s0.particle_q.clone(), y)
+        solver.set_iterate(s0, x_tol, commit=True)
+        ws.observe(solver, s0)
+        solver.end_step(s0, s1, dt)
+        s0.particle_q.copy_(s1.particle_q)
+        s0.particle_qd.copy_(s1.particle_qd)
+
+    feats, targs = [], []
+    for frame in range(24):
+        wind.redraw(frame)
+        for _ in range(sub):
+            substep(lambda f, t, *_: (feats.append(f), targs.append(t)))
+    X, T = torch.cat(feats), torch.cat(targs).contiguous()
+    W = ws.active.float().repeat(len(feats))
+    gen = torch.Generator(device=cuda_device)
+    gen.manual_seed(0)
+    for step in range(600):
+        idx = torch.randint(0, X.shape[0], (16384,), device=cuda_device, generator=gen)
+        ws.mlp.train_step(X[idx].contiguous(), T[idx].contiguous(), W[idx].contiguous(),
+                          lr=max(1e-3 * 0.5 * (1.0 + np.cos(np.pi * step / 600)), 1e-5))
+    act = ws.active
+    err = {"reuse": [], "learned": [], "inertial": []}
+
+    def measure(f, t, cands, x_star, y):
+        for name, c in (("reuse", cands[0]), ("learned", cands[1]), ("inertial", y)):
+            err[name].append(float(torch.linalg.norm((c - x_star)[act]).item()))
+
+    for frame in range(24, 30):
+        wind.redraw(frame)
+        for _ in range(sub):
+            substep(measure)
+    e = {k: float(np.mean(v)) for k, v in err.items()}
+    print(f"held-out distance to x*: inertial {e['inertial']:.4e}, y + c {e['reuse']:.4e}, learned {e['learned']:.4e}")
+    assert e["reuse"] < e["inertial"]
+    assert e["learned"] < 0.9 * e["reuse"], e
