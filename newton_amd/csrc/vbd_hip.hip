@@ -177,7 +177,8 @@ __device__ __forceinline__ float portable_atan2f(float y, float x) {
         r = signbit(x) ? PI_F : 0.0f;
     } else {
         bool swap = ay > ax;
-        float t = swap ? (ax / ay) : (ay / ax);
+        // one division with selected operands (two divisions under a divergent branch otherwise)
+        float t = (swap ? ax : ay) / (swap ? ay : ax);
         float base = 0.0f;
         if (t > 0.41421356237309504880f) {
             base = PIO4_F;
@@ -307,9 +308,16 @@ __device__ void eval_bending(const float* __restrict__ q, const float* __restric
     float theta = portable_atan2f(sin_t, cos_t);
     float k = stiffness * rest_length;
     float dE = k * (theta - rest_angle);
-    const V3 zero = v3(0.0f, 0.0f, 0.0f);
-    V3 s1 = v_order == 0 ? e : (v_order == 1 ? zero : (v_order == 2 ? neg(x03) : x02));
-    V3 s2 = v_order == 0 ? zero : (v_order == 1 ? neg(e) : (v_order == 2 ? x13 : neg(x12)));
+    // per-component selects (the compiler turns selects of whole vectors into branches)
+    const bool o0 = v_order == 0, o1 = v_order == 1, o2 = v_order == 2;
+    const V3 m03 = neg(x03), me = neg(e), m12 = neg(x12);
+    V3 s1, s2;
+    s1.x = o0 ? e.x : (o1 ? 0.0f : (o2 ? m03.x : x02.x));
+    s1.y = o0 ? e.y : (o1 ? 0.0f : (o2 ? m03.y : x02.y));
+    s1.z = o0 ? e.z : (o1 ? 0.0f : (o2 ? m03.z : x02.z));
+    s2.x = o0 ? 0.0f : (o1 ? me.x : (o2 ? x13.x : m12.x));
+    s2.y = o0 ? 0.0f : (o1 ? me.y : (o2 ? x13.y : m12.y));
+    s2.z = o0 ? 0.0f : (o1 ? me.z : (o2 ? x13.z : m12.z));
     M3 dn1 = nvd_skew(n1n, n1h, s1);
     M3 dn2 = nvd_skew(n2n, n2h, s2);
     M3 a = mul_skew_left(n1h, dn2);
@@ -2716,7 +2724,12 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     WorldMask mask(c, particle_world, world_state_dev);
     {
         std::vector<int32_t> off((size_t)W + 1);
-        VBD_HIP_TRY(hipMemcpy(off.data(), world_offsets, sizeof(int32_t) * (size_t)(W + 1), hipMemcpyDeviceToHost));
+        VBD_HIP_TRY(hipMemcpyAsync(off.data(), world_offsets, sizeof(int32_t) * (size_t)(W + 1),
+                                   hipMemcpyDeviceToHost, s));
+        VBD_HIP_TRY(hipStreamSynchronize(s));
+        for (int w = 0; w < W; ++w)
+            if (off[w] < 0 || off[w + 1] < off[w] || off[w + 1] > c->model.particle_count)
+                return fail(VBD_ERR_ARG, "vbd_iterate_to_residual_worlds: world_offsets must ascend within [0, N]");
         batch_worlds(c, off);
     }
     std::vector<int32_t> st_host((size_t)(2 * W + 1), 1), live_prev((size_t)W, 1);
