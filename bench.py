@@ -72,6 +72,8 @@ def parse():
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
+    ap.add_argument("--pass-shape", default="auto",
+                    help="--pmc-mode color-pass shape: 'auto' (autotuner) or lanes:waves, e.g. 256:7")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
     ap.add_argument("--self-contact-steps", type=int, default=10,
                     help="timed frames of the stacked-sheets self-contact drape (0 disables)")
@@ -212,7 +214,13 @@ def pmc_mode(args):
     contacts = pipe.contacts()
     state.clear_forces()
     pipe.collide(state, contacts)
-    tune = solver.autotune(state, contacts, info["dt"])
+    if args.pass_shape == "auto":
+        tune = solver.autotune(state, contacts, info["dt"])
+    else:
+        lanes, waves = (int(x) for x in args.pass_shape.split(":"))
+        solver.set_block_shape(lanes)
+        solver.set_pass_waves(waves)
+        tune = {"block": lanes, "waves": waves, "fixed": True}
     solver.begin_step(state, contacts, info["dt"])
     ncol = len(model.particle_color_groups)
     for _ in range(args.roofline_reps):
@@ -251,7 +259,7 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
                    str(ROOT / "bench.py"), "--pmc-mode", "--pmc-workload", workload, "--verts", str(args.verts),
                    "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--cube-worlds",
-                   str(args.cube_worlds), "--roofline-reps", "5"]
+                   str(args.cube_worlds), "--roofline-reps", "5", "--pass-shape", args.pass_shape]
             try:
                 proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
                                       env={**os.environ, "TMPDIR": "/tmp"})
@@ -717,6 +725,7 @@ def main():
     state0.clear_forces()
     pipe.collide(state0, contacts)
     tune = solver.autotune(state0, contacts, info["dt"])
+    args.pass_shape = f"{tune['block']}:{tune['waves']}"  # the counter runs use the timed shape
     _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames, shape {tune}")
     gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
     value = n * sub * args.steps / (gpu_ms / 1e3)
