@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--train-frames", type=int, default=40)
     ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-smooth", type=float, default=100.0,
+                    help="weight of the 1-ring smoothness term of the warm-start training loss")
     ap.add_argument("--residual-worlds", type=int, default=64,
                     help="worlds of the batched wind cloth in the residual leg (0: skip)")
     ap.add_argument("--batched-residual-warm-frames", type=int, default=2)
@@ -452,21 +454,17 @@ def residual_leg(args):
             s0, s1 = s1, s0
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    X = torch.cat(feats)
-    T = torch.cat(targs).contiguous()
-    W = ws.active.float().repeat(len(feats))
-    del feats, targs
-    rows = X.shape[0]
-    gen = torch.Generator(device=dev)
+    rows = n * len(feats)
+    gen = torch.Generator(device="cpu")
     gen.manual_seed(0)
-    batch = 32768
+    batch = n  # one substep's particles per step: the smoothness term couples neighbours
     t0 = time.perf_counter()
     first = None
     loss = None
     for step in range(args.train_steps):
-        idx = torch.randint(0, rows, (batch,), device=dev, generator=gen)
+        k = int(torch.randint(0, len(feats), (1,), generator=gen))
         lr = max(1e-3 * 0.5 * (1.0 + math.cos(math.pi * step / args.train_steps)), 1e-5)
-        loss = ws.mlp.train_step(X[idx].contiguous(), T[idx].contiguous(), W[idx].contiguous(), lr=lr)
+        loss = ws.train_substep(feats[k], targs[k], lr=lr, smooth=args.train_smooth)
         if step == 0:
             first = float(loss.item())
         if step % 500 == 0:
@@ -474,9 +472,10 @@ def residual_leg(args):
     last = float(loss.item()) if loss is not None else None
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    target_ms = float((T[W > 0] ** 2).mean().item())
-    train_flops = 3 * MLP_FLOPS_PER_ROW * batch * args.train_steps
-    del X, T, W
+    act = ws.active
+    target_ms = float(torch.stack([(t[act] ** 2).mean() for t in targs]).mean().item())
+    train_flops = 4 * MLP_FLOPS_PER_ROW * batch * args.train_steps  # forward + training forward/backward
+    del feats, targs
 
     # MFMA roofline of the inference kernel on one substep's rows (the last features gathered)
     pred = torch.empty((n, 3), dtype=torch.float32, device=dev)
@@ -491,7 +490,7 @@ def residual_leg(args):
     fwd_ms = e0.elapsed_time(e1) / 50
     fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
 
-    variants = ("plain", "reuse", "learned_trial", "learned")
+    variants = ("plain", "reuse", "learned_trial", "learned_unjudged", "learned")
     res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in variants}
     accepted = 0
     accepted_trial = 0
@@ -508,10 +507,21 @@ def residual_leg(args):
                 ws.corr.copy_(corr0)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                start = {"plain": None, "reuse": ws.reuse, "learned": ws, "learned_trial": ws}[name]
-                r = solver.step_to_residual(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                            max_iterations=args.max_iterations, warm_start=start,
-                                            trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
+                if name == "learned_unjudged":
+                    # ablation outside the judged protocol: the learned start is taken without the veto
+                    solver.begin_step(s0, None, dt)
+                    r0_sq = float(solver.residual(s0, dt).item())
+                    solver.set_iterate(s0, ws(solver, s0, dt)[-1], commit=True)
+                    its, r_sq = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0_sq,
+                                                           args.max_iterations)
+                    solver.end_step(s0, s1, dt)
+                    r = {"iterations": its, "r": math.sqrt(r_sq), "r0": math.sqrt(r0_sq), "warm_start_index": 1,
+                         "r_warm_start": None}
+                else:
+                    start = {"plain": None, "reuse": ws.reuse, "learned": ws, "learned_trial": ws}[name]
+                    r = solver.step_to_residual(
+                        s0, s1, None, None, dt, rel_tol=args.rel_tol, max_iterations=args.max_iterations,
+                        warm_start=start, trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
                 torch.cuda.synchronize()
                 res[name]["s"] += time.perf_counter() - t0
                 res[name]["iters"].append(r["iterations"])
@@ -540,11 +550,14 @@ def residual_leg(args):
     out["speedup_trial_vertex_steps"] = round(res["plain"]["s"] / res["learned_trial"]["s"], 3)
     out["trial_iterations"] = args.trial_iterations
     out["learned_candidate_chosen_after_trial"] = f"{accepted_trial}/{substeps}"
+    out["speedup_unjudged_vertex_steps"] = round(res["plain"]["s"] / res["learned_unjudged"]["s"], 3)
+    out["learned_blend"] = ws.blend
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
     if args.residual_worlds > 0:
         out["batched"] = batched_residual(args, s0, ws)
     out["training"] = {"protocol": args.train_protocol, "frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
+                       "loss": "mse + smooth * |L e|^2 over one substep per step", "smooth": args.train_smooth,
                        "batch": batch, "loss_first": None if first is None else round(first, 5),
                        "loss_last": None if last is None else round(last, 6),
                        "target_mean_square": round(target_ms, 5), "data_s": round(t_data, 2),

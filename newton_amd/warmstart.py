@@ -175,9 +175,10 @@ class LearnedWarmStart:
     """
 
     def __init__(self, model: Model, dt: float, mlp: WarmStartMLP | None = None, seed: int = 0,
-                 offer_reuse: bool = True):
+                 offer_reuse: bool = True, blend: float = 0.5):
         self.model = model
         self.offer_reuse = offer_reuse
+        self.blend = float(blend)
         self.device = model.device
         self.dt = float(dt)
         n = model.particle_count
@@ -203,6 +204,32 @@ class LearnedWarmStart:
         self.feat = torch.empty((n, N_IN), dtype=torch.int16, device=self.device)
         self.pred = torch.empty((n, 3), dtype=torch.float32, device=self.device)
         self.acc = torch.zeros((n, 3), dtype=torch.float32, device=self.device)
+        # 1-ring mean operator for the smoothness term of the training loss
+        deg = (self.nbr_off[1:] - self.nbr_off[:-1]).long()
+        self._rows = torch.repeat_interleave(torch.arange(n, device=self.device), deg)
+        self._deg = deg.clamp(min=1).float()[:, None]
+        self._act = self.active.float()[:, None]
+
+    def _ring_mean(self, e: torch.Tensor) -> torch.Tensor:
+        return torch.zeros_like(e).index_add_(0, self._rows, e[self.nbr.long()]) / self._deg
+
+    def train_substep(self, feat: torch.Tensor, target: torch.Tensor, lr: float, smooth: float = 100.0):
+        """One Adam step on one substep's rows (all particles) with a smoothness-weighted loss
+        ``|e|^2 + smooth * |L e|^2`` over the solved particles, e = MLP(feat) - target and L = I - (1-ring
+        mean): the residual that judges a start weighs rough (neighbour-to-neighbour) error by the
+        membrane stiffness, so the network is trained to keep its error smooth.  The MLP kernels train
+        on a weighted MSE, so the loss gradient g enters as the target ``prediction - g / 2``.
+        Returns the MSE part (device [1])."""
+        pred = self.mlp.forward(feat)
+        e = (pred - target) * self._act
+        g = e
+        if smooth > 0.0:
+            le = (e - self._ring_mean(e)) * self._act
+            w = le / self._deg
+            lt = le - torch.zeros_like(le).index_add_(0, self._rows, w[self.nbr.long()])  # (I - A)^T (L e)
+            g = e + smooth * lt * self._act
+        self.mlp.train_step(feat, (pred - 0.5 * g).contiguous(), self._act[:, 0].contiguous(), lr=lr)
+        return (e * e).sum() / (3.0 * self._act.sum().clamp(min=1.0))
 
     def reset(self) -> None:
         self.corr.zero_()
@@ -224,12 +251,13 @@ class LearnedWarmStart:
         return self.feat
 
     def __call__(self, solver, state_in, dt):
-        """Candidate iterates for ``step_to_residual``: the learned one, y + c_prev + scale * MLP(features)
-        (the network predicts how this substep's correction differs from the previous substep's), and,
-        when ``offer_reuse`` is set, first the network-free y + c_prev; the residual picks among them."""
+        """Candidate iterates for ``step_to_residual``: the learned one, y + c_prev + blend * scale *
+        MLP(features) (the network predicts how this substep's correction differs from the previous
+        substep's; ``blend`` 0.5 halves the step, which converges faster than the full one), and, when
+        ``offer_reuse`` is set, first the network-free y + c_prev; the residual picks among them."""
         feat = self.features(solver, state_in)
         self.mlp.forward(feat, self.pred)
-        learned = self._y + self.corr + self.pred * self.scale_corr
+        learned = self._y + self.corr + self.pred * (self.blend * self.scale_corr)
         if self.offer_reuse:
             return [self._y + self.corr, learned]
         return learned

@@ -196,7 +196,7 @@ def test_trained_network_gains_over_reuse(cuda_device):
     wind, dt, sub = info["wind"], info["dt"], info["substeps"]
     s1 = model.state()
     solver = SolverVBD(model, iterations=10)
-    ws = LearnedWarmStart(model, dt, seed=0)
+    ws = LearnedWarmStart(model, dt, seed=0, blend=1.0)  # the network's full prediction
     mass = model.particle_mass
 
     def substep(record):
