@@ -701,6 +701,17 @@ def cpu_baseline_leg(args, model, q_init, qd_init, dt):
         if time.perf_counter() - t0 >= args.cpu_seconds or done >= 200:
             break
     cpu_t = time.perf_counter() - t0
+    # the serial restatement too (Warp's CPU device runs each launch serially): a shorter sample
+    prm1 = oracle.params(args.iterations, num_threads=1)
+    q, qd = q_init, qd_init
+    done1 = 0
+    t0 = time.perf_counter()
+    while True:
+        q, qd, _ = oracle.simulate(om, prm1, q, qd, 1, dt, 0.01, len(om.pairs) // 2)
+        done1 += 1
+        if time.perf_counter() - t0 >= 0.25 * args.cpu_seconds or done1 >= 50:
+            break
+    cpu1_t = time.perf_counter() - t0
     return {
         "value": round(n * done / cpu_t, 1),
         "unit": "vertex-steps/s",
@@ -708,6 +719,8 @@ def cpu_baseline_leg(args, model, q_init, qd_init, dt):
         "kind": "port",
         "sample": f"{done} substeps (collide + step, {args.iterations} iterations) of the same "
                   f"{args.verts}x{args.verts} cloth from its initial state, FP32 C oracle, OpenMP {cores} threads",
+        "single_thread": {"value": round(n * done1 / cpu1_t, 1), "cores": 1,
+                          "sample": f"{done1} substeps of the same trajectory, FP32 C oracle, 1 thread"},
     }
 
 

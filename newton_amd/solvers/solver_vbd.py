@@ -674,6 +674,7 @@ class SolverVBD:
             torch.cuda.synchronize(self.model.device)
             _native.check(_native.hip_lib().vbd_rebuild(self._ctx), "vbd_rebuild")
         self._material_key = None
+        self._world_off = None  # particle worlds may have changed
 
     def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
         """Pick the color-pass shape by timing the candidates on this model: 256-lane work-groups at the
