@@ -490,7 +490,14 @@ def residual_leg(args):
     fwd_ms = e0.elapsed_time(e1) / 50
     fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
 
-    variants = ("plain", "reuse", "learned_trial", "learned_unjudged", "learned")
+    variants = ("plain", "reuse", "learned_trial", "learned_unjudged", "parallel_starts", "learned")
+    # the world's substep solved from three starts at once (inertial, y + c, learned) in a 3-copy model
+    from newton_amd.solvers.parallel_starts import ParallelStarts
+
+    model_k, _, _ = scenes.cloth_wind_batched(dev, worlds=2)
+    ps = ParallelStarts(model_k, SolverVBD(model_k, iterations=10), LearnedWarmStart(model_k, dt, mlp=ws.mlp),
+                        copies=2, candidates=(1,))
+    ps_wins = [0, 0]
     res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in variants}
     accepted = 0
     accepted_trial = 0
@@ -507,7 +514,12 @@ def residual_leg(args):
                 ws.corr.copy_(corr0)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                if name == "learned_unjudged":
+                if name == "parallel_starts":
+                    pr = ps.step(s0, s1, ws.corr, dt, args.rel_tol, args.max_iterations)
+                    ps_wins[pr["winner"]] += 1
+                    r = {"iterations": pr["iterations"], "r": pr["r"], "r0": pr["r0"], "warm_start_index": None,
+                         "r_warm_start": None}
+                elif name == "learned_unjudged":
                     # ablation outside the judged protocol: the learned start is taken without the veto
                     solver.begin_step(s0, None, dt)
                     r0_sq = float(solver.residual(s0, dt).item())
@@ -551,6 +563,8 @@ def residual_leg(args):
     out["trial_iterations"] = args.trial_iterations
     out["learned_candidate_chosen_after_trial"] = f"{accepted_trial}/{substeps}"
     out["speedup_unjudged_vertex_steps"] = round(res["plain"]["s"] / res["learned_unjudged"]["s"], 3)
+    out["speedup_parallel_starts_vertex_steps"] = round(res["plain"]["s"] / res["parallel_starts"]["s"], 3)
+    out["parallel_starts_winner"] = {"inertial": ps_wins[0], "network": ps_wins[1]}
     out["learned_blend"] = ws.blend
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
     if args.residual_worlds > 0:

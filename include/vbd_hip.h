@@ -225,6 +225,11 @@ int32_t vbd_residual(VbdContext* ctx, void* stream, const float* q, float dt, fl
  * with no counterpart in the reference; call after vbd_forward. */
 int32_t vbd_set_iterate(VbdContext* ctx, void* stream, float* q_dst, const float* guess, int32_t commit);
 
+/* vbd_set_iterate for the particles p_begin <= i < p_end only (the others keep their iterate and
+ * displacement): one world of a batched model.  No counterpart in the reference. */
+int32_t vbd_set_iterate_range(VbdContext* ctx, void* stream, float* q_dst, const float* guess, int32_t commit,
+                              int32_t p_begin, int32_t p_end);
+
 /* Runs VBD iterations on q until the squared residual norm is <= target_norm_sq
  * or max_iterations were run, evaluating the residual before the first
  * iteration and after every check_every iterations (host-synchronous: not for
@@ -244,9 +249,9 @@ int32_t vbd_residual_worlds(VbdContext* ctx, void* stream, const float* q, float
 /* vbd_iterate_to_residual over worlds: each world is iterated until its squared norm is
  * <= target_norm_sq[w] (host array; evaluated every check_every iterations), then frozen --
  * its vertices are neither solved nor re-evaluated, so its iterate is the one a solve of
- * that world alone stops at -- until all worlds are frozen or max_iterations were run.  The
- * freezing is decided on the device; the host polls the live-world count every 8
- * evaluations.  particle_world (device int32 [N]) gives each particle's world, consistent
+ * that world alone stops at -- until at most stop_at_live worlds are still live (0: all
+ * frozen) or max_iterations were run.  The freezing is decided on the device; the host
+ * polls the live-world count every 8 evaluations (after every one when stop_at_live > 0).  particle_world (device int32 [N]) gives each particle's world, consistent
  * with world_offsets; world_scratch_dev (2 * world_count doubles) and world_state_dev
  * (2 * world_count + 1 int32) are device scratch.  Reports the iterations run, optionally
  * each world's count (world_iterations_out, host), and the last norms (norm_sq_out, host).
@@ -254,8 +259,8 @@ int32_t vbd_residual_worlds(VbdContext* ctx, void* stream, const float* q, float
 int32_t vbd_iterate_to_residual_worlds(VbdContext* ctx, void* stream, float* q, float dt, int32_t world_count,
                                        const int32_t* world_offsets, const int32_t* particle_world,
                                        const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
-                                       double* world_scratch_dev, int32_t* world_state_dev, int32_t* iterations_out,
-                                       int32_t* world_iterations_out, double* norm_sq_out);
+                                       int32_t stop_at_live, double* world_scratch_dev, int32_t* world_state_dev,
+                                       int32_t* iterations_out, int32_t* world_iterations_out, double* norm_sq_out);
 
 /* Copies the solver's scratch state of the current substep ([N,3] floats)
  * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */

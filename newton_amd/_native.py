@@ -163,6 +163,7 @@ HIP_EXPORTS = {
     "vbd_finalize": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float]),
     "vbd_residual": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "vbd_set_iterate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32]),
+    "vbd_set_iterate_range": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32]),
     "vbd_iterate_to_residual": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_float, ctypes.c_double, c_int32, c_int32, ctypes.POINTER(c_int32),
@@ -172,7 +173,8 @@ HIP_EXPORTS = {
     "vbd_iterate_to_residual_worlds": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_float, c_int32, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_double), c_int32,
-         c_int32, c_void_p, c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32), ctypes.POINTER(ctypes.c_double)],
+         c_int32, c_int32, c_void_p, c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+         ctypes.POINTER(ctypes.c_double)],
     ),
     "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
     "vbd_self_detect": (c_int32, [c_void_p, c_void_p, c_void_p]),

@@ -1441,9 +1441,10 @@ __global__ __launch_bounds__(kNormBlock) void k_residual_worlds(int n, const int
 // d = guess - x0, else d = 0; q_dst = x0 + d (the truncation form forward_step
 // uses).  commit != 0 also stores d as the solver's displacement (q_dst is then
 // the solver's iterate); commit == 0 only fills a trial buffer.
-__global__ void k_set_iterate(KParams P, float* __restrict__ q_dst, const float* __restrict__ guess, int commit) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.N) return;
+__global__ void k_set_iterate(KParams P, float* __restrict__ q_dst, const float* __restrict__ guess, int commit,
+                              int i0, int i1) {
+    int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= i1) return;
     bool solved = (P.flags[i] & 1) && P.inv_mass[i] != 0.0f;
     V3 x0 = ld3(P.q_prev, i);
     V3 d = solved ? sub(ld3(guess, i), x0) : v3(0.0f, 0.0f, 0.0f);
@@ -1562,6 +1563,8 @@ inline int norm_parts(long n) { return n > 0 ? blocks_for(n, kNormBlock * kNormI
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+constexpr int kWorldPinned = 4096;  // worlds whose driver flags go through pinned staging
+
 struct VbdContext {
     VbdModelDesc model;
     VbdSolverParams params;
@@ -1589,6 +1592,7 @@ struct VbdContext {
     double* r_part = nullptr;   // residual partial sums, one per first-stage workgroup
     double* r_dev = nullptr;    // [1] squared residual norm (iterate-to-tolerance driver)
     double* r_host = nullptr;   // [1] pinned host copy of r_dev
+    int32_t* w_host = nullptr;  // [kWorldPinned] pinned staging of the world driver's state
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int capacity = 0;
@@ -2098,6 +2102,8 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
         return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->r_dev, sizeof(double))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipHostMalloc(&c->r_host, sizeof(double))) != hipSuccess) return cleanup("hipHostMalloc", e);
+    if ((e = hipHostMalloc(&c->w_host, sizeof(int32_t) * kWorldPinned)) != hipSuccess)
+        return cleanup("hipHostMalloc", e);
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->c_off, N + 1)) != hipSuccess)
         return cleanup("scan sizing", e);
     if ((e = hipMalloc(&c->cub_tmp, c->cub_bytes > 0 ? c->cub_bytes : 16)) != hipSuccess)
@@ -2184,6 +2190,7 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->r_part);
     hipFree(c->r_dev);
     if (c->r_host) hipHostFree(c->r_host);
+    if (c->w_host) hipHostFree(c->w_host);
     hipFree(c->cub_tmp);
     free_layout(c);
     hipFree(c->pos_of);
@@ -2560,8 +2567,21 @@ int32_t vbd_set_iterate(VbdContext* c, void* stream, float* q_dst, const float* 
     if (!c || !q_dst || !guess) return fail(VBD_ERR_ARG, "vbd_set_iterate: null argument");
     if (c->self_on) return fail(VBD_ERR_STATE, "vbd_set_iterate: not available with self-contact");
     hipLaunchKernelGGL(k_set_iterate, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0,
-                       (hipStream_t)stream, c->kp, q_dst, guess, commit ? 1 : 0);
+                       (hipStream_t)stream, c->kp, q_dst, guess, commit ? 1 : 0, 0, c->model.particle_count);
     VBD_LAUNCH_CHECK("k_set_iterate");
+    return VBD_OK;
+}
+
+int32_t vbd_set_iterate_range(VbdContext* c, void* stream, float* q_dst, const float* guess, int32_t commit,
+                              int32_t p_begin, int32_t p_end) {
+    if (!c || !q_dst || !guess) return fail(VBD_ERR_ARG, "vbd_set_iterate_range: null argument");
+    if (c->self_on) return fail(VBD_ERR_STATE, "vbd_set_iterate_range: not available with self-contact");
+    if (p_begin < 0 || p_end < p_begin || p_end > c->model.particle_count)
+        return fail(VBD_ERR_ARG, "vbd_set_iterate_range: need 0 <= p_begin <= p_end <= particle_count");
+    if (p_end == p_begin) return VBD_OK;
+    hipLaunchKernelGGL(k_set_iterate, dim3(blocks_for(p_end - p_begin, 256)), dim3(256), 0, (hipStream_t)stream,
+                       c->kp, q_dst, guess, commit ? 1 : 0, p_begin, p_end);
+    VBD_LAUNCH_CHECK("k_set_iterate(range)");
     return VBD_OK;
 }
 
@@ -2705,11 +2725,13 @@ constexpr int kWorldPoll = 8;
 int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, float dt, int32_t world_count,
                                        const int32_t* world_offsets, const int32_t* particle_world,
                                        const double* target_norm_sq, int32_t max_iterations, int32_t check_every,
-                                       double* world_scratch_dev, int32_t* world_state_dev, int32_t* iterations_out,
-                                       int32_t* world_iterations_out, double* norm_sq_out) {
+                                       int32_t stop_at_live, double* world_scratch_dev, int32_t* world_state_dev,
+                                       int32_t* iterations_out, int32_t* world_iterations_out, double* norm_sq_out) {
     if (!c || !q || !world_offsets || !particle_world || !target_norm_sq || !world_scratch_dev || !world_state_dev ||
         !norm_sq_out)
         return fail(VBD_ERR_ARG, "vbd_iterate_to_residual_worlds: null argument");
+    if (stop_at_live < 0 || stop_at_live >= world_count)
+        return fail(VBD_ERR_ARG, "vbd_iterate_to_residual_worlds: need 0 <= stop_at_live < world_count");
     if (world_count < 1 || max_iterations < 0 || check_every < 1)
         return fail(VBD_ERR_ARG,
                     "vbd_iterate_to_residual_worlds: world_count and check_every must be >= 1, max_iterations >= 0");
@@ -2738,18 +2760,28 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     if (rc) return rc;
     hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0);
     VBD_LAUNCH_CHECK("k_world_freeze");
-    int32_t since_poll = kWorldPoll;  // poll before the first iteration
+    // stopping before every world is frozen needs the count after every evaluation
+    const int32_t poll_every = stop_at_live > 0 ? 1 : kWorldPoll;
+    int32_t since_poll = poll_every;  // poll before the first iteration
     for (;;) {
-        if (since_poll >= kWorldPoll || done >= max_iterations) {
-            VBD_HIP_TRY(hipMemcpyAsync(st_host.data(), world_state_dev, sizeof(int32_t) * (size_t)(2 * W + 1),
-                                       hipMemcpyDeviceToHost, s));
+        if (since_poll >= poll_every || done >= max_iterations) {
+            // the live count through pinned memory; the flags only when it changed
+            VBD_HIP_TRY(hipMemcpyAsync(c->w_host, world_state_dev + 2 * W, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             VBD_HIP_TRY(hipStreamSynchronize(s));
             since_poll = 0;
-            live = st_host[2 * W];
-            if (live == 0 || done >= max_iterations) break;
-            // launch only the batches of worlds still live (frozen ones inside them are masked)
-            std::vector<int32_t> now(st_host.begin(), st_host.begin() + W);
-            if (now != live_prev) {
+            const int32_t now_live = c->w_host[0];
+            if (now_live <= stop_at_live || done >= max_iterations) {
+                live = now_live;
+                break;
+            }
+            if (now_live != live) {
+                live = now_live;
+                int32_t* flags = W <= kWorldPinned ? c->w_host : st_host.data();
+                VBD_HIP_TRY(hipMemcpyAsync(flags, world_state_dev, sizeof(int32_t) * (size_t)W, hipMemcpyDeviceToHost,
+                                           s));
+                VBD_HIP_TRY(hipStreamSynchronize(s));
+                // launch only the batches of worlds still live (frozen ones inside them are masked)
+                std::vector<int32_t> now(flags, flags + W);
                 live_prev = now;
                 rc = build_live_list(c, s, now);
                 if (rc) return rc;

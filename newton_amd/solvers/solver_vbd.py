@@ -427,6 +427,19 @@ class SolverVBD:
         )
         return dst
 
+    def set_iterate_range(self, state_in: State, guess: torch.Tensor, p_begin: int, p_end: int) -> None:
+        """:meth:`set_iterate` (committed) for the particles ``p_begin <= i < p_end`` only; the others keep
+        their iterate (``guess`` is the full [N,3] array)."""
+        guess = guess.contiguous()
+        if guess.shape != (self.model.particle_count, 3) or guess.dtype != torch.float32:
+            raise ValueError("guess must be a float32 tensor of shape [particle_count, 3]")
+        _native.check(
+            _native.hip_lib().vbd_set_iterate_range(self._ctx, current_stream_handle(self.model.device),
+                                                    _native.ptr(state_in.particle_q), _native.ptr(guess), 1,
+                                                    int(p_begin), int(p_end)),
+            "vbd_set_iterate_range",
+        )
+
     def iterate_to_residual(self, state_in: State, dt: float, target_norm_sq: float, max_iterations: int = 200,
                             check_every: int = 1) -> tuple[int, float]:
         """Iterate until ``|g|^2 <= target_norm_sq`` (checked every ``check_every`` iterations).
@@ -478,13 +491,14 @@ class SolverVBD:
         return out
 
     def iterate_to_residual_worlds(self, state_in: State, dt: float, target_norm_sq, max_iterations: int = 200,
-                                   check_every: int = 1, per_world: bool = False):
+                                   check_every: int = 1, per_world: bool = False, stop_at_live: int = 0):
         """Iterate every world until its ``|g_w|^2 <= target_norm_sq[w]``, then freeze it (its vertices
         are no longer solved, so its iterate is the one a solve of that world alone stops at);
         host-synchronous.
 
         Returns ``(iterations run, last squared norms [world_count])``, plus each world's iteration
-        count when ``per_world``.
+        count when ``per_world``.  ``stop_at_live`` > 0 stops once at most that many worlds are live
+        (e.g. ``world_count - 1``: at the first world that reaches its target).
         """
         off = self.world_offsets()
         nw = off.numel() - 1
@@ -497,7 +511,7 @@ class SolverVBD:
             _native.hip_lib().vbd_iterate_to_residual_worlds(
                 self._ctx, current_stream_handle(self.model.device), _native.ptr(state_in.particle_q), float(dt), nw,
                 _native.ptr(off), _native.ptr(self._world_of_particle_i32), tgt.ctypes.data_as(dptr),
-                int(max_iterations), int(check_every), _native.ptr(self._world_scratch_dev),
+                int(max_iterations), int(check_every), int(stop_at_live), _native.ptr(self._world_scratch_dev),
                 _native.ptr(self._world_state_dev), ctypes.byref(its), wits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                 out.ctypes.data_as(dptr)),
             "vbd_iterate_to_residual_worlds",

@@ -136,3 +136,54 @@ def test_frozen_worlds_equal_single_world_solves(cuda_device):
         torch.cuda.synchronize()
         assert torch.equal(one_out.particle_q, s1.particle_q[sl]), w
         assert torch.equal(one_out.particle_qd, s1.particle_qd[sl]), w
+
+
+def test_parallel_starts_inertial_fallback_and_winner(cuda_device):
+    """ParallelStarts: with poor candidates the inertial copy wins and its result equals a single-world
+    solve of its iteration count bit for bit; a converged candidate wins at once."""
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+    from newton_amd.solvers.parallel_starts import ParallelStarts
+
+    one, st, info = scenes.cloth_wind(cuda_device, verts_per_side=16)
+    dt = info["dt"]
+    st.particle_f.copy_(info["wind"].force(st.particle_q, one.particle_mass))
+    mk, _, _ = scenes.cloth_wind_batched(cuda_device, worlds=3, verts_per_side=16)
+    n = one.particle_count
+    q0, qd0, f0 = st.particle_q.clone(), st.particle_qd.clone(), st.particle_f.clone()
+
+    class Starts:
+        def __init__(self, cands):
+            self.corr = torch.zeros((3 * n, 3), dtype=torch.float32, device=cuda_device)
+            self.cands = cands
+
+        def __call__(self, solver, state, dt):
+            return self.cands
+
+    far = (q0 + 0.05).repeat(3, 1)
+    ps = ParallelStarts(mk, SolverVBD(mk, iterations=2), Starts([far, far]), copies=3)
+    out = one.state()
+    r = ps.step(st, out, torch.zeros((n, 3), dtype=torch.float32, device=cuda_device), dt, 0.3, 300)
+    assert r["winner"] == 0 and r["reached"] and r["iterations"] > 0, r
+    solver = SolverVBD(one, iterations=2)
+    s2, out2 = one.state(), one.state()
+    s2.particle_q.copy_(q0)
+    s2.particle_qd.copy_(qd0)
+    s2.particle_f.copy_(f0)
+    solver.begin_step(s2, None, dt)
+    solver.iterate(s2, dt, r["iterations"])
+    solver.end_step(s2, out2, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(out.particle_q, out2.particle_q) and torch.equal(out.particle_qd, out2.particle_qd)
+    assert torch.equal(st.particle_q, s2.particle_q)  # state_in holds the winner's iterate
+    # a converged candidate (100 iterations from the inertial start) wins at once
+    s2.particle_q.copy_(q0)
+    solver.begin_step(s2, None, dt)
+    solver.iterate(s2, dt, 100)
+    good = s2.particle_q.clone().repeat(3, 1)
+    st.particle_q.copy_(q0)
+    ps = ParallelStarts(mk, SolverVBD(mk, iterations=2), Starts([far, good]), copies=3)
+    r = ps.step(st, out, torch.zeros((n, 3), dtype=torch.float32, device=cuda_device), dt, 0.3, 300)
+    assert r["winner"] == 2 and r["iterations"] == 0 and r["reached"], r
