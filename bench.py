@@ -741,8 +741,8 @@ def cpu_baseline_leg(args, model, q_init, qd_init, dt):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (worlds would shard "
-                          "across GPUs with no per-step exchange, see DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py runs on one GPU only; --gpus must be 1 (worlds would shard "
+                          "across GPUs with no per-step exchange)"}))
         sys.exit(2)
     if args.pmc_mode:
         pmc_mode(args)

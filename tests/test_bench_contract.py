@@ -1,4 +1,4 @@
-"""bench.py's contract on this build (CPU): one GPU only, and the batched wind-cloth scene's layout."""
+"""bench.py's contract (CPU): one GPU only, and the batched wind-cloth scene's layout."""
 
 import json
 import subprocess
