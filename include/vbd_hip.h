@@ -250,8 +250,9 @@ int32_t vbd_residual_worlds(VbdContext* ctx, void* stream, const float* q, float
  * <= target_norm_sq[w] (host array; evaluated every check_every iterations), then frozen --
  * its vertices are neither solved nor re-evaluated, so its iterate is the one a solve of
  * that world alone stops at -- until at most stop_at_live worlds are still live (0: all
- * frozen) or max_iterations were run.  The freezing is decided on the device; the host
- * polls the live-world count every 8 evaluations (after every one when stop_at_live > 0).  particle_world (device int32 [N]) gives each particle's world, consistent
+ * frozen) or max_iterations were run.  The freezing is decided on the device (with
+ * stop_at_live > 0 the remaining worlds are frozen as soon as at most that many are live); the
+ * host polls the live-world count every 8 evaluations.  particle_world (device int32 [N]) gives each particle's world, consistent
  * with world_offsets; world_scratch_dev (2 * world_count doubles) and world_state_dev
  * (2 * world_count + 1 int32) are device scratch.  Reports the iterations run, optionally
  * each world's count (world_iterations_out, host), and the last norms (norm_sq_out, host).

@@ -1591,6 +1591,8 @@ struct VbdContext {
     float* g_tmp = nullptr;     // [N,3] residual scratch
     double* r_part = nullptr;   // residual partial sums, one per first-stage workgroup
     double* r_dev = nullptr;    // [1] squared residual norm (iterate-to-tolerance driver)
+    int* zero_world = nullptr;  // [N] zeros: every particle in world 0 (single-world freezing)
+    int* one_state = nullptr;   // [3] active, iterations, live of the single-world driver
     double* r_host = nullptr;   // [1] pinned host copy of r_dev
     int32_t* w_host = nullptr;  // [kWorldPinned] pinned staging of the world driver's state
     void* cub_tmp = nullptr;
@@ -2101,6 +2103,10 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipMalloc(&c->r_part, sizeof(double) * (size_t)norm_parts(N))) != hipSuccess)
         return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->r_dev, sizeof(double))) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->zero_world, sizeof(int) * (size_t)std::max(N, 1))) != hipSuccess ||
+        (e = hipMemset(c->zero_world, 0, sizeof(int) * (size_t)std::max(N, 1))) != hipSuccess ||
+        (e = hipMalloc(&c->one_state, sizeof(int) * 3)) != hipSuccess)
+        return cleanup("hipMalloc", e);
     if ((e = hipHostMalloc(&c->r_host, sizeof(double))) != hipSuccess) return cleanup("hipHostMalloc", e);
     if ((e = hipHostMalloc(&c->w_host, sizeof(int32_t) * kWorldPinned)) != hipSuccess)
         return cleanup("hipHostMalloc", e);
@@ -2189,6 +2195,8 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->g_tmp);
     hipFree(c->r_part);
     hipFree(c->r_dev);
+    hipFree(c->zero_world);
+    hipFree(c->one_state);
     if (c->r_host) hipHostFree(c->r_host);
     if (c->w_host) hipHostFree(c->w_host);
     hipFree(c->cub_tmp);
@@ -2585,34 +2593,6 @@ int32_t vbd_set_iterate_range(VbdContext* c, void* stream, float* q_dst, const f
     return VBD_OK;
 }
 
-int32_t vbd_iterate_to_residual(VbdContext* c, void* stream, float* q, float dt, double target_norm_sq,
-                                int32_t max_iterations, int32_t check_every, int32_t* iterations_out,
-                                double* norm_sq_out) {
-    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: null argument");
-    if (max_iterations < 0 || check_every < 1)
-        return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: max_iterations must be >= 0 and check_every >= 1");
-    hipStream_t s = (hipStream_t)stream;
-    int32_t done = 0;
-    double r = 0.0;
-    for (;;) {
-        int32_t rc = vbd_residual(c, stream, q, dt, nullptr, c->r_dev);
-        if (rc) return rc;
-        VBD_HIP_TRY(hipMemcpyAsync(c->r_host, c->r_dev, sizeof(double), hipMemcpyDeviceToHost, s));
-        VBD_HIP_TRY(hipStreamSynchronize(s));
-        r = *c->r_host;
-        if (r <= target_norm_sq || done >= max_iterations) break;
-        int32_t n = std::min(check_every, max_iterations - done);
-        for (int32_t k = 0; k < n; ++k) {
-            rc = vbd_iteration(c, stream, q, dt);
-            if (rc) return rc;
-        }
-        done += n;
-    }
-    if (iterations_out) *iterations_out = done;
-    if (norm_sq_out) *norm_sq_out = r;
-    return VBD_OK;
-}
-
 int32_t vbd_residual_worlds(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
                             const int32_t* world_offsets, double* norm_sq_out) {
     if (!c || !q || !world_offsets || !norm_sq_out) return fail(VBD_ERR_ARG, "vbd_residual_worlds: null argument");
@@ -2684,11 +2664,13 @@ int32_t build_live_list(VbdContext* c, hipStream_t s, const std::vector<int32_t>
 }  // namespace
 
 // Freezes the worlds whose squared norm reached their target after `done` iterations (one
-// work-group; st = [active W | iterations W | live count]).  Frozen worlds stay frozen.
+// work-group; st = [active W | iterations W | live count]).  Frozen worlds stay frozen.  Once at
+// most stop_at_live (> 0) worlds are live, the rest are frozen too (the driver stops there).
 __global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __restrict__ norm,
                                                       const double* __restrict__ target, int* __restrict__ st,
-                                                      int done) {
+                                                      int done, int stop_at_live) {
     __shared__ int s_live[1024 / 64];
+    __shared__ int s_total;
     int live = 0;
     for (int w = threadIdx.x; w < nw; w += blockDim.x) {
         int a = st[w];
@@ -2705,8 +2687,19 @@ __global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __r
     if (threadIdx.x == 0) {
         int sum = 0;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) sum += s_live[k];
-        st[2 * nw] = sum;
+        s_total = sum;
     }
+    __syncthreads();
+    const int total = s_total;
+    const bool stop_all = stop_at_live > 0 && total <= stop_at_live;
+    if (stop_all) {
+        for (int w = threadIdx.x; w < nw; w += blockDim.x)
+            if (st[w]) {
+                st[w] = 0;
+                st[nw + w] = done;
+            }
+    }
+    if (threadIdx.x == 0) st[2 * nw] = stop_all ? 0 : total;
 }
 
 __global__ void k_world_init(int nw, int* __restrict__ st) {
@@ -2758,10 +2751,9 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     int32_t done = 0, live = W;
     int32_t rc = vbd_residual_worlds(c, stream, q, dt, W, world_offsets, norm);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0);
+    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0, stop_at_live);
     VBD_LAUNCH_CHECK("k_world_freeze");
-    // stopping before every world is frozen needs the count after every evaluation
-    const int32_t poll_every = stop_at_live > 0 ? 1 : kWorldPoll;
+    const int32_t poll_every = kWorldPoll;
     int32_t since_poll = poll_every;  // poll before the first iteration
     for (;;) {
         if (since_poll >= poll_every || done >= max_iterations) {
@@ -2795,7 +2787,8 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
         done += n;
         rc = vbd_residual_worlds(c, stream, q, dt, W, world_offsets, norm);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, done);
+        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, done,
+                           stop_at_live);
         VBD_LAUNCH_CHECK("k_world_freeze");
         ++since_poll;
     }
@@ -2806,6 +2799,94 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     if (world_iterations_out)
         for (int w = 0; w < W; ++w) world_iterations_out[w] = st[w] ? done : st[W + w];
     if (iterations_out) *iterations_out = done;
+    return VBD_OK;
+}
+
+// Single-world freezing: st = [active, iterations, live]; the substep's passes see it through the
+// world mask (every particle in world 0), so once the residual is within the target every later
+// pass exits at once.
+__global__ void k_single_freeze(const double* __restrict__ norm, double target, int* __restrict__ st, int done) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (st[0] && norm[0] <= target) {
+            st[0] = 0;
+            st[1] = done;
+        }
+        st[2] = st[0];
+    }
+}
+
+__global__ void k_single_init(int* __restrict__ st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st[0] = 1;
+        st[1] = 0;
+        st[2] = 1;
+    }
+}
+
+int32_t vbd_iterate_to_residual(VbdContext* c, void* stream, float* q, float dt, double target_norm_sq,
+                                int32_t max_iterations, int32_t check_every, int32_t* iterations_out,
+                                double* norm_sq_out) {
+    if (!c || !q) return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: null argument");
+    if (max_iterations < 0 || check_every < 1)
+        return fail(VBD_ERR_ARG, "vbd_iterate_to_residual: max_iterations must be >= 0 and check_every >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->self_on) {
+        // self-contact truncation rewrites every q after each color pass: no freezing, a host check
+        // after every check_every iterations
+        int32_t done = 0;
+        double r = 0.0;
+        for (;;) {
+            int32_t rc = vbd_residual(c, stream, q, dt, nullptr, c->r_dev);
+            if (rc) return rc;
+            VBD_HIP_TRY(hipMemcpyAsync(c->r_host, c->r_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+            VBD_HIP_TRY(hipStreamSynchronize(s));
+            r = *c->r_host;
+            if (r <= target_norm_sq || done >= max_iterations) break;
+            int32_t n = std::min(check_every, max_iterations - done);
+            for (int32_t k = 0; k < n; ++k) {
+                rc = vbd_iteration(c, stream, q, dt);
+                if (rc) return rc;
+            }
+            done += n;
+        }
+        if (iterations_out) *iterations_out = done;
+        if (norm_sq_out) *norm_sq_out = r;
+        return VBD_OK;
+    }
+    // The stop is decided on the device after every evaluation (the substep freezes there: its passes
+    // exit at once); the host reads the state every kWorldPoll evaluations, so the iterate and the
+    // count are those of a host check after every evaluation, with one round trip per poll.
+    hipLaunchKernelGGL(k_single_init, dim3(1), dim3(64), 0, s, c->one_state);
+    VBD_LAUNCH_CHECK("k_single_init");
+    WorldMask mask(c, c->zero_world, c->one_state);
+    int32_t rc = vbd_residual(c, stream, q, dt, nullptr, c->r_dev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_single_freeze, dim3(1), dim3(64), 0, s, c->r_dev, target_norm_sq, c->one_state, 0);
+    VBD_LAUNCH_CHECK("k_single_freeze");
+    int32_t done = 0, since_poll = kWorldPoll;
+    for (;;) {
+        if (since_poll >= kWorldPoll || done >= max_iterations) {
+            VBD_HIP_TRY(hipMemcpyAsync(c->w_host, c->one_state, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+            VBD_HIP_TRY(hipStreamSynchronize(s));
+            since_poll = 0;
+            if (c->w_host[2] == 0 || done >= max_iterations) break;
+        }
+        const int32_t n = std::min(check_every, max_iterations - done);
+        for (int32_t k = 0; k < n; ++k) {
+            rc = vbd_iteration(c, stream, q, dt);
+            if (rc) return rc;
+        }
+        done += n;
+        rc = vbd_residual(c, stream, q, dt, nullptr, c->r_dev);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_single_freeze, dim3(1), dim3(64), 0, s, c->r_dev, target_norm_sq, c->one_state, done);
+        VBD_LAUNCH_CHECK("k_single_freeze");
+        ++since_poll;
+    }
+    VBD_HIP_TRY(hipMemcpyAsync(c->r_host, c->r_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    VBD_HIP_TRY(hipStreamSynchronize(s));
+    if (iterations_out) *iterations_out = c->w_host[0] ? done : c->w_host[1];
+    if (norm_sq_out) *norm_sq_out = *c->r_host;
     return VBD_OK;
 }
 
