@@ -187,3 +187,39 @@ def test_parallel_starts_inertial_fallback_and_winner(cuda_device):
     ps = ParallelStarts(mk, SolverVBD(mk, iterations=2), Starts([far, good]), copies=3)
     r = ps.step(st, out, torch.zeros((n, 3), dtype=torch.float32, device=cuda_device), dt, 0.3, 300)
     assert r["winner"] == 2 and r["iterations"] == 0 and r["reached"], r
+
+
+def test_frozen_uneven_worlds_with_ground_contacts(cuda_device):
+    """Worlds of different sizes (batches straddle worlds) with ground contacts: every world frozen by the
+    batched driver holds exactly the iterate of the whole batch run plainly for that world's count."""
+    import torch
+
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = scenes.cloth_batched(cuda_device, worlds=7, side_min=6, side_max=14, height=0.12)
+    dt = info["dt"]
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    state.clear_forces()
+    pipe.collide(state, contacts)
+    assert int(contacts.soft_contact_count.item()) > 0
+    q0, qd0 = state.particle_q.clone(), state.particle_qd.clone()
+    solver = SolverVBD(model, iterations=5)
+    s1, s2 = model.state(), model.state()
+    r = solver.step_to_residual_worlds(state, s1, None, contacts, dt, rel_tol=0.3, max_iterations=300)
+    wits = r["world_iterations"]
+    # a world near free fall can sit at its round-off floor above 0.3 r0: it runs to the cap
+    assert r["reached"] >= 5 and len(set(wits.tolist())) > 1, wits
+    off = solver.world_offsets().cpu().numpy()
+    for k in sorted(set(wits.tolist())):
+        state.particle_q.copy_(q0)
+        state.particle_qd.copy_(qd0)
+        solver.begin_step(state, contacts, dt)
+        solver.iterate(state, dt, k)
+        solver.end_step(state, s2, dt)
+        torch.cuda.synchronize()
+        for w in np.nonzero(wits == k)[0]:
+            sl = slice(int(off[w]), int(off[w + 1]))
+            assert torch.equal(s1.particle_q[sl], s2.particle_q[sl]), (w, k)
+            assert torch.equal(s1.particle_qd[sl], s2.particle_qd[sl]), (w, k)
