@@ -75,7 +75,7 @@ def parse():
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
     ap.add_argument("--pass-shape", default="auto",
-                    help="--pmc-mode color-pass shape: 'auto' (autotuner) or lanes:waves, e.g. 256:7")
+                    help="--pmc-mode color-pass shape: 'auto' (autotuner) or lanes:waves[:aligned], e.g. 256:7:1")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
     ap.add_argument("--self-contact-steps", type=int, default=10,
                     help="timed frames of the stacked-sheets self-contact drape (0 disables)")
@@ -219,10 +219,12 @@ def pmc_mode(args):
     if args.pass_shape == "auto":
         tune = solver.autotune(state, contacts, info["dt"])
     else:
-        lanes, waves = (int(x) for x in args.pass_shape.split(":"))
+        parts = [int(x) for x in args.pass_shape.split(":")]
+        lanes, waves, aligned = parts[0], parts[1], (parts[2] if len(parts) > 2 else 0)
         solver.set_block_shape(lanes)
         solver.set_pass_waves(waves)
-        tune = {"block": lanes, "waves": waves, "fixed": True}
+        solver.set_aligned_items(bool(aligned))
+        tune = {"block": lanes, "waves": waves, "aligned": aligned, "fixed": True}
     solver.begin_step(state, contacts, info["dt"])
     ncol = len(model.particle_color_groups)
     for _ in range(args.roofline_reps):
@@ -765,7 +767,7 @@ def main():
     state0.clear_forces()
     pipe.collide(state0, contacts)
     tune = solver.autotune(state0, contacts, info["dt"])
-    args.pass_shape = f"{tune['block']}:{tune['waves']}"  # the counter runs use the timed shape
+    args.pass_shape = f"{tune['block']}:{tune['waves']}:{tune['aligned']}"  # the counter runs use the timed shape
     _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames, shape {tune}")
     gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
     value = n * sub * args.steps / (gpu_ms / 1e3)

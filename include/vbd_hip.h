@@ -177,18 +177,25 @@ int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
 
 /* Autotuner of the color-pass shape: builds the layout for 256-lane work-groups
  * (batches of <= 21 vertices / 256 element items) and for 128-lane ones (<= 10
- * vertices / 128 items), times three force-assembly passes over every color with
- * each candidate -- 256 lanes at the compiler's register budget, 256 lanes at a
- * seven-waves-per-SIMD budget, 128 lanes (q is only read; call after
- * vbd_contact_prepare and vbd_forward) -- and keeps the fastest.  block_out[2]
- * receives the lanes (256 or 128) and the waves target (1: compiler's choice, 7),
- * ms_out[3] the three per-pass-set times.  Results do not depend on the shape.  No
- * counterpart in the reference; synchronises and allocates (not for capture). */
+ * vertices / 128 items), with item ranges packed or starting on wave boundaries,
+ * times three force-assembly passes over every color with each candidate -- 256
+ * lanes at the compiler's register budget, 256 lanes at a seven-waves-per-SIMD
+ * budget, 128 lanes, 256 lanes wave-aligned at either budget (q is only read; call
+ * after vbd_contact_prepare and vbd_forward) -- and keeps the fastest.
+ * block_out[3] receives the lanes (256 or 128), the waves target (1: compiler's
+ * choice, 7) and the alignment (0 packed, 1 wave-aligned); ms_out[5] the five
+ * per-pass-set times.  Results do not depend on the shape.  No counterpart in the
+ * reference; synchronises and allocates (not for capture). */
 int32_t vbd_autotune(VbdContext* ctx, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out);
 
 /* Register-budget occupancy target of the 256-lane color pass: 1 (compiler's
  * choice) or 7 waves per SIMD; what vbd_autotune chooses between. */
 int32_t vbd_set_pass_waves(VbdContext* ctx, int32_t waves);
+
+/* Item ranges of a color-pass batch packed (0) or each starting on a wave boundary (1), so that a
+ * wave evaluates one element kind only; rebuilds the layout (synchronises, allocates); what
+ * vbd_autotune also chooses between. */
+int32_t vbd_set_aligned_items(VbdContext* ctx, void* stream, int32_t aligned);
 
 /* Selects the work-group shape (256 or 128 lanes) directly, rebuilding the
  * layout; the contact ranges of the current substep are carried over.  Not for

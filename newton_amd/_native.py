@@ -152,6 +152,7 @@ HIP_EXPORTS = {
     "vbd_autotune": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "vbd_set_block_shape": (c_int32, [c_void_p, c_void_p, c_int32]),
     "vbd_set_pass_waves": (c_int32, [c_void_p, c_int32]),
+    "vbd_set_aligned_items": (c_int32, [c_void_p, c_void_p, c_int32]),
     "vbd_step": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc), c_float],

@@ -637,6 +637,7 @@ struct KParams {
     // pass layout (vbd_layout.h)
     const int* batch_blk;
     const int* batch_list;  // when set: launches enumerate these batch indices (live worlds only)
+    int aligned;            // item ranges of a batch start on wave boundaries (layout built so)
     const int* bat_cinfo;  // per batch {first packed contact, contact count} (k_batch_contacts)
     const int* bat_coff;   // per batch, max_v + 1 contact offsets relative to the first
     const int* vert_rec;
@@ -688,6 +689,8 @@ struct KParams {
 // packed contact record: 4 x float4
 //   {particle, local slot, ke, kd} {mu, margin, radius, bx} {by, bz, vx, vy} {vz, nx, ny, nz}
 constexpr int kContactWords = 16;
+
+__device__ __forceinline__ int wave_align(int n) { return (n + 63) & ~63; }
 
 __device__ __forceinline__ bool particle_active(const KParams& P, int v) {
     return (P.flags[v] & 1) && P.mass[v] != 0.0f;
@@ -888,36 +891,42 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
     int kind = 0;  // 1 triangle, 2 edge, 3 tet, 4 contact, 5 spring
     int4 r0 = make_int4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     int meta = 0;
+    // first lane of each item range: packed, or each range on a wave boundary (a wave then runs
+    // one element kind only)
+    const int L1 = P.aligned ? wave_align(ntri) : ntri;
+    const int L2 = L1 + (P.aligned ? wave_align(nedge) : nedge);
+    const int L3 = L2 + (P.aligned ? wave_align(ntet) : ntet);
+    const int L4 = L3 + (P.aligned ? wave_align(nspr) : nspr);
     if (t < ntri) {
         kind = 1;
         const int4* r4 = reinterpret_cast<const int4*>(P.tri_rec) + 3 * (size_t)(tri_b + t);
         r0 = r4[0];
         r1 = r4[1];
         r2 = r4[2];
-    } else if (t < ntri + nedge) {
+    } else if (t >= L1 && t < L1 + nedge) {
         kind = 2;
-        const int k = t - ntri;
+        const int k = t - L1;
         const int4* r4 = reinterpret_cast<const int4*>(P.edge_rec) + 2 * (size_t)(edge_b + k);
         r0 = r4[0];
         r1 = r4[1];
         meta = P.edge_meta[edge_b + k];
-    } else if (t < ntri + nedge + ntet) {
+    } else if (t >= L2 && t < L2 + ntet) {
         kind = 3;
-        const int k = t - ntri - nedge;
+        const int k = t - L2;
         const int4* r4 = reinterpret_cast<const int4*>(P.tet_rec) + 4 * (size_t)(tet_b + k);
         r0 = r4[0];
         r1 = r4[1];
         r2 = r4[2];
         r3 = r4[3];
         meta = P.tet_meta[tet_b + k];
-    } else if (t < nstatic) {
+    } else if (t >= L3 && t < L3 + nspr) {
         kind = 5;
-        const int4* r4 = reinterpret_cast<const int4*>(P.spr_rec) + 2 * (size_t)(spr_b + t - ntri - nedge - ntet);
+        const int4* r4 = reinterpret_cast<const int4*>(P.spr_rec) + 2 * (size_t)(spr_b + t - L3);
         r0 = r4[0];
         r1 = r4[1];
-    } else if (par_contacts && t - nstatic < ncont) {
+    } else if (par_contacts && t >= L4 && t - L4 < ncont) {
         kind = 4;
-        const int4* c4 = reinterpret_cast<const int4*>(P.cpack) + 4 * (size_t)(cbeg + t - nstatic);
+        const int4* c4 = reinterpret_cast<const int4*>(P.cpack) + 4 * (size_t)(cbeg + t - L4);
         r0 = c4[0];
         r1 = c4[1];
         r2 = c4[2];
@@ -977,17 +986,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
             int p = r0.x;
             eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z,
                          B.y, A.z, A.w, B.x, P.friction_epsilon, dt, fe, he);
-            row = t;  // = nstatic + contact index
+            row = nstatic + (t - L4);  // contact rows follow the static rows
         }
         store_row(s_rows + row * kComp, fe, he);
     }
-    // contacts beyond the first BLOCK items (only when nstatic + ncont > BLOCK)
-    const int nitems = nstatic + (par_contacts ? ncont : 0);
-    for (int it = t + BLOCK; it < nitems; it += BLOCK) {
+    // contacts beyond the lanes left after the static items (only when L4 + ncont > BLOCK)
+    const int npar = par_contacts ? ncont : 0;
+    for (int ci = BLOCK - L4 + t; ci < npar; ci += BLOCK) {
         V3 fe;
         M3 he;
-        eval_packed_contact(P, q, cbeg + it - nstatic, dt, fe, he);
-        store_row(s_rows + it * kComp, fe, he);
+        eval_packed_contact(P, q, cbeg + ci, dt, fe, he);
+        store_row(s_rows + (nstatic + ci) * kComp, fe, he);
     }
     __syncthreads();
 
@@ -1617,6 +1626,7 @@ struct VbdContext {
     int* bat_coff = nullptr;   // [B, max_v + 1] per-substep contact offsets within the batch
     int block = 256;           // color-pass work-group shape (256 or 128 lanes)
     int waves = 1;             // occupancy target of the 256-lane pass (1: compiler's choice, or 7)
+    int aligned = 0;           // item ranges of a batch on wave boundaries (layout and kernel)
     // cloth self-contact (vbd_self.h)
     bool self_on = false;
     SelfParams sp{};
@@ -1729,7 +1739,7 @@ int32_t build_layout(VbdContext* c) {
         }
     }
     vbd_layout::PassLayout L;
-    err = vbd_layout::build_pass_layout(hm, L, c->block, c->block / kComp);
+    err = vbd_layout::build_pass_layout(hm, L, c->block, c->block / kComp, c->aligned != 0);
     if (!err.empty()) return fail(VBD_ERR_ARG, "vbd_create: " + err);
     free_layout(c);
     for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_blk), std::make_pair(&c->lay_vert, &L.vert_rec),
@@ -1810,6 +1820,7 @@ int32_t build_layout(VbdContext* c) {
     k.spr_rows = c->spr_rows;
     k.slow_spr_off = c->lay_slow_spr_off;
     k.slow_spr = c->lay_slow_spr;
+    k.aligned = c->aligned;
     return VBD_OK;
 }
 
@@ -2478,6 +2489,18 @@ int32_t vbd_set_block_shape(VbdContext* c, void* stream, int32_t block) {
     return VBD_OK;
 }
 
+int32_t vbd_set_aligned_items(VbdContext* c, void* stream, int32_t aligned) {
+    if (!c || (aligned != 0 && aligned != 1)) return fail(VBD_ERR_ARG, "vbd_set_aligned_items: aligned must be 0 or 1");
+    if (c->aligned == aligned) return VBD_OK;
+    c->aligned = aligned;
+    int32_t rc = build_layout(c);
+    if (rc) return rc;
+    rc = launch_batch_contacts(c, (hipStream_t)stream);
+    if (rc) return rc;
+    VBD_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return VBD_OK;
+}
+
 int32_t vbd_set_pass_waves(VbdContext* c, int32_t waves) {
     if (!c || (waves != 1 && waves != 7)) return fail(VBD_ERR_ARG, "vbd_set_pass_waves: waves must be 1 or 7");
     c->waves = waves;
@@ -2487,17 +2510,20 @@ int32_t vbd_set_pass_waves(VbdContext* c, int32_t waves) {
 int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out) {
     if (!c || !q || !(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_autotune: bad argument");
     hipStream_t s = (hipStream_t)stream;
-    const int shapes[3] = {256, 256, 128};
-    const int waves[3] = {1, 7, 1};
-    float ms[3] = {0.0f, 0.0f, 0.0f};
+    constexpr int kCand = 5;
+    const int shapes[kCand] = {256, 256, 128, 256, 256};
+    const int waves[kCand] = {1, 7, 1, 1, 7};
+    const int align[kCand] = {0, 0, 0, 1, 1};
+    float ms[kCand] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     hipEvent_t e0, e1;
     VBD_HIP_TRY(hipEventCreate(&e0));
     VBD_HIP_TRY(hipEventCreate(&e1));
     int32_t rc = VBD_OK;
-    for (int k = 0; k < 3 && rc == VBD_OK; ++k) {
+    for (int k = 0; k < kCand && rc == VBD_OK; ++k) {
         c->waves = waves[k];
-        if (k == 0 || shapes[k] != shapes[k - 1]) {
+        if (k == 0 || shapes[k] != shapes[k - 1] || align[k] != align[k - 1]) {
             c->block = shapes[k];
+            c->aligned = align[k];
             rc = build_layout(c);
             if (rc) break;
             rc = launch_batch_contacts(c, s);
@@ -2519,11 +2545,12 @@ int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int3
     hipEventDestroy(e1);
     if (rc) return rc;
     int best = 0;
-    for (int k = 1; k < 3; ++k)
+    for (int k = 1; k < kCand; ++k)
         if (ms[k] < ms[best]) best = k;
     c->waves = waves[best];
-    if (c->block != shapes[best]) {
+    if (c->block != shapes[best] || c->aligned != align[best]) {
         c->block = shapes[best];
+        c->aligned = align[best];
         rc = build_layout(c);
         if (rc) return rc;
         rc = launch_batch_contacts(c, s);
@@ -2533,9 +2560,10 @@ int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int3
     if (block_out) {
         block_out[0] = shapes[best];
         block_out[1] = waves[best];
+        block_out[2] = align[best];
     }
     if (ms_out)
-        for (int k = 0; k < 3; ++k) ms_out[k] = ms[k];
+        for (int k = 0; k < kCand; ++k) ms_out[k] = ms[k];
     return VBD_OK;
 }
 

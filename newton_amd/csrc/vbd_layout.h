@@ -86,8 +86,13 @@ inline int32_t fbits(float f) {
 }
 
 // Builds the layout; returns an empty string or an error message.
+// Lanes an item range takes when ranges start on wave (64-lane) boundaries.
+inline int wave_align(int n) { return (n + 63) & ~63; }
+
+// aligned: each kind's items start on a wave boundary of the work-group (a wave then evaluates one
+// kind only); the batch's lane budget counts the padded ranges.
 inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap = kLayoutCap,
-                                     int max_v = kLayoutMaxV) {
+                                     int max_v = kLayoutMaxV, bool aligned = false) {
     const int N = m.N;
     const int C = (int)m.color_off.size() - 1;
     std::vector<char> active((size_t)N), tri_ok((size_t)m.T), edge_ok((size_t)m.E), tet_ok((size_t)m.K);
@@ -168,7 +173,9 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
         while (k < kend) {
             int v0 = m.color_particles[k];
             int items0 = nitems(v0);
-            if (items0 > cap) {  // oversized vertex: generic kernel
+            const int lanes0 = wave_align(ntri[v0]) + wave_align(nedge[v0]) + wave_align(ntet[v0]) +
+                               wave_align(nspr[v0]);
+            if (items0 > cap || (aligned && lanes0 > cap)) {  // oversized vertex: generic kernel
                 L.slow_pos.push_back(k);
                 if (active[v0])
                     for (int p = vs_off[v0]; p < vs_off[v0 + 1]; ++p) {
@@ -180,11 +187,22 @@ inline std::string build_pass_layout(const HostModel& m, PassLayout& L, int cap 
                 continue;
             }
             // greedy run of vertices that fit one workgroup
-            int vb = k, nv = 0, items = 0;
+            int vb = k, nv = 0, items = 0, kt = 0, ke = 0, kk = 0, ks = 0;
             while (k < kend && nv < max_v) {
                 int v = m.color_particles[k];
                 int it = nitems(v);
                 if (it > cap || items + it > cap) break;
+                if (aligned && active[v]) {
+                    const int lanes = wave_align(kt + ntri[v]) + wave_align(ke + nedge[v]) +
+                                      wave_align(kk + ntet[v]) + wave_align(ks + nspr[v]);
+                    if (lanes > cap) break;
+                }
+                if (active[v]) {
+                    kt += ntri[v];
+                    ke += nedge[v];
+                    kk += ntet[v];
+                    ks += nspr[v];
+                }
                 items += it;
                 ++nv;
                 ++k;

@@ -692,7 +692,8 @@ class SolverVBD:
 
     def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
         """Pick the color-pass shape by timing the candidates on this model: 256-lane work-groups at the
-        compiler's register budget, 256 lanes at seven waves per SIMD, 128 lanes.
+        compiler's register budget, 256 lanes at seven waves per SIMD, 128 lanes, and 256 lanes with
+        wave-aligned item ranges at either budget.
 
         Runs on a copy of ``state_in`` (prepared with contacts and the forward
         step); ``state_in`` is left untouched.  Results never depend on the shape.
@@ -703,13 +704,20 @@ class SolverVBD:
         trial.particle_qd = state_in.particle_qd.clone()
         trial.particle_f = state_in.particle_f.clone()
         self.begin_step(trial, contacts, dt)
-        shape = (ctypes.c_int32 * 2)()
-        ms = (ctypes.c_float * 3)()
+        shape = (ctypes.c_int32 * 3)()
+        ms = (ctypes.c_float * 5)()
         _native.check(_native.hip_lib().vbd_autotune(self._ctx, current_stream_handle(self.model.device),
                                                      _native.ptr(trial.particle_q), float(dt), shape, ms),
                       "vbd_autotune")
-        return {"block": int(shape[0]), "waves": int(shape[1]), "ms_256": float(ms[0]), "ms_256_w7": float(ms[1]),
-                "ms_128": float(ms[2])}
+        return {"block": int(shape[0]), "waves": int(shape[1]), "aligned": int(shape[2]), "ms_256": float(ms[0]),
+                "ms_256_w7": float(ms[1]), "ms_128": float(ms[2]), "ms_256_aligned": float(ms[3]),
+                "ms_256_w7_aligned": float(ms[4])}
+
+    def set_aligned_items(self, aligned: bool) -> None:
+        """Item ranges of a color-pass batch packed, or each on a wave boundary (a wave evaluates one
+        element kind only; fewer vertices per batch); what :meth:`autotune` also chooses between."""
+        _native.check(_native.hip_lib().vbd_set_aligned_items(self._ctx, current_stream_handle(self.model.device),
+                                                              1 if aligned else 0), "vbd_set_aligned_items")
 
     def set_pass_waves(self, waves: int) -> None:
         """Occupancy target of the 256-lane color pass: 1 (compiler's register budget) or 7 waves per

@@ -8,7 +8,7 @@ import oracle
 
 
 def gpu_run(model, state, info, substeps: int, record_every: int = 0, margin: float = 0.01, solver_kwargs=None,
-            contact_max=None, block_shape=None, solver=None, waves=None):
+            contact_max=None, block_shape=None, solver=None, waves=None, aligned=None):
     """clear_forces -> collide -> SolverVBD.step -> swap, on the GPU; returns (q, qd, recorded q)."""
     import torch
 
@@ -21,6 +21,8 @@ def gpu_run(model, state, info, substeps: int, record_every: int = 0, margin: fl
         solver.set_block_shape(block_shape)
     if waves is not None:
         solver.set_pass_waves(waves)
+    if aligned is not None:
+        solver.set_aligned_items(aligned)
     pipe = CollisionPipeline(model, soft_contact_margin=margin, soft_contact_max=contact_max)
     contacts = pipe.contacts()
     s0, s1 = state, model.state()

@@ -27,6 +27,19 @@ def test_cloth_worlds_bitexact(cuda_device, worlds):
     np.testing.assert_array_equal(qd_gpu, qd_ref)
 
 
+def test_cloth_worlds_wave_aligned_with_contacts_bitexact(cuda_device):
+    """Wave-aligned item ranges on uneven worlds resting on the ground (contact lanes after the padded
+    static ranges)."""
+    from newton_amd import scenes
+
+    model, state, info = scenes.cloth_batched(cuda_device, worlds=9, side_min=6, side_max=14, height=0.12)
+    q0, qd0 = initial(state)
+    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 8, aligned=True, waves=7)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 8)
+    np.testing.assert_array_equal(q_gpu, q_ref)
+    np.testing.assert_array_equal(qd_gpu, qd_ref)
+
+
 def test_soft_cube_worlds_bitexact(cuda_device):
     from newton_amd import scenes
 
