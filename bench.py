@@ -75,7 +75,8 @@ def parse():
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
     ap.add_argument("--pass-shape", default="auto",
-                    help="--pmc-mode color-pass shape: 'auto' (autotuner) or lanes:waves[:aligned], e.g. 256:7:1")
+                    help="color-pass shape of the headline and --pmc-mode runs: 'auto' (autotuner) or lanes:waves[:aligned], "
+                         "e.g. 256:7:1")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
     ap.add_argument("--self-contact-steps", type=int, default=10,
                     help="timed frames of the stacked-sheets self-contact drape (0 disables)")
@@ -766,8 +767,15 @@ def main():
     qd_init = state0.particle_qd.cpu().numpy().copy()
     state0.clear_forces()
     pipe.collide(state0, contacts)
-    tune = solver.autotune(state0, contacts, info["dt"])
-    args.pass_shape = f"{tune['block']}:{tune['waves']}:{tune['aligned']}"  # the counter runs use the timed shape
+    if args.pass_shape == "auto":
+        tune = solver.autotune(state0, contacts, info["dt"])
+        args.pass_shape = f"{tune['block']}:{tune['waves']}:{tune['aligned']}"  # the counter runs use the timed shape
+    else:
+        parts = [int(x) for x in args.pass_shape.split(":")]
+        solver.set_block_shape(parts[0])
+        solver.set_pass_waves(parts[1])
+        solver.set_aligned_items(bool(parts[2]) if len(parts) > 2 else False)
+        tune = {"block": parts[0], "waves": parts[1], "aligned": parts[2] if len(parts) > 2 else 0, "fixed": True}
     _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames, shape {tune}")
     gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
     value = n * sub * args.steps / (gpu_ms / 1e3)
