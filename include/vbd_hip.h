@@ -176,16 +176,18 @@ int32_t vbd_rebuild(VbdContext* ctx);
 int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
 
 /* Autotuner of the color-pass shape: builds the layout for 256-lane work-groups
- * (batches of <= 21 vertices / 256 element items) and for 128-lane ones (<= 10
- * vertices / 128 items), with item ranges packed or starting on wave boundaries,
- * times three force-assembly passes over every color with each candidate -- 256
- * lanes at the compiler's register budget, 256 lanes at a seven-waves-per-SIMD
- * budget, 128 lanes, 256 lanes wave-aligned at either budget (q is only read; call
- * after vbd_contact_prepare and vbd_forward) -- and keeps the fastest.
- * block_out[3] receives the lanes (256 or 128), the waves target (1: compiler's
- * choice, 7) and the alignment (0 packed, 1 wave-aligned); ms_out[5] the five
- * per-pass-set times.  Results do not depend on the shape.  No counterpart in the
- * reference; synchronises and allocates (not for capture). */
+ * (batches of <= 21 vertices / 256 element items), 192-lane ones (<= 16 / 192) and
+ * 128-lane ones (<= 10 / 128), with item ranges packed or starting on wave
+ * boundaries, times three force-assembly passes over every color with each
+ * candidate -- 256 lanes at the compiler's register budget, 256 lanes at a
+ * seven-waves-per-SIMD budget, 128 lanes, 256 lanes wave-aligned at either budget,
+ * 192 lanes wave-aligned at either budget (q is only read; call after
+ * vbd_contact_prepare and vbd_forward) -- and keeps the fastest.  block_out[3]
+ * receives the lanes (256, 192 or 128), the waves target (1: compiler's choice, 7)
+ * and the alignment (0 packed, 1 wave-aligned); ms_out[7] the seven per-pass-set
+ * times.  Results do not depend on
+ * the shape.  No counterpart in the reference; synchronises and allocates (not for
+ * capture). */
 int32_t vbd_autotune(VbdContext* ctx, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out);
 
 /* Register-budget occupancy target of the 256-lane color pass: 1 (compiler's

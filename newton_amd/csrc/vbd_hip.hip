@@ -2372,6 +2372,10 @@ namespace {
 int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, float dt, int mode, float* g) {
     if (c->block == 128) {
         hipLaunchKernelGGL((k_solve_fast<128, 1>), dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
+    } else if (c->block == 192 && c->waves == 7) {
+        hipLaunchKernelGGL((k_solve_fast<192, 7>), dim3(nb), dim3(192), 0, s, c->kp, q, b0, nb, dt, mode, g);
+    } else if (c->block == 192) {
+        hipLaunchKernelGGL((k_solve_fast<192, 1>), dim3(nb), dim3(192), 0, s, c->kp, q, b0, nb, dt, mode, g);
     } else if (c->waves == 7) {
         hipLaunchKernelGGL((k_solve_fast<256, 7>), dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
     } else {
@@ -2478,7 +2482,8 @@ int32_t vbd_step(VbdContext* c, void* stream, float* q_in, const float* qd_in, c
 }
 
 int32_t vbd_set_block_shape(VbdContext* c, void* stream, int32_t block) {
-    if (!c || (block != 256 && block != 128)) return fail(VBD_ERR_ARG, "vbd_set_block_shape: block must be 256 or 128");
+    if (!c || (block != 256 && block != 192 && block != 128))
+        return fail(VBD_ERR_ARG, "vbd_set_block_shape: block must be 256, 192 or 128");
     if (c->block == block) return VBD_OK;
     c->block = block;
     int32_t rc = build_layout(c);
@@ -2510,11 +2515,11 @@ int32_t vbd_set_pass_waves(VbdContext* c, int32_t waves) {
 int32_t vbd_autotune(VbdContext* c, void* stream, const float* q, float dt, int32_t* block_out, float* ms_out) {
     if (!c || !q || !(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_autotune: bad argument");
     hipStream_t s = (hipStream_t)stream;
-    constexpr int kCand = 5;
-    const int shapes[kCand] = {256, 256, 128, 256, 256};
-    const int waves[kCand] = {1, 7, 1, 1, 7};
-    const int align[kCand] = {0, 0, 0, 1, 1};
-    float ms[kCand] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    constexpr int kCand = 7;
+    const int shapes[kCand] = {256, 256, 128, 256, 256, 192, 192};
+    const int waves[kCand] = {1, 7, 1, 1, 7, 1, 7};
+    const int align[kCand] = {0, 0, 0, 1, 1, 1, 1};
+    float ms[kCand] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     hipEvent_t e0, e1;
     VBD_HIP_TRY(hipEventCreate(&e0));
     VBD_HIP_TRY(hipEventCreate(&e1));

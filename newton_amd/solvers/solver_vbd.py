@@ -692,8 +692,8 @@ class SolverVBD:
 
     def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
         """Pick the color-pass shape by timing the candidates on this model: 256-lane work-groups at the
-        compiler's register budget, 256 lanes at seven waves per SIMD, 128 lanes, and 256 lanes with
-        wave-aligned item ranges at either budget.
+        compiler's register budget, 256 lanes at seven waves per SIMD, 128 lanes, 256 lanes with
+        wave-aligned item ranges at either budget, and 192 lanes wave-aligned at either budget.
 
         Runs on a copy of ``state_in`` (prepared with contacts and the forward
         step); ``state_in`` is left untouched.  Results never depend on the shape.
@@ -705,13 +705,14 @@ class SolverVBD:
         trial.particle_f = state_in.particle_f.clone()
         self.begin_step(trial, contacts, dt)
         shape = (ctypes.c_int32 * 3)()
-        ms = (ctypes.c_float * 5)()
+        ms = (ctypes.c_float * 7)()
         _native.check(_native.hip_lib().vbd_autotune(self._ctx, current_stream_handle(self.model.device),
                                                      _native.ptr(trial.particle_q), float(dt), shape, ms),
                       "vbd_autotune")
         return {"block": int(shape[0]), "waves": int(shape[1]), "aligned": int(shape[2]), "ms_256": float(ms[0]),
                 "ms_256_w7": float(ms[1]), "ms_128": float(ms[2]), "ms_256_aligned": float(ms[3]),
-                "ms_256_w7_aligned": float(ms[4])}
+                "ms_256_w7_aligned": float(ms[4]), "ms_192_aligned": float(ms[5]),
+                "ms_192_w7_aligned": float(ms[6])}
 
     def set_aligned_items(self, aligned: bool) -> None:
         """Item ranges of a color-pass batch packed, or each on a wave boundary (a wave evaluates one
@@ -725,7 +726,7 @@ class SolverVBD:
         _native.check(_native.hip_lib().vbd_set_pass_waves(self._ctx, int(waves)), "vbd_set_pass_waves")
 
     def set_block_shape(self, block: int) -> None:
-        """Use 256- or 128-lane color-pass work-groups (what :meth:`autotune` chooses between)."""
+        """Use 256-, 192- or 128-lane color-pass work-groups (what :meth:`autotune` chooses between)."""
         _native.check(_native.hip_lib().vbd_set_block_shape(self._ctx, current_stream_handle(self.model.device),
                                                             int(block)), "vbd_set_block_shape")
 

@@ -338,10 +338,10 @@ def test_seven_wave_pass_bitexact(cuda_device, scene):
 def test_wave_aligned_items_bitexact(cuda_device, scene):
     """Item ranges starting on wave boundaries (fewer vertices per batch, contact rows after the static
     rows) give the oracle's bits too, at both register budgets."""
-    for waves in (1, 7):
+    for lanes, waves in ((256, 1), (256, 7), (192, 1), (192, 7)):
         model, state, info = _scene(scene, cuda_device)
         q0, qd0 = initial(state)
-        q_gpu, qd_gpu, _ = gpu_run(model, state, info, 6, aligned=True, waves=waves)
+        q_gpu, qd_gpu, _ = gpu_run(model, state, info, 6, aligned=True, waves=waves, block_shape=lanes)
         q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 6)
         np.testing.assert_array_equal(q_gpu, q_ref)
         np.testing.assert_array_equal(qd_gpu, qd_ref)
@@ -364,7 +364,7 @@ def test_autotune_keeps_results(cuda_device):
     before = state.particle_q.clone()
     choice = solver.autotune(state, contacts, info["dt"])
     torch.cuda.synchronize()
-    assert choice["block"] in (128, 256) and choice["waves"] in (1, 7) and choice["aligned"] in (0, 1)
+    assert choice["block"] in (128, 192, 256) and choice["waves"] in (1, 7) and choice["aligned"] in (0, 1)
     assert choice["ms_256"] > 0 and choice["ms_256_w7"] > 0 and choice["ms_128"] > 0
     assert torch.equal(state.particle_q, before)
     q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4, solver=solver)
