@@ -421,6 +421,7 @@ def residual_leg(args):
     n = model.particle_count
     solver = SolverVBD(model, iterations=10)
     ws = LearnedWarmStart(model, dt, seed=0)
+    shape = solver.autotune(state, None, dt)  # pass shape of this cloth (results do not depend on it)
     mass = model.particle_mass
     s0, s1 = state, model.state()
     # Training data.  "deployed": along the trajectory the deployed solver follows -- each substep
@@ -498,8 +499,9 @@ def residual_leg(args):
     from newton_amd.solvers.parallel_starts import ParallelStarts
 
     model_k, _, _ = scenes.cloth_wind_batched(dev, worlds=2)
-    ps = ParallelStarts(model_k, SolverVBD(model_k, iterations=10), LearnedWarmStart(model_k, dt, mlp=ws.mlp),
-                        copies=2, candidates=(1,))
+    solver_k = SolverVBD(model_k, iterations=10)
+    solver_k.autotune(model_k.state(), None, dt)
+    ps = ParallelStarts(model_k, solver_k, LearnedWarmStart(model_k, dt, mlp=ws.mlp), copies=2, candidates=(1,))
     ps_wins = [0, 0]
     res = {k: {"iters": [], "s": 0.0, "reached": 0} for k in variants}
     accepted = 0
@@ -551,7 +553,8 @@ def residual_leg(args):
             s0, s1 = s1, s0
     substeps = args.residual_frames * sub
     out = {"workload": info["workload"], "vertices": n, "rel_tol": args.rel_tol, "max_iterations": args.max_iterations,
-           "substeps": substeps, "check_every": 1}
+           "substeps": substeps, "check_every": 1,
+           "pass_shape": {k: shape[k] for k in ("block", "waves", "aligned")}}
     for name in variants:
         d = res[name]
         out[name] = {"mean_iterations": round(float(np.mean(d["iters"])), 2),
@@ -609,6 +612,7 @@ def batched_residual(args, s_single, ws):
     s0.particle_q.copy_(s_single.particle_q.repeat(W, 1))
     s0.particle_qd.copy_(s_single.particle_qd.repeat(W, 1))
     solver = SolverVBD(model, iterations=10)
+    solver.autotune(s0, None, dt)
     wsb = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0)
     wsb.corr.copy_(ws.corr.repeat(W, 1))
     mass = model.particle_mass
@@ -626,6 +630,7 @@ def batched_residual(args, s_single, ws):
     res = {k: {"iters": [], "witers": [], "s": 0.0, "reached": 0, "chosen": None} for k in variants}
     one_model, one_state, _ = scenes.cloth_wind(dev)
     one_solver = SolverVBD(one_model, iterations=10)
+    one_solver.autotune(one_state, None, dt)
     one_out = one_model.state()
     per = one_model.particle_count
     sample = list(range(0, W, max(args.single_world_stride, 1)))
