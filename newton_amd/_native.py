@@ -187,6 +187,11 @@ HIP_EXPORTS = {
 }
 
 HOST_EXPORTS = {
+    "vbd_host_number_edges": (
+        c_int32,
+        [ctypes.c_int64, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_int64),
+         ctypes.POINTER(ctypes.c_int64)],
+    ),
     "vbd_host_edge_tri_maps": (c_int32, [ctypes.c_int64, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "vbd_host_vertex_csr": (
         c_int32,

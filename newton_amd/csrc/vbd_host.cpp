@@ -261,6 +261,100 @@ int32_t vbd_host_edge_tri_maps(int64_t edge_count, const int32_t* edges, int64_t
     return 0;
 }
 
+// Edge numbering of a triangle mesh (MeshAdjacency._compute_edge_adjacency,
+// newton/_src/utils/mesh.py:464-529).  The rule: walk the triangles in order
+// and, inside each, its sides k = 0, 1, 2 spanning (v_k, v_{k+1}) with opposite
+// vertex v_{k+2}; an unordered vertex pair gets the next edge id the first time
+// it is met.  Its row (o0, o1, v0, v1) takes v0, v1 and o0 from that first
+// side and o1 from the second; edge_tri holds the two triangles (+ tri_start);
+// a third or later side on one pair counts as non-manifold and changes nothing.
+//
+// Two linear passes: the sides are bucketed by their smaller vertex (counting
+// sort, so each bucket lists its sides in walk order); then, in walk order,
+// each side looks for the first side of its bucket with the same larger
+// vertex -- the side that introduced the pair -- and takes over its edge.
+// edges_out needs room for 3 * tri_count rows (the worst case); the number of
+// edges is returned in *edge_count_out and the number of ignored non-manifold
+// sides in *nonmanifold_out.  Returns -1 on a bad argument, -2 on a negative
+// vertex id.
+int32_t vbd_host_number_edges(int64_t tri_count, const int32_t* tris, int32_t tri_start, int32_t* edges_out,
+                              int32_t* edge_tri_out, int32_t* tri_edge_out, int64_t* edge_count_out,
+                              int64_t* nonmanifold_out) {
+    if (tri_count < 0 || (tri_count > 0 && (!tris || !edges_out || !edge_tri_out || !tri_edge_out)) ||
+        !edge_count_out || !nonmanifold_out)
+        return -1;
+    *edge_count_out = 0;
+    *nonmanifold_out = 0;
+    if (tri_count == 0) return 0;
+    const int64_t sides = 3 * tri_count;
+    int32_t vmax = -1;
+    for (int64_t i = 0; i < sides; ++i) {
+        if (tris[i] < 0) return -2;
+        vmax = std::max(vmax, tris[i]);
+    }
+    // bucket start per smaller vertex, then the sides of each bucket in walk order
+    std::vector<int64_t> start((size_t)vmax + 2, 0);
+    for (int64_t s = 0; s < sides; ++s) {
+        const int64_t t = s / 3;
+        const int k = (int)(s % 3);
+        start[(size_t)std::min(tris[3 * t + k], tris[3 * t + (k + 1) % 3]) + 1]++;
+    }
+    for (int32_t v = 0; v <= vmax; ++v) start[(size_t)v + 1] += start[(size_t)v];
+    std::vector<int64_t> bucket((size_t)sides);
+    {
+        std::vector<int64_t> cursor(start.begin(), start.end() - 1);
+        for (int64_t s = 0; s < sides; ++s) {
+            const int64_t t = s / 3;
+            const int k = (int)(s % 3);
+            bucket[(size_t)cursor[(size_t)std::min(tris[3 * t + k], tris[3 * t + (k + 1) % 3])]++] = s;
+        }
+    }
+    std::vector<int32_t> side_edge((size_t)sides, -1);
+    std::vector<int32_t> uses;  // sides met so far per edge
+    int32_t next_edge = 0;
+    for (int64_t s = 0; s < sides; ++s) {
+        const int64_t t = s / 3;
+        const int k = (int)(s % 3);
+        const int32_t a = tris[3 * t + k], b = tris[3 * t + (k + 1) % 3], opp = tris[3 * t + (k + 2) % 3];
+        const int32_t lo = std::min(a, b), hi = std::max(a, b);
+        int64_t first = s;
+        for (int64_t j = start[(size_t)lo]; j < start[(size_t)lo + 1]; ++j) {
+            const int64_t u = bucket[(size_t)j];
+            const int64_t tu = u / 3;
+            const int ku = (int)(u % 3);
+            if (std::max(tris[3 * tu + ku], tris[3 * tu + (ku + 1) % 3]) == hi) {
+                first = u;  // buckets are in walk order: the first match introduced the pair
+                break;
+            }
+        }
+        int32_t e;
+        if (first == s) {
+            e = next_edge++;
+            uses.push_back(0);
+            int32_t* row = edges_out + 4 * (int64_t)e;
+            row[0] = opp;
+            row[1] = -1;
+            row[2] = a;
+            row[3] = b;
+            edge_tri_out[2 * (int64_t)e] = tri_start + (int32_t)t;
+            edge_tri_out[2 * (int64_t)e + 1] = -1;
+        } else {
+            e = side_edge[(size_t)first];
+            if (uses[(size_t)e] == 1) {
+                edges_out[4 * (int64_t)e + 1] = opp;
+                edge_tri_out[2 * (int64_t)e + 1] = tri_start + (int32_t)t;
+            } else {
+                ++*nonmanifold_out;
+            }
+        }
+        uses[(size_t)e]++;
+        side_edge[(size_t)s] = e;
+        tri_edge_out[s] = e;
+    }
+    *edge_count_out = next_edge;
+    return 0;
+}
+
 // Per-vertex CSR of (element, slot) pairs (values_per_entry 2) or element ids
 // (values_per_entry 1) in the serial count/fill order of the reference's
 // adjacency kernels (newton/_src/utils/mesh.py:674-857): elements in increasing

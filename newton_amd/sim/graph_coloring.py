@@ -1,9 +1,10 @@
 """Particle graph construction, coloring and per-world color merging.
 
-Graph construction restates the reference's numpy pipeline exactly
+Graph construction gives the reference's graph exactly
 (``newton/_src/sim/graph_coloring.py:45-240``): canonical ``(min, max)``
 unique edges from active triangles, all six vertex pairs of active bending
-hinges (when ``include_bending``), and all six pairs of active tetrahedra.
+hinges (when ``include_bending``), and all six pairs of active tetrahedra,
+built here from int64 pair keys.
 The coloring itself runs in this package's native host library
 (``csrc/vbd_host.cpp``: MCS ordering + greedy assignment + balancing); the
 reference delegates that step to warp-lang's native code, so the color of each
@@ -27,79 +28,64 @@ class ColoringAlgorithm(Enum):
     GREEDY = 1
 
 
-def _canonicalize_edges_np(edges_np: np.ndarray) -> np.ndarray:
-    """Sorted endpoints, duplicates dropped (``graph_coloring.py:45-51``)."""
-    if edges_np.size == 0:
+def _pairs_to_graph_edges(pairs: np.ndarray) -> np.ndarray:
+    """Undirected graph edges from vertex pairs: each pair as ``(min, max)``, duplicates and pairs
+    with a negative vertex dropped, rows in ascending ``(min, max)`` order -- the canonical form the
+    reference's ``_canonicalize_edges_np`` produces (``graph_coloring.py:45-51``).
+
+    Each pair is encoded as one int64 key ``min * 2^32 + max``; one ``np.unique`` over the keys sorts
+    and de-duplicates."""
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    pairs = pairs[(pairs >= 0).all(axis=1)]
+    if pairs.shape[0] == 0:
         return np.empty((0, 2), dtype=np.int32)
-    edges_sorted = np.sort(edges_np, axis=1)
-    return np.unique(edges_sorted, axis=0).astype(np.int32)
+    keys = np.unique((pairs.min(axis=1) << 32) | pairs.max(axis=1))
+    out = np.empty((keys.size, 2), dtype=np.int32)
+    out[:, 0] = keys >> 32
+    out[:, 1] = keys & 0xFFFFFFFF
+    return out
 
 
-def _apply_mask(arr: np.ndarray, mask) -> np.ndarray:
+def _element_pairs(rows: np.ndarray) -> np.ndarray:
+    """Every pair of columns of every element row, as an (M, 2) array (order irrelevant: the caller
+    canonicalises)."""
+    k = rows.shape[1]
+    return np.concatenate([rows[:, [a, b]] for a in range(k) for b in range(a + 1, k)], axis=0)
+
+
+def _active_rows(indices, width: int, mask) -> np.ndarray:
+    rows = np.asarray(indices, dtype=np.int32).reshape(-1, width)
     if mask is None:
-        return arr
-    mask_arr = np.asarray(mask, dtype=bool)
-    if mask_arr.ndim == 0:
-        return arr if bool(mask_arr) else arr[:0]
-    return arr[mask_arr]
+        return rows
+    m = np.asarray(mask, dtype=bool)
+    if m.ndim == 0:
+        return rows if bool(m) else rows[:0]
+    return rows[m]
 
 
 def construct_tetmesh_graph_edges(tet_indices, tet_active_mask=None) -> np.ndarray:
-    """Unique graph edges of tetrahedra (``graph_coloring.py:54-96``)."""
+    """Unique graph edges of active tetrahedra: all six vertex pairs (``graph_coloring.py:54-96``)."""
     if tet_indices is None:
         return np.empty((0, 2), dtype=np.int32)
-    tet_np = _apply_mask(np.asarray(tet_indices, dtype=np.int32).reshape(-1, 4), tet_active_mask)
-    if tet_np.size == 0:
-        return np.empty((0, 2), dtype=np.int32)
-    v0, v1, v2, v3 = tet_np[:, 0], tet_np[:, 1], tet_np[:, 2], tet_np[:, 3]
-    edges = np.stack(
-        [
-            np.stack([v0, v1], axis=1),
-            np.stack([v0, v2], axis=1),
-            np.stack([v0, v3], axis=1),
-            np.stack([v1, v2], axis=1),
-            np.stack([v1, v3], axis=1),
-            np.stack([v2, v3], axis=1),
-        ],
-        axis=0,
-    ).reshape(-1, 2)
-    return _canonicalize_edges_np(edges)
+    tets = _active_rows(tet_indices, 4, tet_active_mask)
+    return _pairs_to_graph_edges(_element_pairs(tets)) if tets.size else np.empty((0, 2), dtype=np.int32)
 
 
 def construct_trimesh_graph_edges(
     tri_indices, tri_active_mask=None, bending_edge_indices=None, bending_edge_active_mask=None
 ) -> np.ndarray:
-    """Unique graph edges of triangles plus active bending hinges (``graph_coloring.py:99-197``)."""
-    parts = []
+    """Unique graph edges of active triangles (three sides) plus all six vertex pairs of active bending
+    hinges (``graph_coloring.py:99-197``); hinge pairs with a missing (-1) vertex are dropped."""
+    parts = [np.empty((0, 2), dtype=np.int32)]
     if tri_indices is not None:
-        tris = _apply_mask(np.asarray(tri_indices, dtype=np.int32).reshape(-1, 3), tri_active_mask)
-        if tris.size > 0:
-            v0, v1, v2 = tris[:, 0], tris[:, 1], tris[:, 2]
-            parts.append(
-                np.stack(
-                    [np.stack([v0, v1], axis=1), np.stack([v0, v2], axis=1), np.stack([v1, v2], axis=1)], axis=0
-                ).reshape(-1, 2)
-            )
+        tris = _active_rows(tri_indices, 3, tri_active_mask)
+        if tris.size:
+            parts.append(_element_pairs(tris))
     if bending_edge_indices is not None:
-        bend = _apply_mask(np.asarray(bending_edge_indices, dtype=np.int32).reshape(-1, 4), bending_edge_active_mask)
-        if bend.size > 0:
-            c = [bend[:, k : k + 1] for k in range(4)]
-            pairs = np.concatenate(
-                [
-                    np.concatenate([c[0], c[1]], axis=1),
-                    np.concatenate([c[0], c[2]], axis=1),
-                    np.concatenate([c[0], c[3]], axis=1),
-                    np.concatenate([c[1], c[2]], axis=1),
-                    np.concatenate([c[1], c[3]], axis=1),
-                    np.concatenate([c[2], c[3]], axis=1),
-                ],
-                axis=0,
-            )
-            pairs = pairs[np.all(pairs >= 0, axis=1)]
-            if pairs.size > 0:
-                parts.append(pairs)
-    edges = np.concatenate(parts, axis=0) if parts else np.empty((0, 2), dtype=np.int32)
-    return _canonicalize_edges_np(edges)
+        hinges = _active_rows(bending_edge_indices, 4, bending_edge_active_mask)
+        if hinges.size:
+            parts.append(_element_pairs(hinges))
+    return _pairs_to_graph_edges(np.concatenate(parts, axis=0))
 
 
 def construct_particle_graph(
@@ -108,7 +94,7 @@ def construct_particle_graph(
     """Unified particle graph (``graph_coloring.py:200-240``)."""
     tri_edges = construct_trimesh_graph_edges(tri_indices, tri_active_mask, bending_edge_indices, bending_edge_active_mask)
     tet_edges = construct_tetmesh_graph_edges(tet_indices, tet_active_mask)
-    return _canonicalize_edges_np(np.vstack([tri_edges, tet_edges]).astype(np.int32))
+    return _pairs_to_graph_edges(np.vstack([tri_edges, tet_edges]))
 
 
 def color_graph(

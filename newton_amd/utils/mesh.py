@@ -1,14 +1,14 @@
 """Soft-mesh adjacency: edge numbering, edge/triangle maps and per-vertex CSR.
 
-Restates the integer pipelines of the reference's ``newton/_src/utils/mesh.py``
-bit for bit, vectorised with numpy so that multi-million-vertex batches build
-in seconds:
+Reproduces the integer results of the reference's ``newton/_src/utils/mesh.py``
+bit for bit, with linear-time C++ routines (``libvbd_host.so``) so that
+multi-million-vertex batches build in seconds:
 
 * edge numbering = order of first occurrence while walking each triangle's
   local edges (v0,v1), (v1,v2), (v2,v0); row ``[o0, o1, v0, v1]`` takes
   ``o0, v0, v1`` from the first triangle and ``o1`` from the second
-  (``mesh.py:464-529``);
-* edge/triangle maps against pre-numbered edges (``mesh.py:531-576``);
+  (``mesh.py:464-529``), numbered in C++;
+* edge/triangle maps against pre-numbered edges (``mesh.py:531-576``), in C++;
 * per-vertex CSR of ``(element, slot)`` int32 pairs whose order is the serial
   fill order of the reference's count/fill kernels (``mesh.py:674-857``):
   elements in increasing id; within one bending edge the fill order is
@@ -35,93 +35,34 @@ def _int_rows(data, width: int) -> np.ndarray:
 
 
 def compute_edge_adjacency(tri_indices, tri_start: int = 0):
-    """Edge table, edge->triangle and triangle->edge maps from triangles (``mesh.py:464-529``).
+    """Edge table, edge->triangle and triangle->edge maps from triangles (``mesh.py:464-529``), in C++
+    (``vbd_host_number_edges``: sides bucketed by their smaller vertex, numbered in one walk).
 
-    Returns ``(edge_indices [E,4], edge_tri_indices [E,2], tri_edge_indices [T,3])``.
+    Returns ``(edge_indices [E,4], edge_tri_indices [E,2], tri_edge_indices [T,3])``; warns once if
+    a vertex pair is shared by more than two triangle sides (the extra sides are ignored).
     """
-    tris = _int_rows(tri_indices, 3)
+    import ctypes  # noqa: PLC0415
+
+    from .. import _native  # noqa: PLC0415
+
+    tris = np.ascontiguousarray(_int_rows(tri_indices, 3))
     tri_count = tris.shape[0]
-    if tri_count == 0:
-        return (
-            np.empty((0, 4), dtype=np.int32),
-            np.empty((0, 2), dtype=np.int32),
-            np.empty((0, 3), dtype=np.int32),
-        )
-    entry_v0 = np.stack((tris[:, 0], tris[:, 1], tris[:, 2]), axis=1).reshape(-1)
-    entry_v1 = np.stack((tris[:, 1], tris[:, 2], tris[:, 0]), axis=1).reshape(-1)
-    entry_opp = np.stack((tris[:, 2], tris[:, 0], tris[:, 1]), axis=1).reshape(-1)
-    entry_tri = np.repeat(np.arange(tri_count, dtype=np.int32), 3)
-    entry_slot = np.tile(np.arange(3, dtype=np.int32), tri_count)
-
-    lo = np.minimum(entry_v0, entry_v1).astype(np.int64)
-    hi = np.maximum(entry_v0, entry_v1).astype(np.int64)
-    stride = int(max(hi.max(), 0)) + 1
-    keys = lo * stride + hi
-    _, first_entries, inverse = np.unique(keys, return_index=True, return_inverse=True)
-    first_order = np.argsort(first_entries, kind="stable")
-    edge_remap = np.empty(len(first_order), dtype=np.int32)
-    edge_remap[first_order] = np.arange(len(first_order), dtype=np.int32)
-    entry_edge = edge_remap[inverse.reshape(-1)]
-    edge_count = len(first_order)
-
-    # rank of each entry among the entries of its edge, in entry order
-    order = np.argsort(entry_edge, kind="stable")
-    sorted_edges = entry_edge[order]
-    group_start = np.searchsorted(sorted_edges, sorted_edges, side="left")
-    rank = np.empty_like(order)
-    rank[order] = np.arange(len(order)) - group_start
-
-    edge_indices = np.full((edge_count, 4), -1, dtype=np.int32)
-    edge_tri_indices = np.full((edge_count, 2), -1, dtype=np.int32)
-    tri_edge_indices = np.full((tri_count, 3), -1, dtype=np.int32)
-    tri_edge_indices[entry_tri, entry_slot] = entry_edge
-
-    s0 = rank == 0
-    e0 = entry_edge[s0]
-    edge_indices[e0, 0] = entry_opp[s0]
-    edge_indices[e0, 2] = entry_v0[s0]
-    edge_indices[e0, 3] = entry_v1[s0]
-    edge_tri_indices[e0, 0] = tri_start + entry_tri[s0]
-    s1 = rank == 1
-    e1 = entry_edge[s1]
-    edge_indices[e1, 1] = entry_opp[s1]
-    edge_tri_indices[e1, 1] = tri_start + entry_tri[s1]
-    if np.any(rank >= 2):
+    edges = np.empty((max(3 * tri_count, 1), 4), dtype=np.int32)
+    edge_tri = np.empty((max(3 * tri_count, 1), 2), dtype=np.int32)
+    tri_edge = np.empty((tri_count, 3), dtype=np.int32)
+    n_edges = ctypes.c_int64(0)
+    n_extra = ctypes.c_int64(0)
+    rc = _native.host_lib().vbd_host_number_edges(tri_count, tris.ctypes.data, int(tri_start), edges.ctypes.data,
+                                                  edge_tri.ctypes.data, tri_edge.ctypes.data, ctypes.byref(n_edges),
+                                                  ctypes.byref(n_extra))
+    if rc == -2:
+        raise ValueError("triangle indices must be non-negative")
+    if rc != 0:
+        raise ValueError(f"vbd_host_number_edges failed ({rc})")
+    if n_extra.value:
         warnings.warn("Detected non-manifold edge", stacklevel=2)
-    return edge_indices, edge_tri_indices, tri_edge_indices
-
-
-def build_edge_tri_maps_np(edge_indices, tri_indices):
-    """Edge/triangle maps for pre-numbered edges (``mesh.py:531-576``), numpy restatement.
-
-    Kept as the readable statement of the rule; :func:`build_edge_tri_maps`
-    (C++, linear time) is what :class:`MeshAdjacency` uses.
-    """
-    edges = _int_rows(edge_indices, 4)
-    tris = _int_rows(tri_indices, 3)
-    edge_count, tri_count = edges.shape[0], tris.shape[0]
-    edge_tri = np.full((edge_count, 2), -1, dtype=np.int32)
-    tri_edge = np.full((tri_count, 3), -1, dtype=np.int32)
-    if edge_count == 0 or tri_count == 0:
-        return edge_tri, tri_edge
-    e64 = edges.astype(np.int64)
-    t64 = tris.astype(np.int64)
-    stride = max(int(e64[:, 2:4].max()), int(t64.max())) + 1
-    edge_key = np.minimum(e64[:, 2], e64[:, 3]) * stride + np.maximum(e64[:, 2], e64[:, 3])
-    nxt = np.roll(t64, -1, axis=1)
-    slot_key = np.minimum(t64, nxt) * stride + np.maximum(t64, nxt)
-    slot_opp = np.roll(t64, -2, axis=1)
-    order = np.argsort(edge_key, kind="stable")
-    pos = np.clip(np.searchsorted(edge_key[order], slot_key), 0, edge_count - 1)
-    cand = order[pos]
-    found = edge_key[cand] == slot_key
-    side0 = found & (slot_opp == e64[cand, 0])
-    side1 = found & ~side0 & (slot_opp == e64[cand, 1])
-    tri_edge = np.where(side0 | side1, cand, -1).astype(np.int32)
-    tri_ids = np.broadcast_to(np.arange(tri_count, dtype=np.int32)[:, None], (tri_count, 3))
-    edge_tri[cand[side0], 0] = tri_ids[side0]
-    edge_tri[cand[side1], 1] = tri_ids[side1]
-    return edge_tri, tri_edge
+    e = n_edges.value
+    return edges[:e].copy(), edge_tri[:e].copy(), tri_edge
 
 
 def _vertex_csr_np(rows: np.ndarray, slots_in_fill_order, particle_count: int, values_per_entry: int = 2):
@@ -290,5 +231,4 @@ def _csr_offsets_or_zeros(offsets: np.ndarray, particle_count: int) -> np.ndarra
     return offsets
 
 
-__all__ = ["MeshAdjacency", "MeshAdjacencyData", "build_edge_tri_maps", "build_edge_tri_maps_np",
-           "compute_edge_adjacency"]
+__all__ = ["MeshAdjacency", "MeshAdjacencyData", "build_edge_tri_maps", "compute_edge_adjacency"]

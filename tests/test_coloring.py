@@ -36,7 +36,7 @@ def test_simple_path_graph():
 def test_random_graph_valid_and_balanced(alg, seed):
     rng = np.random.default_rng(seed)
     n = 500
-    edges = gc._canonicalize_edges_np(rng.integers(0, n, size=(2000, 2)))
+    edges = gc._pairs_to_graph_edges(rng.integers(0, n, size=(2000, 2)))
     edges = edges[edges[:, 0] != edges[:, 1]]
     groups = gc.color_graph(n, edges, balance_colors=True, target_max_min_color_ratio=1.1, algorithm=alg)
     colors = _colors_from_groups(n, groups)
@@ -134,3 +134,17 @@ def test_world_merge_keeps_colorings_valid():
     np.testing.assert_array_equal(model.particle_colors.numpy(), colors)
     assert model.world_count == 4
     assert set(np.unique(model.particle_world.numpy())) == {0, 1, 2, 3}
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_graph_edges_equal_set_of_sorted_pairs(seed):
+    """Canonical edges == sorted set of (min, max) pairs without negatives (graph_coloring.py:45-51)."""
+    rng = np.random.default_rng(seed)
+    pairs = rng.integers(-1, 40, size=(500, 2))
+    expect = sorted({(min(a, b), max(a, b)) for a, b in pairs.tolist() if a >= 0 and b >= 0})
+    np.testing.assert_array_equal(gc._pairs_to_graph_edges(pairs), np.array(expect, dtype=np.int32))
+    hinges = rng.integers(-1, 30, size=(60, 4)).astype(np.int32)
+    expect = sorted({(min(h[i], h[j]), max(h[i], h[j])) for h in hinges.tolist() for i in range(4)
+                     for j in range(i + 1, 4) if h[i] >= 0 and h[j] >= 0})
+    np.testing.assert_array_equal(gc.construct_trimesh_graph_edges(None, None, hinges, None),
+                                  np.array(expect, dtype=np.int32).reshape(-1, 2))

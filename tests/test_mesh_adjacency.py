@@ -249,10 +249,35 @@ def _random_soup(rng, n_verts, n_tris):
     return tris
 
 
+def _loop_edge_tri_maps(edges, tris):
+    """The linking rule of mesh.py:531-576 as plain loops: each triangle side takes the lowest-numbered
+    edge on its vertex pair, and links only if its opposite vertex is that edge's o0 (side 0) or o1
+    (side 1); a later side overwrites an earlier one on the same (edge, side)."""
+    edge_tri = np.full((len(edges), 2), -1, dtype=np.int32)
+    tri_edge = np.full((len(tris), 3), -1, dtype=np.int32)
+    lowest = {}
+    for e in range(len(edges) - 1, -1, -1):
+        a, b = int(edges[e, 2]), int(edges[e, 3])
+        if a >= 0 and b >= 0:
+            lowest[frozenset((a, b))] = e
+    for t, tri in enumerate(tris.tolist()):
+        for k in range(3):
+            e = lowest.get(frozenset((tri[k], tri[(k + 1) % 3])))
+            if e is None:
+                continue
+            opp = tri[(k + 2) % 3]
+            for side in (0, 1):
+                if opp == edges[e, side]:
+                    tri_edge[t, k] = e
+                    edge_tri[e, side] = t
+                    break
+    return edge_tri, tri_edge
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_native_edge_tri_maps_equal_numpy_restatement(seed):
-    """C++ vbd_host_edge_tri_maps == the numpy statement of mesh.py:531-576, incl. duplicates and placeholders."""
-    from newton_amd.utils.mesh import build_edge_tri_maps, build_edge_tri_maps_np
+def test_native_edge_tri_maps_equal_loop_rule(seed):
+    """C++ vbd_host_edge_tri_maps == the rule of mesh.py:531-576 as loops, incl. duplicates and placeholders."""
+    from newton_amd.utils.mesh import build_edge_tri_maps
 
     rng = np.random.default_rng(seed)
     tris = _random_soup(rng, 12, 40)  # dense soup: non-manifold edges, repeated pairs
@@ -269,9 +294,29 @@ def test_native_edge_tri_maps_equal_numpy_restatement(seed):
     perm = rng.permutation(len(edges))
     edges = np.ascontiguousarray(edges[perm])
     a = build_edge_tri_maps(edges, tris)
-    b = build_edge_tri_maps_np(edges, tris)
+    b = _loop_edge_tri_maps(edges, tris)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_edge_numbering_non_manifold_soup_matches_serial_loop(seed):
+    """A dense soup (pairs shared by three or more sides): the extra sides change nothing and warn."""
+    rng = np.random.default_rng(seed)
+    tris = _random_soup(rng, 9, 30)
+    with pytest.warns(UserWarning, match="non-manifold"):
+        e, et, te = compute_edge_adjacency(tris, tri_start=5)
+    e_ref, et_ref, te_ref = _serial_edge_numbering(tris)
+    np.testing.assert_array_equal(e, e_ref)
+    np.testing.assert_array_equal(et, np.where(et_ref >= 0, et_ref + 5, -1))
+    np.testing.assert_array_equal(te, te_ref)
+
+
+def test_edge_numbering_empty_and_negative():
+    e, et, te = compute_edge_adjacency(np.empty((0, 3), dtype=np.int32))
+    assert e.shape == (0, 4) and et.shape == (0, 2) and te.shape == (0, 3)
+    with pytest.raises(ValueError):
+        compute_edge_adjacency(np.array([[0, 1, -2]], dtype=np.int32))
 
 
 @pytest.mark.parametrize("cols,vpe", [([0, 1, 2], 2), ([2, 3, 0, 1], 2), ([0, 1], 1)])
