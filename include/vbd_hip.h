@@ -23,6 +23,7 @@
  *   _finalize_particles/update_velocity solver_vbd.py:3158-3169, particle_vbd_kernels.py:1468-1471 -> vbd_finalize
  *   create_soft_contacts (particle vs static plane/sphere) newton/_src/sim/collide.py:1553-1592,
  *     newton/_src/geometry/kernels.py:1020-1184 -> vbd_collide_generate
+ *   SolverBase.integrate_particles     newton/_src/solvers/solver.py:20-56, :297-330 -> vbd_integrate_particles
  */
 #ifndef VBD_HIP_H
 #define VBD_HIP_H
@@ -169,8 +170,22 @@ int32_t vbd_destroy(VbdContext* ctx);
 
 /* Rebuilds the color-pass layout after the caller changed static model arrays
  * (element materials, rest data, flags, masses).  SolverVBD.notify_model_changed
- * (newton/_src/solvers/solver.py:375).  Allocates; not for use during capture. */
+ * (newton/_src/solvers/solver.py:375-410, solver_vbd.py:851-856).  When the new layout has the
+ * shape of the old one (same array sizes, per-color batch ranges and batch width: the case of
+ * material and rest-data edits that leave every element's stiffness non-zero) it is written into
+ * the existing device buffers, so graphs captured before stay valid and see the new values.
+ * Otherwise (elements switched to or from zero stiffness, particles activated or deactivated,
+ * new color groups) the layout is re-allocated and graphs captured before this call must be
+ * captured again.  Synchronises the device; not for use during capture. */
 int32_t vbd_rebuild(VbdContext* ctx);
+
+/* Scalar settings read per launch: iterations of vbd_step, friction_epsilon, and the model's
+ * soft_contact_ke / kd / mu (averaged with each shape's material in vbd_contact_prepare).  The
+ * reference reads SolverVBD.iterations / friction_epsilon and Model.soft_contact_* at every step
+ * (solver_vbd.py:1773-1825, :2493-2514); graphs captured before keep the values they were captured
+ * with. */
+int32_t vbd_set_scalars(VbdContext* ctx, int32_t iterations, float friction_epsilon, float soft_contact_ke,
+                        float soft_contact_kd, float soft_contact_mu);
 
 /* Layout statistics: out4 = {fast batches, generic-path vertices, packed element items, colors}. */
 int32_t vbd_layout_stats(VbdContext* ctx, int64_t* out4);
@@ -199,10 +214,16 @@ int32_t vbd_set_pass_waves(VbdContext* ctx, int32_t waves);
  * vbd_autotune also chooses between. */
 int32_t vbd_set_aligned_items(VbdContext* ctx, void* stream, int32_t aligned);
 
-/* Selects the work-group shape (256 or 128 lanes) directly, rebuilding the
+/* Selects the work-group shape (256, 192 or 128 lanes) directly, rebuilding the
  * layout; the contact ranges of the current substep are carried over.  Not for
  * capture. */
 int32_t vbd_set_block_shape(VbdContext* ctx, void* stream, int32_t block);
+
+/* All three shape choices at once (lanes 256 / 192 / 128, waves 1 / 7, aligned 0 / 1), with one
+ * layout rebuild; what vbd_autotune picks between.  Synchronises; not for capture. */
+int32_t vbd_set_pass_shape(VbdContext* ctx, void* stream, int32_t block, int32_t waves, int32_t aligned);
+/* out3 = {lanes, waves, aligned} of the current pass shape. */
+int32_t vbd_get_pass_shape(VbdContext* ctx, int32_t* out3);
 
 /* Full substep: contact prepare, forward, iterations x colors, finalize.
  * q_in is overwritten with the iterate (as the reference does); q_out/qd_out
@@ -216,6 +237,16 @@ int32_t vbd_forward(VbdContext* ctx, void* stream, float* q, const float* qd, co
 int32_t vbd_iteration(VbdContext* ctx, void* stream, float* q, float dt);
 int32_t vbd_color_pass(VbdContext* ctx, void* stream, int32_t color, float* q, float dt);
 int32_t vbd_finalize(VbdContext* ctx, void* stream, const float* q, float* q_out, float* qd_out, float dt);
+
+/* SolverBase.integrate_particles (newton/_src/solvers/solver.py:20-56, :297-330): one semi-implicit
+ * Euler step v' = v + (f * w + g * [w > 0]) * dt, |v'| limited to v_max, x' = x + v' * dt, for the n
+ * particles of x/v/f ([n,3]) with inverse masses w and flags (bit 0 ACTIVE).  Inactive particles get
+ * x' = x and leave v_new unwritten.  g is the gravity row ([gravity_count,3]) of the particle's world
+ * (world may be NULL: world 0; -1 selects world 0).  Needs no context; enqueued on stream. */
+int32_t vbd_integrate_particles(void* stream, int32_t n, const float* x, const float* v, const float* f,
+                                const float* inv_mass, const int32_t* flags, const int32_t* world,
+                                const float* gravity, int32_t gravity_count, float dt, float v_max, float* x_new,
+                                float* v_new);
 
 /* Incremental-potential force residual at q: g_i = m_i/h^2 (y_i - q_i) +
  * elastic + damping + contact forces (the force the block solve assembles),

@@ -2,13 +2,13 @@
 
 Public names mirror the reference's ``newton`` package for this path:
 ``ModelBuilder``, ``Model``, ``State``, ``Control``, ``Contacts``,
-``CollisionPipeline``, ``ParticleFlags``, ``ShapeFlags``, ``GeoType`` and
-``newton_amd.solvers.SolverVBD``.
+``CollisionPipeline``, ``ParticleFlags``, ``ShapeFlags``, ``GeoType``,
+``ModelFlags``, ``StateFlags`` and ``newton_amd.solvers.SolverBase`` / ``SolverVBD``.
 """
 
 from . import solvers
 from .geometry import GeoType, ParticleFlags, ShapeFlags
-from .sim import CollisionPipeline, Contacts, Control, Model, ModelBuilder, State
+from .sim import CollisionPipeline, Contacts, Control, Model, ModelBuilder, ModelFlags, State, StateFlags
 
 __version__ = "0.1.0"
 
@@ -19,8 +19,10 @@ __all__ = [
     "GeoType",
     "Model",
     "ModelBuilder",
+    "ModelFlags",
     "ParticleFlags",
     "ShapeFlags",
     "State",
+    "StateFlags",
     "solvers",
 ]

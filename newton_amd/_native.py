@@ -153,6 +153,9 @@ HIP_EXPORTS = {
     "vbd_set_block_shape": (c_int32, [c_void_p, c_void_p, c_int32]),
     "vbd_set_pass_waves": (c_int32, [c_void_p, c_int32]),
     "vbd_set_aligned_items": (c_int32, [c_void_p, c_void_p, c_int32]),
+    "vbd_set_pass_shape": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32]),
+    "vbd_get_pass_shape": (c_int32, [c_void_p, c_void_p]),
+    "vbd_set_scalars": (c_int32, [c_void_p, c_int32, c_float, c_float, c_float, c_float]),
     "vbd_step": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(VbdContactsDesc), c_float],
@@ -162,6 +165,11 @@ HIP_EXPORTS = {
     "vbd_iteration": (c_int32, [c_void_p, c_void_p, c_void_p, c_float]),
     "vbd_color_pass": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_float]),
     "vbd_finalize": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float]),
+    "vbd_integrate_particles": (
+        c_int32,
+        [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_float,
+         c_float, c_void_p, c_void_p],
+    ),
     "vbd_residual": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "vbd_set_iterate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32]),
     "vbd_set_iterate_range": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32]),

@@ -12,6 +12,7 @@
 //                      update   (particle_vbd_kernels.py:2289-2401, :2568-2706)
 //   k_solve_generic    the same for vertices with more than 256 elements (CSR walk)
 //   k_finalize         velocity update   (particle_vbd_kernels.py:1468-1471)
+//   k_integrate_particles  SolverBase.integrate_particles, semi-implicit Euler (solvers/solver.py:20-56)
 //   k_collide_*        particle vs static plane/sphere soft contacts (geometry/kernels.py:1020-1184)
 //
 // Numerics: every FP32 expression follows the reference's CPU kernel order with
@@ -542,10 +543,47 @@ __device__ void eval_contact(V3 x, V3 x_prev, V3 bx, V3 body_vel, V3 n, float ra
     h_out = hess;
 }
 
-// Guarded 3x3 solve, particle_vbd_kernels.py:2701-2706: returns false when |det H| <= 1e-8.
+// Guarded 3x3 solve, particle_vbd_kernels.py:2701-2706: returns false when |det H| <= 1e-8.  With
+// fp64_ill the Frobenius condition number ||H||_F ||adj H||_F / |det H| is evaluated in double from
+// the stored entries (a single-precision estimate is unreliable exactly for ill-conditioned blocks);
+// above 1e6 the block is solved in double, otherwise in the reference's single-precision adjugate order.
 __device__ bool block_solve(const M3& h, V3 f, int fp64_ill, V3& dx) {
     float det = mdet(h);
     if (!(fabsf(det) > 1e-8f)) return false;
+    if (fp64_ill) {
+        double a[3][3], c[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) a[i][j] = (double)h.m[i][j];
+        c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+        c[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+        c[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+        c[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+        c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+        c[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+        c[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+        c[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+        c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+        const double det_d = a[0][0] * c[0][0] + a[0][1] * c[1][0] + a[0][2] * c[2][0];
+        double n1 = 0.0, n2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                n1 += a[i][j] * a[i][j];
+                n2 += c[i][j] * c[i][j];
+            }
+        if (!(sqrt(n1 * n2) <= 1.0e6 * fabs(det_d))) {
+            const double inv = 1.0 / det_d;
+            const double fd0 = (double)f.x, fd1 = (double)f.y, fd2 = (double)f.z;
+            const double r0 = (c[0][0] * inv) * fd0 + (c[0][1] * inv) * fd1 + (c[0][2] * inv) * fd2;
+            const double r1 = (c[1][0] * inv) * fd0 + (c[1][1] * inv) * fd1 + (c[1][2] * inv) * fd2;
+            const double r2 = (c[2][0] * inv) * fd0 + (c[2][1] * inv) * fd1 + (c[2][2] * inv) * fd2;
+            dx = v3((float)r0, (float)r1, (float)r2);
+            return true;
+        }
+    }
     M3 b;
     b.m[0][0] = h.m[1][1] * h.m[2][2] - h.m[1][2] * h.m[2][1];
     b.m[1][0] = h.m[1][2] * h.m[2][0] - h.m[1][0] * h.m[2][2];
@@ -561,43 +599,6 @@ __device__ bool block_solve(const M3& h, V3 f, int fp64_ill, V3& dx) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) b.m[i][j] = b.m[i][j] * inv_det;
-    if (fp64_ill) {
-        float n1 = 0.0f, n2 = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                n1 = n1 + h.m[i][j] * h.m[i][j];
-                n2 = n2 + b.m[i][j] * b.m[i][j];
-            }
-        float cond = sqrtf(n1) * sqrtf(n2);
-        if (!(cond <= 1.0e6f)) {
-            double a[3][3], c[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) a[i][j] = (double)h.m[i][j];
-            double det_d = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) +
-                           a[0][1] * (a[1][2] * a[2][0] - a[1][0] * a[2][2]) +
-                           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-            c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
-            c[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
-            c[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
-            c[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
-            c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
-            c[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
-            c[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
-            c[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
-            c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
-            double inv = 1.0 / det_d;
-            double fd0 = (double)f.x, fd1 = (double)f.y, fd2 = (double)f.z;
-            double r0 = (c[0][0] * inv) * fd0 + (c[0][1] * inv) * fd1 + (c[0][2] * inv) * fd2;
-            double r1 = (c[1][0] * inv) * fd0 + (c[1][1] * inv) * fd1 + (c[1][2] * inv) * fd2;
-            double r2 = (c[2][0] * inv) * fd0 + (c[2][1] * inv) * fd1 + (c[2][2] * inv) * fd2;
-            dx = v3((float)r0, (float)r1, (float)r2);
-            return true;
-        }
-    }
     dx = mvec(b, f);
     return true;
 }
@@ -1234,6 +1235,33 @@ __global__ void k_finalize(KParams P, const float* __restrict__ q, float* __rest
     st3(qd_out, i, divs(sub(x, ld3(P.q_prev, i)), dt));
 }
 
+// SolverBase.integrate_particles: semi-implicit Euler with a speed limit (solver.py:20-56).  Inactive
+// particles keep x and leave v_new unwritten; gravity is that of the particle's world (-1: world 0);
+// world ids at or beyond the gravity table read its last row instead of memory outside it.
+__global__ void k_integrate_particles(int n, const float* __restrict__ x, const float* __restrict__ v,
+                                      const float* __restrict__ f, const float* __restrict__ w,
+                                      const int* __restrict__ flags, const int* __restrict__ world,
+                                      const float* __restrict__ gravity, int gravity_count, float dt, float v_max,
+                                      float* __restrict__ x_new, float* __restrict__ v_new) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 x0 = ld3(x, i);
+    if (!(flags[i] & 1)) {
+        st3(x_new, i, x0);
+        return;
+    }
+    const float inv_mass = w[i];
+    int wi = world ? world[i] : 0;
+    wi = wi < 0 ? 0 : (wi >= gravity_count ? gravity_count - 1 : wi);
+    const V3 g = ld3(gravity, wi);
+    const float has_mass = -inv_mass < 0.0f ? 1.0f : 0.0f;  // wp.step(-inv_mass)
+    V3 v1 = add(ld3(v, i), scl(add(scl(ld3(f, i), inv_mass), scl(g, has_mass)), dt));
+    const float mag = length(v1);
+    if (mag > v_max) v1 = scl(v1, v_max / mag);
+    st3(x_new, i, add(x0, scl(v1, dt)));
+    st3(v_new, i, v1);
+}
+
 __global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
@@ -1641,6 +1669,7 @@ struct VbdContext {
     int max_v = vbd_layout::kLayoutMaxV;
     int blk_words = vbd_layout::block_words(vbd_layout::kLayoutMaxV);
     int64_t fast_items = 0;
+    std::vector<size_t> lay_sizes;  // element counts of the uploaded layout arrays (rebuild in place when equal)
     VbdContactsDesc contacts{};
     bool have_contacts = false;
 };
@@ -1686,6 +1715,25 @@ void free_layout(VbdContext* c) {
     c->bw_key.clear();
     hipFree(c->spr_rows);
     c->spr_rows = nullptr;
+}
+
+// Smallest / largest particle of each packed batch (the batched drivers launch live worlds' batches).
+void set_batch_particle_ranges(VbdContext* c, const vbd_layout::PassLayout& L) {
+    const int nb = L.fast_color_off.back();
+    c->bat_pmin.assign((size_t)nb, 0);
+    c->bat_pmax.assign((size_t)nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        const int32_t* blk = &L.batch_blk[(size_t)b * L.blk_words];
+        int lo = INT32_MAX, hi = -1;
+        for (int i = 0; i < blk[1]; ++i) {
+            const int v = blk[kBatchHdr + 4 * i];
+            lo = std::min(lo, v);
+            hi = std::max(hi, v);
+        }
+        c->bat_pmin[b] = hi < 0 ? 0 : lo;
+        c->bat_pmax[b] = hi < 0 ? 0 : hi;
+    }
+    c->bw_key.clear();
 }
 
 // Reads the model arrays back once and builds the color-pass layout on the host (vbd_layout.h).
@@ -1741,17 +1789,50 @@ int32_t build_layout(VbdContext* c) {
     vbd_layout::PassLayout L;
     err = vbd_layout::build_pass_layout(hm, L, c->block, c->block / kComp, c->aligned != 0);
     if (!err.empty()) return fail(VBD_ERR_ARG, "vbd_create: " + err);
+    const std::pair<int**, std::vector<int32_t>*> arrays[] = {
+        {&c->lay_batch, &L.batch_blk}, {&c->lay_vert, &L.vert_rec}, {&c->lay_tri, &L.tri_rec},
+        {&c->lay_edge, &L.edge_rec}, {&c->lay_edge_meta, &L.edge_meta}, {&c->lay_tet, &L.tet_rec},
+        {&c->lay_tet_meta, &L.tet_meta}, {&c->lay_slow, &L.slow_pos}, {&c->lay_spr, &L.spr_rec},
+        {&c->lay_pre, &L.pre_item}, {&c->lay_slow_spr_off, &L.slow_spr_off}, {&c->lay_slow_spr, &L.slow_spr}};
+    std::vector<size_t> sizes;
+    for (const auto& pr : arrays) sizes.push_back(pr.second->size());
+    // A rebuild whose layout has the same shape (array sizes, per-color batch / generic / pre-row
+    // ranges, batch width) -- the case of material or rest-data edits -- rewrites the existing
+    // device buffers, so every pointer and launch shape a captured graph holds stays valid.
+    const bool in_place = c->lay_batch != nullptr && sizes == c->lay_sizes && L.fast_color_off == c->fast_color_off &&
+                          L.slow_color_off == c->slow_color_off && L.pre_color_off == c->pre_color_off &&
+                          L.max_v == c->max_v && L.blk_words == c->blk_words;
+    if (in_place) {
+        for (const auto& pr : arrays) {
+            if (pr.second->empty()) continue;
+            hipError_t e = hipMemcpy(*pr.first, pr.second->data(), sizeof(int32_t) * pr.second->size(),
+                                     hipMemcpyHostToDevice);
+            if (e != hipSuccess)
+                return fail(VBD_ERR_HIP, std::string("vbd_rebuild: rewriting the layout: ") + hipGetErrorString(e));
+        }
+        std::vector<int32_t> pos_of((size_t)hm.N);
+        for (int k = 0; k < hm.N; ++k) pos_of[hm.color_particles[k]] = k;
+        hipError_t e = hipMemcpy(c->pos_of, pos_of.data(), sizeof(int32_t) * (size_t)hm.N, hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return fail(VBD_ERR_HIP, std::string("vbd_rebuild: color positions: ") + hipGetErrorString(e));
+        c->fast_items = L.fast_items;
+        set_batch_particle_ranges(c, L);
+        // the batches may hold other vertices now: recompute their contact ranges of the current substep
+        const int nbat = c->fast_color_off.back();
+        if (nbat > 0) {
+            hipLaunchKernelGGL(k_batch_contacts, dim3(blocks_for(nbat, 256)), dim3(256), 0, 0, nbat, c->lay_batch,
+                               c->blk_words, c->max_v, c->pos_coff, c->bat_cinfo, c->bat_coff);
+            VBD_LAUNCH_CHECK("k_batch_contacts");
+        }
+        VBD_HIP_TRY(hipDeviceSynchronize());
+        return VBD_OK;
+    }
     free_layout(c);
-    for (auto pr : {std::make_pair(&c->lay_batch, &L.batch_blk), std::make_pair(&c->lay_vert, &L.vert_rec),
-                    std::make_pair(&c->lay_tri, &L.tri_rec), std::make_pair(&c->lay_edge, &L.edge_rec),
-                    std::make_pair(&c->lay_edge_meta, &L.edge_meta), std::make_pair(&c->lay_tet, &L.tet_rec),
-                    std::make_pair(&c->lay_tet_meta, &L.tet_meta), std::make_pair(&c->lay_slow, &L.slow_pos),
-                    std::make_pair(&c->lay_spr, &L.spr_rec), std::make_pair(&c->lay_pre, &L.pre_item),
-                    std::make_pair(&c->lay_slow_spr_off, &L.slow_spr_off),
-                    std::make_pair(&c->lay_slow_spr, &L.slow_spr)}) {
+    for (const auto& pr : arrays) {
         err = h2d(pr.first, *pr.second);
         if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_create: uploading the layout: " + err);
     }
+    c->lay_sizes = sizes;
     c->fast_color_off = L.fast_color_off;
     c->slow_color_off = L.slow_color_off;
     c->pre_color_off = L.pre_color_off;
@@ -1772,19 +1853,7 @@ int32_t build_layout(VbdContext* c) {
     c->blk_words = L.blk_words;
     {
         const int nb = L.fast_color_off.back();
-        c->bat_pmin.assign((size_t)nb, 0);
-        c->bat_pmax.assign((size_t)nb, 0);
-        for (int b = 0; b < nb; ++b) {
-            const int32_t* blk = &L.batch_blk[(size_t)b * L.blk_words];
-            int lo = INT32_MAX, hi = -1;
-            for (int i = 0; i < blk[1]; ++i) {
-                const int v = blk[kBatchHdr + 4 * i];
-                lo = std::min(lo, v);
-                hi = std::max(hi, v);
-            }
-            c->bat_pmin[b] = hi < 0 ? 0 : lo;
-            c->bat_pmax[b] = hi < 0 ? 0 : hi;
-        }
+        set_batch_particle_ranges(c, L);
         hipError_t e = hipMalloc(&c->batch_list, sizeof(int) * (size_t)std::max(nb, 1));
         if (e != hipSuccess) return fail(VBD_ERR_HIP, std::string("vbd_create: batch list: ") + hipGetErrorString(e));
     }
@@ -2458,6 +2527,20 @@ int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
     return VBD_OK;
 }
 
+int32_t vbd_integrate_particles(void* stream, int32_t n, const float* x, const float* v, const float* f,
+                                const float* inv_mass, const int32_t* flags, const int32_t* world,
+                                const float* gravity, int32_t gravity_count, float dt, float v_max, float* x_new,
+                                float* v_new) {
+    if (n < 0) return fail(VBD_ERR_ARG, "vbd_integrate_particles: negative particle count");
+    if (n == 0) return VBD_OK;
+    if (!x || !v || !f || !inv_mass || !flags || !gravity || !x_new || !v_new || gravity_count <= 0)
+        return fail(VBD_ERR_ARG, "vbd_integrate_particles: null array or empty gravity table");
+    hipLaunchKernelGGL(k_integrate_particles, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x, v,
+                       f, inv_mass, flags, world, gravity, gravity_count, dt, v_max, x_new, v_new);
+    VBD_LAUNCH_CHECK("k_integrate_particles");
+    return VBD_OK;
+}
+
 int32_t vbd_finalize(VbdContext* c, void* stream, const float* q, float* q_out, float* qd_out, float dt) {
     if (!c || !q || !q_out || !qd_out) return fail(VBD_ERR_ARG, "vbd_finalize: null argument");
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(c->model.particle_count, 256)), dim3(256), 0, (hipStream_t)stream,
@@ -2509,6 +2592,42 @@ int32_t vbd_set_aligned_items(VbdContext* c, void* stream, int32_t aligned) {
 int32_t vbd_set_pass_waves(VbdContext* c, int32_t waves) {
     if (!c || (waves != 1 && waves != 7)) return fail(VBD_ERR_ARG, "vbd_set_pass_waves: waves must be 1 or 7");
     c->waves = waves;
+    return VBD_OK;
+}
+
+int32_t vbd_set_pass_shape(VbdContext* c, void* stream, int32_t block, int32_t waves, int32_t aligned) {
+    if (!c || (block != 256 && block != 192 && block != 128) || (waves != 1 && waves != 7) ||
+        (aligned != 0 && aligned != 1))
+        return fail(VBD_ERR_ARG, "vbd_set_pass_shape: block must be 256, 192 or 128, waves 1 or 7, aligned 0 or 1");
+    c->waves = waves;
+    if (c->block == block && c->aligned == aligned) return VBD_OK;
+    c->block = block;
+    c->aligned = aligned;
+    int32_t rc = build_layout(c);
+    if (rc) return rc;
+    rc = launch_batch_contacts(c, (hipStream_t)stream);
+    if (rc) return rc;
+    VBD_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return VBD_OK;
+}
+
+int32_t vbd_get_pass_shape(VbdContext* c, int32_t* out3) {
+    if (!c || !out3) return fail(VBD_ERR_ARG, "vbd_get_pass_shape: null argument");
+    out3[0] = c->block;
+    out3[1] = c->waves;
+    out3[2] = c->aligned;
+    return VBD_OK;
+}
+
+int32_t vbd_set_scalars(VbdContext* c, int32_t iterations, float friction_epsilon, float soft_contact_ke,
+                        float soft_contact_kd, float soft_contact_mu) {
+    if (!c || iterations < 0) return fail(VBD_ERR_ARG, "vbd_set_scalars: null context or negative iterations");
+    c->params.iterations = iterations;
+    c->params.friction_epsilon = friction_epsilon;
+    c->kp.friction_epsilon = friction_epsilon;
+    c->model.soft_contact_ke = soft_contact_ke;
+    c->model.soft_contact_kd = soft_contact_kd;
+    c->model.soft_contact_mu = soft_contact_mu;
     return VBD_OK;
 }
 

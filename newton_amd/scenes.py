@@ -120,6 +120,27 @@ def cloth_strip_pinned(device="cuda:0", nx: int = 6, ny: int = 4, seed: int = 0,
     return model, state, info
 
 
+def bead_rope_light(device="cuda:0", beads: int = 12, mass: float = 1.0e-8, spring_ke: float = 1.0e4,
+                    height: float = 0.22, iterations: int = 4):
+    """A zig-zag rope of nearly massless beads joined by stiff springs, first bead pinned, swinging under
+    gravity.  Two alternating colors.
+
+    With ``mass`` 1e-8 kg the inertia term m/h^2 is ~4e-3 against spring stiffness 1e4, so the vertex
+    blocks have Frobenius condition numbers far above 1e6: the case the FP64 block solve is for.
+    """
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    for i in range(beads):
+        b.add_particle((0.05 * i, height - 0.02 * i, 0.01 * (i % 2)), (0.0, 0.0, 0.0), 0.0 if i == 0 else mass,
+                       radius=0.01, flags=0 if i == 0 else int(ParticleFlags.ACTIVE))
+    for i in range(beads - 1):
+        b.add_spring(i, i + 1, spring_ke, 0.0)
+    b.set_coloring([np.arange(0, beads, 2), np.arange(1, beads, 2)])
+    model, state = _finish(b, device, 0.0, 0, 1.0e5, 0.0, 0.5)
+    info = {"workload": f"bead_rope_{beads}", "dt": FRAME_DT / SUBSTEPS, "substeps": SUBSTEPS,
+            "iterations": iterations}
+    return model, state, info
+
+
 def cloth_batched(device="cuda:0", worlds: int = 4096, side_min: int = 16, side_max: int = 64, seed: int = 2,
                   iterations: int = 5, cell: float = 1.0 / 31.0, height: float = 1.0, max_tilt_deg: float = 30.0):
     """Many independent cloth worlds of uneven size solved together (4096 worlds ~ 7.4M vertices by default).

@@ -155,6 +155,7 @@ class ModelBuilder:
         self.current_world = -1
         self.default_shape_cfg = ModelBuilder.ShapeConfig()
         self.default_particle_radius = 0.1
+        self.particle_max_velocity = 1e5
         self.default_tri_ke = 100.0
         self.default_tri_ka = 100.0
         self.default_tri_kd = 10.0
@@ -902,6 +903,7 @@ class ModelBuilder:
         m.particle_flags = i32(self._pflags.array())
         m.particle_world = i32(self._pworld.array())
         m.particle_max_radius = float(np.max(self._pradius.array())) if n else 0.0
+        m.particle_max_velocity = float(self.particle_max_velocity)
         colors = np.full(n, -1, dtype=np.int32)
         for c, g in enumerate(self.particle_color_groups):
             colors[np.asarray(g, dtype=np.int64)] = c

@@ -85,6 +85,7 @@ class Model:
         self.particle_flags = None
         self.particle_world = None
         self.particle_max_radius = 0.0
+        self.particle_max_velocity = 1e5
         self.particle_colors = None
         self.particle_color_groups: list[torch.Tensor] = []
         self.tri_count = 0

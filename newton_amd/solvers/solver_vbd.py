@@ -13,13 +13,17 @@ from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 
 import numpy as np
 import torch
 
 from .. import _native
 from ..sim.collide import current_stream_handle
+from ..sim.enums import StateFlags
 from ..sim.model import Contacts, Control, Model, State
+from ..utils.deprecation import deprecate_nonkeyword_arguments
+from .solver import SolverBase
 
 
 class _TrialState:
@@ -38,9 +42,20 @@ def color_layout(model: Model):
     return color_offsets, color_particles
 
 
-class SolverVBD:
-    """Vertex Block Descent for particles (cloth triangles, bending edges, tetrahedra) with soft contact."""
+class SolverVBD(SolverBase):
+    """Vertex Block Descent for particles (cloth triangles, bending edges, tetrahedra) with soft contact.
 
+    The keyword list is the reference's (``solver_vbd.py:221-269``); the ``rigid_*`` keywords are
+    validated as the reference validates them (``:403-423``, ``:648-669``) and otherwise only matter
+    for rigid bodies, which this package does not simulate.  Two keywords extend the list:
+    ``fp64_ill_conditioned`` (default on: a vertex block whose Frobenius condition number, evaluated
+    in FP64, exceeds 1e6 is solved in FP64; off gives the reference's all-FP32 solve, and the two
+    agree bit for bit on every block below that condition number) and ``soft_contact_capacity``
+    (largest ``Contacts.soft_contact_max`` the solver accepts without re-allocating; default
+    particles x shapes).
+    """
+
+    @deprecate_nonkeyword_arguments
     def __init__(
         self,
         model: Model,
@@ -61,32 +76,49 @@ class SolverVBD:
         particle_rest_shape_contact_exclusion_radius: float = 0.0,
         particle_external_vertex_contact_filtering_map: dict | None = None,
         particle_external_edge_contact_filtering_map: dict | None = None,
+        rigid_avbd_alpha: float = 0.95,
+        rigid_avbd_joint_alpha: float | None = None,
+        rigid_avbd_contact_alpha: float | None = None,
         rigid_avbd_beta: float = 0.0,
         rigid_avbd_linear_beta: float | None = None,
+        rigid_avbd_angular_beta: float | None = None,
+        rigid_avbd_gamma: float = 0.999,
+        rigid_contact_hard: bool = True,
+        rigid_contact_history: bool = False,
+        rigid_contact_stick_motion_eps: float | None = None,
+        rigid_contact_stick_freeze_translation_eps: float | None = None,
+        rigid_contact_stick_freeze_angular_eps: float | None = None,
         rigid_contact_k_start: float = 1.0e2,
-        fp64_ill_conditioned: bool = False,
+        rigid_body_contact_buffer_size: int = 64,
+        rigid_body_particle_contact_buffer_size: int = 256,
+        rigid_joint_linear_ke: float = 1.0e5,
+        rigid_joint_angular_ke: float = 1.0e5,
+        rigid_joint_linear_k_start: float = 1.0e2,
+        rigid_joint_angular_k_start: float = 1.0e1,
+        rigid_joint_linear_kd: float = 0.0,
+        rigid_joint_angular_kd: float = 0.0,
+        deterministic=None,
+        fp64_ill_conditioned: bool = True,
         soft_contact_capacity: int | None = None,
-        **unsupported,
     ):
         if rigid_avbd_beta < 0:
             raise ValueError(f"rigid_avbd_beta must be >= 0, got {rigid_avbd_beta}")
         linear_beta = rigid_avbd_beta if rigid_avbd_linear_beta is None else rigid_avbd_linear_beta
-        if linear_beta < 0:
-            raise ValueError(f"rigid_avbd_linear_beta must be >= 0, got {linear_beta}")
-        if rigid_contact_k_start < 0:
-            raise ValueError(f"rigid_contact_k_start must be >= 0, got {rigid_contact_k_start}")
-        if particle_enable_self_contact and particle_self_contact_margin < particle_self_contact_radius:
-            raise ValueError(
-                "particle_self_contact_margin is smaller than particle_self_contact_radius, this will result in "
-                "missing contacts and cause instability.\nIt is advisable to make particle_self_contact_margin "
-                "1.5-2 times larger than particle_self_contact_radius."
+        angular_beta = rigid_avbd_beta if rigid_avbd_angular_beta is None else rigid_avbd_angular_beta
+        if (rigid_contact_stick_motion_eps is not None or rigid_contact_stick_freeze_translation_eps is not None
+                or rigid_contact_stick_freeze_angular_eps is not None):
+            warnings.warn(
+                "SolverVBD rigid_contact_stick_motion_eps, rigid_contact_stick_freeze_translation_eps, and "
+                "rigid_contact_stick_freeze_angular_eps are deprecated and ignored, and will be removed in a "
+                "future release. Use CollisionPipeline(contact_matching='sticky', "
+                "contact_matching_pos_threshold=...) for persistent contact geometry. The SolverVBD body-level "
+                "contact deadzone was removed.",
+                DeprecationWarning,
+                stacklevel=3,
             )
-        for key in unsupported:
-            if not (key.startswith("particle_") or key.startswith("rigid_") or key == "deterministic"):
-                raise TypeError(f"SolverVBD got an unexpected keyword argument {key!r}")
+        super().__init__(model)
         if model.body_count > 0:
             raise NotImplementedError("rigid bodies are not part of this VBD path")
-        self.model = model
         self.iterations = int(iterations)
         self.friction_epsilon = float(friction_epsilon)
         self.integrate_with_external_rigid_solver = integrate_with_external_rigid_solver
@@ -103,20 +135,53 @@ class SolverVBD:
         self._external_vt_map = particle_external_vertex_contact_filtering_map
         self._external_ee_map = particle_external_edge_contact_filtering_map
         self.particle_enable_tile_solve = particle_enable_tile_solve
+        self.deterministic = deterministic  # every reduction here has a fixed order already
+        self.fp64_ill_conditioned = bool(fp64_ill_conditioned)
+        self._ctx = None
+        self._pass_shape = None  # (block, waves, aligned) chosen by autotune() / set_*(), kept across rebuilds
+        if model.particle_count > 0:
+            if model.soft_mesh_adjacency_device is None or model.soft_mesh_adjacency is None:
+                raise ValueError("model.soft_mesh_adjacency_device is missing; finalize the model with ModelBuilder.")
+            if self.particle_enable_self_contact and particle_self_contact_margin < particle_self_contact_radius:
+                raise ValueError(
+                    "particle_self_contact_margin is smaller than particle_self_contact_radius, this will result in "
+                    "missing contacts and cause instability.\nIt is advisable to make particle_self_contact_margin "
+                    "1.5-2 times larger than particle_self_contact_radius."
+                )
+            if len(model.particle_color_groups) == 0:
+                raise ValueError(
+                    "model.particle_color_groups is empty! When using the SolverVBD you must call "
+                    "ModelBuilder.color() or ModelBuilder.set_coloring() before calling ModelBuilder.finalize()."
+                )
+        # rigid AVBD hyper-parameters (solver_vbd.py:648-669)
+        if not (0.0 <= rigid_avbd_alpha <= 1.0):
+            raise ValueError(f"rigid_avbd_alpha must be in [0, 1], got {rigid_avbd_alpha}")
+        if rigid_avbd_joint_alpha is not None and not (0.0 <= rigid_avbd_joint_alpha <= 1.0):
+            raise ValueError(f"rigid_avbd_joint_alpha must be in [0, 1], got {rigid_avbd_joint_alpha}")
+        if rigid_avbd_contact_alpha is not None and not (0.0 <= rigid_avbd_contact_alpha <= 1.0):
+            raise ValueError(f"rigid_avbd_contact_alpha must be in [0, 1], got {rigid_avbd_contact_alpha}")
+        if linear_beta < 0:
+            raise ValueError(f"rigid_avbd_linear_beta must be >= 0, got {linear_beta}")
+        if angular_beta < 0:
+            raise ValueError(f"rigid_avbd_angular_beta must be >= 0, got {angular_beta}")
+        if not (0.0 <= rigid_avbd_gamma <= 1.0):
+            raise ValueError(f"rigid_avbd_gamma must be in [0, 1], got {rigid_avbd_gamma}")
+        if rigid_contact_k_start < 0:
+            raise ValueError(f"rigid_contact_k_start must be >= 0, got {rigid_contact_k_start}")
+        if rigid_joint_linear_k_start < 0:
+            raise ValueError(f"rigid_joint_linear_k_start must be >= 0, got {rigid_joint_linear_k_start}")
+        if rigid_joint_angular_k_start < 0:
+            raise ValueError(f"rigid_joint_angular_k_start must be >= 0, got {rigid_joint_angular_k_start}")
+        if rigid_joint_linear_ke < 0:
+            raise ValueError(f"rigid_joint_linear_ke must be >= 0, got {rigid_joint_linear_ke}")
+        if rigid_joint_angular_ke < 0:
+            raise ValueError(f"rigid_joint_angular_ke must be >= 0, got {rigid_joint_angular_ke}")
+        self.rigid_avbd_gamma = float(rigid_avbd_gamma)
         self.rigid_linear_beta = float(linear_beta)
         # fixed penalty (k = averaged ke) when the linear beta is 0 (solver_vbd.py:671)
         self.rigid_contact_k_start_value = -1.0 if linear_beta == 0.0 else float(rigid_contact_k_start)
-        self.fp64_ill_conditioned = bool(fp64_ill_conditioned)
-        self._ctx = None
         if model.particle_count == 0:
             return
-        if model.soft_mesh_adjacency_device is None or model.soft_mesh_adjacency is None:
-            raise ValueError("model.soft_mesh_adjacency_device is missing; finalize the model with ModelBuilder.")
-        if len(model.particle_color_groups) == 0:
-            raise ValueError(
-                "model.particle_color_groups is empty! When using the SolverVBD you must call ModelBuilder.color() "
-                "or ModelBuilder.set_coloring() before calling ModelBuilder.finalize()."
-            )
         if model.device.type != "cuda":
             raise RuntimeError("SolverVBD runs on the GPU through libvbd_hip; finalize the model on a cuda device")
         co, cp = color_layout(model)
@@ -232,11 +297,12 @@ class SolverVBD:
         with torch.cuda.device(m.device):
             _native.check(lib.vbd_create(ctypes.byref(d), ctypes.byref(prm), ctypes.byref(h)), "vbd_create")
         self._ctx = h
-        self._material_key = self._contact_material_key()
+        self._push_scalars()
 
     def _contact_material_key(self):
         m = self.model
-        return (float(m.soft_contact_ke), float(m.soft_contact_kd), float(m.soft_contact_mu), self.iterations)
+        return (float(m.soft_contact_ke), float(m.soft_contact_kd), float(m.soft_contact_mu), self.iterations,
+                self.friction_epsilon)
 
     def _destroy_context(self):
         if self._ctx is not None and self._ctx.value:
@@ -249,20 +315,34 @@ class SolverVBD:
         except Exception:
             pass
 
+    def _push_scalars(self):
+        """Hands the solver's and the model's scalar settings (iterations, friction epsilon, soft-contact
+        ke / kd / mu) to the context; they are read per launch, as the reference reads them."""
+        m = self.model
+        _native.check(_native.hip_lib().vbd_set_scalars(self._ctx, self.iterations, self.friction_epsilon,
+                                                         float(m.soft_contact_ke), float(m.soft_contact_kd),
+                                                         float(m.soft_contact_mu)), "vbd_set_scalars")
+        self._material_key = self._contact_material_key()
+
+    def _apply_pass_shape(self):
+        if self._pass_shape is not None:
+            self.set_pass_shape(*self._pass_shape)
+
     def _ensure_context(self, contacts: Contacts | None):
-        capturing = torch.cuda.is_current_stream_capturing()
         need = contacts.soft_contact_max if contacts is not None else 0
-        rebuild = need > self._capacity or self._contact_material_key() != self._material_key
-        if rebuild:
-            if capturing:
+        if need > self._capacity:
+            if torch.cuda.is_current_stream_capturing():
                 raise RuntimeError(
-                    "SolverVBD: solver buffers would need to be rebuilt (larger soft_contact_max or changed "
-                    "contact material / iterations) during graph capture; run one step before capturing"
+                    "SolverVBD: the soft-contact buffers would need to grow (larger soft_contact_max) during graph "
+                    "capture; run one step before capturing"
                 )
             torch.cuda.synchronize(self.model.device)
-            self._capacity = max(self._capacity, need)
+            self._capacity = need
             self._destroy_context()
             self._create_context()
+            self._apply_pass_shape()
+        elif self._contact_material_key() != self._material_key:
+            self._push_scalars()
 
     @staticmethod
     def _contacts_desc(contacts: Contacts | None):
@@ -279,10 +359,6 @@ class SolverVBD:
         return c
 
     # ------------------------------------------------------------ public API
-    @property
-    def device(self):
-        return self.model.device
-
     def step(self, state_in: State, state_out: State, control: Control | None, contacts: Contacts | None,
              dt: float) -> None:
         """One VBD step (``solver_vbd.py:1773-1825``).
@@ -678,16 +754,17 @@ class SolverVBD:
         )
         return out
 
-    def notify_model_changed(self, flags: int = 0) -> None:
-        """Re-read the model after the caller edited it (``solver.py:375``).
+    def notify_model_changed(self, flags=0) -> None:
+        """Re-read the model after the caller edited it (``solver.py:375-410``, ``solver_vbd.py:851-856``).
 
-        Element arrays (materials, rest data), masses and flags are re-read into
-        the color-pass layout now; contact-material scalars on the next step.
-        """
+        Element arrays (materials, rest data), masses and flags are re-read into the color-pass
+        layout now, into the same device buffers when the layout's shape is unchanged (so a graph
+        captured before stays valid); the soft-contact scalars are handed over now as well.  The
+        solver keeps its context and its pass shape."""
         if self._ctx is not None:
             torch.cuda.synchronize(self.model.device)
             _native.check(_native.hip_lib().vbd_rebuild(self._ctx), "vbd_rebuild")
-        self._material_key = None
+            self._push_scalars()
         self._world_off = None  # particle worlds may have changed
 
     def autotune(self, state_in: State, contacts: Contacts | None, dt: float) -> dict:
@@ -709,26 +786,43 @@ class SolverVBD:
         _native.check(_native.hip_lib().vbd_autotune(self._ctx, current_stream_handle(self.model.device),
                                                      _native.ptr(trial.particle_q), float(dt), shape, ms),
                       "vbd_autotune")
+        self._pass_shape = (int(shape[0]), int(shape[1]), int(shape[2]))
         return {"block": int(shape[0]), "waves": int(shape[1]), "aligned": int(shape[2]), "ms_256": float(ms[0]),
                 "ms_256_w7": float(ms[1]), "ms_128": float(ms[2]), "ms_256_aligned": float(ms[3]),
                 "ms_256_w7_aligned": float(ms[4]), "ms_192_aligned": float(ms[5]),
                 "ms_192_w7_aligned": float(ms[6])}
 
+    def pass_shape(self) -> tuple[int, int, int]:
+        """The color pass's current shape: (lanes per work-group, waves target, wave-aligned items)."""
+        out = (ctypes.c_int32 * 3)()
+        _native.check(_native.hip_lib().vbd_get_pass_shape(self._ctx, out), "vbd_get_pass_shape")
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def set_pass_shape(self, block: int, waves: int = 1, aligned: bool = False) -> None:
+        """Use ``block``-lane color-pass work-groups (256, 192 or 128) at an occupancy target of ``waves``
+        (1: the compiler's register budget, 7: seven waves per SIMD) with item ranges packed or each on a
+        wave boundary (``aligned``): what :meth:`autotune` chooses between.  Rebuilds the layout once."""
+        _native.check(_native.hip_lib().vbd_set_pass_shape(self._ctx, current_stream_handle(self.model.device),
+                                                           int(block), int(waves), 1 if aligned else 0),
+                      "vbd_set_pass_shape")
+        self._pass_shape = (int(block), int(waves), 1 if aligned else 0)
+
     def set_aligned_items(self, aligned: bool) -> None:
         """Item ranges of a color-pass batch packed, or each on a wave boundary (a wave evaluates one
         element kind only; fewer vertices per batch); what :meth:`autotune` also chooses between."""
-        _native.check(_native.hip_lib().vbd_set_aligned_items(self._ctx, current_stream_handle(self.model.device),
-                                                              1 if aligned else 0), "vbd_set_aligned_items")
+        block, waves, _ = self.pass_shape()
+        self.set_pass_shape(block, waves, aligned)
 
     def set_pass_waves(self, waves: int) -> None:
-        """Occupancy target of the 256-lane color pass: 1 (compiler's register budget) or 7 waves per
-        SIMD (what :meth:`autotune` also chooses between)."""
-        _native.check(_native.hip_lib().vbd_set_pass_waves(self._ctx, int(waves)), "vbd_set_pass_waves")
+        """Occupancy target of the color pass: 1 (compiler's register budget) or 7 waves per SIMD (what
+        :meth:`autotune` also chooses between)."""
+        block, _, aligned = self.pass_shape()
+        self.set_pass_shape(block, waves, bool(aligned))
 
     def set_block_shape(self, block: int) -> None:
         """Use 256-, 192- or 128-lane color-pass work-groups (what :meth:`autotune` chooses between)."""
-        _native.check(_native.hip_lib().vbd_set_block_shape(self._ctx, current_stream_handle(self.model.device),
-                                                            int(block)), "vbd_set_block_shape")
+        _, waves, aligned = self.pass_shape()
+        self.set_pass_shape(block, waves, bool(aligned))
 
     def layout_stats(self) -> dict:
         """Color-pass layout: packed batches, vertices on the generic path, packed element items, colors."""
@@ -737,14 +831,32 @@ class SolverVBD:
         return {"batches": out[0], "generic_vertices": out[1], "items": out[2], "colors": out[3]}
 
     def reset(self, state: State, world_mask=None, flags=None) -> None:
-        """The particle path keeps no state across steps (``CHANGELOG.md:23``): nothing to reset."""
-        return None
-
-    def update_contacts(self, contacts: Contacts) -> None:
-        return None
+        """``solver_vbd.py:1828-1935`` for a body-free model: the arguments are checked as the reference
+        checks them; the particle path keeps no history across steps, and, as in the reference, a
+        request to reset particle positions or velocities is ignored with a warning."""
+        if state is None:
+            raise ValueError("'state' argument is required.")
+        model = self.model
+        if world_mask is not None:
+            if not isinstance(world_mask, torch.Tensor) or world_mask.ndim != 1 or world_mask.dtype != torch.bool:
+                raise ValueError("world_mask must be a one-dimensional boolean tensor.")
+            n = int(world_mask.shape[0])
+            if n not in (model.world_count, model.world_count + 1):
+                raise ValueError(f"world_mask has length {n}, expected {model.world_count} or {model.world_count + 1}.")
+            if world_mask.device != torch.device(self.device):
+                raise ValueError(f"world_mask is on device {world_mask.device}, expected solver device {self.device}.")
+        flags_value = int(StateFlags.ALL if flags is None else flags)
+        if model.particle_count > 0 and (flags_value & (int(StateFlags.PARTICLE_Q) | int(StateFlags.PARTICLE_QD))):
+            warnings.warn(
+                "SolverVBD.reset() does not yet support particle resets; StateFlags.PARTICLE_Q and "
+                "StateFlags.PARTICLE_QD are ignored; particle and body-particle solver history is unchanged.",
+                stacklevel=2,
+            )
 
     @classmethod
-    def register_custom_attributes(cls, builder) -> None:
+    def register_custom_attributes(cls, builder, *, dahl_defaults_enabled: bool = False) -> None:
+        """``solver_vbd.py:1534-1560`` registers joint and cable attributes (``vbd:joint_is_hard``,
+        ``vbd:dahl_*``); this package has no joints or cables, so there is nothing to register."""
         return None
 
 

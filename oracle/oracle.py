@@ -289,7 +289,7 @@ class OracleModel:
 
 
 def params(iterations: int, friction_epsilon: float = 1e-2, avbd_beta: float = 0.0, k_start: float = -1.0,
-           use_libm_atan2: bool = False, fp64_ill_conditioned: bool = False, num_threads: int = 1,
+           use_libm_atan2: bool = False, fp64_ill_conditioned: bool = True, num_threads: int = 1,
            self_contact: "SelfContact | None" = None) -> _OracleParams:
     prm = _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
                         int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads),
@@ -579,3 +579,29 @@ def truncation_ts(kind, x, delta, parallel_eps=1e-5, gamma=0.85, precision="f64"
     getattr(lib(), f"vbd_oracle_truncation_ts_{precision}")(i32(kind), _p(x), _p(d), ct(parallel_eps), ct(gamma),
                                                             _p(t))
     return t
+
+
+# ---- SolverBase.integrate_particles (solvers/solver.py:20-56) -------------------------------
+def integrate_particles(x, v, f, inv_mass, flags, world, gravity, dt: float, v_max: float):
+    """Semi-implicit Euler of every particle in float32, one IEEE rounding per operation in the
+    reference kernel's order: ``v1 = v0 + (f * w + g * step(-w)) * dt``; ``|v1| = sqrt((x*x + y*y) + z*z)``;
+    above ``v_max`` ``v1 *= v_max / |v1|``; ``x1 = x0 + v1 * dt``.  Inactive particles keep x and v
+    (the kernel leaves their v_new unwritten).  ``world`` -1 selects gravity row 0.  Returns (x1, v1)."""
+    f32 = np.float32
+    x = np.asarray(x, f32).reshape(-1, 3)
+    v = np.asarray(v, f32).reshape(-1, 3)
+    f = np.asarray(f, f32).reshape(-1, 3)
+    w = np.asarray(inv_mass, f32)
+    g = np.asarray(gravity, f32).reshape(-1, 3)
+    wi = np.clip(np.asarray(world, np.int64), 0, g.shape[0] - 1)
+    dt, v_max = f32(dt), f32(v_max)
+    active = (np.asarray(flags) & 1) != 0
+    has_mass = (-w < f32(0.0)).astype(f32)
+    acc = f * w[:, None] + g[wi] * has_mass[:, None]
+    v1 = v + acc * dt
+    mag = np.sqrt((v1[:, 0] * v1[:, 0] + v1[:, 1] * v1[:, 1]) + v1[:, 2] * v1[:, 2])
+    over = mag > v_max
+    scale = np.where(over, v_max / np.where(over, mag, f32(1.0)), f32(1.0)).astype(f32)
+    v1 = np.where(over[:, None], v1 * scale[:, None], v1)
+    x1 = x + v1 * dt
+    return np.where(active[:, None], x1, x).astype(f32), np.where(active[:, None], v1, v).astype(f32)

@@ -480,36 +480,37 @@ static void body_contact_eval(V3 x, V3 x_prev, V3 body_pos, V3 body_vel, V3 n, R
     *h_out = madd(hess, fh);
 }
 
-/* ---- guarded 3x3 block solve: K:2701-2706 -------------------------------- */
+/* Guarded 3x3 solve (K:2701-2706): no update when |det H| <= 1e-8.  With fp64_ill, the Frobenius
+ * condition number ||H||_F ||H^-1||_F = ||H||_F ||adj H||_F / |det H| is evaluated in double from
+ * the stored entries (a single-precision estimate is unreliable exactly for ill-conditioned
+ * blocks); above 1e6 the block is solved in double, otherwise in the reference's single-precision
+ * adjugate order. */
 static int block_solve(M3 h, V3 f, int fp64_ill, V3* dx) {
     REAL det = mdet(h);
     if (!(RABS(det) > R(1e-8))) return 0;
-    M3 hinv = minverse(h);
     if (fp64_ill) {
-        /* cond_F(H) = |H|_F |H^-1|_F; above 1e6 the block is re-solved in double */
-        REAL n1 = R(0.0), n2 = R(0.0);
+        double a[3][3], b[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) a[i][j] = (double)h.m[i][j];
+        b[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+        b[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+        b[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+        b[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+        b[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+        b[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+        b[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+        b[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+        b[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+        double det_d = a[0][0] * b[0][0] + a[0][1] * b[1][0] + a[0][2] * b[2][0];
+        double n1 = 0.0, n2 = 0.0;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) {
-                n1 = n1 + h.m[i][j] * h.m[i][j];
-                n2 = n2 + hinv.m[i][j] * hinv.m[i][j];
+                n1 += a[i][j] * a[i][j];
+                n2 += b[i][j] * b[i][j];
             }
-        REAL cond = RSQRT(n1) * RSQRT(n2);
-        if (!(cond <= R(1.0e6))) {
-            double a[3][3], b[3][3], fd[3] = {(double)f.x, (double)f.y, (double)f.z};
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) a[i][j] = (double)h.m[i][j];
-            double det_d = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) +
-                           a[0][1] * (a[1][2] * a[2][0] - a[1][0] * a[2][2]) +
-                           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-            b[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
-            b[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
-            b[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
-            b[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
-            b[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
-            b[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
-            b[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
-            b[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
-            b[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+        /* cond > 1e6  <=>  sqrt(n1 n2) > 1e6 |det|  (also taken when det_d is 0 or not finite) */
+        if (!(sqrt(n1 * n2) <= 1.0e6 * fabs(det_d))) {
+            double fd[3] = {(double)f.x, (double)f.y, (double)f.z};
             double inv = 1.0 / det_d;
             double r[3];
             for (int i = 0; i < 3; ++i)
@@ -518,7 +519,7 @@ static int block_solve(M3 h, V3 f, int fp64_ill, V3* dx) {
             return 1;
         }
     }
-    *dx = mvec(hinv, f);
+    *dx = mvec(minverse(h), f);
     return 1;
 }
 

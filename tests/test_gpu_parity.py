@@ -95,7 +95,8 @@ def test_serial_contact_mode_bitexact(cuda_device):
 
 
 def test_notify_model_changed_rebuilds_layout(cuda_device):
-    """Editing element materials after construction takes effect after notify_model_changed()."""
+    """Editing element materials after construction takes effect after notify_model_changed(), also when
+    the edit switches elements to zero stiffness (the layout then changes shape and is re-allocated)."""
     import torch
 
     from newton_amd.solvers import SolverVBD
@@ -103,9 +104,12 @@ def test_notify_model_changed_rebuilds_layout(cuda_device):
     model, state, info = _scene("strip", cuda_device)
     q0, qd0 = initial(state)
     solver = SolverVBD(model, iterations=info["iterations"])
+    stats0 = solver.layout_stats()
     model.tri_materials[:, 0] *= 3.0
+    model.tri_materials[:4, :2] = 0.0  # these triangles now contribute nothing and leave the layout
     model.edge_bending_properties[:, 0] *= 0.5
     solver.notify_model_changed()
+    assert solver.layout_stats()["items"] < stats0["items"]
     s1 = model.state()
     solver.step(state, s1, None, None, info["dt"])
     torch.cuda.synchronize()
@@ -184,12 +188,20 @@ def test_contact_overflow_step_bitexact(cuda_device):
 
 
 def test_fp64_ill_conditioned_mode_bitexact(cuda_device):
-    model, state, info = _scene("strip", cuda_device)
-    q0, qd0 = initial(state)
-    q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4, solver_kwargs={"fp64_ill_conditioned": True})
-    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 4, fp64_ill_conditioned=True)
-    np.testing.assert_array_equal(q_gpu, q_ref)
-    np.testing.assert_array_equal(qd_gpu, qd_ref)
+    """Nearly massless beads on stiff springs (block condition numbers >> 1e6): the FP64 block solve is
+    taken, changes the result, and the GPU equals the oracle bit for bit with it on and off."""
+    from newton_amd import scenes
+
+    out = {}
+    for fp64 in (True, False):
+        model, state, info = scenes.bead_rope_light(cuda_device)
+        q0, qd0 = initial(state)
+        q_gpu, qd_gpu, _ = gpu_run(model, state, info, 10, solver_kwargs={"fp64_ill_conditioned": fp64})
+        q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 10, fp64_ill_conditioned=fp64)
+        np.testing.assert_array_equal(q_gpu, q_ref)
+        np.testing.assert_array_equal(qd_gpu, qd_ref)
+        out[fp64] = q_gpu
+    assert np.abs(out[True] - out[False]).max() > 1e-4
 
 
 def test_penalty_ramp_bitexact(cuda_device):

@@ -131,3 +131,62 @@ def test_inactive_and_zero_mass_particles_do_not_move():
                                0.05, len(om.pairs) // 2)
     np.testing.assert_array_equal(q[[0, 5]], q0[[0, 5]])
     np.testing.assert_array_equal(qd[[0, 5]], 0.0)
+
+
+def test_fp64_block_solve_changes_and_improves_ill_conditioned_steps():
+    """Nearly massless beads on stiff springs: blocks with condition numbers >> 1e6.  The FP64 solve
+    takes another path than the all-FP32 one and lands far closer to the FP64 oracle of the same
+    iteration; on a well-conditioned cloth both modes give identical bits."""
+    from newton_amd import scenes
+
+    model, state, info = scenes.bead_rope_light("cpu")
+    om = oracle.OracleModel(model)
+    q0 = state.particle_q.numpy()
+    qd0 = state.particle_qd.numpy()
+    cmax = len(om.pairs) // 2
+    runs = {}
+    for fp64 in (False, True):
+        runs[fp64], _, _ = oracle.simulate(om, oracle.params(info["iterations"], fp64_ill_conditioned=fp64), q0, qd0,
+                                           10, info["dt"], 0.01, cmax)
+    ref, _, _ = oracle.simulate(om, oracle.params(info["iterations"]), q0.astype(np.float64),
+                                qd0.astype(np.float64), 10, info["dt"], 0.01, cmax, precision="f64")
+    err_fp32 = np.abs(runs[False] - ref).max()
+    err_fp64 = np.abs(runs[True] - ref).max()
+    assert np.abs(runs[True] - runs[False]).max() > 1e-4
+    assert err_fp64 < 1e-5 and err_fp32 > 100 * err_fp64, (err_fp32, err_fp64)
+    model, state, info = scenes.cloth_hanging("cpu")
+    om = oracle.OracleModel(model)
+    q0 = state.particle_q.numpy()
+    outs = [oracle.simulate(om, oracle.params(info["iterations"], fp64_ill_conditioned=f), q0,
+                            state.particle_qd.numpy(), 3, info["dt"], 0.01, len(om.pairs) // 2)[0] for f in (False, True)]
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_integrate_particles_closed_forms():
+    """SolverBase.integrate_particles (solver.py:20-56): free fall v + g dt, x + v' dt; forces scale by 1/m;
+    massless particles ignore gravity; the speed limit caps |v'| at v_max; inactive particles stay put."""
+    rng = np.random.default_rng(3)
+    n = 64
+    x = rng.normal(size=(n, 3)).astype(np.float32)
+    v = rng.normal(size=(n, 3)).astype(np.float32)
+    f = np.zeros((n, 3), np.float32)
+    w = np.full(n, 2.0, np.float32)
+    flags = np.ones(n, np.int32)
+    world = rng.integers(-1, 3, n)
+    gravity = np.array([[0, -9.81, 0], [0, 0, -3.0], [1.0, 0, 0]], np.float32)
+    dt = 1.0 / 60.0
+    x1, v1 = oracle.integrate_particles(x, v, f, w, flags, world, gravity, dt, 1e5)
+    g = gravity[np.maximum(world, 0)].astype(np.float64)
+    np.testing.assert_allclose(v1, v + g * dt, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(x1, x + (v + g * dt) * dt, rtol=1e-6, atol=1e-6)
+    f = rng.normal(size=(n, 3)).astype(np.float32)
+    w[:8] = 0.0
+    flags[8:12] = 0
+    x1, v1 = oracle.integrate_particles(x, v, f, w, flags, world, gravity, dt, 1e5)
+    np.testing.assert_array_equal(v1[:8], v[:8])  # no mass: neither force nor gravity
+    np.testing.assert_array_equal(x1[8:12], x[8:12])
+    np.testing.assert_allclose(v1[12:], v[12:] + (2.0 * f[12:] + g[12:]) * dt, rtol=1e-5, atol=1e-6)
+    x1, v1 = oracle.integrate_particles(x, 100.0 * v, f, w, np.ones(n, np.int32), world, gravity, dt, 5.0)
+    speed = np.linalg.norm(v1.astype(np.float64), axis=1)
+    np.testing.assert_allclose(speed, 5.0, rtol=1e-6)
+    np.testing.assert_allclose(x1, x + v1.astype(np.float64) * dt, rtol=1e-6, atol=1e-6)
