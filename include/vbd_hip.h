@@ -333,6 +333,19 @@ int32_t vbd_collide_destroy(VbdCollide* col);
 int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* particle_q, float margin,
                              const VbdContactsOut* out);
 
+/* Largest pair_count vbd_collide_generate_noalloc accepts. */
+#define VBD_NOALLOC_MAX_PAIRS (16 * 1024 * 1024)
+
+/* vbd_collide_generate that allocates nothing: no counterpart in the reference.  It is for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation), and needs no VbdCollide: the descriptor is read per call.  Same records, same
+ * order and same soft_contact_count as vbd_collide_generate.  Its device scratch (one int per
+ * 1,024 pairs, 64 KiB) is a static array each GPU holds until the process ends; calls that overlap
+ * on one GPU share it and race.  pair_count above VBD_NOALLOC_MAX_PAIRS is rejected with
+ * VBD_ERR_ARG.  The first call on a device looks the scratch address up (not inside a capture). */
+int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* desc, void* stream, const float* particle_q, float margin,
+                                     const VbdContactsOut* out);
+
 #ifdef __cplusplus
 }
 #endif

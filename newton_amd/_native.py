@@ -192,6 +192,8 @@ HIP_EXPORTS = {
     "vbd_collide_create": (c_int32, [ctypes.POINTER(VbdCollideDesc), ctypes.POINTER(c_void_p)]),
     "vbd_collide_destroy": (c_int32, [c_void_p]),
     "vbd_collide_generate": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),
+    "vbd_collide_generate_noalloc": (
+        c_int32, [ctypes.POINTER(VbdCollideDesc), c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),
 }
 
 HOST_EXPORTS = {

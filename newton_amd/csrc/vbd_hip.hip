@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1592,6 +1593,95 @@ __global__ void k_collide_write(ColParams C, const float* __restrict__ q, float 
     st3(out.soft_contact_normal, k, g.normal);
 }
 
+// ---- soft-contact generation from per-tile hit counts ------------------------
+// The same records in the same order as k_collide_flag / scan / k_collide_write (stable compaction
+// in pair order), with no per-pair flag or index array: a tile of kColTile pairs is one 256-lane
+// work-group, four rounds of 256 consecutive pairs; a round's hits are ranked with a wave ballot.
+constexpr int kColTile = 1024;
+constexpr int kColRounds = kColTile / 256;
+
+__device__ __forceinline__ int wave_rank(bool hit, int lane, int* total) {
+    unsigned long long m = __ballot(hit ? 1 : 0);
+    *total = __popcll(m);
+    return __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(256) void k_collide_tile_count(ColParams C, const float* __restrict__ q, float margin,
+                                                            int* __restrict__ tile_count) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int count = 0;
+    for (int r = 0; r < kColRounds; ++r) {
+        int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
+        bool hit = tid < C.P && soft_contact_geometry(C, q, margin, tid).hit;
+        int t;
+        wave_rank(hit, lane, &t);
+        count += t;
+    }
+    if (lane == 0) wsum[wave] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Exclusive scan of the tile counts in place (one 1024-lane work-group, each lane a run of
+// consecutive tiles); the total is the contact count (may exceed soft_contact_max, as the
+// reference's atomic counter does).
+__global__ __launch_bounds__(1024) void k_collide_tile_scan(int tiles, int* __restrict__ tile_count,
+                                                            int* __restrict__ count_out) {
+    __shared__ int part[1024];
+    const int per = (tiles + 1023) / 1024;
+    const int lo = min(threadIdx.x * per, tiles), hi = min(lo + per, tiles);
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += tile_count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+    for (int i = lo; i < hi; ++i) {
+        int c = tile_count[i];
+        tile_count[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) *count_out = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_collide_tile_write(ColParams C, const float* __restrict__ q, float margin,
+                                                            const int* __restrict__ tile_off, VbdContactsOut out) {
+    __shared__ int wsum[kColRounds][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int run = tile_off[blockIdx.x];
+    for (int r = 0; r < kColRounds; ++r) {
+        int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
+        ContactGeo g;
+        g.hit = false;
+        if (tid < C.P) g = soft_contact_geometry(C, q, margin, tid);
+        int t;
+        int rank = wave_rank(g.hit, lane, &t);
+        if (lane == 0) wsum[r][wave] = t;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += wsum[r][w];
+        int k = run + before + rank;
+        run += wsum[r][0] + wsum[r][1] + wsum[r][2] + wsum[r][3];
+        if (g.hit && k < out.soft_contact_max) {
+            out.soft_contact_shape[k] = g.shape;
+            st3(out.soft_contact_body_pos, k, g.body_pos);
+            st3(out.soft_contact_body_vel, k, g.body_vel);
+            if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
+            out.soft_contact_indices[3 * k] = g.particle;
+            out.soft_contact_indices[3 * k + 1] = -1;
+            out.soft_contact_indices[3 * k + 2] = -1;
+            if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, v3(1.0f, 0.0f, 0.0f));
+            st3(out.soft_contact_normal, k, g.normal);
+        }
+    }
+}
+
 inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
 inline int norm_parts(long n) { return n > 0 ? blocks_for(n, kNormBlock * kNormItems) : 1; }
 
@@ -2121,6 +2211,41 @@ int32_t self_truncate(VbdContext* c, hipStream_t s, float* q) {
     hipLaunchKernelGGL(k_sc_trunc_apply, dim3(blocks_for(S.N, 256)), dim3(256), 0, s, S, c->kp.disp, q);
     VBD_LAUNCH_CHECK("k_sc_trunc_apply");
     return VBD_OK;
+}
+
+}  // namespace
+
+// Device scratch of vbd_collide_generate_noalloc: one int per tile of kColTile pairs (hit counts,
+// then their exclusive scan).  Each GPU holds its own copy for the life of the process; every
+// call on a GPU reuses it, so calls that overlap on one GPU race.  Nothing in it needs clearing:
+// k_collide_tile_count writes every slot the call reads.
+static_assert(VBD_NOALLOC_MAX_PAIRS % kColTile == 0, "whole tiles");
+constexpr int kNoallocTiles = VBD_NOALLOC_MAX_PAIRS / kColTile;
+static __device__ __align__(256) int32_t scratch[kNoallocTiles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device (the array's
+// name in host code is only a host-side handle, never a device address).
+std::string noalloc_scratch(int32_t** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return std::string("hipGetDevice: ") + hipGetErrorString(e);
+    if (dev < 0 || dev >= kMaxDevices) return "device index " + std::to_string(dev) + " out of range";
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        if (!p) return "hipGetSymbolAddress(scratch) returned null";
+        g_scratch_dev[dev] = static_cast<int32_t*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return std::string();
 }
 
 }  // namespace
@@ -3106,6 +3231,53 @@ int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* q, floa
     hipLaunchKernelGGL(k_collide_write, dim3(blocks_for(P, 256)), dim3(256), 0, s, col->cp, q, margin, col->flag,
                        col->index, *out);
     VBD_LAUNCH_CHECK("k_collide_write");
+    return VBD_OK;
+}
+
+int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, const float* q, float margin,
+                                     const VbdContactsOut* out) {
+    if (!d || !q || !out || !out->soft_contact_count)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: null argument");
+    if (d->pair_count < 0) return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: negative pair_count");
+    if (d->pair_count > VBD_NOALLOC_MAX_PAIRS)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: pair_count " + std::to_string(d->pair_count) +
+                                     " exceeds the maximum of " + std::to_string(VBD_NOALLOC_MAX_PAIRS) +
+                                     " (use vbd_collide_create / vbd_collide_generate)");
+    if (d->pair_count > 0 && (!d->pairs || !d->particle_radius || !d->particle_flags || !d->particle_world ||
+                              !d->shape_type || !d->shape_flags || !d->shape_world || !d->shape_transform ||
+                              !d->shape_scale || !d->shape_margin))
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: missing arrays");
+    if (d->pair_count > 0 && (!out->soft_contact_indices || !out->soft_contact_shape || !out->soft_contact_body_pos ||
+                              !out->soft_contact_body_vel || !out->soft_contact_normal))
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: missing output arrays");
+    hipStream_t s = (hipStream_t)stream;
+    if (d->pair_count == 0 || out->soft_contact_max <= 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out->soft_contact_count, 0, 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        return VBD_OK;
+    }
+    int32_t* tiles_dev = nullptr;
+    std::string err = noalloc_scratch(&tiles_dev);
+    if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_collide_generate_noalloc: " + err);
+    ColParams cp;
+    cp.P = d->pair_count;
+    cp.pairs = d->pairs;
+    cp.radius = d->particle_radius;
+    cp.pflags = d->particle_flags;
+    cp.pworld = d->particle_world;
+    cp.stype = d->shape_type;
+    cp.sflags = d->shape_flags;
+    cp.sworld = d->shape_world;
+    cp.sxf = d->shape_transform;
+    cp.sscale = d->shape_scale;
+    cp.smargin = d->shape_margin;
+    const int tiles = blocks_for(cp.P, kColTile);
+    hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev);
+    VBD_LAUNCH_CHECK("k_collide_tile_count");
+    hipLaunchKernelGGL(k_collide_tile_scan, dim3(1), dim3(1024), 0, s, tiles, tiles_dev, out->soft_contact_count);
+    VBD_LAUNCH_CHECK("k_collide_tile_scan");
+    hipLaunchKernelGGL(k_collide_tile_write, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev, *out);
+    VBD_LAUNCH_CHECK("k_collide_tile_write");
     return VBD_OK;
 }
 

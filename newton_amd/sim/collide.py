@@ -86,6 +86,7 @@ class CollisionPipeline:
             if np.any((types != 1) & (types != 3)):
                 raise NotImplementedError("particle contact supports plane and sphere shapes only")
         self._handle = None
+        self.desc = None
         if self.device.type == "cuda" and self.soft_rigid_contact_pair_count > 0:
             lib = _native.hip_lib()
             d = _native.VbdCollideDesc()
@@ -105,6 +106,7 @@ class CollisionPipeline:
             h = ctypes.c_void_p()
             _native.check(lib.vbd_collide_create(ctypes.byref(d), ctypes.byref(h)), "vbd_collide_create")
             self._handle = h
+            self.desc = d  # the pair / shape arrays as the C ABI reads them
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -129,20 +131,26 @@ class CollisionPipeline:
         if self._handle is None:
             contacts.soft_contact_count.zero_()
             return
-        out = _native.VbdContactsOut()
-        out.soft_contact_count = _native.ptr(contacts.soft_contact_count)
-        out.soft_contact_max = contacts.soft_contact_max
-        out.soft_contact_particle = _native.ptr(contacts.soft_contact_particle)
-        out.soft_contact_indices = _native.ptr(contacts.soft_contact_indices)
-        out.soft_contact_barycentric = _native.ptr(contacts.soft_contact_barycentric)
-        out.soft_contact_shape = _native.ptr(contacts.soft_contact_shape)
-        out.soft_contact_body_pos = _native.ptr(contacts.soft_contact_body_pos)
-        out.soft_contact_body_vel = _native.ptr(contacts.soft_contact_body_vel)
-        out.soft_contact_normal = _native.ptr(contacts.soft_contact_normal)
         rc = _native.hip_lib().vbd_collide_generate(
-            self._handle, current_stream_handle(self.device), _native.ptr(state.particle_q), margin, ctypes.byref(out)
+            self._handle, current_stream_handle(self.device), _native.ptr(state.particle_q), margin,
+            ctypes.byref(contacts_out(contacts))
         )
         _native.check(rc, "vbd_collide_generate")
 
 
-__all__ = ["CollisionPipeline", "world_compatible_pairs"]
+def contacts_out(contacts: Contacts) -> _native.VbdContactsOut:
+    """The writable soft-contact arrays of ``contacts`` as the C ABI's ``VbdContactsOut``."""
+    out = _native.VbdContactsOut()
+    out.soft_contact_count = _native.ptr(contacts.soft_contact_count)
+    out.soft_contact_max = contacts.soft_contact_max
+    out.soft_contact_particle = _native.ptr(contacts.soft_contact_particle)
+    out.soft_contact_indices = _native.ptr(contacts.soft_contact_indices)
+    out.soft_contact_barycentric = _native.ptr(contacts.soft_contact_barycentric)
+    out.soft_contact_shape = _native.ptr(contacts.soft_contact_shape)
+    out.soft_contact_body_pos = _native.ptr(contacts.soft_contact_body_pos)
+    out.soft_contact_body_vel = _native.ptr(contacts.soft_contact_body_vel)
+    out.soft_contact_normal = _native.ptr(contacts.soft_contact_normal)
+    return out
+
+
+__all__ = ["CollisionPipeline", "contacts_out", "world_compatible_pairs"]
