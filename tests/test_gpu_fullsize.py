@@ -67,3 +67,116 @@ def test_hanging_self_contact_example_matches_oracle_then_stays_valid(cuda_devic
     lo = np.array([-64 * 0.11, -4.0, -1.8])
     hi = np.array([0.1, 7.0, 4.0])
     assert ((q >= lo) & (q <= hi)).all(), "particles are within a reasonable volume"
+
+
+def test_wind_cloth_128_frame_bitexact_and_residual_stop(cuda_device):
+    """The learned-warm-start workload's cloth at its stated size (128 x 128, top edge pinned, Fourier
+    wind re-evaluated at every substep's positions): one frame of 10 substeps x 10 iterations bit-exact
+    against the oracle, then one substep driven to relative residual 0.6 (the oracle needs ~110
+    iterations this early in the run), which must stop at the oracle's first iteration within that
+    tolerance with the oracle's result for that many iterations."""
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.solvers import SolverVBD
+
+    model, state, info = scenes.cloth_wind(cuda_device)
+    assert model.particle_count == 128 * 128
+    wind, dt = info["wind"], info["dt"]
+    solver = SolverVBD(model, iterations=info["iterations"])
+    om = oracle.OracleModel(model)
+    prm = oracle.params(info["iterations"], num_threads=16)
+    q_ref, qd_ref = initial(state)
+    s0, s1 = state, model.state()
+    for _ in range(info["substeps"]):
+        f = wind.force(s0.particle_q, model.particle_mass)
+        s0.particle_f.copy_(f)
+        _, q_ref, qd_ref = oracle.step(om, prm, q_ref, qd_ref, f.cpu().numpy(), None, dt)
+        solver.step(s0, s1, None, None, dt)
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(s0.particle_q.cpu().numpy(), q_ref)
+    np.testing.assert_array_equal(s0.particle_qd.cpu().numpy(), qd_ref)
+
+    f = wind.force(s0.particle_q, model.particle_mass)
+    s0.particle_f.copy_(f)
+    f_host = f.cpu().numpy()
+    cap = 200
+    _, _, _, hist = oracle.step(om, oracle.params(cap, num_threads=16), q_ref, qd_ref, f_host, None, dt,
+                                residual_history=True)
+    hit = np.nonzero(hist <= 0.6 * hist[0])[0]
+    assert hit.size, "oracle never reached the tolerance within the cap"
+    k = int(hit[0])
+    res = solver.step_to_residual(s0, s1, None, None, dt, rel_tol=0.6, max_iterations=cap)
+    torch.cuda.synchronize()
+    assert res["iterations"] == k, (res, hist[: k + 2])
+    _, q_k, qd_k = oracle.step(om, oracle.params(k, num_threads=16), q_ref, qd_ref, f_host, None, dt)
+    np.testing.assert_array_equal(s1.particle_q.cpu().numpy(), q_k)
+    np.testing.assert_array_equal(s1.particle_qd.cpu().numpy(), qd_k)
+
+
+def _tet_volumes(q, tets):
+    a, b, c, d = (q[tets[:, i]] for i in range(4))
+    return np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) / 6.0
+
+
+def test_soft_cubes_512_substep_bitexact_then_drop(cuda_device):
+    """The batched tet workload at its stated size (512 cubes of 6..12 cells, ~587 k particles,
+    2.2 M tets, 8 iterations): one substep bit-exact against the 16-thread oracle; then 3 frames of
+    free fall, in which every tet of a cube clear of the ground keeps its volume within 1 %
+    (test_softbody_simulation.py:92-121 checks the same on one grid); then 87 more frames, in which
+    every cube lands on the ground plane and stays finite and above it, with less kinetic
+    energy than the potential energy it released."""
+    import torch
+
+    from newton_amd import scenes
+
+    model, state, info = scenes.soft_cubes_batched(cuda_device)
+    assert model.tet_count > 2_000_000 and model.particle_count > 500_000
+    q0, qd0 = initial(state)
+    q1, qd1, _ = gpu_run(model, state, info, 1)
+    q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 1, threads=16)
+    np.testing.assert_array_equal(q1, q_ref)
+    np.testing.assert_array_equal(qd1, qd_ref)
+
+    from newton_amd import CollisionPipeline
+    from newton_amd.solvers import SolverVBD
+
+    solver = SolverVBD(model, iterations=info["iterations"])
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    s0, s1 = state, model.state()
+    s0.particle_q.copy_(torch.from_numpy(q0))
+    s0.particle_qd.copy_(torch.from_numpy(qd0))
+    tets = model.tet_indices.cpu().numpy()
+    v0 = _tet_volumes(q0, tets)
+    ps, ts = info["world_particle_start"], info["world_tet_start"]
+    clear = np.array([q0[ps[w]:ps[w + 1], 1].min() > 0.2 for w in range(len(ps) - 1)])
+    assert clear.sum() > 100
+
+    def frames(k):
+        nonlocal s0, s1
+        for _ in range(k * info["substeps"]):
+            s0.clear_forces()
+            pipe.collide(s0, contacts)
+            solver.step(s0, s1, None, contacts, info["dt"])
+            s0, s1 = s1, s0
+        torch.cuda.synchronize()
+        return s0.particle_q.cpu().numpy(), s0.particle_qd.cpu().numpy()
+
+    q, qd = frames(3)
+    ratio = _tet_volumes(q, tets) / v0
+    tet_clear = np.repeat(clear, np.diff(ts))
+    assert ratio[tet_clear].min() > 0.99 and ratio[tet_clear].max() < 1.01
+    q, qd = frames(87)
+    assert np.isfinite(q).all() and np.isfinite(qd).all()
+    assert q[:, 1].min() > -0.005, "the ground holds every cube (particle radius 0.01 m)"
+    # the implicit step dissipates: the kinetic energy cannot exceed the potential energy released
+    mass = model.particle_mass.cpu().numpy().astype(np.float64)
+    ke = 0.5 * float((mass * (qd.astype(np.float64) ** 2).sum(axis=1)).sum())
+    released = 9.81 * float((mass * (q0[:, 1].astype(np.float64) - q[:, 1])).sum())
+    assert 0.0 < ke < released, (ke, released)
+    assert np.linalg.norm(qd, axis=1).max() < 25.0, "no particle faster than ~5x the impact speed"
+    com0 = np.add.reduceat(q0[:, 1], ps[:-1]) / np.diff(ps)
+    com = np.add.reduceat(q[:, 1], ps[:-1]) / np.diff(ps)
+    assert (com < com0).all(), "every cube fell"

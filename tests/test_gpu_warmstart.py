@@ -124,7 +124,10 @@ def test_adam_matches_oracle(cuda_device):
         np.testing.assert_allclose(mlp.m.cpu().numpy(), m, rtol=1e-6, atol=1e-12)
 
 
-def test_features_match_oracle(cuda_device):
+@pytest.mark.parametrize("verts", [12, 128])
+def test_features_match_oracle(cuda_device, verts):
+    """Features of the wind cloth, also at its stated size (128 x 128, 16,384 particles); then the
+    MLP's forward pass over all of that cloth's rows against the oracle's."""
     import torch
 
     from newton_amd import _native
@@ -132,7 +135,7 @@ def test_features_match_oracle(cuda_device):
     from newton_amd.warmstart import vertex_neighbours
     from newton_amd import scenes
 
-    model, state, info = scenes.cloth_wind(cuda_device, verts_per_side=12)
+    model, state, info = scenes.cloth_wind(cuda_device, verts_per_side=verts)
     off, nbr = vertex_neighbours(model)
     n = model.particle_count
     rng = np.random.default_rng(11)
@@ -157,6 +160,16 @@ def test_features_match_oracle(cuda_device):
     ulp = np.abs(ref) * 2.0 ** -7 + 1e-30
     bad = np.abs(got - ref) > ulp
     assert not bad.any(), (np.argwhere(bad)[:5], got[bad][:5], ref[bad][:5])
+    if verts == 128:
+        from newton_amd.warmstart import WarmStartMLP
+
+        mlp = WarmStartMLP(cuda_device, seed=5)
+        out = mlp.forward(feat)
+        torch.cuda.synchronize()
+        fwd = out.cpu().numpy().astype(np.float64)
+        fref, _, _ = mo.forward(got, mlp.params.cpu().numpy().astype(np.float64), bf16=True)
+        assert np.all(np.abs(fwd - fref) <= 1e-2 * (1.0 + np.abs(fref)))
+        assert np.linalg.norm(fwd - fref) <= 5e-3 * np.linalg.norm(fref)
 
 
 def test_training_reduces_loss(cuda_device):
