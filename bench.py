@@ -1,25 +1,27 @@
-"""Benchmark of the MI355X VBD solver: vertex-steps/s, HBM roofline, residual-driven comparison.
+"""Benchmark of the MI355X VBD solver: vertex-steps/s at a fixed force residual, HBM roofline, CPU baseline.
 
-Prints ONE JSON line.  Its ``value`` is the throughput of the reference's
-configured loop on the 256x256 cloth-on-sphere workload (65,536 vertices):
-one step = one frame = 10 substeps, each clear_forces ->
-CollisionPipeline.collide -> SolverVBD.step (10 iterations), captured once in a
-HIP graph and replayed; vertex-steps/s = vertices x substeps x frames / GPU time.
-The same line carries
-* ``roofline``: the color-pass kernel's algorithmic bytes / its live launch time
-  (HIP events on its stream) against 8 TB/s, plus HBM traffic from two child
-  ``rocprofv3 --pmc`` runs (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy;
-* ``batched`` / ``batched_tets``: the 4096-world cloth (~7.3M vertices, 5
-  iterations) and the 512 tetrahedral cubes (~0.59M vertices, 8 iterations),
-  each one graph-captured frame per step with its own color-pass roofline;
-* ``springs``: the headline cloth with a spring on every mesh edge and hinge;
-* ``self_contact``: two stacked 128x128 sheets draping a sphere with cloth self-contact;
-* ``residual``: the 128x128 wind cloth driven to a relative force residual,
-  plain VBD (inertial guess), the previous substep's correction carried over
-  ("reuse", no network) and the learned warm start (trained on the GPU in the
-  same run from plain-VBD data): iterations and vertex-steps/s of each;
-* ``cpu_baseline``: the FP32 CPU oracle (OpenMP) on a bounded sample of the
-  headline workload.
+Prints ONE JSON line.  Its ``value`` is the north-star metric: vertex-steps/s of the batched +
+learned-warm-start solver at a fixed relative force residual, on the 128x128 wind cloth (top edge
+pinned, 8-mode Fourier wind; 16,384 vertices) run as 64 worlds with their own winds (1,048,576
+vertices).  One step = one substep (dt 1/600 s) of every world, each world iterated until its
+residual is within ``rel_tol`` (0.3) of its value at the inertial guess; vertex-steps/s = vertices x
+substeps / wall time (the drivers read the residual on the host, so time is wall-clock with the
+stream synchronised).  The warm-start network (BF16 MFMA MLP) is trained on the GPU in the same run
+from plain-VBD solves.  The same line carries
+* ``comparison``: from identical states per substep, plain VBD solving the worlds one at a time (the
+  plain GPU VBD of one cloth), plain VBD on the batch, and the batch with the learned warm start;
+* ``tolerance_sweep``: the same comparison at tighter tolerances (0.1, 0.03; iteration cap 400);
+* ``fp32_floor``: where plain VBD's relative residual stops falling on one world (3,000 iterations);
+* ``roofline``: the color-pass kernel on the headline model, in-frame (iterations captured in a HIP
+  graph, HIP events) against 8 TB/s, plus HBM traffic from two rocprofv3 --pmc child runs
+  (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy;
+* ``single_world``: the single 128x128 cloth with each warm-start protocol, and the training record;
+* ``fixed_iterations``: the reference's configured loop (10 iterations) on the 256x256
+  cloth-on-sphere workload, timed while falling and after landing on the sphere;
+* ``batched`` / ``batched_tets`` / ``springs`` / ``self_contact``: the 4096-world cloth, the 512 tet
+  cubes, springs on every edge and hinge, and two stacked sheets with self-contact;
+* ``cpu_baseline``: the FP32 CPU oracle (OpenMP, and single-threaded) on a bounded sample of the
+  headline workload, and the FP64 residual of the FP32 floor iterate.
 """
 
 from __future__ import annotations
@@ -43,47 +45,58 @@ MLP_FLOPS_PER_ROW = 2 * (48 * 128 + 2 * 128 * 128 + 128 * 3)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed frames of the headline workload")
-    ap.add_argument("--warmup", type=int, default=3, help="untimed frames")
-    ap.add_argument("--verts", type=int, default=256, help="cloth vertices per side (headline workload)")
-    ap.add_argument("--iterations", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time budget (0 disables)")
-    ap.add_argument("--roofline-reps", type=int, default=20)
-    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
-                    help="skip the two rocprofv3 --pmc child runs that measure HBM traffic per color pass")
-    ap.add_argument("--worlds", type=int, default=4096, help="worlds of the batched workload (0 disables the leg)")
-    ap.add_argument("--batched-steps", type=int, default=4)
-    ap.add_argument("--residual-frames", type=int, default=3, help="evaluation frames of the residual leg (0 disables)")
-    ap.add_argument("--rel-tol", type=float, default=0.3, help="relative force residual of the residual leg")
-    ap.add_argument("--max-iterations", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=30,
+                    help="timed substeps of the headline (every world of the batch driven to --rel-tol)")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed substeps of the headline batch before timing")
+    ap.add_argument("--rel-tol", type=float, default=0.3, help="relative force residual of the headline")
+    ap.add_argument("--tol-sweep", type=lambda t: [float(x) for x in t.split(",") if x], default=[0.1, 0.03],
+                    help="further relative residuals the three solvers are compared at (comma separated)")
+    ap.add_argument("--sweep-substeps", type=int, default=10, help="substeps per tolerance of the sweep")
+    ap.add_argument("--max-iterations", type=int, default=400, help="iteration cap of every residual-driven solve")
+    ap.add_argument("--floor-iterations", type=int, default=3000,
+                    help="plain-VBD iterations of the FP32-floor measurement (one substep of one world)")
+    ap.add_argument("--residual-worlds", type=int, default=64, help="worlds of the headline batch")
+    ap.add_argument("--single-world-stride", type=int, default=4,
+                    help="headline: every k-th world is also solved alone (the plain single-world baseline)")
+    ap.add_argument("--trial-iterations", type=int, default=4,
+                    help="iterations each start runs before the residual judges it (the trial-judged variants)")
+    ap.add_argument("--smooth-sweeps", type=int, default=64,
+                    help="1-ring low-pass sweeps of the deployed warm-start candidates")
+    ap.add_argument("--blend", type=float, default=1.0, help="weight of the network's correction (deployed)")
+    ap.add_argument("--residual-frames", type=int, default=3, help="evaluation frames of the single-world leg")
     ap.add_argument("--train-frames", type=int, default=40)
     ap.add_argument("--train-iterations", type=int, default=400, help="plain-VBD iterations per training substep")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-smooth", type=float, default=100.0,
                     help="weight of the 1-ring smoothness term of the warm-start training loss")
-    ap.add_argument("--residual-worlds", type=int, default=64,
-                    help="worlds of the batched wind cloth in the residual leg (0: skip)")
-    ap.add_argument("--batched-residual-warm-frames", type=int, default=2)
-    ap.add_argument("--single-world-stride", type=int, default=4,
-                    help="batched residual leg: every k-th world is also solved alone (the plain baseline)")
-    ap.add_argument("--trial-iterations", type=int, default=4,
-                    help="iterations each start runs before the residual judges it (learned_trial variant)")
     ap.add_argument("--train-protocol", choices=("deployed", "converged"), default="deployed",
                     help="trajectory of the training data: the deployed tolerance solve, or converged solves")
+    ap.add_argument("--verts", type=int, default=256, help="cloth vertices per side of the fixed-iteration leg")
+    ap.add_argument("--iterations", type=int, default=10, help="iterations of the fixed-iteration leg")
+    ap.add_argument("--fixed-steps", type=int, default=20, help="timed frames per window of the fixed-iteration leg")
+    ap.add_argument("--fixed-warmup", type=int, default=5, help="first timed frame of its falling window")
+    ap.add_argument("--contact-frame", type=int, default=60,
+                    help="first timed frame of its on-sphere window (the cloth lands at about frame 27)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time budget (0 disables)")
+    ap.add_argument("--roofline-reps", type=int, default=20)
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the rocprofv3 --pmc child runs that measure HBM traffic per color pass")
+    ap.add_argument("--worlds", type=int, default=4096, help="worlds of the batched cloth leg (0 disables the leg)")
+    ap.add_argument("--batched-steps", type=int, default=4)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
-    ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs"), default="drop")
+    ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs", "wind"), default="drop")
     ap.add_argument("--pass-shape", default="auto",
-                    help="color-pass shape of the headline and --pmc-mode runs: 'auto' (autotuner) or lanes:waves[:aligned], "
-                         "e.g. 256:7:1")
+                    help="color-pass shape of the fixed-iteration and --pmc-mode runs: 'auto' (autotuner) or "
+                         "lanes:waves[:aligned], e.g. 256:7:1")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
     ap.add_argument("--self-contact-steps", type=int, default=10,
                     help="timed frames of the stacked-sheets self-contact drape (0 disables)")
     ap.add_argument("--self-contact-warm", type=int, default=20, help="frames before timing (sheets land)")
     ap.add_argument("--self-contact-verts", type=int, default=128, help="vertices per side of each sheet")
     ap.add_argument("--spring-steps", type=int, default=10,
-                    help="timed frames of the headline cloth with springs on every edge and hinge (0 disables)")
+                    help="timed frames of the 256x256 cloth with springs on every edge and hinge (0 disables)")
     return ap.parse_args()
 
 
@@ -102,6 +115,8 @@ def _scene(name, args, dev):
         return scenes.soft_cubes_batched(dev, worlds=args.cube_worlds)
     if name == "springs":
         return scenes.cloth_drop_on_sphere(dev, verts_per_side=args.verts, iterations=args.iterations, springs=True)
+    if name == "wind":
+        return scenes.cloth_wind_batched(dev, worlds=args.residual_worlds)
     return scenes.cloth_batched(dev, worlds=args.worlds)
 
 
@@ -158,36 +173,47 @@ def time_frames(model, state0, info, solver, pipe, contacts, steps, warmup):
     return gpu_ms, wall, finite, states[0]
 
 
-def color_roofline(model, state, info, solver, pipe, contacts, reps):
-    """Algorithmic bytes of each color pass over its live launch time (back-to-back launches, HIP events)."""
+def inframe_roofline(model, state, info, solver, pipe, contacts, iters, reps):
+    """Color-pass roofline from the passes as a substep runs them: ``iters`` iterations (every color in
+    order, repeated) captured in one HIP graph after contact preparation and the forward step, replayed
+    ``reps`` times; HIP events on the graph's stream.  The average pass duration is the elapsed time
+    over reps x iters x colors launches (the graph's gaps between kernels included), the algorithmic
+    bytes are the exact per-color counts of ``roofline.color_pass_bytes`` for the live contacts."""
     import numpy as np
     import torch
 
     from newton_amd.roofline import color_pass_bytes
 
     dt = info["dt"]
+    ncol = len(model.particle_color_groups)
     stream = torch.cuda.Stream()
     stream.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(stream):
         state.clear_forces()
-        pipe.collide(state, contacts)
+        if pipe is not None:
+            pipe.collide(state, contacts)
         solver.begin_step(state, contacts, dt)
+        solver.iterate(state, dt, 1)
         torch.cuda.synchronize()
-        per_color_bytes = color_pass_bytes(model, contacts_per_particle=_contact_counts(model, contacts))
-        ncol = len(model.particle_color_groups)
-        color_ms = np.zeros(ncol)
-        for c in range(ncol):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            solver.color_pass(state, c, dt)  # warm
-            e0.record(stream)
-            for _ in range(reps):
-                solver.color_pass(state, c, dt)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            color_ms[c] = e0.elapsed_time(e1) / reps
-    total_bytes = float(per_color_bytes.sum())
-    achieved = total_bytes / (float(color_ms.sum()) / 1e3) / 1e9
+        per_color_bytes = color_pass_bytes(model, contacts_per_particle=_contact_counts(model, contacts)
+                                           if contacts is not None else None)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            solver.iterate(state, dt, iters)
+        graph.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            graph.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+    del graph
+    launches = reps * iters * ncol
+    total_bytes = float(per_color_bytes.sum()) * iters * reps
+    achieved = total_bytes / (ms / 1e3) / 1e9
     return {
         "kernel": "k_solve_fast",
         "bound": "hbm",
@@ -197,8 +223,11 @@ def color_roofline(model, state, info, solver, pipe, contacts, reps):
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "traffic_detail": None,
-        "bytes_per_launch": round(total_bytes / ncol, 1),
-        "avg_launch_us": round(float(color_ms.mean()) * 1e3, 3),
+        "bytes_per_launch": round(float(per_color_bytes.mean()), 1),
+        "avg_launch_us": round(ms * 1e3 / launches, 3),
+        "launches_timed": launches,
+        "timing": f"{iters} iterations x {ncol} colors captured in one HIP graph, replayed {reps} times "
+                  "(HIP events; inter-kernel gaps of the graph included)",
     }
 
 
@@ -264,7 +293,8 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
                    str(ROOT / "bench.py"), "--pmc-mode", "--pmc-workload", workload, "--verts", str(args.verts),
                    "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--cube-worlds",
-                   str(args.cube_worlds), "--roofline-reps", "5", "--pass-shape", args.pass_shape]
+                   str(args.cube_worlds), "--residual-worlds", str(args.residual_worlds), "--roofline-reps", "5",
+                   "--pass-shape", args.pass_shape if workload == "drop" else "auto"]
             try:
                 proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
                                       env={**os.environ, "TMPDIR": "/tmp"})
@@ -290,6 +320,63 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
 
 
 # ----------------------------------------------------------------------------- legs
+def fixed_iterations_leg(args):
+    """The reference's configured loop on the 256x256 cloth-on-sphere workload (65,536 vertices; one
+    frame = 10 x (clear_forces -> collide -> SolverVBD.step with 10 iterations), captured in a HIP
+    graph): timed while the cloth falls and again after it has landed on the sphere, each window with
+    its in-frame color-pass roofline; HBM traffic per pass from two rocprofv3 --pmc child runs."""
+    import torch
+
+    from newton_amd import CollisionPipeline
+    from newton_amd.solvers import SolverVBD
+
+    dev = "cuda:0"
+    model, state0, info = _scene("drop", args, dev)
+    n = model.particle_count
+    sub = info["substeps"]
+    solver = SolverVBD(model, iterations=args.iterations)
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    state0.clear_forces()
+    pipe.collide(state0, contacts)
+    if args.pass_shape == "auto":
+        tune = solver.autotune(state0, contacts, info["dt"])
+        args.pass_shape = f"{tune['block']}:{tune['waves']}:{tune['aligned']}"  # the counter runs use the timed shape
+    else:
+        parts = [int(x) for x in args.pass_shape.split(":")]
+        solver.set_pass_shape(parts[0], parts[1], parts[2] if len(parts) > 2 else 0)
+        tune = {"block": parts[0], "waves": parts[1], "aligned": parts[2] if len(parts) > 2 else 0, "fixed": True}
+    windows = {}
+    state = state0
+    frame = 0
+    for name, start in (("falling", args.fixed_warmup), ("on_sphere", args.contact_frame)):
+        warm = max(start - frame - 2, 1)  # time_frames runs max(warmup, 1) eager frames and 2 replays first
+        _progress(f"fixed-iteration cloth, {name}: frames {frame + warm + 2}..{frame + warm + 2 + args.fixed_steps}")
+        gpu_ms, wall, finite, state = time_frames(model, state, info, solver, pipe, contacts, args.fixed_steps, warm)
+        first = frame + warm + 2
+        frame = first + args.fixed_steps
+        pipe.collide(state, contacts)
+        torch.cuda.synchronize()
+        n_contacts = int(contacts.soft_contact_count.item())
+        probe = model.state()
+        probe.particle_q.copy_(state.particle_q)
+        probe.particle_qd.copy_(state.particle_qd)
+        roof = inframe_roofline(model, probe, info, solver, pipe, contacts, args.iterations, args.roofline_reps)
+        windows[name] = {"frames": [first, frame], "ms_per_step": round(gpu_ms / args.fixed_steps, 4),
+                         "vertex_steps_per_s": round(n * sub * args.fixed_steps / (gpu_ms / 1e3), 1),
+                         "soft_contacts": n_contacts, "finite": finite, "wall_s": round(wall, 4), "roofline": roof}
+    out = {"workload": info["workload"], "vertices": n, "triangles": model.tri_count,
+           "bending_edges": model.edge_count, "colors": len(model.particle_color_groups), "substeps_per_step": sub,
+           "iterations": args.iterations, "solver": "plain VBD, the reference's fixed iteration count",
+           "hip_graph": True, "steps": args.fixed_steps, "pass_shape": tune, **windows}
+    if args.traffic:
+        _progress("HBM traffic: two rocprofv3 --pmc child runs")
+        traffic, detail = measure_traffic(args, "drop")
+        out["falling"]["roofline"]["traffic"] = None if traffic is None else round(traffic, 1)
+        out["falling"]["roofline"]["traffic_detail"] = detail
+    return out, model, info
+
+
 def batched_leg(args, name="batched", steps=None):
     """Many worlds in one launch per color pass (4096 cloths or 512 tet cubes), or the headline cloth
     with a spring on every edge and hinge; graph-captured frames; color-pass roofline; the work-group
@@ -312,7 +399,7 @@ def batched_leg(args, name="batched", steps=None):
     tune = solver.autotune(state, contacts, info["dt"])
     gpu_ms, wall, finite, last = time_frames(model, state, info, solver, pipe, contacts, steps, 1)
     n = model.particle_count
-    roof = color_roofline(model, last, info, solver, pipe, contacts, max(3, args.roofline_reps // 4))
+    roof = inframe_roofline(model, last, info, solver, pipe, contacts, info["iterations"], max(3, args.roofline_reps // 4))
     return {
         "workload": info["workload"],
         "vertices": n,
@@ -494,7 +581,12 @@ def residual_leg(args):
     fwd_ms = e0.elapsed_time(e1) / 50
     fwd_tflops = MLP_FLOPS_PER_ROW * n / (fwd_ms * 1e-3) / 1e12
 
-    variants = ("plain", "reuse", "learned_trial", "learned_unjudged", "parallel_starts", "learned")
+    variants = ("plain", "reuse", "learned_trial", "learned_unjudged", "parallel_starts", "learned_smoothed",
+                "learned")
+    # the deployed batch's candidates on the single world: low-passed, judged at the start
+    ws_s = LearnedWarmStart(model, dt, mlp=ws.mlp, blend=args.blend, smooth_sweeps=args.smooth_sweeps)
+    ws_s.corr = ws.corr
+    accepted_smoothed = 0
     # the world's substep solved from three starts at once (inertial, y + c, learned) in a 3-copy model
     from newton_amd.solvers.parallel_starts import ParallelStarts
 
@@ -535,7 +627,8 @@ def residual_leg(args):
                     r = {"iterations": its, "r": math.sqrt(r_sq), "r0": math.sqrt(r0_sq), "warm_start_index": 1,
                          "r_warm_start": None}
                 else:
-                    start = {"plain": None, "reuse": ws.reuse, "learned": ws, "learned_trial": ws}[name]
+                    start = {"plain": None, "reuse": ws.reuse, "learned": ws, "learned_trial": ws,
+                             "learned_smoothed": ws_s}[name]
                     r = solver.step_to_residual(
                         s0, s1, None, None, dt, rel_tol=args.rel_tol, max_iterations=args.max_iterations,
                         warm_start=start, trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
@@ -545,6 +638,8 @@ def residual_leg(args):
                 res[name]["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
                 if name == "learned_trial":
                     accepted_trial += int(r["warm_start_index"] == 1)
+                if name == "learned_smoothed":
+                    accepted_smoothed += int(r["warm_start_index"] == 1)
                 if name == "learned":
                     accepted += int(r["warm_start_index"] == 1)
                     if r["r_warm_start"] is not None and r["r0"] > 0:
@@ -568,13 +663,14 @@ def residual_leg(args):
     out["speedup_trial_vertex_steps"] = round(res["plain"]["s"] / res["learned_trial"]["s"], 3)
     out["trial_iterations"] = args.trial_iterations
     out["learned_candidate_chosen_after_trial"] = f"{accepted_trial}/{substeps}"
+    out["speedup_smoothed_vertex_steps"] = round(res["plain"]["s"] / res["learned_smoothed"]["s"], 3)
+    out["learned_candidate_chosen_smoothed"] = f"{accepted_smoothed}/{substeps}"
+    out["smooth_sweeps"] = args.smooth_sweeps
     out["speedup_unjudged_vertex_steps"] = round(res["plain"]["s"] / res["learned_unjudged"]["s"], 3)
     out["speedup_parallel_starts_vertex_steps"] = round(res["plain"]["s"] / res["parallel_starts"]["s"], 3)
     out["parallel_starts_winner"] = {"inertial": ps_wins[0], "network": ps_wins[1]}
     out["learned_blend"] = ws.blend
     out["warm_start_rel_residual_mean"] = round(float(np.mean(r_ws_rel)), 4) if r_ws_rel else None
-    if args.residual_worlds > 0:
-        out["batched"] = batched_residual(args, s0, ws)
     out["training"] = {"protocol": args.train_protocol, "frames": args.train_frames, "substeps": args.train_frames * sub,
                        "iterations_per_substep": args.train_iterations, "rows": rows, "steps": args.train_steps,
                        "loss": "mse + smooth * |L e|^2 over one substep per step", "smooth": args.train_smooth,
@@ -586,16 +682,99 @@ def residual_leg(args):
                            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": round(fwd_tflops / BF16_PEAK_TFLOPS, 5), "rows": n,
                            "avg_launch_us": round(fwd_ms * 1e3, 2)}
+    return out, s0, ws
+
+
+def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
+    """``substeps`` substeps of the batched wind cloth from identical states per substep, each driven to
+    relative residual ``tol`` by three solvers: plain VBD one world at a time (every
+    ``single_world_stride``-th world on a single-world solver, time scaled to all worlds), plain VBD on the
+    batch, and the batch with the learned warm start.  The batched warm-started iterate carries on."""
+    import numpy as np
+    import torch
+
+    model, s0, s1, solver, wsb, wind, info = (ctx[k] for k in ("model", "s0", "s1", "solver", "ws", "wind", "info"))
+    one_state, one_out, one_solver, per, W = (ctx[k] for k in ("one_state", "one_out", "one_solver", "per", "W"))
+    dt, sub, mass = info["dt"], info["substeps"], model.particle_mass
+    cap = args.max_iterations
+    sample = list(range(0, W, max(args.single_world_stride, 1)))
+    batched = ctx["variants"] if with_extra else ("plain_batched", "batched_learned")
+    res = {k: {"iters": [], "witers": [], "s": 0.0, "reached": 0, "chosen": None}
+           for k in ("plain_single_world",) + tuple(batched)}
+    for i in range(substeps):
+        if i % sub == 0:
+            wind.redraw(frame0 + i // sub)
+        q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), wsb.corr.clone()
+        f_ext = wind.force(s0.particle_q, mass)
+        d = res["plain_single_world"]
+        for w in sample:
+            sl = slice(w * per, (w + 1) * per)
+            one_state.particle_q.copy_(q0[sl])
+            one_state.particle_qd.copy_(qd0[sl])
+            one_state.particle_f.copy_(f_ext[sl])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = one_solver.step_to_residual(one_state, one_out, None, None, dt, rel_tol=tol, max_iterations=cap)
+            torch.cuda.synchronize()
+            d["s"] += time.perf_counter() - t0
+            d["iters"].append(r["iterations"])
+            d["reached"] += int(r["r"] <= tol * r["r0"])
+        for name in batched:
+            s0.particle_q.copy_(q0)
+            s0.particle_qd.copy_(qd0)
+            s0.particle_f.copy_(f_ext)
+            wsb.corr.copy_(corr0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=tol, max_iterations=cap,
+                                               **ctx["protocol"][name])
+            torch.cuda.synchronize()
+            d = res[name]
+            d["s"] += time.perf_counter() - t0
+            d["iters"].append(r["iterations"])
+            d["witers"].append(r["world_iterations"])
+            d["reached"] += r["reached"]
+            if r["candidate_chosen"]:
+                d["chosen"] = r["candidate_chosen"] if d["chosen"] is None else \
+                    [a + b for a, b in zip(d["chosen"], r["candidate_chosen"])]
+        wsb.observe(solver, s0)
+        ctx["s0"], ctx["s1"] = s1, s0
+        s0, s1 = s1, s0
+    n = model.particle_count
+    out = {"rel_tol": tol, "substeps": substeps, "max_iterations": cap}
+    d = res["plain_single_world"]
+    single_vs = len(sample) * per * substeps / d["s"]
+    out["plain_single_world"] = {"worlds_timed": len(sample), "mean_iterations": round(float(np.mean(d["iters"])), 2),
+                                 "reached_tol": f"{d['reached']}/{len(sample) * substeps} world-substeps",
+                                 "vertex_steps_per_s": round(single_vs, 1)}
+    for name in batched:
+        d = res[name]
+        wit = np.stack(d["witers"])
+        out[name] = {"mean_iterations_launched": round(float(np.mean(d["iters"])), 2),
+                     "mean_world_iterations": round(float(wit.mean()), 2),
+                     "mean_slowest_world_iterations": round(float(wit.max(axis=1).mean()), 2),
+                     "world_substeps_at_cap": int((wit >= cap).sum()),
+                     "reached_tol": f"{d['reached']}/{substeps * W} world-substeps",
+                     "vertex_steps_per_s": round(n * substeps / d["s"], 1), "seconds": round(d["s"], 4)}
+        if d["chosen"] is not None:
+            out[name]["world_substeps_choosing"] = dict(zip(ctx["candidate_names"], d["chosen"]))
+    out["speedup_vs_plain_single_world"] = round(out["batched_learned"]["vertex_steps_per_s"] / single_vs, 3)
+    out["speedup_vs_plain_batched"] = round(res["plain_batched"]["s"] / res["batched_learned"]["s"], 3)
+    out["speedup_batching_only"] = round(out["plain_batched"]["vertex_steps_per_s"] / single_vs, 3)
+    for name in batched:
+        if name not in ("plain_batched", "batched_learned"):
+            out[f"speedup_{name}_vs_plain_batched"] = round(res["plain_batched"]["s"] / res[name]["s"], 3)
     return out
 
 
-def batched_residual(args, s_single, ws):
-    """The wind cloth batched: ``residual_worlds`` worlds in one model, every world from the single-world
-    leg's last state and its own wind, each judged by its own residual.  From identical states each
-    substep: plain VBD solving the worlds one at a time (a single-world solver, every
-    ``single_world_stride``-th world, time scaled to all worlds), plain VBD on the batch, and the batch
-    with the warm start (``y + c`` and the trained network, chosen per world after
-    ``trial_iterations``)."""
+def headline_leg(args, s_single, ws):
+    """The headline: the 128x128 wind cloth (pinned top edge, Fourier wind) as ``residual_worlds`` worlds
+    in one model, every world driven by its own wind and judged by its own force residual.  Every world
+    starts from the single-world leg's last state; ``warmup`` substeps let the worlds drift apart; then
+    ``steps`` substeps are timed at ``rel_tol`` for plain VBD one world at a time, plain VBD batched and
+    batched + learned warm start (from identical states per substep), then the same at each tolerance
+    of ``tol_sweep``; the color-pass roofline of this model is measured in-frame; and one substep shows
+    where plain VBD's relative residual stops falling (the FP32 floor)."""
     import numpy as np
     import torch
 
@@ -612,93 +791,71 @@ def batched_residual(args, s_single, ws):
     s0.particle_q.copy_(s_single.particle_q.repeat(W, 1))
     s0.particle_qd.copy_(s_single.particle_qd.repeat(W, 1))
     solver = SolverVBD(model, iterations=10)
-    solver.autotune(s0, None, dt)
-    wsb = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0)
+    tune = solver.autotune(s0, None, dt)
+    # the deployed warm start: both candidates low-passed, judged at the start (no trial iterations)
+    wsb = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=args.blend, smooth_sweeps=args.smooth_sweeps)
     wsb.corr.copy_(ws.corr.repeat(W, 1))
-    mass = model.particle_mass
-    first = args.train_frames + args.residual_frames
-    # the worlds drift apart under their own winds before the measured frames
-    for frame in range(args.batched_residual_warm_frames):
-        wind.redraw(first + frame)
-        for _ in range(sub):
-            s0.particle_f.copy_(wind.force(s0.particle_q, mass))
-            solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                           max_iterations=args.max_iterations, warm_start=wsb.reuse)
-            wsb.observe(solver, s0)
-            s0, s1 = s1, s0
-    variants = ("plain", "learned_trial")
-    res = {k: {"iters": [], "witers": [], "s": 0.0, "reached": 0, "chosen": None} for k in variants}
+    # the unsmoothed candidates judged after trial iterations, for comparison (shares the correction buffer)
+    wst = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.5, smooth_sweeps=0)
+    wst.corr = wsb.corr
     one_model, one_state, _ = scenes.cloth_wind(dev)
     one_solver = SolverVBD(one_model, iterations=10)
     one_solver.autotune(one_state, None, dt)
-    one_out = one_model.state()
-    per = one_model.particle_count
-    sample = list(range(0, W, max(args.single_world_stride, 1)))
-    single = {"iters": [], "s": 0.0, "reached": 0}
-    first += args.batched_residual_warm_frames
-    for frame in range(args.residual_frames):
-        wind.redraw(first + frame)
-        _progress(f"batched residual: frame {frame}/{args.residual_frames}")
-        for _ in range(sub):
-            q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), wsb.corr.clone()
-            f_ext = wind.force(s0.particle_q, mass)
-            for w in sample:
-                sl = slice(w * per, (w + 1) * per)
-                one_state.particle_q.copy_(q0[sl])
-                one_state.particle_qd.copy_(qd0[sl])
-                one_state.particle_f.copy_(f_ext[sl])
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                r = one_solver.step_to_residual(one_state, one_out, None, None, dt, rel_tol=args.rel_tol,
-                                                max_iterations=args.max_iterations)
-                torch.cuda.synchronize()
-                single["s"] += time.perf_counter() - t0
-                single["iters"].append(r["iterations"])
-                single["reached"] += int(r["r"] <= args.rel_tol * r["r0"])
-            for name in variants:
-                s0.particle_q.copy_(q0)
-                s0.particle_qd.copy_(qd0)
-                s0.particle_f.copy_(f_ext)
-                wsb.corr.copy_(corr0)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                r = solver.step_to_residual_worlds(
-                    s0, s1, None, None, dt, rel_tol=args.rel_tol, max_iterations=args.max_iterations,
-                    warm_start=None if name == "plain" else wsb,
-                    trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
-                torch.cuda.synchronize()
-                d = res[name]
-                d["s"] += time.perf_counter() - t0
-                d["iters"].append(r["iterations"])
-                d["witers"].append(r["world_iterations"])
-                d["reached"] += r["reached"]
-                if r["candidate_chosen"]:
-                    d["chosen"] = r["candidate_chosen"] if d["chosen"] is None else \
-                        [a + b for a, b in zip(d["chosen"], r["candidate_chosen"])]
-            wsb.observe(solver, s0)
-            s0, s1 = s1, s0
-    substeps = args.residual_frames * sub
-    out = {"workload": info["workload"], "worlds": W, "vertices": n, "stop": "every world at rel_tol",
-           "warm_frames": args.batched_residual_warm_frames, "substeps": substeps}
-    single_vs = len(sample) * per * substeps / single["s"]
-    out["single_world_plain"] = {"worlds_timed": len(sample), "mean_iterations": round(float(np.mean(single["iters"])), 2),
-                                 "reached_tol": f"{single['reached']}/{len(sample) * substeps} world-substeps",
-                                 "vertex_steps_per_s": round(single_vs, 1)}
-    for name in variants:
-        d = res[name]
-        wit = np.stack(d["witers"])
-        out[name] = {"mean_iterations_launched": round(float(np.mean(d["iters"])), 2),
-                     "mean_world_iterations": round(float(wit.mean()), 2),
-                     "mean_slowest_world_iterations": round(float(wit.max(axis=1).mean()), 2),
-                     "reached_tol": f"{d['reached']}/{substeps * W} world-substeps",
-                     "vertex_steps_per_s": round(n * substeps / d["s"], 1)}
-        if d["chosen"] is not None:
-            keys = ("reuse", "network") if len(d["chosen"]) == 2 else [str(k) for k in range(len(d["chosen"]))]
-            out[name]["world_substeps_choosing"] = dict(zip(keys, d["chosen"]))
-    out["speedup_vs_single_world_plain"] = round(out["learned_trial"]["vertex_steps_per_s"] / single_vs, 3)
-    out["speedup_batching_only"] = round(out["plain"]["vertex_steps_per_s"] / single_vs, 3)
-    out["speedup_warm_start_in_batch"] = round(res["plain"]["s"] / res["learned_trial"]["s"], 3)
-    return out
+    ctx = {"model": model, "s0": s0, "s1": s1, "solver": solver, "ws": wsb, "wind": wind, "info": info,
+           "one_state": one_state, "one_out": one_model.state(), "one_solver": one_solver,
+           "per": one_model.particle_count, "W": W, "candidate_names": ("reuse", "network"),
+           "variants": ("plain_batched", "batched_learned_trial", "batched_learned"),  # the deployed one last
+           "protocol": {"plain_batched": {}, "batched_learned": {"warm_start": wsb},
+                        "batched_learned_trial": {"warm_start": wst, "trial_iterations": args.trial_iterations}}}
+    frame = args.train_frames + args.residual_frames
+    mass = model.particle_mass
+    _progress(f"headline: {W} worlds x {one_model.particle_count} vertices, {args.warmup} warm-up substeps")
+    for i in range(args.warmup):
+        if i % sub == 0:
+            wind.redraw(frame + i // sub)
+        s0, s1 = ctx["s0"], ctx["s1"]
+        s0.particle_f.copy_(wind.force(s0.particle_q, mass))
+        solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=args.rel_tol,
+                                       max_iterations=args.max_iterations, warm_start=wsb)
+        wsb.observe(solver, s0)
+        ctx["s0"], ctx["s1"] = s1, s0
+    frame += -(-args.warmup // sub)
+    _progress(f"headline: {args.steps} timed substeps at rel_tol {args.rel_tol}")
+    main = _solve_variants(args, args.rel_tol, args.steps, frame, ctx, with_extra=True)
+    frame += -(-args.steps // sub)
+    sweep = []
+    for tol in args.tol_sweep:
+        _progress(f"headline: tolerance sweep, rel_tol {tol}, {args.sweep_substeps} substeps")
+        sweep.append(_solve_variants(args, tol, args.sweep_substeps, frame, ctx))
+        frame += -(-args.sweep_substeps // sub)
+    # in-frame color-pass roofline of this model (all worlds live, 10 iterations of one substep)
+    probe = model.state()
+    probe.particle_q.copy_(ctx["s0"].particle_q)
+    probe.particle_qd.copy_(ctx["s0"].particle_qd)
+    probe.particle_f.copy_(wind.force(probe.particle_q, mass))
+    roof = inframe_roofline(model, probe, info, solver, None, None, 10, max(2, args.roofline_reps // 4))
+    roof["pass_shape"] = {k: tune[k] for k in ("block", "waves", "aligned")}
+    # where plain VBD's relative residual stops falling: one world's substep iterated far past the tolerances
+    st = ctx["one_state"]
+    st.particle_q.copy_(ctx["s0"].particle_q[:ctx["per"]])
+    st.particle_qd.copy_(ctx["s0"].particle_qd[:ctx["per"]])
+    st.particle_f.copy_(wind.force(st.particle_q, mass[:ctx["per"]]))
+    one_solver.begin_step(st, None, dt)
+    r0 = float(one_solver.residual(st, dt).item())
+    hist = []
+    for _ in range(args.floor_iterations // 50):
+        one_solver.iterate(st, dt, 50)
+        hist.append(float(one_solver.residual(st, dt).item()))
+    rel = np.sqrt(np.asarray(hist) / r0)
+    floor = {"workload": "one world of the batch, one substep of plain VBD", "iterations": len(hist) * 50,
+             "rel_residual_every_50": [round(float(x), 5) for x in rel],
+             "min_rel_residual": round(float(rel.min()), 5),
+             "iterations_at_min": int(50 * (int(rel.argmin()) + 1))}
+    ctx["floor_state"] = (st.particle_q.clone(), one_solver.scratch("q_prev"), one_solver.scratch("inertia"),
+                          st.particle_f.clone())
+    return {"workload": info["workload"], "worlds": W, "vertices": n, "stop": "every world at rel_tol",
+            "warmup_substeps": args.warmup, "pass_shape": {k: tune[k] for k in ("block", "waves", "aligned")},
+            "main": main, "tolerance_sweep": sweep, "roofline": roof, "fp32_floor": floor}, ctx
 
 
 class _trial:  # noqa: N801  (positions-only state for a residual evaluation)
@@ -706,44 +863,81 @@ class _trial:  # noqa: N801  (positions-only state for a residual evaluation)
         self.particle_q = q
 
 
-def cpu_baseline_leg(args, model, q_init, qd_init, dt):
+def cpu_baseline_leg(args, ctx, fixed_model, fixed_q, fixed_qd, fixed_dt):
+    """CPU baseline: the FP32 C oracle on the GPU host's cores.
+
+    Headline workload: plain VBD on one 128x128 wind-cloth world from the batch's last state, each
+    substep iterated to ``rel_tol`` (the residual evaluated after every iteration, as the GPU driver
+    does), OpenMP over each color's vertices, for about ``cpu_seconds``; also single-threaded (Warp's
+    CPU device runs a launch serially) on a shorter sample.  Beside it: the fixed-iteration 256x256
+    cloth, and the FP64 residual of the FP32 iterate the FP32 floor measurement ended at."""
+    import numpy as np
+    import torch
+
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle  # noqa: PLC0415  (the checker, timed as the CPU baseline)
 
-    n = model.particle_count
     cores = min(len(os.sched_getaffinity(0)), 16)
-    om = oracle.OracleModel(model)
-    prm = oracle.params(args.iterations, num_threads=cores)
-    q, qd = q_init, qd_init
-    done = 0
-    t0 = time.perf_counter()
-    while True:
-        q, qd, _ = oracle.simulate(om, prm, q, qd, 1, dt, 0.01, len(om.pairs) // 2)
-        done += 1
-        if time.perf_counter() - t0 >= args.cpu_seconds or done >= 200:
-            break
-    cpu_t = time.perf_counter() - t0
-    # the serial restatement too (Warp's CPU device runs each launch serially): a shorter sample
-    prm1 = oracle.params(args.iterations, num_threads=1)
-    q, qd = q_init, qd_init
-    done1 = 0
-    t0 = time.perf_counter()
-    while True:
-        q, qd, _ = oracle.simulate(om, prm1, q, qd, 1, dt, 0.01, len(om.pairs) // 2)
-        done1 += 1
-        if time.perf_counter() - t0 >= 0.25 * args.cpu_seconds or done1 >= 50:
-            break
-    cpu1_t = time.perf_counter() - t0
-    return {
-        "value": round(n * done / cpu_t, 1),
+    one_model, per, info = ctx["one_solver"].model, ctx["per"], ctx["info"]
+    wind, dt, sub = info["wind"], info["dt"], info["substeps"]
+    om = oracle.OracleModel(one_model)
+    q = ctx["s0"].particle_q[:per].cpu().numpy().copy()
+    qd = ctx["s0"].particle_qd[:per].cpu().numpy().copy()
+    mass = one_model.particle_mass
+
+    def run(threads, budget, limit):
+        nonlocal q, qd
+        prm = oracle.params(args.max_iterations, num_threads=threads, residual_stop=args.rel_tol)
+        done, spent, its = 0, 0.0, []
+        while spent < budget and done < limit:
+            # the world's wind (world 0 of the batch) at the current positions; not timed
+            f = (wind.winds[0].acceleration(torch.as_tensor(q, device=mass.device)) * mass[:, None]).cpu().numpy()
+            t0 = time.perf_counter()
+            _, q, qd, hist = oracle.step(om, prm, q, qd, f, None, dt, residual_history=True)
+            spent += time.perf_counter() - t0
+            its.append(int(np.count_nonzero(hist[1:])))
+            done += 1
+        return done, spent, its
+
+    done, spent, its = run(cores, args.cpu_seconds, 200)
+    done1, spent1, its1 = run(1, 0.3 * args.cpu_seconds, 50)
+    out = {
+        "value": round(per * done / spent, 1),
         "unit": "vertex-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{done} substeps (collide + step, {args.iterations} iterations) of the same "
-                  f"{args.verts}x{args.verts} cloth from its initial state, FP32 C oracle, OpenMP {cores} threads",
-        "single_thread": {"value": round(n * done1 / cpu1_t, 1), "cores": 1,
-                          "sample": f"{done1} substeps of the same trajectory, FP32 C oracle, 1 thread"},
+        "sample": f"{done} substeps of one {info['workload'].split('_')[-1]} wind-cloth world (the headline's "
+                  f"world 0, from its last state), plain VBD iterated to relative force residual {args.rel_tol} "
+                  f"(cap {args.max_iterations}, mean {np.mean(its):.1f} iterations), FP32 C oracle, "
+                  f"OpenMP {cores} threads",
+        "single_thread": {"value": round(per * done1 / spent1, 1), "cores": 1,
+                          "sample": f"{done1} substeps of the same trajectory, mean {np.mean(its1):.1f} "
+                                    "iterations, FP32 C oracle, 1 thread"},
     }
+    # the reference's fixed-iteration loop on the 256x256 cloth (10 iterations)
+    omf = oracle.OracleModel(fixed_model)
+    prm = oracle.params(args.iterations, num_threads=cores)
+    qf, qdf = fixed_q, fixed_qd
+    nf, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 0.3 * args.cpu_seconds and nf < 100:
+        qf, qdf, _ = oracle.simulate(omf, prm, qf, qdf, 1, fixed_dt, 0.01, len(omf.pairs) // 2)
+        nf += 1
+    out["fixed_iterations"] = {"value": round(fixed_model.particle_count * nf / (time.perf_counter() - t0), 1),
+                               "cores": cores, "sample": f"{nf} substeps (collide + step, {args.iterations} "
+                               "iterations) of the 256x256 cloth from its initial state"}
+    # FP32 floor: the FP32 and FP64 residuals of the FP32 iterate the floor measurement ended at
+    if "floor_state" in ctx:
+        qe, q_prev, inertia, f_ext = (t.cpu().numpy() for t in ctx["floor_state"])
+        prm64 = oracle.params(1, num_threads=cores)
+        _, r_end64 = oracle.residual(om, prm64, None, qe, q_prev, inertia, dt, precision="f64")
+        _, r0_64 = oracle.residual(om, prm64, None, inertia, q_prev, inertia, dt, precision="f64")
+        _, r_end32 = oracle.residual(om, prm64, None, qe, q_prev, inertia, dt, precision="f32")
+        _, r0_32 = oracle.residual(om, prm64, None, inertia, q_prev, inertia, dt, precision="f32")
+        out["fp32_floor_check"] = {
+            "what": "residual of the FP32 iterate after the floor measurement's iterations, evaluated by the "
+                    "oracle in FP32 and in FP64 (relative to each precision's residual at the inertial guess)",
+            "rel_fp32": round(r_end32 / r0_32, 6), "rel_fp64": round(r_end64 / r0_64, 6)}
+    return out
 
 
 def main():
@@ -757,79 +951,49 @@ def main():
         return
     import torch
 
-    from newton_amd import CollisionPipeline
-    from newton_amd.solvers import SolverVBD
-
-    dev = "cuda:0"
     torch.cuda.set_device(0)
-    model, state0, info = _scene("drop", args, dev)
-    n = model.particle_count
-    sub = info["substeps"]
-    solver = SolverVBD(model, iterations=args.iterations)
-    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
-    contacts = pipe.contacts()
-    q_init = state0.particle_q.cpu().numpy().copy()
-    qd_init = state0.particle_qd.cpu().numpy().copy()
-    state0.clear_forces()
-    pipe.collide(state0, contacts)
-    if args.pass_shape == "auto":
-        tune = solver.autotune(state0, contacts, info["dt"])
-        args.pass_shape = f"{tune['block']}:{tune['waves']}:{tune['aligned']}"  # the counter runs use the timed shape
-    else:
-        parts = [int(x) for x in args.pass_shape.split(":")]
-        solver.set_block_shape(parts[0])
-        solver.set_pass_waves(parts[1])
-        solver.set_aligned_items(bool(parts[2]) if len(parts) > 2 else False)
-        tune = {"block": parts[0], "waves": parts[1], "aligned": parts[2] if len(parts) > 2 else 0, "fixed": True}
-    _progress(f"headline workload {info['workload']}: {n} vertices, timing {args.steps} frames, shape {tune}")
-    gpu_ms, wall, finite, last = time_frames(model, state0, info, solver, pipe, contacts, args.steps, args.warmup)
-    value = n * sub * args.steps / (gpu_ms / 1e3)
-    roofline = color_roofline(model, last, info, solver, pipe, contacts, args.roofline_reps)
-    roofline["launches_per_substep"] = len(model.particle_color_groups) * args.iterations
-    roofline["pass_shape"] = tune
-    _progress(f"headline: {value:.4g} vertex-steps/s, color pass {roofline['avg_launch_us']} us")
+    _progress("residual leg: training data, GPU training, single-world comparisons")
+    single, s_single, ws = residual_leg(args)
+    _progress(f"single world: learned after trial sweeps {single['speedup_trial_vertex_steps']}x plain")
+    head, ctx = headline_leg(args, s_single, ws)
+    main_rec = head["main"]
+    value = main_rec["batched_learned"]["vertex_steps_per_s"]
+    ms_per_step = 1e3 * main_rec["batched_learned"]["seconds"] / args.steps
+    _progress(f"headline: {value:.4g} vertex-steps/s ({main_rec['speedup_vs_plain_single_world']}x plain single "
+              f"world, {main_rec['speedup_vs_plain_batched']}x plain batched)")
+    torch.cuda.empty_cache()
+
+    fixed, fixed_model, fixed_info = fixed_iterations_leg(args)
+    _progress(f"fixed iterations: {fixed['falling']['vertex_steps_per_s']:.4g} / "
+              f"{fixed['on_sphere']['vertex_steps_per_s']:.4g} vertex-steps/s")
+    torch.cuda.empty_cache()
+    roofline = dict(head["roofline"])
     if args.traffic:
-        _progress("HBM traffic: two rocprofv3 --pmc child runs")
-        traffic, detail = measure_traffic(args, "drop")
+        _progress("HBM traffic of the headline model: two rocprofv3 --pmc child runs")
+        traffic, detail = measure_traffic(args, "wind")
         roofline["traffic"] = None if traffic is None else round(traffic, 1)
         roofline["traffic_detail"] = detail
-    del solver, pipe, contacts
-    torch.cuda.empty_cache()
 
-    batched = None
-    if args.worlds > 0:
-        _progress(f"batched leg: {args.worlds} cloth worlds")
-        batched = batched_leg(args, "batched")
-        _progress(f"batched: {batched['vertex_steps_per_s']:.4g} vertex-steps/s")
-    torch.cuda.empty_cache()
-    cubes = None
-    if args.cube_worlds > 0:
-        _progress(f"batched leg: {args.cube_worlds} tet cubes")
-        cubes = batched_leg(args, "cubes")
-        _progress(f"cubes: {cubes['vertex_steps_per_s']:.4g} vertex-steps/s")
-    torch.cuda.empty_cache()
-    springs = None
-    if args.spring_steps > 0:
-        _progress("springs leg: the headline cloth with a spring on every edge and hinge")
-        springs = batched_leg(args, "springs", args.spring_steps)
-        _progress(f"springs: {springs['vertex_steps_per_s']:.4g} vertex-steps/s")
-    torch.cuda.empty_cache()
-    self_contact = None
+    legs = {}
+    for name, enabled, steps in (("batched", args.worlds > 0, None), ("batched_tets", args.cube_worlds > 0, None),
+                                 ("springs", args.spring_steps > 0, args.spring_steps)):
+        if enabled:
+            _progress(f"{name} leg")
+            legs[name] = batched_leg(args, {"batched_tets": "cubes"}.get(name, name), steps)
+            _progress(f"{name}: {legs[name]['vertex_steps_per_s']:.4g} vertex-steps/s")
+            torch.cuda.empty_cache()
     if args.self_contact_steps > 0:
         _progress("self-contact leg: stacked sheets on a sphere")
-        self_contact = self_contact_leg(args)
-        _progress(f"self-contact: {self_contact['vertex_steps_per_s']:.4g} vertex-steps/s")
-    torch.cuda.empty_cache()
-    residual = None
-    if args.residual_frames > 0:
-        _progress("residual leg: data generation, training, evaluation")
-        residual = residual_leg(args)
-        _progress(f"residual: speedup {residual['speedup_vertex_steps']}")
+        legs["self_contact"] = self_contact_leg(args)
+        torch.cuda.empty_cache()
     cpu = None
     if args.cpu_seconds > 0:
-        _progress(f"CPU baseline: {args.cpu_seconds} s")
-        cpu = cpu_baseline_leg(args, model, q_init, qd_init, info["dt"])
+        _progress(f"CPU baseline: about {1.6 * args.cpu_seconds:.0f} s")
+        fq = fixed_model.particle_q.cpu().numpy().copy()
+        fqd = fixed_model.particle_qd.cpu().numpy().copy()
+        cpu = cpu_baseline_leg(args, ctx, fixed_model, fq, fqd, fixed_info["dt"])
 
+    per = ctx["per"]
     rec = {
         "metric": "vertex-steps/s at fixed force residual; plain VBD vs batched+learned warm-start",
         "value": round(value, 1),
@@ -837,31 +1001,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(gpu_ms / args.steps, 4),
+        "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"cloth_drop_on_sphere_{args.verts}x{args.verts}",
-            "vertices": n,
-            "triangles": model.tri_count,
-            "bending_edges": model.edge_count,
-            "colors": len(model.particle_color_groups),
-            "substeps_per_step": sub,
-            "iterations": args.iterations,
-            "solver": "plain VBD, the reference's fixed iteration count",
-            "hip_graph": True,
-            "wall_s": round(wall, 4),
-            "finite": finite,
+            "workload": f"{head['workload']}: the 128x128 wind cloth (top edge pinned, 8-mode Fourier wind, "
+                        f"16,384 vertices) as {head['worlds']} worlds with their own winds, every world driven "
+                        f"to relative force residual {args.rel_tol}",
+            "worlds": head["worlds"],
+            "vertices": head["vertices"],
+            "vertices_per_world": per,
+            "rel_tol": args.rel_tol,
+            "max_iterations": args.max_iterations,
+            "step": "one substep (dt 1/600 s) of every world, each world iterated until r_w <= rel_tol * r0_w",
+            "solver": "batched + learned warm start: per world the inertial start y, y + S(c_prev) and "
+                      "y + S(c_prev + MLP), judged by their force residuals at the start (S: "
+                      f"{args.smooth_sweeps} sweeps of the 1-ring mean; MLP: the BF16 network trained in this run)",
+            "pass_shape": head["pass_shape"],
         },
+        "comparison": main_rec,
+        "tolerance_sweep": head["tolerance_sweep"],
+        "fp32_floor": head["fp32_floor"],
         "roofline": roofline,
-        "batched": batched,
-        "batched_tets": cubes,
-        "springs": springs,
-        "self_contact": self_contact,
-        "residual": residual,
+        "single_world": single,
+        "fixed_iterations": fixed,
+        **legs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec))

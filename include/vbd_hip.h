@@ -307,6 +307,12 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* ctx, void* stream, float* q, 
  * into dst (device) on the stream: which = 0 inertia y, 1 q_prev, 2 displacement. */
 int32_t vbd_copy_scratch(VbdContext* ctx, void* stream, int32_t which, float* dst);
 
+/* Restores the displacement ([N,3] floats, device) saved with vbd_copy_scratch(which = 2) together
+ * with the positions it belongs to, so a saved iterate (q, d) continues bit for bit; rewriting q
+ * through vbd_set_iterate would store d = fl(q - x0) instead, which may differ from d in its last
+ * bits.  No counterpart in the reference (warm-start drivers). */
+int32_t vbd_set_displacement(VbdContext* ctx, void* stream, const float* src);
+
 /* Cloth self-contact (contexts created with params.self_contact = 1).  With it on, vbd_forward
  * detects before the forward step and truncates after it, vbd_iteration re-detects on the
  * collision-detection interval and truncates after every color pass (solver_vbd.py:2003-2063,

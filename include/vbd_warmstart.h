@@ -52,6 +52,12 @@ int32_t vbd_ws_features(void* stream, int32_t n, const int32_t* nbr_off, const i
                         const float* y, const float* corr, const float* acc, const float* mass, const float* area,
                         float dt, const float* scales, uint16_t* feat_out);
 
+/* Low-pass of correction fields v ([n, width] floats, width 1..12, in place): `sweeps` sweeps of
+ * v_i <- (v_i + sum_j v_j) / (1 + degree_i) over the 1-ring CSR (nbr_off [n+1], nbr), with particles
+ * whose active[i] == 0 read and written as 0; tmp is [n, width] device scratch. */
+int32_t vbd_ws_smooth(void* stream, int32_t n, int32_t width, const int32_t* nbr_off, const int32_t* nbr,
+                      const float* active, int32_t sweeps, float* v, float* tmp);
+
 /* FP32 parameters -> packed BF16 weights. */
 int32_t vbd_ws_pack(void* stream, const float* params, uint16_t* packed);
 

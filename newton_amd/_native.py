@@ -186,6 +186,7 @@ HIP_EXPORTS = {
          ctypes.POINTER(ctypes.c_double)],
     ),
     "vbd_copy_scratch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
+    "vbd_set_displacement": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "vbd_self_detect": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "vbd_self_copy_lists": (c_int32, [c_void_p, c_void_p] + [c_void_p] * 7),
     "vbd_self_status": (c_int32, [c_void_p, ctypes.POINTER(c_int32)]),
@@ -233,6 +234,8 @@ WS_EXPORTS = {
         [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
          c_void_p, c_u16p],
     ),
+    "vbd_ws_smooth": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                c_void_p]),
     "vbd_ws_pack": (c_int32, [c_void_p, c_void_p, c_u16p]),
     "vbd_ws_forward": (c_int32, [c_void_p, c_int32, c_u16p, c_u16p, c_void_p, c_void_p]),
     "vbd_ws_forward_train": (c_int32, [c_void_p, c_int32, c_u16p, c_u16p, c_void_p, c_void_p, c_u16p, c_void_p]),

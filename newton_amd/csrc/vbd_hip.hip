@@ -2455,6 +2455,13 @@ int32_t vbd_copy_scratch(VbdContext* c, void* stream, int32_t which, float* dst)
     return VBD_OK;
 }
 
+int32_t vbd_set_displacement(VbdContext* c, void* stream, const float* src) {
+    if (!c || !src) return fail(VBD_ERR_ARG, "vbd_set_displacement: null argument");
+    const size_t n3 = 3 * (size_t)c->model.particle_count;
+    VBD_HIP_TRY(hipMemcpyAsync(c->kp.disp, src, sizeof(float) * n3, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return VBD_OK;
+}
+
 int32_t vbd_layout_stats(VbdContext* c, int64_t* out4) {
     if (!c || !out4) return fail(VBD_ERR_ARG, "vbd_layout_stats: null argument");
     out4[0] = c->fast_color_off.back();

@@ -233,6 +233,37 @@ __global__ __launch_bounds__(kFeatBlock) void k_features(int n, const int* __res
 }
 
 // ---- weight packing ------------------------------------------------------------
+// ---- low-pass of correction fields --------------------------------------------
+// One sweep of the 1-ring mean with the particle itself, for `width` floats per particle (e.g. two
+// [n,3] fields side by side): out_i = (v_i + sum_j v_j) / (1 + degree_i) over solved particles
+// (inactive ones read and write 0).  Repeated sweeps damp rough, neighbour-to-neighbour components
+// of a predicted correction fast and its smooth ones slowly.
+constexpr int kSmoothMaxWidth = 12;
+
+__global__ __launch_bounds__(256) void k_smooth(int n, int width, const int* __restrict__ nbr_off,
+                                                const int* __restrict__ nbr, const float* __restrict__ active,
+                                                const float* __restrict__ in, float* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* o = out + (size_t)width * i;
+    if (active[i] == 0.0f) {
+        for (int c = 0; c < width; ++c) o[c] = 0.0f;
+        return;
+    }
+    float acc[kSmoothMaxWidth];
+    const float* vi = in + (size_t)width * i;
+    for (int c = 0; c < width; ++c) acc[c] = vi[c];
+    const int b = nbr_off[i], e = nbr_off[i + 1];
+    for (int k = b; k < e; ++k) {
+        const int j = nbr[k];
+        const float w = active[j];
+        const float* vj = in + (size_t)width * j;
+        for (int c = 0; c < width; ++c) acc[c] += w * vj[c];
+    }
+    const float inv = 1.0f / (float)(1 + e - b);
+    for (int c = 0; c < width; ++c) o[c] = acc[c] * inv;
+}
+
 __global__ void k_pack(const float* __restrict__ prm, uint16_t* __restrict__ pk) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kPacked) return;
@@ -559,6 +590,30 @@ int32_t vbd_ws_features(void* stream, int32_t n, const int32_t* nbr_off, const i
     hipLaunchKernelGGL(k_features, dim3(blocks_for(n, kFeatBlock)), dim3(kFeatBlock), 0, (hipStream_t)stream, n,
                        nbr_off, nbr, x0, y, corr, acc, mass, area, dt, scales, feat_out);
     WS_LAUNCH_CHECK("k_features");
+    return 0;
+}
+
+int32_t vbd_ws_smooth(void* stream, int32_t n, int32_t width, const int32_t* nbr_off, const int32_t* nbr,
+                      const float* active, int32_t sweeps, float* v, float* tmp) {
+    if (n < 0 || sweeps < 0 || width < 1 || width > kSmoothMaxWidth ||
+        (n > 0 && (!nbr_off || !active || !v || (sweeps > 0 && !tmp))))
+        return fail(-1, "vbd_ws_smooth: bad argument (width 1..12)");
+    if (n == 0 || sweeps == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    float* src = v;
+    float* dst = tmp;
+    for (int k = 0; k < sweeps; ++k) {
+        hipLaunchKernelGGL(k_smooth, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, width, nbr_off, nbr, active, src,
+                           dst);
+        WS_LAUNCH_CHECK("k_smooth");
+        float* t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != v) {
+        hipError_t e = hipMemcpyAsync(v, src, sizeof(float) * (size_t)width * (size_t)n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return fail(-2, std::string("vbd_ws_smooth: ") + hipGetErrorString(e));
+    }
     return 0;
 }
 

@@ -599,7 +599,7 @@ class SolverVBD(SolverBase):
     def step_to_residual_worlds(self, state_in: State, state_out: State, control: Control | None,
                                 contacts: Contacts | None, dt: float, rel_tol: float = 1.0e-3,
                                 max_iterations: int = 200, check_every: int = 1, warm_start=None,
-                                trial_iterations: int = 0) -> dict:
+                                trial_iterations: int = 0, abs_tol=0.0) -> dict:
         """:meth:`step_to_residual` for a model of several worlds, each judged by its own residual.
 
         Each world is frozen once ``r_w <= rel_tol * r0_w`` (its vertices are no longer solved, and a
@@ -609,6 +609,10 @@ class SolverVBD(SolverBase):
         chosen per world: after ``trial_iterations`` iterations from each start (0: at the start
         itself), each world keeps the start with its lowest residual (a candidate must beat the
         inertial start strictly); the mixed iterate then restarts through :meth:`set_iterate`.
+
+        ``abs_tol`` (a norm in force units, scalar or one per world) floors each world's target at
+        ``abs_tol**2``: a world whose inertial guess is already at round-off (free fall of an unstressed
+        body) stops there instead of chasing a relative tolerance its FP32 residual cannot reach.
         """
         self.world_offsets()
         self.begin_step(state_in, contacts, dt)
@@ -630,6 +634,7 @@ class SolverVBD(SolverBase):
                     self.iterate(state_in, dt, trial_iterations)
                     best_r = self.residual_worlds(state_in, dt).cpu().numpy()
                     best_q = state_in.particle_q.clone()
+                    best_d = self.scratch("displacement")
                     for k, g in enumerate(guesses):
                         self.set_iterate(state_in, g, commit=True)
                         self.iterate(state_in, dt, trial_iterations)
@@ -638,15 +643,20 @@ class SolverVBD(SolverBase):
                         if better.any():
                             best_r = np.where(better, r, best_r)
                             chosen[better] = k
-                            take = torch.as_tensor(better, device=wp.device)[wp]
-                            best_q = torch.where(take[:, None], state_in.particle_q, best_q)
+                            take = torch.as_tensor(better, device=wp.device)[wp][:, None]
+                            best_q = torch.where(take, state_in.particle_q, best_q)
+                            best_d = torch.where(take, self.scratch("displacement"), best_d)
                     trial_its = trial_iterations * (1 + n_cand)
-                    self.set_iterate(state_in, best_q, commit=True)
+                    # every world continues from its winning start's iterate exactly (positions and
+                    # displacement as saved; a world that kept the inertial start equals a plain solve)
+                    state_in.particle_q.copy_(best_q)
+                    self.set_displacement(best_d)
                 else:
                     # judged at the starts themselves; the solver state is untouched unless a
                     # candidate wins somewhere, and then the mixed guess restarts every world
                     best_r = r0.copy()
                     best_g = state_in.particle_q.clone()
+                    d_inertial = self.scratch("displacement")
                     trial = None
                     for k, g in enumerate(guesses):
                         trial = self.set_iterate(state_in, g, commit=False, out=trial)
@@ -659,13 +669,18 @@ class SolverVBD(SolverBase):
                             best_g = torch.where(take[:, None], g, best_g)
                     if (chosen >= 0).any():
                         self.set_iterate(state_in, best_g, commit=True)
-        its, r_sq, wits = self.iterate_to_residual_worlds(state_in, dt, (rel_tol * rel_tol) * r0,
+                        # worlds that keep the inertial start keep its displacement bit for bit
+                        keep = torch.as_tensor(chosen < 0, device=wp.device)[wp][:, None]
+                        self.set_displacement(torch.where(keep, d_inertial, self.scratch("displacement")))
+        target = np.maximum((rel_tol * rel_tol) * r0, np.square(np.broadcast_to(np.asarray(abs_tol, np.float64),
+                                                                                  (nw,))))
+        its, r_sq, wits = self.iterate_to_residual_worlds(state_in, dt, target,
                                                           max(max_iterations - trial_iterations, 0) if trial_its else
                                                           max_iterations, check_every, per_world=True)
         self.end_step(state_in, state_out, dt)
         return {"iterations": its + trial_its, "world_iterations": wits + trial_its, "r0": np.sqrt(r0),
                 "r": np.sqrt(r_sq),
-                "reached": int((r_sq <= (rel_tol * rel_tol) * r0).sum()), "worlds": nw,
+                "reached": int((r_sq <= target).sum()), "worlds": nw,
                 "candidate_chosen": [int((chosen == k).sum()) for k in range(n_cand)]}
 
     def step_to_residual(self, state_in: State, state_out: State, control: Control | None,
@@ -701,6 +716,7 @@ class SolverVBD(SolverBase):
             r0_sq = float(r0_dev.item())
             self.iterate(state_in, dt, trial_iterations)
             best_q = state_in.particle_q.clone()
+            best_d = self.scratch("displacement")
             best_r = float(self.residual(state_in, dt).item())
             r_candidates = [math.sqrt(best_r)]
             for k, g in enumerate(guesses):
@@ -711,7 +727,10 @@ class SolverVBD(SolverBase):
                 if r < best_r:
                     best_r, chosen = r, k
                     best_q.copy_(state_in.particle_q)
-            self.set_iterate(state_in, best_q, commit=True)
+                    best_d = self.scratch("displacement")
+            # the winning start's iterate continues exactly: positions and displacement restored as saved
+            state_in.particle_q.copy_(best_q)
+            self.set_displacement(best_d)
             r_ws = math.sqrt(best_r)
             its, r_sq = self.iterate_to_residual(state_in, dt, (rel_tol * rel_tol) * r0_sq,
                                                  max(max_iterations - trial_iterations, 0), check_every)
@@ -742,6 +761,13 @@ class SolverVBD(SolverBase):
         return {"iterations": its, "r0": math.sqrt(r0_sq), "r": math.sqrt(r_sq),
                 "warm_start_accepted": chosen is not None, "warm_start_index": chosen, "r_warm_start": r_ws,
                 "r_candidates": r_candidates}
+
+    def set_displacement(self, disp: torch.Tensor) -> None:
+        """Restore a displacement saved with ``scratch("displacement")`` together with the positions it
+        belongs to (``vbd_set_displacement``), so that saved iterate continues bit for bit."""
+        _native.check(_native.hip_lib().vbd_set_displacement(self._ctx, current_stream_handle(self.model.device),
+                                                             _native.ptr(disp.contiguous())),
+                      "vbd_set_displacement")
 
     def scratch(self, which: str) -> torch.Tensor:
         """Copy of the current substep's ``inertia`` / ``q_prev`` / ``displacement`` array ([N, 3])."""

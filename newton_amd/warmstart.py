@@ -175,10 +175,11 @@ class LearnedWarmStart:
     """
 
     def __init__(self, model: Model, dt: float, mlp: WarmStartMLP | None = None, seed: int = 0,
-                 offer_reuse: bool = True, blend: float = 0.5):
+                 offer_reuse: bool = True, blend: float = 0.5, smooth_sweeps: int = 0):
         self.model = model
         self.offer_reuse = offer_reuse
         self.blend = float(blend)
+        self.smooth_sweeps = int(smooth_sweeps)
         self.device = model.device
         self.dt = float(dt)
         n = model.particle_count
@@ -204,6 +205,7 @@ class LearnedWarmStart:
         self.feat = torch.empty((n, N_IN), dtype=torch.int16, device=self.device)
         self.pred = torch.empty((n, 3), dtype=torch.float32, device=self.device)
         self.acc = torch.zeros((n, 3), dtype=torch.float32, device=self.device)
+        self._tmp = torch.empty((n, 3), dtype=torch.float32, device=self.device)
         # 1-ring mean operator for the smoothness term of the training loss
         deg = (self.nbr_off[1:] - self.nbr_off[:-1]).long()
         self._rows = torch.repeat_interleave(torch.arange(n, device=self.device), deg)
@@ -250,21 +252,42 @@ class LearnedWarmStart:
         self._y = y
         return self.feat
 
+    def smooth(self, v: torch.Tensor, sweeps: int | None = None) -> torch.Tensor:
+        """Low-pass of correction fields ([N, 3k] floats, in place): ``sweeps`` sweeps of the 1-ring
+        mean (``vbd_ws_smooth``) over the solved particles."""
+        k = self.smooth_sweeps if sweeps is None else int(sweeps)
+        if k > 0:
+            width = v.shape[1]
+            if self._tmp.numel() < v.numel():
+                self._tmp = torch.empty_like(v)
+            _native.ws_check(_native.ws_lib().vbd_ws_smooth(current_stream_handle(self.device),
+                                                            self.model.particle_count, width,
+                                                            self.nbr_off.data_ptr(), self.nbr.data_ptr(),
+                                                            self._act.data_ptr(), k, v.data_ptr(),
+                                                            self._tmp.data_ptr()),
+                             "vbd_ws_smooth")
+        return v
+
     def __call__(self, solver, state_in, dt):
-        """Candidate iterates for ``step_to_residual``: the learned one, y + c_prev + blend * scale *
-        MLP(features) (the network predicts how this substep's correction differs from the previous
-        substep's; ``blend`` 0.5 halves the step, which converges faster than the full one), and, when
-        ``offer_reuse`` is set, first the network-free y + c_prev; the residual picks among them."""
+        """Candidate iterates for ``step_to_residual``: the learned one, y + S(c_prev + blend * scale *
+        MLP(features)) (the network predicts how this substep's correction differs from the previous
+        substep's), and, when ``offer_reuse`` is set, first the network-free y + S(c_prev); the residual
+        picks among them.  S is ``smooth_sweeps`` sweeps of the 1-ring mean (identity at 0): the
+        per-particle network leaves rough, neighbour-to-neighbour error that the stiff membrane turns
+        into large forces, and the low-pass removes it while keeping the smooth correction the
+        Gauss-Seidel sweeps are slow to find."""
         feat = self.features(solver, state_in)
         self.mlp.forward(feat, self.pred)
-        learned = self._y + self.corr + self.pred * (self.blend * self.scale_corr)
-        if self.offer_reuse:
-            return [self._y + self.corr, learned]
-        return learned
+        learned = self.corr + self.pred * (self.blend * self.scale_corr)
+        if not self.offer_reuse:
+            return self._y + self.smooth(learned)
+        both = self.smooth(torch.cat([self.corr, learned], dim=1))  # one set of sweeps for both fields
+        return [self._y + both[:, :3], self._y + both[:, 3:]]
 
     def reuse(self, solver, state_in, dt) -> torch.Tensor:
-        """The no-network baseline: y + c_prev (the previous substep's correction carried over)."""
-        return solver.scratch("inertia") + self.corr
+        """The no-network baseline: y + S(c_prev) (the previous substep's correction carried over,
+        low-passed as the learned candidate is)."""
+        return solver.scratch("inertia") + self.smooth(self.corr.clone())
 
     def target(self, solver, state_in) -> torch.Tensor:
         """Training target (x - y - c_prev) / scale of the current iterate (call after iterating,

@@ -2,7 +2,8 @@
 
 Restates, in numpy, what libvbd_ws.so computes (newton_amd/csrc/vbd_warmstart.hip):
 the 48 per-particle features, the 48-128-128-128-3 SiLU MLP forward pass, its
-backward pass for the weighted mean-squared error, and Adam -- with the same
+backward pass for the weighted mean-squared error, Adam, and the 1-ring low-pass
+of predicted corrections (``smooth``) -- with the same
 BF16 rounding points (features; H_l = bf16(silu(Z_l)); dZ_l = bf16(dH_l *
 silu'(Z_l)); dZ_4 = bf16(dY)), the products accumulated in float64 here and in
 FP32 on the MFMA units there.  ``bf16=False`` drops every rounding, giving the
@@ -201,3 +202,27 @@ def features(nbr_off, nbr, x0, y, corr, acc, mass, area, dt, scales) -> np.ndarr
 
 __all__ = ["N_PARAMS", "adam", "bf16_bits", "bf16_round", "bf16_to_f32", "features", "forward", "loss_and_grad",
            "pack", "unpack"]
+
+
+def smooth(nbr_off, nbr, active, v, sweeps: int) -> np.ndarray:
+    """``vbd_ws_smooth``: ``sweeps`` sweeps of v_i <- (v_i + sum_j active_j v_j) / (1 + degree_i) for
+    active particles (inactive ones become 0), in float32 with the kernel's order of operations
+    (the particle's own value, then its neighbours in CSR order, then one multiply by the float32
+    reciprocal of 1 + degree)."""
+    off = np.asarray(nbr_off, dtype=np.int64)
+    nb = np.asarray(nbr, dtype=np.int64)
+    act = np.asarray(active, dtype=np.float32).reshape(-1)
+    v = np.array(v, dtype=np.float32, copy=True)
+    n = v.shape[0]
+    deg = np.diff(off)
+    inv = (np.float32(1.0) / (1 + deg).astype(np.float32)).astype(np.float32)
+    for _ in range(sweeps):
+        acc = v.copy()
+        for slot in range(int(deg.max()) if n else 0):
+            has = np.nonzero(deg > slot)[0]
+            j = nb[off[has] + slot]
+            acc[has] = (acc[has] + act[j, None] * v[j]).astype(np.float32)
+        out = (acc * inv[:, None]).astype(np.float32)
+        out[act == 0] = 0.0
+        v = out
+    return v

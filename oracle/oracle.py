@@ -89,6 +89,7 @@ class _OracleParams(ctypes.Structure):
         ("fp64_ill_conditioned", i32),
         ("num_threads", i32),
         ("self_contact", vp),
+        ("residual_stop", ctypes.c_double),
     ]
 
 
@@ -290,10 +291,14 @@ class OracleModel:
 
 def params(iterations: int, friction_epsilon: float = 1e-2, avbd_beta: float = 0.0, k_start: float = -1.0,
            use_libm_atan2: bool = False, fp64_ill_conditioned: bool = True, num_threads: int = 1,
-           self_contact: "SelfContact | None" = None) -> _OracleParams:
+           self_contact: "SelfContact | None" = None, residual_stop: float = 0.0) -> _OracleParams:
+    """Solver settings of the oracle.  ``residual_stop`` > 0 ends a step with a residual history once
+    the residual norm is <= ``residual_stop`` x its value at the inertial guess (iterate to a relative
+    tolerance, checked after every iteration); later history entries stay 0."""
     prm = _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
                         int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads),
-                        None if self_contact is None else ctypes.addressof(self_contact.struct))
+                        None if self_contact is None else ctypes.addressof(self_contact.struct),
+                        float(residual_stop))
     prm._self_contact = self_contact  # keeps the settings and their arrays alive
     return prm
 

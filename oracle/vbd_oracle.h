@@ -86,6 +86,7 @@ typedef struct OracleParams {
     int32_t fp64_ill_conditioned; /* 1: solve blocks with cond > 1e6 in double */
     int32_t num_threads;      /* OpenMP threads for the per-color vertex loop (<=1: serial) */
     const struct OracleSelf* self_contact; /* NULL: particle_enable_self_contact = False */
+    double residual_stop;     /* > 0 with a residual history: stop once residual <= residual_stop * initial residual */
 } OracleParams;
 
 /* Cloth self-contact settings (SolverVBD particle_* arguments, solver_vbd.py:231-242) and the

@@ -819,16 +819,25 @@ int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, in
         self_lists_free(&L);
     }
     Ctx c = {m, q, q_prev, dt};
-    double acc = 0.0;
+    REAL* gb = g_out ? g_out : (REAL*)calloc((size_t)N * 3, sizeof(REAL));
+#ifdef _OPENMP
+    int nt = p->num_threads > 1 ? p->num_threads : 1;
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
+#endif
     for (int i = 0; i < N; ++i) {
         V3 f;
         M3 h;
         V3 g = v3(R(0.0), R(0.0), R(0.0));
         if (assemble_vertex(&c, i, inertia, &f, &h)) g = vadd(f, vld(pf, i));
-        if (g_out) vst(g_out, i, g);
+        vst(gb, i, g);
+    }
+    double acc = 0.0; /* serial, in particle order: the same sum for any thread count */
+    for (int i = 0; i < N; ++i) {
+        V3 g = vld(gb, i);
         acc += (double)g.x * (double)g.x + (double)g.y * (double)g.y + (double)g.z * (double)g.z;
     }
     *norm_out = sqrt(acc);
+    if (!g_out) free(gb);
     free(pk);
     free(pf);
     free(ph);
@@ -896,11 +905,13 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
         for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
 
     Ctx c = {m, q, q_prev, dt};
+    int run = p->iterations;
     if (residual_hist) {
         API(vbd_oracle_residual)(m, p, contact_count, contact_max, c_indices, c_shape, c_body_pos, c_body_vel,
                                  c_normal, q, q_prev, inertia, dt, NULL, &residual_hist[0]);
+        if (p->residual_stop > 0.0 && residual_hist[0] <= 0.0) run = 0;
     }
-    for (int it = 0; it < p->iterations; ++it) {
+    for (int it = 0; it < run; ++it) {
         /* self-contact re-detection (S:2540-2545): positions and displacement base reset */
         if (sc && ((sc->detection_interval == 0 && it == 0) ||
                    (sc->detection_interval >= 1 && it % sc->detection_interval == 0))) {
@@ -954,9 +965,10 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
         if (residual_hist) {
             API(vbd_oracle_residual)(m, p, contact_count, contact_max, c_indices, c_shape, c_body_pos,
                                      c_body_vel, c_normal, q, q_prev, inertia, dt, NULL, &residual_hist[it + 1]);
+            if (p->residual_stop > 0.0 && residual_hist[it + 1] <= p->residual_stop * residual_hist[0]) break;
         }
     }
-    if (p->iterations <= 0) memcpy(q_out, q, sizeof(REAL) * 3 * (size_t)N);
+    if (run <= 0) memcpy(q_out, q, sizeof(REAL) * 3 * (size_t)N);
     /* _finalize_particles -> update_velocity (K:1468-1471) */
     for (int i = 0; i < N; ++i) vst(qd_out, i, vdivs(vsub(vld(q_out, i), vld(q_prev, i)), dt));
     if (sc) {

@@ -240,7 +240,7 @@ def test_batched_driver_full_size_4096_worlds(cuda_device):
     contacts = pipe.contacts()
     state.clear_forces()
     pipe.collide(state, contacts)
-    q0 = state.particle_q.clone()
+    q0, qd0, f0 = state.particle_q.clone(), state.particle_qd.clone(), state.particle_f.clone()
     solver = SolverVBD(model, iterations=5)
     s1 = model.state()
     r = solver.step_to_residual_worlds(state, s1, None, contacts, dt, rel_tol=0.5, max_iterations=40)
@@ -253,3 +253,22 @@ def test_batched_driver_full_size_4096_worlds(cuda_device):
     assert torch.isfinite(s1.particle_q).all() and torch.isfinite(s1.particle_qd).all()
     pinned = model.particle_inv_mass == 0
     assert torch.equal(s1.particle_q[pinned], q0[pinned])
+    at_cap = int((wits >= 40).sum())
+    print(f"reached {r['reached']}/4096, at the cap {at_cap}, iterations {wits.min()}..{wits.max()}")
+    # a sample of worlds equals the plain batched run of that world's iteration count, bit for bit
+    ps = info["world_particle_start"]
+    order = np.argsort(wits, kind="stable")
+    sample = sorted({int(order[0]), int(order[len(order) // 2]), int(order[-1])})
+    s2, s3 = model.state(), model.state()
+    for w in sample:
+        k = int(wits[w])
+        s2.particle_q.copy_(q0)
+        s2.particle_qd.copy_(qd0)
+        s2.particle_f.copy_(f0)
+        solver.begin_step(s2, contacts, dt)
+        solver.iterate(s2, dt, k)
+        solver.end_step(s2, s3, dt)
+        torch.cuda.synchronize()
+        sl = slice(int(ps[w]), int(ps[w + 1]))
+        assert torch.equal(s3.particle_q[sl], s1.particle_q[sl]), (w, k)
+        assert torch.equal(s3.particle_qd[sl], s1.particle_qd[sl]), (w, k)

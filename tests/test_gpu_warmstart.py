@@ -260,3 +260,21 @@ def test_trained_network_gains_over_reuse(cuda_device):
     print(f"held-out distance to x*: inertial {e['inertial']:.4e}, y + c {e['reuse']:.4e}, learned {e['learned']:.4e}")
     assert e["reuse"] < e["inertial"]
     assert e["learned"] < 0.9 * e["reuse"], e
+
+
+def test_smooth_matches_oracle(cuda_device):
+    """vbd_ws_smooth on the 128x128 wind cloth's 1-ring (pinned top row inactive), two fields side by
+    side, 64 sweeps: bit-identical to the oracle's float32 restatement."""
+    import torch
+
+    from newton_amd import scenes
+    from newton_amd.warmstart import LearnedWarmStart
+
+    model, state, info = scenes.cloth_wind(cuda_device)
+    ws = LearnedWarmStart(model, info["dt"], smooth_sweeps=64)
+    rng = np.random.default_rng(4)
+    v = rng.normal(0, 1e-4, (model.particle_count, 6)).astype(np.float32)
+    got = ws.smooth(torch.as_tensor(v, device=cuda_device)).cpu().numpy()
+    torch.cuda.synchronize()
+    ref = mo.smooth(ws.nbr_off.cpu().numpy(), ws.nbr.cpu().numpy(), ws._act.cpu().numpy()[:, 0], v, 64)
+    np.testing.assert_array_equal(got, ref)

@@ -117,3 +117,32 @@ def test_forward_shapes_and_zero_last_layer(bf16):
     out, hs, zs = mo.forward(x, mo.pack(P), bf16)
     assert out.shape == (5, 3) and len(hs) == 3 and hs[0].shape == (5, 128)
     np.testing.assert_allclose(out, np.tile([0.5, -1.0, 2.0], (5, 1)))
+
+
+def test_smooth_oracle_is_the_masked_ring_mean_operator():
+    """The oracle's low-pass equals k applications of the dense operator P = D^-1 (I + A M) with the
+    inactive rows zeroed (A: adjacency, M: active mask, D: 1 + degree), in float64 up to float32
+    rounding; it keeps a constant field on a mesh without inactive particles constant and leaves
+    inactive particles at 0."""
+    rng = np.random.default_rng(2)
+    n = 30
+    pairs = {(int(a), int(b)) for a, b in rng.integers(0, n, size=(70, 2)) if a != b}
+    pairs |= {(b, a) for a, b in pairs}
+    adj = [sorted(b for a, b in pairs if a == i) for i in range(n)]
+    off = np.concatenate([[0], np.cumsum([len(x) for x in adj])]).astype(np.int32)
+    nbr = np.array([j for x in adj for j in x], dtype=np.int32)
+    active = (rng.random(n) > 0.2).astype(np.float32)
+    v = rng.normal(size=(n, 6)).astype(np.float32)
+    A = np.zeros((n, n))
+    for i, x in enumerate(adj):
+        A[i, x] = 1.0
+    P = (np.eye(n) + A * active[None, :]) / (1.0 + A.sum(axis=1))[:, None]
+    P[active == 0] = 0.0
+    ref = v.astype(np.float64)
+    for _ in range(5):
+        ref = P @ ref
+    got = mo.smooth(off, nbr, active, v, 5)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
+    assert (got[active == 0] == 0).all()
+    ones = mo.smooth(off, nbr, np.ones(n, np.float32), np.ones((n, 3), np.float32), 7)
+    np.testing.assert_allclose(ones, 1.0, rtol=1e-6)

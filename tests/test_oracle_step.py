@@ -190,3 +190,26 @@ def test_integrate_particles_closed_forms():
     speed = np.linalg.norm(v1.astype(np.float64), axis=1)
     np.testing.assert_allclose(speed, 5.0, rtol=1e-6)
     np.testing.assert_allclose(x1, x + v1.astype(np.float64) * dt, rtol=1e-6, atol=1e-6)
+
+
+def test_residual_stop_ends_at_first_iteration_within_tolerance():
+    """``residual_stop`` (iterate to a relative tolerance): the step stops at the first iteration of
+    the full residual history that is within the tolerance, with the result of a step of exactly
+    that many iterations; the residual norm does not depend on the thread count."""
+    import torch
+
+    model, state, info = scenes.cloth_wind("cpu", verts_per_side=24)
+    om = oracle.OracleModel(model)
+    q0, qd0 = state.particle_q.numpy(), state.particle_qd.numpy()
+    f = info["wind"].force(torch.from_numpy(q0), model.particle_mass).numpy()
+    cap = 80
+    _, _, _, hist = oracle.step(om, oracle.params(cap), q0, qd0, f, None, info["dt"], residual_history=True)
+    k = int(np.nonzero(hist <= 0.7 * hist[0])[0][0])
+    assert 0 < k < cap
+    _, q_s, qd_s, h_s = oracle.step(om, oracle.params(cap, num_threads=4, residual_stop=0.7), q0, qd0, f, None,
+                                    info["dt"], residual_history=True)
+    np.testing.assert_array_equal(h_s[: k + 1], hist[: k + 1])
+    assert (h_s[k + 1:] == 0).all()
+    _, q_k, qd_k = oracle.step(om, oracle.params(k), q0, qd0, f, None, info["dt"])
+    np.testing.assert_array_equal(q_s, q_k)
+    np.testing.assert_array_equal(qd_s, qd_k)
