@@ -686,48 +686,65 @@ def residual_leg(args):
 
 
 def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
-    """``substeps`` substeps of the batched wind cloth from identical states per substep, each driven to
-    relative residual ``tol`` by three solvers: plain VBD one world at a time (every
-    ``single_world_stride``-th world on a single-world solver, time scaled to all worlds), plain VBD on the
-    batch, and the batch with the learned warm start.  The batched warm-started iterate carries on."""
+    """``substeps`` substeps of the batched wind cloth, every world driven to relative residual ``tol``
+    (cap ``max_iterations``) by each solver on its own trajectory from one common start (the headline
+    batch's current state): plain VBD one world at a time (every ``single_world_stride``-th world on
+    a single-world solver, its rate taken as the rate for all worlds), plain VBD on the batch, and
+    the batch with the learned warm start (``ctx["variants"]`` lists the batched solvers; each keeps
+    its own positions, velocities and carried correction).  The winds are the same functions of
+    position for every solver.  Times are wall-clock around each solve, stream synchronised.  The
+    deployed solver's final state becomes the common start of the next call."""
     import numpy as np
     import torch
 
-    model, s0, s1, solver, wsb, wind, info = (ctx[k] for k in ("model", "s0", "s1", "solver", "ws", "wind", "info"))
-    one_state, one_out, one_solver, per, W = (ctx[k] for k in ("one_state", "one_out", "one_solver", "per", "W"))
+    model, wind, info, solver = (ctx[k] for k in ("model", "wind", "info", "solver"))
+    one_solver, per, W = (ctx[k] for k in ("one_solver", "per", "W"))
     dt, sub, mass = info["dt"], info["substeps"], model.particle_mass
+    one_mass = one_solver.model.particle_mass
     cap = args.max_iterations
     sample = list(range(0, W, max(args.single_world_stride, 1)))
-    batched = ctx["variants"] if with_extra else ("plain_batched", "batched_learned")
+    batched = ctx["variants"] if with_extra else ("plain_batched", ctx["deployed"])
+    base = ctx["state"]
+    # per-solver trajectories from the common start
+    traj = {}
+    for name in batched:
+        a, b = model.state(), model.state()
+        a.particle_q.copy_(base.particle_q)
+        a.particle_qd.copy_(base.particle_qd)
+        ws = ctx["protocol"][name].get("warm_start")
+        if ws is not None:
+            ws.corr.copy_(ctx["corr"])
+        traj[name] = [a, b]
+    single = {w: (base.particle_q[w * per:(w + 1) * per].clone(), base.particle_qd[w * per:(w + 1) * per].clone())
+              for w in sample}
+    one_in, one_out = ctx["one_state"], ctx["one_out"]
     res = {k: {"iters": [], "witers": [], "s": 0.0, "reached": 0, "chosen": None}
            for k in ("plain_single_world",) + tuple(batched)}
     for i in range(substeps):
         if i % sub == 0:
             wind.redraw(frame0 + i // sub)
-        q0, qd0, corr0 = s0.particle_q.clone(), s0.particle_qd.clone(), wsb.corr.clone()
-        f_ext = wind.force(s0.particle_q, mass)
         d = res["plain_single_world"]
         for w in sample:
-            sl = slice(w * per, (w + 1) * per)
-            one_state.particle_q.copy_(q0[sl])
-            one_state.particle_qd.copy_(qd0[sl])
-            one_state.particle_f.copy_(f_ext[sl])
+            q, qd = single[w]
+            one_in.particle_q.copy_(q)
+            one_in.particle_qd.copy_(qd)
+            one_in.particle_f.copy_(wind.winds[w].force(q, one_mass))
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = one_solver.step_to_residual(one_state, one_out, None, None, dt, rel_tol=tol, max_iterations=cap)
+            r = one_solver.step_to_residual(one_in, one_out, None, None, dt, rel_tol=tol, max_iterations=cap)
             torch.cuda.synchronize()
             d["s"] += time.perf_counter() - t0
             d["iters"].append(r["iterations"])
             d["reached"] += int(r["r"] <= tol * r["r0"])
+            q.copy_(one_out.particle_q)
+            qd.copy_(one_out.particle_qd)
         for name in batched:
-            s0.particle_q.copy_(q0)
-            s0.particle_qd.copy_(qd0)
-            s0.particle_f.copy_(f_ext)
-            wsb.corr.copy_(corr0)
+            a, b = traj[name]
+            a.particle_f.copy_(wind.force(a.particle_q, mass))
+            prot = ctx["protocol"][name]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=tol, max_iterations=cap,
-                                               **ctx["protocol"][name])
+            r = solver.step_to_residual_worlds(a, b, None, None, dt, rel_tol=tol, max_iterations=cap, **prot)
             torch.cuda.synchronize()
             d = res[name]
             d["s"] += time.perf_counter() - t0
@@ -736,17 +753,22 @@ def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
             d["reached"] += r["reached"]
             if r["candidate_chosen"]:
                 d["chosen"] = r["candidate_chosen"] if d["chosen"] is None else \
-                    [a + b for a, b in zip(d["chosen"], r["candidate_chosen"])]
-        wsb.observe(solver, s0)
-        ctx["s0"], ctx["s1"] = s1, s0
-        s0, s1 = s1, s0
+                    [x + y for x, y in zip(d["chosen"], r["candidate_chosen"])]
+            if prot.get("warm_start") is not None:
+                prot["warm_start"].observe(solver, a)
+            traj[name] = [b, a]
+    # the deployed solver's state carries on
+    dep = ctx["deployed"]
+    ctx["state"].particle_q.copy_(traj[dep][0].particle_q)
+    ctx["state"].particle_qd.copy_(traj[dep][0].particle_qd)
+    ctx["corr"].copy_(ctx["protocol"][dep]["warm_start"].corr)
     n = model.particle_count
-    out = {"rel_tol": tol, "substeps": substeps, "max_iterations": cap}
+    out = {"rel_tol": tol, "substeps": substeps, "max_iterations": cap, "trajectories": "each solver its own"}
     d = res["plain_single_world"]
     single_vs = len(sample) * per * substeps / d["s"]
     out["plain_single_world"] = {"worlds_timed": len(sample), "mean_iterations": round(float(np.mean(d["iters"])), 2),
                                  "reached_tol": f"{d['reached']}/{len(sample) * substeps} world-substeps",
-                                 "vertex_steps_per_s": round(single_vs, 1)}
+                                 "vertex_steps_per_s": round(single_vs, 1), "seconds": round(d["s"], 4)}
     for name in batched:
         d = res[name]
         wit = np.stack(d["witers"])
@@ -758,22 +780,22 @@ def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
                      "vertex_steps_per_s": round(n * substeps / d["s"], 1), "seconds": round(d["s"], 4)}
         if d["chosen"] is not None:
             out[name]["world_substeps_choosing"] = dict(zip(ctx["candidate_names"], d["chosen"]))
-    out["speedup_vs_plain_single_world"] = round(out["batched_learned"]["vertex_steps_per_s"] / single_vs, 3)
-    out["speedup_vs_plain_batched"] = round(res["plain_batched"]["s"] / res["batched_learned"]["s"], 3)
-    out["speedup_batching_only"] = round(out["plain_batched"]["vertex_steps_per_s"] / single_vs, 3)
     for name in batched:
-        if name not in ("plain_batched", "batched_learned"):
+        if name != "plain_batched":
+            out[f"speedup_{name}_vs_plain_single_world"] = round(out[name]["vertex_steps_per_s"] / single_vs, 3)
             out[f"speedup_{name}_vs_plain_batched"] = round(res["plain_batched"]["s"] / res[name]["s"], 3)
+    out["speedup_batching_only"] = round(out["plain_batched"]["vertex_steps_per_s"] / single_vs, 3)
     return out
 
 
 def headline_leg(args, s_single, ws):
     """The headline: the 128x128 wind cloth (pinned top edge, Fourier wind) as ``residual_worlds`` worlds
     in one model, every world driven by its own wind and judged by its own force residual.  Every world
-    starts from the single-world leg's last state; ``warmup`` substeps let the worlds drift apart; then
-    ``steps`` substeps are timed at ``rel_tol`` for plain VBD one world at a time, plain VBD batched and
-    batched + learned warm start (from identical states per substep), then the same at each tolerance
-    of ``tol_sweep``; the color-pass roofline of this model is measured in-frame; and one substep shows
+    starts from the single-world leg's last state; ``warmup`` substeps of the deployed solver let the
+    worlds drift apart; then each solver -- plain VBD one world at a time, plain VBD batched, batched +
+    learned warm start (the deployed protocol) and the alternative protocol -- runs ``steps`` substeps at
+    ``rel_tol`` on its own trajectory from that common state, and again at each tolerance of
+    ``tol_sweep``; the color-pass roofline of this model is measured in-frame; and one substep shows
     where plain VBD's relative residual stops falling (the FP32 floor)."""
     import numpy as np
     import torch
@@ -792,35 +814,39 @@ def headline_leg(args, s_single, ws):
     s0.particle_qd.copy_(s_single.particle_qd.repeat(W, 1))
     solver = SolverVBD(model, iterations=10)
     tune = solver.autotune(s0, None, dt)
-    # the deployed warm start: both candidates low-passed, judged at the start (no trial iterations)
-    wsb = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=args.blend, smooth_sweeps=args.smooth_sweeps)
-    wsb.corr.copy_(ws.corr.repeat(W, 1))
-    # the unsmoothed candidates judged after trial iterations, for comparison (shares the correction buffer)
-    wst = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.5, smooth_sweeps=0)
-    wst.corr = wsb.corr
+    # deployed: y + c_prev and y + c_prev + blend * MLP, judged against the inertial start after
+    # trial_iterations VBD iterations each
+    ws_dep = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.5, smooth_sweeps=0)
+    # alternative: both candidates low-passed (smooth_sweeps of the 1-ring mean), judged at the start
+    ws_alt = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=args.blend, smooth_sweeps=args.smooth_sweeps)
     one_model, one_state, _ = scenes.cloth_wind(dev)
     one_solver = SolverVBD(one_model, iterations=10)
     one_solver.autotune(one_state, None, dt)
-    ctx = {"model": model, "s0": s0, "s1": s1, "solver": solver, "ws": wsb, "wind": wind, "info": info,
+    corr = ws.corr.repeat(W, 1).contiguous()
+    ctx = {"model": model, "solver": solver, "wind": wind, "info": info, "state": s0, "corr": corr,
            "one_state": one_state, "one_out": one_model.state(), "one_solver": one_solver,
            "per": one_model.particle_count, "W": W, "candidate_names": ("reuse", "network"),
-           "variants": ("plain_batched", "batched_learned_trial", "batched_learned"),  # the deployed one last
-           "protocol": {"plain_batched": {}, "batched_learned": {"warm_start": wsb},
-                        "batched_learned_trial": {"warm_start": wst, "trial_iterations": args.trial_iterations}}}
+           "deployed": "batched_learned",
+           "variants": ("plain_batched", "batched_learned", "batched_learned_smoothed"),
+           "protocol": {"plain_batched": {},
+                        "batched_learned": {"warm_start": ws_dep, "trial_iterations": args.trial_iterations},
+                        "batched_learned_smoothed": {"warm_start": ws_alt}}}
     frame = args.train_frames + args.residual_frames
     mass = model.particle_mass
     _progress(f"headline: {W} worlds x {one_model.particle_count} vertices, {args.warmup} warm-up substeps")
+    ws_dep.corr.copy_(corr)
     for i in range(args.warmup):
         if i % sub == 0:
             wind.redraw(frame + i // sub)
-        s0, s1 = ctx["s0"], ctx["s1"]
         s0.particle_f.copy_(wind.force(s0.particle_q, mass))
         solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                       max_iterations=args.max_iterations, warm_start=wsb)
-        wsb.observe(solver, s0)
-        ctx["s0"], ctx["s1"] = s1, s0
+                                       max_iterations=args.max_iterations, **ctx["protocol"]["batched_learned"])
+        ws_dep.observe(solver, s0)
+        s0, s1 = s1, s0
+    ctx["state"] = s0
+    corr.copy_(ws_dep.corr)
     frame += -(-args.warmup // sub)
-    _progress(f"headline: {args.steps} timed substeps at rel_tol {args.rel_tol}")
+    _progress(f"headline: {args.steps} timed substeps at rel_tol {args.rel_tol}, each solver on its own trajectory")
     main = _solve_variants(args, args.rel_tol, args.steps, frame, ctx, with_extra=True)
     frame += -(-args.steps // sub)
     sweep = []
@@ -828,6 +854,7 @@ def headline_leg(args, s_single, ws):
         _progress(f"headline: tolerance sweep, rel_tol {tol}, {args.sweep_substeps} substeps")
         sweep.append(_solve_variants(args, tol, args.sweep_substeps, frame, ctx))
         frame += -(-args.sweep_substeps // sub)
+    ctx["s0"] = ctx["state"]
     # in-frame color-pass roofline of this model (all worlds live, 10 iterations of one substep)
     probe = model.state()
     probe.particle_q.copy_(ctx["s0"].particle_q)
@@ -839,7 +866,7 @@ def headline_leg(args, s_single, ws):
     st = ctx["one_state"]
     st.particle_q.copy_(ctx["s0"].particle_q[:ctx["per"]])
     st.particle_qd.copy_(ctx["s0"].particle_qd[:ctx["per"]])
-    st.particle_f.copy_(wind.force(st.particle_q, mass[:ctx["per"]]))
+    st.particle_f.copy_(wind.winds[0].force(st.particle_q, mass[:ctx["per"]]))
     one_solver.begin_step(st, None, dt)
     r0 = float(one_solver.residual(st, dt).item())
     hist = []
@@ -959,8 +986,8 @@ def main():
     main_rec = head["main"]
     value = main_rec["batched_learned"]["vertex_steps_per_s"]
     ms_per_step = 1e3 * main_rec["batched_learned"]["seconds"] / args.steps
-    _progress(f"headline: {value:.4g} vertex-steps/s ({main_rec['speedup_vs_plain_single_world']}x plain single "
-              f"world, {main_rec['speedup_vs_plain_batched']}x plain batched)")
+    _progress(f"headline: {value:.4g} vertex-steps/s ({main_rec['speedup_batched_learned_vs_plain_single_world']}x "
+              f"plain single world, {main_rec['speedup_batched_learned_vs_plain_batched']}x plain batched)")
     torch.cuda.empty_cache()
 
     fixed, fixed_model, fixed_info = fixed_iterations_leg(args)
@@ -1017,9 +1044,10 @@ def main():
             "rel_tol": args.rel_tol,
             "max_iterations": args.max_iterations,
             "step": "one substep (dt 1/600 s) of every world, each world iterated until r_w <= rel_tol * r0_w",
-            "solver": "batched + learned warm start: per world the inertial start y, y + S(c_prev) and "
-                      "y + S(c_prev + MLP), judged by their force residuals at the start (S: "
-                      f"{args.smooth_sweeps} sweeps of the 1-ring mean; MLP: the BF16 network trained in this run)",
+            "solver": "batched + learned warm start: per world the inertial start y, y + c_prev and "
+                      "y + c_prev + 0.5 MLP (the BF16 network trained in this run), judged by their force "
+                      f"residuals after {args.trial_iterations} VBD iterations each; every solver on its own "
+                      "trajectory from one common state",
             "pass_shape": head["pass_shape"],
         },
         "comparison": main_rec,
