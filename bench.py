@@ -87,6 +87,11 @@ def parse():
                     help="counter runs: color passes of one prepared substep plus a plain 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), then exit")
     ap.add_argument("--pmc-workload", choices=("drop", "batched", "cubes", "springs", "wind"), default="drop")
+    ap.add_argument("--roofline-only", choices=("wind", "drop", "batched", "cubes"), default=None,
+                    help="profiling runs: only the in-frame color-pass roofline of the headline model ('wind', all "
+                         "worlds live, after --warmup substeps of 10 iterations), of the 256x256 cloth after "
+                         "--contact-frame frames ('drop'), or of the 4096-world cloth / 512 tet cubes after "
+                         "--fixed-warmup frames, at a fixed --pass-shape; prints its JSON and exits")
     ap.add_argument("--pass-shape", default="auto",
                     help="color-pass shape of the fixed-iteration and --pmc-mode runs: 'auto' (autotuner) or "
                          "lanes:waves[:aligned], e.g. 256:7:1")
@@ -269,6 +274,47 @@ def pmc_mode(args):
     print(json.dumps({"pmc_mode": True, "workload": info["workload"], "autotune": tune,
                       "color_pass_launches": args.roofline_reps * ncol,
                       "algorithmic_bytes_per_color_pass": [int(x) for x in b], "copy_bytes": nbytes}))
+
+
+def roofline_only(args):
+    """The in-frame roofline alone, for kernel-trace profiles: no autotuning (``--pass-shape`` is
+    required), so the only color-pass launches are the measured ones and one prepared substep's."""
+    import torch
+
+    from newton_amd import CollisionPipeline
+    from newton_amd.solvers import SolverVBD
+
+    if args.pass_shape == "auto":
+        raise SystemExit("--roofline-only needs a fixed --pass-shape lanes:waves:aligned")
+    dev = "cuda:0"
+    model, state, info = _scene(args.roofline_only, args, dev)
+    parts = [int(x) for x in args.pass_shape.split(":")]
+    iters = {"wind": 10, "drop": args.iterations}.get(args.roofline_only, info["iterations"])
+    solver = SolverVBD(model, iterations=iters)
+    solver.set_pass_shape(parts[0], parts[1], parts[2] if len(parts) > 2 else 0)
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts() if pipe.soft_rigid_contact_pair_count else None
+    s0, s1 = state, model.state()
+    if args.roofline_only == "wind":
+        wind = info["wind"]
+        for _ in range(args.warmup):
+            s0.particle_f.copy_(wind.force(s0.particle_q, model.particle_mass))
+            solver.step(s0, s1, None, None, info["dt"])
+            s0, s1 = s1, s0
+        s0.particle_f.copy_(wind.force(s0.particle_q, model.particle_mass))
+    else:
+        frames = args.contact_frame if args.roofline_only == "drop" else args.fixed_warmup
+        for _ in range(frames * info["substeps"]):
+            s0.clear_forces()
+            pipe.collide(s0, contacts)
+            solver.step(s0, s1, None, contacts, info["dt"])
+            s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    roof = inframe_roofline(model, s0, info, solver, pipe if contacts is not None else None, contacts, iters,
+                            args.roofline_reps)
+    kernel = f"k_solve_fast<{parts[0]}, {parts[1]}>"
+    print(json.dumps({"roofline_only": args.roofline_only, "workload": info["workload"], "kernel": kernel,
+                      "pass_shape": args.pass_shape, **roof}))
 
 
 def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
@@ -975,6 +1021,9 @@ def main():
         sys.exit(2)
     if args.pmc_mode:
         pmc_mode(args)
+        return
+    if args.roofline_only:
+        roofline_only(args)
         return
     import torch
 
