@@ -95,7 +95,17 @@ def parse():
     ap.add_argument("--pass-shape", default="auto",
                     help="color-pass shape of the fixed-iteration and --pmc-mode runs: 'auto' (autotuner) or "
                          "lanes:waves[:aligned], e.g. 256:7:1")
-    ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube leg (0 disables)")
+    ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube legs (0 disables)")
+    ap.add_argument("--cube-train-frames", type=int, default=6,
+                    help="frames of plain-VBD tet-cube solves that train the cube warm start (0 disables the leg)")
+    ap.add_argument("--cube-train-steps", type=int, default=300)
+    ap.add_argument("--cube-warm-frames", type=int, default=30,
+                    help="frames of the configured loop before the cube warm start's training data")
+    ap.add_argument("--cube-eval-substeps", type=int, default=30)
+    ap.add_argument("--cube-trial-iterations", type=int, default=0,
+                    help="tet-cube warm start: iterations each start runs before the residual judges it")
+    ap.add_argument("--cube-rel-tol", type=float, default=1e-3,
+                    help="relative force residual of the tet-cube warm-start leg (mass-dominated blocks converge fast)")
     ap.add_argument("--self-contact-steps", type=int, default=10,
                     help="timed frames of the stacked-sheets self-contact drape (0 disables)")
     ap.add_argument("--self-contact-warm", type=int, default=20, help="frames before timing (sheets land)")
@@ -931,6 +941,134 @@ def headline_leg(args, s_single, ws):
             "main": main, "tolerance_sweep": sweep, "roofline": roof, "fp32_floor": floor}, ctx
 
 
+def cubes_residual_leg(args, ws_cloth):
+    """Batched tet cubes with the learned warm start: the 512 cubes (6..12 cells, ground contact) as one
+    model, every world judged by its own force residual (relative ``cube_rel_tol``, floored at 1 % of
+    the world's weight norm, since a free-falling unstressed cube starts at round-off).  The MLP starts
+    from the cloth network and is retrained on the GPU on tet-mesh 1-ring features from
+    ``cube_train_frames`` frames of plain-VBD solves (targets: ``train_iterations`` iterations); then
+    plain VBD batched and batched + learned warm start (the deployed protocol) each run
+    ``cube_eval_substeps`` substeps on their own trajectory from one common state.  Training starts
+    after ``cube_warm_frames`` frames of the configured loop, when the cubes reach the ground."""
+    import numpy as np
+    import torch
+
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.solvers import SolverVBD
+    from newton_amd.warmstart import LearnedWarmStart, WarmStartMLP
+
+    dev = "cuda:0"
+    model, s0, info = scenes.soft_cubes_batched(dev, worlds=args.cube_worlds)
+    dt, sub, W = info["dt"], info["substeps"], model.world_count
+    n = model.particle_count
+    solver = SolverVBD(model, iterations=info["iterations"])
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01)
+    contacts = pipe.contacts()
+    s0.clear_forces()
+    pipe.collide(s0, contacts)
+    tune = solver.autotune(s0, contacts, dt)
+    mlp = WarmStartMLP(dev, seed=0)
+    mlp.load_params(ws_cloth.mlp.params.cpu().numpy())
+    ws = LearnedWarmStart(model, dt, mlp=mlp, seed=0, blend=0.5)
+    ps = info["world_particle_start"]
+    w_of = np.repeat(np.arange(W), np.diff(ps))
+    m = model.particle_mass.cpu().numpy().astype(np.float64)
+    abs_tol = 1e-2 * 9.81 * np.sqrt(np.bincount(w_of, weights=m * m, minlength=W))
+    tol = args.cube_rel_tol
+    s1 = model.state()
+    # the configured loop (8 iterations) until the cubes have reached the ground (the drops take up
+    # to ~33 frames): training and evaluation then see landings, bounces and resting contact
+    for _ in range(args.cube_warm_frames * sub):
+        s0.clear_forces()
+        pipe.collide(s0, contacts)
+        solver.step(s0, s1, None, contacts, dt)
+        s0, s1 = s1, s0
+    feats, targs = [], []
+    t0 = time.perf_counter()
+    for frame in range(args.cube_train_frames):
+        for _ in range(sub):
+            s0.clear_forces()
+            pipe.collide(s0, contacts)
+            solver.begin_step(s0, contacts, dt)
+            feats.append(ws.features(solver, s0).clone())
+            r0 = solver.residual_worlds(s0, dt).cpu().numpy()
+            target = np.maximum(tol * tol * r0, abs_tol * abs_tol)
+            solver.iterate_to_residual_worlds(s0, dt, target, args.max_iterations)
+            x_tol, d_tol = s0.particle_q.clone(), solver.scratch("displacement")
+            solver.iterate(s0, dt, args.train_iterations // 2)
+            targs.append(ws.target(solver, s0))
+            s0.particle_q.copy_(x_tol)
+            solver.set_displacement(d_tol)
+            ws.observe(solver, s0)
+            solver.end_step(s0, s1, dt)
+            s0, s1 = s1, s0
+        _progress(f"cubes residual: training data frame {frame + 1}/{args.cube_train_frames}")
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    gen = torch.Generator(device="cpu")
+    gen.manual_seed(1)
+    t0 = time.perf_counter()
+    first = last = None
+    for step in range(args.cube_train_steps):
+        k = int(torch.randint(0, len(feats), (1,), generator=gen))
+        lr = max(1e-3 * 0.5 * (1.0 + math.cos(math.pi * step / args.cube_train_steps)), 1e-5)
+        loss = ws.train_substep(feats[k], targs[k], lr=lr, smooth=args.train_smooth)
+        if step == 0:
+            first = float(loss.item())
+    last = float(loss.item()) if args.cube_train_steps else None
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del feats, targs
+    # evaluation: each solver on its own trajectory from the common state
+    variants = {"plain_batched": {},
+                "batched_learned": {"warm_start": ws, "trial_iterations": args.cube_trial_iterations}}
+    base_q, base_qd, base_corr = s0.particle_q.clone(), s0.particle_qd.clone(), ws.corr.clone()
+    out = {"workload": info["workload"], "worlds": W, "vertices": n, "tets": model.tet_count, "rel_tol": tol,
+           "abs_floor": "1 % of each world's weight norm", "max_iterations": args.max_iterations,
+           "substeps": args.cube_eval_substeps, "first_frame": args.cube_warm_frames + args.cube_train_frames,
+           "pass_shape": {k: tune[k] for k in ("block", "waves", "aligned")}, "trajectories": "each solver its own"}
+    for name, prot in variants.items():
+        a, b = model.state(), model.state()
+        a.particle_q.copy_(base_q)
+        a.particle_qd.copy_(base_qd)
+        ws.corr.copy_(base_corr)
+        c = pipe.contacts()
+        secs, witers, reached, chosen = 0.0, [], 0, None
+        for _ in range(args.cube_eval_substeps):
+            a.clear_forces()
+            pipe.collide(a, c)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = solver.step_to_residual_worlds(a, b, None, c, dt, rel_tol=tol, max_iterations=args.max_iterations,
+                                               abs_tol=abs_tol, **prot)
+            torch.cuda.synchronize()
+            secs += time.perf_counter() - t0
+            witers.append(r["world_iterations"])
+            reached += r["reached"]
+            if r["candidate_chosen"]:
+                chosen = r["candidate_chosen"] if chosen is None else [x + y for x, y in zip(chosen,
+                                                                                             r["candidate_chosen"])]
+            if prot:
+                ws.observe(solver, a)
+            a, b = b, a
+        wit = np.stack(witers)
+        out[name] = {"mean_world_iterations": round(float(wit.mean()), 2),
+                     "mean_slowest_world_iterations": round(float(wit.max(axis=1).mean()), 2),
+                     "world_substeps_at_cap": int((wit >= args.max_iterations).sum()),
+                     "reached_tol": f"{reached}/{W * args.cube_eval_substeps} world-substeps",
+                     "vertex_steps_per_s": round(n * args.cube_eval_substeps / secs, 1), "seconds": round(secs, 4),
+                     "finite": bool(torch.isfinite(a.particle_q).all().item())}
+        if chosen is not None:
+            out[name]["world_substeps_choosing"] = dict(zip(("reuse", "network"), chosen))
+    out["speedup_batched_learned_vs_plain_batched"] = round(out["plain_batched"]["seconds"] /
+                                                           out["batched_learned"]["seconds"], 3)
+    out["training"] = {"frames": args.cube_train_frames, "substeps": args.cube_train_frames * sub,
+                       "target_iterations": args.train_iterations // 2, "steps": args.cube_train_steps,
+                       "init": "the cloth network", "loss_first": first, "loss_last": last,
+                       "data_s": round(t_data, 2), "train_s": round(t_train, 2)}
+    return out
+
+
 class _trial:  # noqa: N801  (positions-only state for a residual evaluation)
     def __init__(self, q):
         self.particle_q = q
@@ -1058,6 +1196,12 @@ def main():
             legs[name] = batched_leg(args, {"batched_tets": "cubes"}.get(name, name), steps)
             _progress(f"{name}: {legs[name]['vertex_steps_per_s']:.4g} vertex-steps/s")
             torch.cuda.empty_cache()
+    if args.cube_worlds > 0 and args.cube_train_frames > 0:
+        _progress("tet cubes with the learned warm start")
+        legs["batched_tets_residual"] = cubes_residual_leg(args, ws)
+        _progress(f"tet cubes: learned {legs['batched_tets_residual']['speedup_batched_learned_vs_plain_batched']}x "
+                  "plain batched")
+        torch.cuda.empty_cache()
     if args.self_contact_steps > 0:
         _progress("self-contact leg: stacked sheets on a sphere")
         legs["self_contact"] = self_contact_leg(args)

@@ -606,9 +606,12 @@ class SolverVBD(SolverBase):
         batch of frozen vertices costs one header read), so each world's result is the iterate a solve
         of that world alone stops at; the batch runs until all worlds are frozen (or
         ``max_iterations``).  ``world_iterations`` reports each world's count.  Warm-start candidates are
-        chosen per world: after ``trial_iterations`` iterations from each start (0: at the start
+        chosen per world: after up to ``trial_iterations`` iterations from each start (0: at the start
         itself), each world keeps the start with its lowest residual (a candidate must beat the
-        inertial start strictly); the mixed iterate then restarts through :meth:`set_iterate`.
+        inertial start strictly) and continues from that start's iterate.  The trial iterations are the
+        first iterations of an ordinary per-world solve: a world within its tolerance stops there (a
+        world whose inertial start is already within it runs no trial iterations), and each world's
+        count includes the trial iterations every start ran for it.
 
         ``abs_tol`` (a norm in force units, scalar or one per world) floors each world's target at
         ``abs_tol**2``: a world whose inertial guess is already at round-off (free fall of an unstressed
@@ -618,9 +621,12 @@ class SolverVBD(SolverBase):
         self.begin_step(state_in, contacts, dt)
         r0 = self.residual_worlds(state_in, dt).cpu().numpy()
         nw = r0.size
+        target = np.maximum((rel_tol * rel_tol) * r0, np.square(np.broadcast_to(np.asarray(abs_tol, np.float64),
+                                                                                  (nw,))))
         n_cand = 0
         chosen = np.full(nw, -1, dtype=np.int64)
         trial_its = 0
+        trial_world = np.zeros(nw, dtype=np.int64)
         if warm_start is not None:
             guesses = warm_start(self, state_in, dt)
             if guesses is not None and not isinstance(guesses, (list, tuple)):
@@ -630,23 +636,31 @@ class SolverVBD(SolverBase):
             if guesses:
                 wp = self._world_of_particle
                 if trial_iterations > 0:
-                    # each start runs k iterations; its iterate is kept for the worlds it wins
-                    self.iterate(state_in, dt, trial_iterations)
-                    best_r = self.residual_worlds(state_in, dt).cpu().numpy()
+                    # each start runs up to k iterations of the per-world solve; its iterate is kept
+                    # for the worlds it wins
+                    its, best_r, wits = self.iterate_to_residual_worlds(state_in, dt, target, trial_iterations,
+                                                                        check_every, per_world=True)
+                    trial_its += its
+                    trial_world += wits
                     best_q = state_in.particle_q.clone()
                     best_d = self.scratch("displacement")
+                    # worlds the inertial start already brought within tolerance keep it; the candidates
+                    # are tried on the others only (an infinite target freezes a world at once)
+                    done = best_r <= target
+                    cand_target = np.where(done, np.inf, target)
                     for k, g in enumerate(guesses):
                         self.set_iterate(state_in, g, commit=True)
-                        self.iterate(state_in, dt, trial_iterations)
-                        r = self.residual_worlds(state_in, dt).cpu().numpy()
-                        better = r < best_r
+                        its, r, wits = self.iterate_to_residual_worlds(state_in, dt, cand_target, trial_iterations,
+                                                                       check_every, per_world=True)
+                        trial_its += its
+                        trial_world += wits
+                        better = (r < best_r) & ~done
                         if better.any():
                             best_r = np.where(better, r, best_r)
                             chosen[better] = k
                             take = torch.as_tensor(better, device=wp.device)[wp][:, None]
                             best_q = torch.where(take, state_in.particle_q, best_q)
                             best_d = torch.where(take, self.scratch("displacement"), best_d)
-                    trial_its = trial_iterations * (1 + n_cand)
                     # every world continues from its winning start's iterate exactly (positions and
                     # displacement as saved; a world that kept the inertial start equals a plain solve)
                     state_in.particle_q.copy_(best_q)
@@ -672,13 +686,11 @@ class SolverVBD(SolverBase):
                         # worlds that keep the inertial start keep its displacement bit for bit
                         keep = torch.as_tensor(chosen < 0, device=wp.device)[wp][:, None]
                         self.set_displacement(torch.where(keep, d_inertial, self.scratch("displacement")))
-        target = np.maximum((rel_tol * rel_tol) * r0, np.square(np.broadcast_to(np.asarray(abs_tol, np.float64),
-                                                                                  (nw,))))
         its, r_sq, wits = self.iterate_to_residual_worlds(state_in, dt, target,
-                                                          max(max_iterations - trial_iterations, 0) if trial_its else
-                                                          max_iterations, check_every, per_world=True)
+                                                          max(max_iterations - trial_iterations, 0) if trial_iterations
+                                                          and n_cand else max_iterations, check_every, per_world=True)
         self.end_step(state_in, state_out, dt)
-        return {"iterations": its + trial_its, "world_iterations": wits + trial_its, "r0": np.sqrt(r0),
+        return {"iterations": its + trial_its, "world_iterations": wits + trial_world, "r0": np.sqrt(r0),
                 "r": np.sqrt(r_sq),
                 "reached": int((r_sq <= target).sum()), "worlds": nw,
                 "candidate_chosen": [int((chosen == k).sum()) for k in range(n_cand)]}

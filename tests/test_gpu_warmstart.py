@@ -124,10 +124,11 @@ def test_adam_matches_oracle(cuda_device):
         np.testing.assert_allclose(mlp.m.cpu().numpy(), m, rtol=1e-6, atol=1e-12)
 
 
-@pytest.mark.parametrize("verts", [12, 128])
+@pytest.mark.parametrize("verts", [12, 128, "cubes"])
 def test_features_match_oracle(cuda_device, verts):
-    """Features of the wind cloth, also at its stated size (128 x 128, 16,384 particles); then the
-    MLP's forward pass over all of that cloth's rows against the oracle's."""
+    """Features of the wind cloth, also at its stated size (128 x 128, 16,384 particles), and of 24
+    tet cubes (the batched tet workload's 1-ring: tet and surface-triangle edges, tet rest volumes);
+    then the MLP's forward pass over all rows of the 128 x 128 cloth against the oracle's."""
     import torch
 
     from newton_amd import _native
@@ -135,7 +136,11 @@ def test_features_match_oracle(cuda_device, verts):
     from newton_amd.warmstart import vertex_neighbours
     from newton_amd import scenes
 
-    model, state, info = scenes.cloth_wind(cuda_device, verts_per_side=verts)
+    if verts == "cubes":
+        model, state, info = scenes.soft_cubes_batched(cuda_device, worlds=24, cells_min=2, cells_max=5)
+        assert model.tet_count > 0
+    else:
+        model, state, info = scenes.cloth_wind(cuda_device, verts_per_side=verts)
     off, nbr = vertex_neighbours(model)
     n = model.particle_count
     rng = np.random.default_rng(11)
@@ -144,7 +149,13 @@ def test_features_match_oracle(cuda_device, verts):
     corr = rng.normal(0, 3e-5, x0.shape).astype(np.float32)
     acc = rng.normal(0, 2, x0.shape).astype(np.float32)
     mass = model.particle_mass.cpu().numpy()
-    area = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    if verts == "cubes":
+        from newton_amd.warmstart import vertex_areas
+
+        area = vertex_areas(model)
+        assert (area > 0).all()
+    else:
+        area = rng.uniform(0.5, 1.5, n).astype(np.float32)
     scales = np.array([1.0 / 600, 9.81 / 360000, 0.09, 1e-3, 7e-3], np.float32)
     dt = 1.0 / 600
     dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=cuda_device)  # noqa: E731
