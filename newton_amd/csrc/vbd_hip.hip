@@ -13,7 +13,7 @@
 //   k_solve_generic    the same for vertices with more than 256 elements (CSR walk)
 //   k_finalize         velocity update   (particle_vbd_kernels.py:1468-1471)
 //   k_integrate_particles  SolverBase.integrate_particles, semi-implicit Euler (solvers/solver.py:20-56)
-//   k_collide_*        particle vs static plane/sphere soft contacts (geometry/kernels.py:1020-1184)
+//   k_collide_tile_*   particle vs static plane/sphere soft contacts (geometry/kernels.py:1020-1184)
 //
 // Numerics: every FP32 expression follows the reference's CPU kernel order with
 // no FMA contraction (built with -ffp-contract=off), IEEE division and square
@@ -669,9 +669,7 @@ struct KParams {
     // worlds with w_active[p_world[v]] == 0 are neither solved nor re-evaluated
     const int* p_world;
     const int* w_active;
-    // per-particle contact lists (contact-index order) and per-contact state
-    const int* c_off;
-    const int* c_ids;
+    // per-contact state
     const float* c_pk;
     const float* c_kd;
     const float* c_mu;
@@ -679,7 +677,7 @@ struct KParams {
     const float* c_body_pos;
     const float* c_body_vel;
     const float* c_normal;
-    // contacts packed per color position (k_contact_pack)
+    // contacts packed per color position (k_contact_sort_pack)
     const int* pos_coff;
     const float* cpack;
     const float* shape_margin;
@@ -1156,18 +1154,18 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
         if (active) {
             V3 cf = v3(0.0f, 0.0f, 0.0f);
             M3 ch = mzero();
-            const int c0 = P.c_off[v], c1 = P.c_off[v + 1];
+            // the vertex's contacts as packed at its color position (contact-index order)
+            const int vpos = P.slow_pos[slow_base + blockIdx.x];
+            const int c0 = P.pos_coff[vpos], c1 = P.pos_coff[vpos + 1];
             if (c1 > c0) {
                 V3 x = ld3(q, v), xp = ld3(P.q_prev, v);
-                float rad = P.radius[v];
                 for (int j = c0; j < c1; ++j) {
-                    int cid = P.c_ids[j];
-                    int sh = P.c_shape[cid];
-                    float margin = P.shape_count > 0 ? P.shape_margin[sh] : 0.0f;
+                    const float4* r = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)j;
+                    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
                     V3 fc;
                     M3 hc;
-                    eval_contact(x, xp, ld3(P.c_body_pos, cid), ld3(P.c_body_vel, cid), ld3(P.c_normal, cid), rad,
-                                 margin, P.c_pk[cid], P.c_kd[cid], P.c_mu[cid], P.friction_epsilon, dt, fc, hc);
+                    eval_contact(x, xp, v3(r1.w, r2.x, r2.y), v3(r2.z, r2.w, r3.x), v3(r3.y, r3.z, r3.w), r1.z, r1.y,
+                                 r0.z, r0.w, r1.x, P.friction_epsilon, dt, fc, hc);
                     cf = add(cf, fc);
                     ch = madd(ch, hc);
                 }
@@ -1268,13 +1266,15 @@ __global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
     if (i < n) p[i] = value;
 }
 
-// init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + per-particle counts
-__global__ void k_contact_init(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
-                               const int* __restrict__ c_shape, float soft_ke, float soft_kd, float soft_mu,
-                               const float* __restrict__ shape_ke, const float* __restrict__ shape_kd,
-                               const float* __restrict__ shape_mu, float k_start, float* __restrict__ pk,
-                               float* __restrict__ mke, float* __restrict__ mkd, float* __restrict__ mmu,
-                               int* __restrict__ cnt, int n_particles, int n_shapes) {
+// init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + counts per color position of the
+// contact's particle (pos_cnt must be zero on entry; k_contact_place leaves it zero again)
+__global__ void k_contact_init_pos(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
+                                   const int* __restrict__ c_shape, float soft_ke, float soft_kd, float soft_mu,
+                                   const float* __restrict__ shape_ke, const float* __restrict__ shape_kd,
+                                   const float* __restrict__ shape_mu, float k_start, float* __restrict__ pk,
+                                   float* __restrict__ mke, float* __restrict__ mkd, float* __restrict__ mmu,
+                                   const int* __restrict__ pos_of, int* __restrict__ pos_cnt, int n_particles,
+                                   int n_shapes) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
@@ -1295,13 +1295,15 @@ __global__ void k_contact_init(const int* __restrict__ count_ptr, int cmax, cons
     mmu[i] = mu;
     pk[i] = k_start < 0.0f ? ke : fminf(k_start, ke);
     int p = c_indices[3 * i];
-    if (c_indices[3 * i + 1] < 0 && p >= 0 && p < n_particles) atomicAdd(&cnt[p], 1);
+    if (c_indices[3 * i + 1] < 0 && p >= 0 && p < n_particles) atomicAdd(&pos_cnt[pos_of[p]], 1);
 }
 
-// same record filter as k_contact_init, so each particle's fill count ends at its counted total
-__global__ void k_contact_scatter(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
-                                  const int* __restrict__ c_shape, const int* __restrict__ off, int* __restrict__ fillc,
-                                  int* __restrict__ ids, int n_particles, int n_shapes) {
+// Places every counted contact id in its color position's range (any order within the range) by
+// counting the position's counter back down to zero, ready for the next substep.
+__global__ void k_contact_place(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
+                                const int* __restrict__ c_shape, const int* __restrict__ pos_of,
+                                const int* __restrict__ pos_coff, int* __restrict__ pos_cnt, int* __restrict__ ids,
+                                int n_particles, int n_shapes) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
@@ -1310,15 +1312,25 @@ __global__ void k_contact_scatter(const int* __restrict__ count_ptr, int cmax, c
     if (p < 0 || p >= n_particles) return;
     int sh = c_shape[i];
     if (sh < 0 || sh >= n_shapes) return;
-    int slot = atomicAdd(&fillc[p], 1);
-    ids[off[p] + slot] = i;
+    const int pos = pos_of[p];
+    const int slot = atomicSub(&pos_cnt[pos], 1) - 1;
+    ids[pos_coff[pos] + slot] = i;
 }
 
-// sort each particle's contact ids ascending (the serial accumulation order)
-__global__ void k_contact_sort(int n, const int* __restrict__ off, int* __restrict__ ids) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int b = off[p], e = off[p + 1];
+// Sorts each color position's contact ids ascending (the serial accumulation order) and packs the
+// records there, so a batch's contacts are one contiguous, vertex-ordered range.
+__global__ void k_contact_sort_pack(int n, const int* __restrict__ color_particles, const int* __restrict__ vert_rec,
+                                    const int* __restrict__ pos_coff, int* __restrict__ ids,
+                                    const float* __restrict__ pk, const float* __restrict__ mkd,
+                                    const float* __restrict__ mmu, const int* __restrict__ c_shape,
+                                    const float* __restrict__ body_pos, const float* __restrict__ body_vel,
+                                    const float* __restrict__ normal, const float* __restrict__ shape_margin,
+                                    int shape_count, const float* __restrict__ radius, float* __restrict__ cpack,
+                                    int* __restrict__ cid_to_pack) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int b = pos_coff[k], e = pos_coff[k + 1];
+    if (e == b) return;
     for (int i = b + 1; i < e; ++i) {
         int key = ids[i];
         int j = i - 1;
@@ -1328,49 +1340,18 @@ __global__ void k_contact_sort(int n, const int* __restrict__ off, int* __restri
         }
         ids[j + 1] = key;
     }
-}
-
-// contacts per color position (input of the position-order scan)
-__global__ void k_pos_contact_count(int n, const int* __restrict__ color_particles, const int* __restrict__ c_off,
-                                    int* __restrict__ pos_cnt) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > n) return;
-    if (k == n) {
-        pos_cnt[n] = 0;
-        return;
-    }
-    int v = color_particles[k];
-    pos_cnt[k] = c_off[v + 1] - c_off[v];
-}
-
-// Packs every particle's contacts (in contact-index order) at its color
-// position, so a batch's contacts are one contiguous, vertex-ordered range.
-__global__ void k_contact_pack(int n, const int* __restrict__ color_particles, const int* __restrict__ vert_rec,
-                               const int* __restrict__ c_off, const int* __restrict__ c_ids,
-                               const int* __restrict__ pos_coff, const float* __restrict__ pk,
-                               const float* __restrict__ mkd, const float* __restrict__ mmu,
-                               const int* __restrict__ c_shape, const float* __restrict__ body_pos,
-                               const float* __restrict__ body_vel, const float* __restrict__ normal,
-                               const float* __restrict__ shape_margin, int shape_count,
-                               const float* __restrict__ radius, float* __restrict__ cpack,
-                               int* __restrict__ cid_to_pack) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int v = color_particles[k];
-    int c0 = c_off[v], c1 = c_off[v + 1];
-    if (c1 == c0) return;
-    int base = pos_coff[k];
-    int local = vert_rec[4 * k + 3] & 0xffff;
-    float rad = radius[v];
-    for (int j = 0; j < c1 - c0; ++j) {
-        int cid = c_ids[c0 + j];
-        int sh = c_shape[cid];
-        float4* o = reinterpret_cast<float4*>(cpack) + 4 * (size_t)(base + j);
+    const int v = color_particles[k];
+    const int local = vert_rec[4 * k + 3] & 0xffff;
+    const float rad = radius[v];
+    for (int j = b; j < e; ++j) {
+        const int cid = ids[j];
+        const int sh = c_shape[cid];
+        float4* o = reinterpret_cast<float4*>(cpack) + 4 * (size_t)j;
         o[0] = make_float4(__int_as_float(v), __int_as_float(local), pk[cid], mkd[cid]);
         o[1] = make_float4(mmu[cid], shape_count > 0 ? shape_margin[sh] : 0.0f, rad, body_pos[3 * cid]);
         o[2] = make_float4(body_pos[3 * cid + 1], body_pos[3 * cid + 2], body_vel[3 * cid], body_vel[3 * cid + 1]);
         o[3] = make_float4(body_vel[3 * cid + 2], normal[3 * cid], normal[3 * cid + 1], normal[3 * cid + 2]);
-        cid_to_pack[cid] = base + j;
+        cid_to_pack[cid] = j;
     }
 }
 
@@ -1567,36 +1548,11 @@ __device__ ContactGeo soft_contact_geometry(const ColParams& C, const float* __r
     return g;
 }
 
-__global__ void k_collide_flag(ColParams C, const float* __restrict__ q, float margin, int* __restrict__ flag) {
-    int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= C.P) return;
-    flag[tid] = soft_contact_geometry(C, q, margin, tid).hit ? 1 : 0;
-}
-
-__global__ void k_collide_write(ColParams C, const float* __restrict__ q, float margin, const int* __restrict__ flag,
-                                const int* __restrict__ index, VbdContactsOut out) {
-    int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= C.P) return;
-    if (tid == C.P - 1) *out.soft_contact_count = index[tid] + flag[tid];
-    if (!flag[tid]) return;
-    int k = index[tid];
-    if (k >= out.soft_contact_max) return;
-    ContactGeo g = soft_contact_geometry(C, q, margin, tid);
-    out.soft_contact_shape[k] = g.shape;
-    st3(out.soft_contact_body_pos, k, g.body_pos);
-    st3(out.soft_contact_body_vel, k, g.body_vel);
-    if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
-    out.soft_contact_indices[3 * k] = g.particle;
-    out.soft_contact_indices[3 * k + 1] = -1;
-    out.soft_contact_indices[3 * k + 2] = -1;
-    if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, v3(1.0f, 0.0f, 0.0f));
-    st3(out.soft_contact_normal, k, g.normal);
-}
-
 // ---- soft-contact generation from per-tile hit counts ------------------------
-// The same records in the same order as k_collide_flag / scan / k_collide_write (stable compaction
-// in pair order), with no per-pair flag or index array: a tile of kColTile pairs is one 256-lane
-// work-group, four rounds of 256 consecutive pairs; a round's hits are ranked with a wave ballot.
+// Records in pair order (a stable compaction of the hitting pairs; the reference's atomic counter
+// gives an arbitrary order), with no per-pair flag or index array: a tile of kColTile pairs is one
+// 256-lane work-group, four rounds of 256 consecutive pairs; a round's hits are ranked with a wave
+// ballot; one work-group scans the tile counts.
 constexpr int kColTile = 1024;
 constexpr int kColRounds = kColTile / 256;
 
@@ -1708,9 +1664,8 @@ struct VbdContext {
     bool use_list = false;
     KParams kp;
     float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
-    int* c_cnt = nullptr;       // [N+1]
-    int* c_off = nullptr;       // [N+1]
-    int* c_ids = nullptr;       // [cap]
+    int* c_cnt = nullptr;       // [N+1] contacts per color position; zero between substeps
+    int* c_ids = nullptr;       // [cap] contact ids in color-position order
     float* c_state = nullptr;   // pk | mke | mkd | mmu, each [cap]
     int* pos_coff = nullptr;    // [N+1]
     float* cpack = nullptr;     // [cap * 16]
@@ -1767,10 +1722,7 @@ struct VbdContext {
 struct VbdCollide {
     VbdCollideDesc desc;
     ColParams cp;
-    int* flag = nullptr;
-    int* index = nullptr;
-    void* cub_tmp = nullptr;
-    size_t cub_bytes = 0;
+    int* tiles = nullptr;  // hit count per tile of kColTile pairs, then its exclusive scan
 };
 
 namespace {
@@ -2248,6 +2200,19 @@ std::string noalloc_scratch(int32_t** out) {
     return std::string();
 }
 
+// Soft contacts in pair order from per-tile hit counts (count, scan of the tile counts, write).
+int32_t launch_collide_tiles(const ColParams& cp, hipStream_t s, const float* q, float margin, int* tiles_dev,
+                             const VbdContactsOut& out) {
+    const int tiles = blocks_for(cp.P, kColTile);
+    hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev);
+    VBD_LAUNCH_CHECK("k_collide_tile_count");
+    hipLaunchKernelGGL(k_collide_tile_scan, dim3(1), dim3(1024), 0, s, tiles, tiles_dev, out.soft_contact_count);
+    VBD_LAUNCH_CHECK("k_collide_tile_scan");
+    hipLaunchKernelGGL(k_collide_tile_write, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev, out);
+    VBD_LAUNCH_CHECK("k_collide_tile_write");
+    return VBD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2298,7 +2263,6 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     const size_t cap = (size_t)c->capacity;
     if ((e = hipMalloc(&c->scratch3, sizeof(float) * 9 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_cnt, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->c_off, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->pos_coff, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_ids, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->cid_to_pack, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
@@ -2315,12 +2279,14 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipHostMalloc(&c->r_host, sizeof(double))) != hipSuccess) return cleanup("hipHostMalloc", e);
     if ((e = hipHostMalloc(&c->w_host, sizeof(int32_t) * kWorldPinned)) != hipSuccess)
         return cleanup("hipHostMalloc", e);
-    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->c_off, N + 1)) != hipSuccess)
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, c->cub_bytes, c->c_cnt, c->pos_coff, N + 1)) != hipSuccess)
         return cleanup("scan sizing", e);
     if ((e = hipMalloc(&c->cub_tmp, c->cub_bytes > 0 ? c->cub_bytes : 16)) != hipSuccess)
         return cleanup("hipMalloc", e);
     // no contacts until vbd_contact_prepare: every particle's list is empty
-    if ((e = hipMemset(c->c_off, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
+    // per-color-position contact counters: zero between substeps (vbd_contact_prepare counts them
+    // up and back down to zero)
+    if ((e = hipMemset(c->c_cnt, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipMemset(c->pos_coff, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipMemset(c->cid_to_pack, 0xff, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipDeviceSynchronize()) != hipSuccess) return cleanup("sync", e);
@@ -2357,8 +2323,6 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.q_prev = c->scratch3;
     k.inertia = c->scratch3 + 3 * (size_t)N;
     k.disp = c->scratch3 + 6 * (size_t)N;
-    k.c_off = c->c_off;
-    k.c_ids = c->c_ids;
     k.c_pk = c->c_state;
     k.c_kd = c->c_state + 2 * cap;
     k.c_mu = c->c_state + 3 * cap;
@@ -2391,7 +2355,6 @@ int32_t vbd_destroy(VbdContext* c) {
     if (!c) return VBD_OK;
     hipFree(c->scratch3);
     hipFree(c->c_cnt);
-    hipFree(c->c_off);
     hipFree(c->pos_coff);
     hipFree(c->c_ids);
     hipFree(c->cid_to_pack);
@@ -2486,8 +2449,6 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
     const int N = c->model.particle_count;
     if (!cd || cd->soft_contact_max <= 0) {
         c->have_contacts = false;
-        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_off, 0, N + 1);
-        VBD_LAUNCH_CHECK("k_fill_i32");
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->pos_coff, 0, N + 1);
         VBD_LAUNCH_CHECK("k_fill_i32");
         return launch_batch_contacts(c, s);
@@ -2509,40 +2470,29 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
     float* mkd = c->c_state + 2 * (size_t)cap;
     float* mmu = c->c_state + 3 * (size_t)cap;
     const int cb = blocks_for(cd->soft_contact_max, 256);
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_cnt, 0, N + 1);
-    VBD_LAUNCH_CHECK("k_fill_i32");
-    hipLaunchKernelGGL(k_contact_init, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
+    // counts per color position (c_cnt is all zero between substeps: k_contact_place counts it back down)
+    hipLaunchKernelGGL(k_contact_init_pos, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
                        cd->soft_contact_indices, cd->soft_contact_shape, c->model.soft_contact_ke,
                        c->model.soft_contact_kd, c->model.soft_contact_mu, c->model.shape_material_ke,
                        c->model.shape_material_kd, c->model.shape_material_mu, c->params.k_start, pk, mke, mkd, mmu,
-                       c->c_cnt, N, c->model.shape_count);
-    VBD_LAUNCH_CHECK("k_contact_init");
+                       c->pos_of, c->c_cnt, N, c->model.shape_count);
+    VBD_LAUNCH_CHECK("k_contact_init_pos");
     size_t bytes = c->cub_bytes;
-    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, c->c_cnt, c->c_off, N + 1, s));
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, c->c_cnt, 0, N + 1);
-    VBD_LAUNCH_CHECK("k_fill_i32");
-    hipLaunchKernelGGL(k_contact_scatter, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
-                       cd->soft_contact_indices, cd->soft_contact_shape, c->c_off, c->c_cnt, c->c_ids, N,
-                       c->model.shape_count);
-    VBD_LAUNCH_CHECK("k_contact_scatter");
-    hipLaunchKernelGGL(k_contact_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->c_off, c->c_ids);
-    VBD_LAUNCH_CHECK("k_contact_sort");
-    // position-order packing for the color-pass kernel
-    hipLaunchKernelGGL(k_pos_contact_count, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, N,
-                       c->model.color_particles, c->c_off, c->c_cnt);
-    VBD_LAUNCH_CHECK("k_pos_contact_count");
-    bytes = c->cub_bytes;
     VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, c->c_cnt, c->pos_coff, N + 1, s));
+    hipLaunchKernelGGL(k_contact_place, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
+                       cd->soft_contact_indices, cd->soft_contact_shape, c->pos_of, c->pos_coff, c->c_cnt, c->c_ids,
+                       N, c->model.shape_count);
+    VBD_LAUNCH_CHECK("k_contact_place");
     if (c->params.avbd_beta != 0.0f) {
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(cap, 256)), dim3(256), 0, s, c->cid_to_pack, -1, cap);
         VBD_LAUNCH_CHECK("k_fill_i32");
     }
-    hipLaunchKernelGGL(k_contact_pack, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->model.color_particles,
-                       c->lay_vert, c->c_off, c->c_ids, c->pos_coff, pk, mkd, mmu, cd->soft_contact_shape,
+    hipLaunchKernelGGL(k_contact_sort_pack, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->model.color_particles,
+                       c->lay_vert, c->pos_coff, c->c_ids, pk, mkd, mmu, cd->soft_contact_shape,
                        cd->soft_contact_body_pos, cd->soft_contact_body_vel, cd->soft_contact_normal,
                        c->model.shape_margin, c->model.shape_count, c->model.particle_radius, c->cpack,
                        c->cid_to_pack);
-    VBD_LAUNCH_CHECK("k_contact_pack");
+    VBD_LAUNCH_CHECK("k_contact_sort_pack");
     int32_t rc = launch_batch_contacts(c, s);
     if (rc) return rc;
     KParams& k = c->kp;
@@ -3196,12 +3146,9 @@ int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
     cp.sxf = d->shape_transform;
     cp.sscale = d->shape_scale;
     cp.smargin = d->shape_margin;
-    int P = d->pair_count > 0 ? d->pair_count : 1;
-    hipError_t e;
-    if ((e = hipMalloc(&col->flag, sizeof(int) * (size_t)P)) != hipSuccess ||
-        (e = hipMalloc(&col->index, sizeof(int) * (size_t)P)) != hipSuccess ||
-        (e = hipcub::DeviceScan::ExclusiveSum(nullptr, col->cub_bytes, col->flag, col->index, P)) != hipSuccess ||
-        (e = hipMalloc(&col->cub_tmp, col->cub_bytes > 0 ? col->cub_bytes : 16)) != hipSuccess) {
+    const int ntiles = std::max(blocks_for(d->pair_count, kColTile), 1);
+    hipError_t e = hipMalloc(&col->tiles, sizeof(int) * (size_t)ntiles);
+    if (e != hipSuccess) {
         std::string msg = std::string("vbd_collide_create: ") + hipGetErrorString(e);
         vbd_collide_destroy(col);
         return fail(VBD_ERR_HIP, msg);
@@ -3212,9 +3159,7 @@ int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
 
 int32_t vbd_collide_destroy(VbdCollide* col) {
     if (!col) return VBD_OK;
-    hipFree(col->flag);
-    hipFree(col->index);
-    hipFree(col->cub_tmp);
+    hipFree(col->tiles);
     delete col;
     return VBD_OK;
 }
@@ -3230,15 +3175,7 @@ int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* q, floa
         VBD_LAUNCH_CHECK("k_fill_i32");
         return VBD_OK;
     }
-    const int P = col->cp.P;
-    hipLaunchKernelGGL(k_collide_flag, dim3(blocks_for(P, 256)), dim3(256), 0, s, col->cp, q, margin, col->flag);
-    VBD_LAUNCH_CHECK("k_collide_flag");
-    size_t bytes = col->cub_bytes;
-    VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(col->cub_tmp, bytes, col->flag, col->index, P, s));
-    hipLaunchKernelGGL(k_collide_write, dim3(blocks_for(P, 256)), dim3(256), 0, s, col->cp, q, margin, col->flag,
-                       col->index, *out);
-    VBD_LAUNCH_CHECK("k_collide_write");
-    return VBD_OK;
+    return launch_collide_tiles(col->cp, s, q, margin, col->tiles, *out);
 }
 
 int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, const float* q, float margin,
@@ -3278,14 +3215,7 @@ int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, cons
     cp.sxf = d->shape_transform;
     cp.sscale = d->shape_scale;
     cp.smargin = d->shape_margin;
-    const int tiles = blocks_for(cp.P, kColTile);
-    hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev);
-    VBD_LAUNCH_CHECK("k_collide_tile_count");
-    hipLaunchKernelGGL(k_collide_tile_scan, dim3(1), dim3(1024), 0, s, tiles, tiles_dev, out->soft_contact_count);
-    VBD_LAUNCH_CHECK("k_collide_tile_scan");
-    hipLaunchKernelGGL(k_collide_tile_write, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev, *out);
-    VBD_LAUNCH_CHECK("k_collide_tile_write");
-    return VBD_OK;
+    return launch_collide_tiles(cp, s, q, margin, tiles_dev, *out);
 }
 
 }  // extern "C"

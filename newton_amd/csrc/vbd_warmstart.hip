@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <string>
 
 #include "vbd_warmstart.h"
@@ -375,6 +376,139 @@ __global__ __launch_bounds__(kBlock) void k_forward(int n, const uint16_t* __res
     }
 }
 
+// ---- inference: weights resident in LDS, 16x16x32 BF16 MFMA ------------------------
+// k_forward re-reads every weight tile from L2 for each 16 rows (about 6 KB per row), which bounds
+// it far below the matrix cores.  The inference kernel keeps all four BF16 weight matrices in LDS
+// (92 KB, loaded once per work-group), runs one 512-lane work-group per CU over 128-row groups
+// (8 waves x 16 rows), and uses gfx950's v_mfma_f32_16x16x32_bf16 (A: lane = row l % 16, k =
+// 8 (l / 16) .. + 7 of the 32-wide k step; B: lane = column l % 16, same k; C: lane holds rows
+// 4 (l / 16) .. + 3 of column l % 16).  Each layer is computed transposed, Z^T = W^T H^T: the weight
+// tile is the A operand and the activation rows the B operand, so a lane ends with four
+// consecutive output columns of one row and writes them to LDS as one 8-byte store.  Layer 1's k
+// runs to 64 with the columns past 48 zero.  SiLU uses the hardware exp2 and reciprocal (the
+// activations are rounded to BF16 right after).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kK1 = 64;                 // layer-1 k padded to two 32-wide steps
+constexpr int kS1 = kK1 + 8;            // LDS row strides (BF16 elements; 16-byte skew)
+constexpr int kSH = kHid + 8;
+constexpr int kInfWaves = 8;
+constexpr int kInfBlock = 64 * kInfWaves;
+constexpr int kInfRows = 16 * kInfWaves;  // rows per work-group iteration
+constexpr int kLdsW1 = kHid * kS1;
+constexpr int kLdsW = kHid * kSH;
+constexpr int kLdsW4 = kOutP * kSH;
+constexpr int kLdsH = kInfWaves * 16 * kSH;
+
+__device__ inline floatx4 mfma32(bf16x8 a, bf16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> two BF16 (round to nearest even) in one v_cvt_pk_bf16_f32
+__device__ inline uint32_t pack_bf16x2(float a, float b) {
+    const bf16x2 p = __builtin_convertvector((floatx2){a, b}, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &p, 4);
+    return u;
+}
+
+// bias + SiLU of the wave's tile (lane: row li, columns 16 nt + 4 lq .. + 3) into its LDS rows
+__device__ inline void infer_epilogue(const floatx4 (&acc)[8], const float* __restrict__ bias, uint16_t* h, int li,
+                                      int lq) {
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+        const int j = nt * 16 + lq * 4;
+        const uint32_t lo = pack_bf16x2(silu_fast(acc[nt][0] + bias[j]), silu_fast(acc[nt][1] + bias[j + 1]));
+        const uint32_t hi = pack_bf16x2(silu_fast(acc[nt][2] + bias[j + 2]), silu_fast(acc[nt][3] + bias[j + 3]));
+        *reinterpret_cast<uint2*>(h + li * kSH + j) = make_uint2(lo, hi);
+    }
+}
+
+template <int KSTEPS, int STRIDE>
+__device__ inline void infer_layer(const uint16_t* h_rows, const uint16_t* w, int li, int lq, floatx4 (&acc)[8]) {
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) acc[nt] = zero4();
+#pragma unroll
+    for (int kt = 0; kt < KSTEPS; ++kt) {
+        const bf16x8 hb = ld8(h_rows + li * kSH + kt * 32 + lq * 8);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt)
+            acc[nt] = mfma32(ld8(w + (nt * 16 + li) * STRIDE + kt * 32 + lq * 8), hb, acc[nt]);
+    }
+}
+
+__global__ __launch_bounds__(kInfBlock) void k_infer(int n, const uint16_t* __restrict__ feat,
+                                                    const uint16_t* __restrict__ pk, const float* __restrict__ prm,
+                                                    float* __restrict__ out) {
+    __shared__ __align__(16) uint16_t s_w1[kLdsW1];
+    __shared__ __align__(16) uint16_t s_w2[kLdsW];
+    __shared__ __align__(16) uint16_t s_w3[kLdsW];
+    __shared__ __align__(16) uint16_t s_w4[kLdsW4];
+    __shared__ __align__(16) uint16_t s_h[kLdsH];
+    for (int e = threadIdx.x; e < kLdsW1; e += kInfBlock) {
+        const int r = e / kS1, k = e % kS1;
+        s_w1[e] = k < kIn ? pk[pWt1 + r * kIn + k] : (uint16_t)0;
+    }
+    for (int e = threadIdx.x; e < kLdsW; e += kInfBlock) {
+        const int r = e / kSH, k = e % kSH;
+        s_w2[e] = k < kHid ? pk[pWt2 + r * kHid + k] : (uint16_t)0;
+        s_w3[e] = k < kHid ? pk[pWt3 + r * kHid + k] : (uint16_t)0;
+    }
+    for (int e = threadIdx.x; e < kLdsW4; e += kInfBlock) {
+        const int r = e / kSH, k = e % kSH;
+        s_w4[e] = k < kHid ? pk[pWt4 + r * kHid + k] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
+    uint16_t* H = s_h + wave * 16 * kSH;
+    floatx4 acc[8];
+    // every wave runs the same number of iterations (the barriers below are work-group wide)
+    for (int base = blockIdx.x * kInfRows; base < n; base += gridDim.x * kInfRows) {
+        const int r = base + wave * 16 + li;  // the row this lane carries
+        // layer 1: the B operand straight from the feature rows (48 BF16 each); k >= 48 and rows >= n
+        // read as 0
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) acc[nt] = zero4();
+#pragma unroll
+        for (int kt = 0; kt < kK1 / 32; ++kt) {
+            const int k0 = kt * 32 + lq * 8;
+            bf16x8 fb = {};
+            if (r < n && k0 < kIn) fb = ld8(feat + (size_t)r * kIn + k0);
+#pragma unroll
+            for (int nt = 0; nt < 8; ++nt) acc[nt] = mfma32(ld8(s_w1 + (nt * 16 + li) * kS1 + k0), fb, acc[nt]);
+        }
+        infer_epilogue(acc, prm + ob1, H, li, lq);
+        __syncthreads();
+        infer_layer<kHid / 32, kSH>(H, s_w2, li, lq, acc);
+        __syncthreads();
+        infer_epilogue(acc, prm + ob2, H, li, lq);
+        __syncthreads();
+        infer_layer<kHid / 32, kSH>(H, s_w3, li, lq, acc);
+        __syncthreads();
+        infer_epilogue(acc, prm + ob3, H, li, lq);
+        __syncthreads();
+        floatx4 o = zero4();
+#pragma unroll
+        for (int kt = 0; kt < kHid / 32; ++kt)
+            o = mfma32(ld8(s_w4 + li * kSH + kt * 32 + lq * 8), ld8(H + li * kSH + kt * 32 + lq * 8), o);
+        // lane (li, lq = 0) holds outputs 0..3 of row r; the first three are the network's
+        if (lq == 0 && r < n) {
+#pragma unroll
+            for (int c = 0; c < kOut; ++c) out[(size_t)r * kOut + c] = o[c] + prm[ob4 + c];
+        }
+        __syncthreads();  // H is rewritten by the next group's first epilogue
+    }
+}
+
 // ---- backward --------------------------------------------------------------------
 // dA (16 x Kout) of the wave's rows times W^T: dH[r][j] = sum_k dZ[r][k] W[j][k];
 // B operand B[k][j] = W[j][k] read from the natural W layout [in][out] (stride KO).
@@ -628,9 +762,13 @@ int32_t vbd_ws_forward(void* stream, int32_t n, const uint16_t* feat, const uint
                        float* out) {
     if (n < 0 || (n > 0 && (!feat || !packed || !params || !out))) return fail(-1, "vbd_ws_forward: bad argument");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_forward, dim3(blocks_for(n, kRowsWG)), dim3(kBlock), 0, (hipStream_t)stream, n, feat, packed,
-                       params, out, (uint16_t*)nullptr, (float*)nullptr);
-    WS_LAUNCH_CHECK("k_forward");
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0)
+        cus = 256;
+    const int grid = std::min(blocks_for(n, kInfRows), cus);
+    hipLaunchKernelGGL(k_infer, dim3(grid), dim3(kInfBlock), 0, (hipStream_t)stream, n, feat, packed, params, out);
+    WS_LAUNCH_CHECK("k_infer");
     return 0;
 }
 

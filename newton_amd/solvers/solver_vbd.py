@@ -638,8 +638,10 @@ class SolverVBD(SolverBase):
                 if trial_iterations > 0:
                     # each start runs up to k iterations of the per-world solve; its iterate is kept
                     # for the worlds it wins
+                    # residuals before and after the k iterations only (a check after every trial
+                    # iteration would cost a residual pass each)
                     its, best_r, wits = self.iterate_to_residual_worlds(state_in, dt, target, trial_iterations,
-                                                                        check_every, per_world=True)
+                                                                        trial_iterations, per_world=True)
                     trial_its += its
                     trial_world += wits
                     best_q = state_in.particle_q.clone()
@@ -651,7 +653,7 @@ class SolverVBD(SolverBase):
                     for k, g in enumerate(guesses):
                         self.set_iterate(state_in, g, commit=True)
                         its, r, wits = self.iterate_to_residual_worlds(state_in, dt, cand_target, trial_iterations,
-                                                                       check_every, per_world=True)
+                                                                       trial_iterations, per_world=True)
                         trial_its += its
                         trial_world += wits
                         better = (r < best_r) & ~done

@@ -36,7 +36,8 @@ def test_forward_matches_oracle(cuda_device):
     mlp = WarmStartMLP(cuda_device, seed=1)
     p = rng.normal(0, 0.15, mo.N_PARAMS).astype(np.float32)
     mlp.load_params(p)
-    for n in (1, 15, 64, 200, 1000):
+    # 40,000 rows: more 128-row groups than work-groups, so each work-group loops
+    for n in (1, 15, 64, 200, 1000, 40000):
         x = _rand_feat(rng, n)
         out = mlp.forward(_to_dev_bf16(x, cuda_device))
         torch.cuda.synchronize()
