@@ -53,6 +53,9 @@ def parse():
                     help="further relative residuals the three solvers are compared at (comma separated)")
     ap.add_argument("--sweep-substeps", type=int, default=10, help="substeps per tolerance of the sweep")
     ap.add_argument("--max-iterations", type=int, default=400, help="iteration cap of every residual-driven solve")
+    ap.add_argument("--check-every", type=int, default=1,
+                    help="iterations between residual checks of every residual-driven solve (a per-world residual "
+                         "evaluation costs about one iteration; a solve stops at the first check within tolerance)")
     ap.add_argument("--floor-iterations", type=int, default=3000,
                     help="plain-VBD iterations of the FP32-floor measurement (one substep of one world)")
     ap.add_argument("--residual-worlds", type=int, default=64, help="worlds of the headline batch")
@@ -684,7 +687,7 @@ def residual_leg(args):
                     r0_sq = float(solver.residual(s0, dt).item())
                     solver.set_iterate(s0, ws(solver, s0, dt)[-1], commit=True)
                     its, r_sq = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0_sq,
-                                                           args.max_iterations)
+                                                           args.max_iterations, args.check_every)
                     solver.end_step(s0, s1, dt)
                     r = {"iterations": its, "r": math.sqrt(r_sq), "r0": math.sqrt(r0_sq), "warm_start_index": 1,
                          "r_warm_start": None}
@@ -693,7 +696,8 @@ def residual_leg(args):
                              "learned_smoothed": ws_s}[name]
                     r = solver.step_to_residual(
                         s0, s1, None, None, dt, rel_tol=args.rel_tol, max_iterations=args.max_iterations,
-                        warm_start=start, trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
+                        check_every=args.check_every, warm_start=start,
+                        trial_iterations=args.trial_iterations if name == "learned_trial" else 0)
                 torch.cuda.synchronize()
                 res[name]["s"] += time.perf_counter() - t0
                 res[name]["iters"].append(r["iterations"])
@@ -797,7 +801,8 @@ def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
             one_in.particle_f.copy_(wind.winds[w].force(q, one_mass))
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = one_solver.step_to_residual(one_in, one_out, None, None, dt, rel_tol=tol, max_iterations=cap)
+            r = one_solver.step_to_residual(one_in, one_out, None, None, dt, rel_tol=tol, max_iterations=cap,
+                                            check_every=args.check_every)
             torch.cuda.synchronize()
             d["s"] += time.perf_counter() - t0
             d["iters"].append(r["iterations"])
@@ -810,7 +815,8 @@ def _solve_variants(args, tol, substeps, frame0, ctx, with_extra=False):
             prot = ctx["protocol"][name]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = solver.step_to_residual_worlds(a, b, None, None, dt, rel_tol=tol, max_iterations=cap, **prot)
+            r = solver.step_to_residual_worlds(a, b, None, None, dt, rel_tol=tol, max_iterations=cap,
+                                               check_every=args.check_every, **prot)
             torch.cuda.synchronize()
             d = res[name]
             d["s"] += time.perf_counter() - t0
@@ -906,7 +912,8 @@ def headline_leg(args, s_single, ws):
             wind.redraw(frame + i // sub)
         s0.particle_f.copy_(wind.force(s0.particle_q, mass))
         solver.step_to_residual_worlds(s0, s1, None, None, dt, rel_tol=args.rel_tol,
-                                       max_iterations=args.max_iterations, **ctx["protocol"]["batched_learned"])
+                                       max_iterations=args.max_iterations, check_every=args.check_every,
+                                       **ctx["protocol"]["batched_learned"])
         ws_dep.observe(solver, s0)
         s0, s1 = s1, s0
     ctx["state"] = s0
@@ -1050,6 +1057,7 @@ def cubes_residual_leg(args, ws_cloth):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r = solver.step_to_residual_worlds(a, b, None, c, dt, rel_tol=tol, max_iterations=args.max_iterations,
+                                               check_every=args.check_every,
                                                abs_tol=abs_tol, **prot)
             torch.cuda.synchronize()
             secs += time.perf_counter() - t0
@@ -1247,6 +1255,7 @@ def main():
             "rel_tol": args.rel_tol,
             "max_iterations": args.max_iterations,
             "step": "one substep (dt 1/600 s) of every world, each world iterated until r_w <= rel_tol * r0_w",
+            "check_every": args.check_every,
             "solver": "batched + learned warm start: per world the inertial start y, y + c_prev and "
                       "y + c_prev + 0.5 MLP (the BF16 network trained in this run), judged by their force "
                       f"residuals after {args.trial_iterations} VBD iterations each; every solver on its own "
