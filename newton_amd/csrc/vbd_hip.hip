@@ -716,6 +716,12 @@ __device__ __forceinline__ void store_row(float* __restrict__ row, V3 f, const M
         for (int j = 0; j < 3; ++j) row[3 + 3 * i + j] = h.m[i][j];
 }
 
+__device__ __forceinline__ void store_force(float* __restrict__ row, V3 f) {
+    row[0] = f.x;
+    row[1] = f.y;
+    row[2] = f.z;
+}
+
 // component c (0..11) of a row {f, H row-major}, by selects (no dynamic register indexing)
 __device__ __forceinline__ float row_component(V3 f, const M3& h, int c) {
     const float r[12] = {f.x, f.y, f.z, h.m[0][0], h.m[0][1], h.m[0][2],
@@ -847,7 +853,9 @@ constexpr int kBatchHdr = vbd_layout::kBatchWords;
 // 1 leaves it to the compiler (77 VGPRs, six waves per SIMD), 7 fits seven waves per SIMD at the
 // price of a few values kept in scratch -- faster on cloth, slower on tets (whose evaluation then
 // spills more).  The autotuner picks the shape per model.
-template <int BLOCK, int WAVES>
+// FORCE_ONLY (the residual, mode 1): only the force rows are stored and summed, so the Hessian
+// arithmetic of every item is dead code; the force bits are those of the solving pass.
+template <int BLOCK, int WAVES, bool FORCE_ONLY = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_solve_fast(KParams P, float* __restrict__ q, int batch_base, int nb,
                                                       float dt, int mode, float* __restrict__ g_out) {
     constexpr int MAXV = BLOCK / kComp;
@@ -988,7 +996,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
                          B.y, A.z, A.w, B.x, P.friction_epsilon, dt, fe, he);
             row = nstatic + (t - L4);  // contact rows follow the static rows
         }
-        store_row(s_rows + row * kComp, fe, he);
+        if constexpr (FORCE_ONLY)
+            store_force(s_rows + row * kComp, fe);
+        else
+            store_row(s_rows + row * kComp, fe, he);
     }
     // contacts beyond the lanes left after the static items (only when L4 + ncont > BLOCK)
     const int npar = par_contacts ? ncont : 0;
@@ -996,14 +1007,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         V3 fe;
         M3 he;
         eval_packed_contact(P, q, cbeg + ci, dt, fe, he);
-        store_row(s_rows + (nstatic + ci) * kComp, fe, he);
+        if constexpr (FORCE_ONLY)
+            store_force(s_rows + (nstatic + ci) * kComp, fe);
+        else
+            store_row(s_rows + (nstatic + ci) * kComp, fe, he);
     }
     __syncthreads();
 
     // ---- phase b
     const int cs = t / kComp;
     const int cc = t - cs * kComp;
-    if (cs < nv) {
+    if (cs < nv && (!FORCE_ONLY || cc < 3)) {
         float acc = s_acc[cs * kComp + cc];
         const int q0 = s_row[cs], q1 = q0 + s_nrow[cs];
         // the LDS reads of four rows are issued together; the additions stay serial, in row order
@@ -1052,7 +1066,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) h.m[i][j] = a[3 + 3 * i + j];
-        if (own_live) finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, dt, mode, g_out);
+        if constexpr (FORCE_ONLY) {
+            if (own_live) st3(g_out, own_v, own_active ? f : v3(0.0f, 0.0f, 0.0f));
+        } else {
+            if (own_live) finish_vertex(P, q, own_v, own_active, own_x0, own_d, f, h, dt, mode, g_out);
+        }
     }
 }
 
@@ -2520,7 +2538,21 @@ int32_t vbd_forward(VbdContext* c, void* stream, float* q, const float* qd, cons
 }
 
 namespace {
-int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, float dt, int mode, float* g) {
+// mode 0 solves; mode 1 writes the assembled force g: by the force-only instantiations when
+// force_only (the residual), else by the solving kernels themselves (the autotuner times those)
+int32_t launch_fast(VbdContext* c, hipStream_t s, int b0, int nb, float* q, float dt, int mode, float* g,
+                    bool force_only = false) {
+    if (mode == 1 && force_only) {
+        if (c->block == 128) {
+            hipLaunchKernelGGL((k_solve_fast<128, 1, true>), dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
+        } else if (c->block == 192) {
+            hipLaunchKernelGGL((k_solve_fast<192, 1, true>), dim3(nb), dim3(192), 0, s, c->kp, q, b0, nb, dt, mode, g);
+        } else {
+            hipLaunchKernelGGL((k_solve_fast<256, 1, true>), dim3(nb), dim3(256), 0, s, c->kp, q, b0, nb, dt, mode, g);
+        }
+        VBD_LAUNCH_CHECK("k_solve_fast");
+        return VBD_OK;
+    }
     if (c->block == 128) {
         hipLaunchKernelGGL((k_solve_fast<128, 1>), dim3(nb), dim3(128), 0, s, c->kp, q, b0, nb, dt, mode, g);
     } else if (c->block == 192 && c->waves == 7) {
@@ -2564,6 +2596,28 @@ int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt
     if (s1 > s0) {
         hipLaunchKernelGGL(k_solve_generic, dim3(s1 - s0), dim3(kBlock), 0, s, c->kp, q, s0, dt, mode, g);
         VBD_LAUNCH_CHECK("k_solve_generic");
+    }
+    return VBD_OK;
+}
+// The force residual assembled by the color-pass kernels' force-only instantiations (one launch
+// over every batch) and, for vertices with more than 256 items, the generic kernel.
+int32_t residual_by_passes(VbdContext* c, hipStream_t s, const float* q, float dt, float* g) {
+    int32_t rc0 = launch_spring_rows(c, s, 0, c->pre_color_off.back(), q, dt);
+    if (rc0) return rc0;
+    if (c->self_on) {
+        hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev, 0,
+                           c->model.particle_count, dt);
+        VBD_LAUNCH_CHECK("k_sc_rows");
+    }
+    int nb = c->use_list ? c->live_off.back() : c->fast_color_off.back();
+    if (nb > 0) {
+        int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g, true);
+        if (rc) return rc;
+    }
+    int ns = c->slow_color_off.back();
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_solve_generic, dim3(ns), dim3(kBlock), 0, s, c->kp, (float*)q, 0, dt, 1, g);
+        VBD_LAUNCH_CHECK("k_solve_generic(residual)");
     }
     return VBD_OK;
 }
@@ -2777,22 +2831,9 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
     if (!c || !q) return fail(VBD_ERR_ARG, "vbd_residual: null argument");
     hipStream_t s = (hipStream_t)stream;
     float* g = g_out ? g_out : c->g_tmp;
-    int32_t rc0 = launch_spring_rows(c, s, 0, c->pre_color_off.back(), q, dt);
-    if (rc0) return rc0;
-    if (c->self_on) {
-        hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev, 0,
-                           c->model.particle_count, dt);
-        VBD_LAUNCH_CHECK("k_sc_rows");
-    }
-    int nb = c->use_list ? c->live_off.back() : c->fast_color_off.back();
-    if (nb > 0) {
-        int32_t rc = launch_fast(c, s, 0, nb, (float*)q, dt, 1, g);
+    {
+        int32_t rc = residual_by_passes(c, s, q, dt, g);
         if (rc) return rc;
-    }
-    int ns = c->slow_color_off.back();
-    if (ns > 0) {
-        hipLaunchKernelGGL(k_solve_generic, dim3(ns), dim3(kBlock), 0, s, c->kp, (float*)q, 0, dt, 1, g);
-        VBD_LAUNCH_CHECK("k_solve_generic(residual)");
     }
     if (norm_sq_out) {
         const int N = c->model.particle_count;
@@ -2804,6 +2845,8 @@ int32_t vbd_residual(VbdContext* c, void* stream, const float* q, float dt, floa
     }
     return VBD_OK;
 }
+
+
 
 int32_t vbd_set_iterate(VbdContext* c, void* stream, float* q_dst, const float* guess, int32_t commit) {
     if (!c || !q_dst || !guess) return fail(VBD_ERR_ARG, "vbd_set_iterate: null argument");

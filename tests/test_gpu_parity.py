@@ -233,13 +233,17 @@ def test_no_contacts_step(cuda_device):
     np.testing.assert_array_equal(s1.particle_qd.cpu().numpy(), qd_ref)
 
 
-def test_residual_matches_oracle(cuda_device):
+@pytest.mark.parametrize("scene", ["strip", "cube", "fan"])
+def test_residual_matches_oracle(cuda_device, scene):
+    """The force residual (the force-only color-pass kernels and the generic kernel) equals the oracle's
+    per-vertex assembly bit for bit: the damped strip with contacts, a damped tet cube on the ground,
+    and a fan whose centre has > 256 items."""
     import torch
 
     from newton_amd import CollisionPipeline
     from newton_amd.solvers import SolverVBD
 
-    model, state, info = _scene("strip", cuda_device)
+    model, state, info = _fan_scene(cuda_device) if scene == "fan" else _scene(scene, cuda_device)
     solver = SolverVBD(model, iterations=info["iterations"])
     pipe = CollisionPipeline(model, soft_contact_margin=0.05)
     contacts = pipe.contacts()
