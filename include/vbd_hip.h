@@ -21,8 +21,10 @@
  *   _solve_particle_iteration (one color: accumulate_particle_body_contact_force_and_hessian
  *     + solve_elasticity(_tile) + apply_truncation_ts) solver_vbd.py:2552-2702 -> vbd_color_pass
  *   _finalize_particles/update_velocity solver_vbd.py:3158-3169, particle_vbd_kernels.py:1468-1471 -> vbd_finalize
- *   create_soft_contacts (particle vs static plane/sphere) newton/_src/sim/collide.py:1553-1592,
- *     newton/_src/geometry/kernels.py:1020-1184 -> vbd_collide_generate
+ *   create_soft_contacts (particle vs plane/sphere) newton/_src/sim/collide.py:1553-1592,
+ *     newton/_src/geometry/kernels.py:1020-1184 -> vbd_collide_generate / vbd_collide_generate_bodies
+ *   launch_soft_ef_contacts (full-surface edge/face records) collide.py:1593-1610,
+ *     newton/_src/geometry/soft_contacts_sdf.py:240-551 -> the same calls (edge/face pairs in the desc)
  *   SolverBase.integrate_particles     newton/_src/solvers/solver.py:20-56, :297-330 -> vbd_integrate_particles
  */
 #ifndef VBD_HIP_H
@@ -88,6 +90,11 @@ typedef struct VbdModelDesc {
     const float* spring_rest_length;       /* [S] */
     const float* spring_stiffness;         /* [S] */
     const float* spring_damping;           /* [S] */
+    /* Model.shape_body ([shape_count]; NULL: every shape static).  Shapes attached to bodies move with
+     * poses an external rigid solver gives each step (SolverVBD(integrate_with_external_rigid_solver=True),
+     * solver_vbd.py:280-281, :2523-2526), handed over in VbdContactsDesc.body_q / body_q_prev */
+    const int32_t* shape_body;
+    int32_t body_count;
 } VbdModelDesc;
 
 typedef struct VbdSolverParams {
@@ -115,17 +122,27 @@ typedef struct VbdSolverParams {
     const int32_t* vt_filter_offsets;       /* [N+1] */
     const int32_t* ee_filter;               /* sorted edge ids per edge */
     const int32_t* ee_filter_offsets;       /* [E+1] */
+    /* 1: the contacts may hold full-surface edge / face records (CollisionPipeline(
+     * enable_rigid_soft_full_surface_contact=True), collide.py:785-793); each adds up to three corner
+     * entries, so the per-corner buffers are sized 3 x soft_contact_capacity */
+    int32_t soft_contact_full_surface;
 } VbdSolverParams;
 
-/* Contacts.soft_contact_* (newton/_src/sim/contacts.py:327-371). */
+/* Contacts.soft_contact_* (newton/_src/sim/contacts.py:327-371) and the body poses of the step. */
 typedef struct VbdContactsDesc {
     const int32_t* soft_contact_count;  /* [1] device */
     int32_t soft_contact_max;
-    const int32_t* soft_contact_indices; /* [S,3] (p,-1,-1) particle records */
+    const int32_t* soft_contact_indices; /* [S,3] (p,-1,-1) particle, (v0,v1,-1) edge, (v0,v1,v2) face records */
     const int32_t* soft_contact_shape;
-    const float* soft_contact_body_pos;
+    const float* soft_contact_body_pos;  /* body frame of the shape's body (world for static shapes) */
     const float* soft_contact_body_vel;
-    const float* soft_contact_normal;
+    const float* soft_contact_normal;    /* world */
+    const float* soft_contact_barycentric; /* [S,3] corner weights; NULL: edge/face records are skipped */
+    /* poses of the shapes' bodies ([body_count,7], p then q xyzw): body_q at the step's end (the external
+     * rigid solver's state_out.body_q), body_q_prev at its start (state_in.body_q); NULL for static shapes */
+    const float* body_q;
+    const float* body_q_prev;
+    float dt;                            /* the step's dt (velocity of body points); vbd_step sets it itself */
 } VbdContactsDesc;
 
 /* Writable soft-contact outputs of the collision step. */
@@ -141,7 +158,7 @@ typedef struct VbdContactsOut {
     float* soft_contact_normal;         /* [S,3] */
 } VbdContactsOut;
 
-/* Particle and static-shape data for soft-contact generation. */
+/* Particle and shape data for soft-contact generation. */
 typedef struct VbdCollideDesc {
     int32_t particle_count;
     int32_t pair_count;
@@ -153,9 +170,17 @@ typedef struct VbdCollideDesc {
     const int32_t* shape_type;          /* 1 plane, 3 sphere */
     const int32_t* shape_flags;
     const int32_t* shape_world;
-    const float* shape_transform;       /* [S,7] (p, q xyzw) */
+    const float* shape_transform;       /* [S,7] (p, q xyzw), relative to the shape's body */
     const float* shape_scale;           /* [S,3] */
     const float* shape_margin;          /* [S] */
+    const int32_t* shape_body;          /* [S]; NULL: every shape static */
+    /* full-surface passes (collide.py:1598-1610, soft_contacts_sdf.py:277-551); counts 0 when off */
+    int32_t edge_pair_count;
+    const int32_t* edge_pairs;          /* [E,2] (edge, shape) */
+    const int32_t* edge_indices;        /* [edges,4] (o0,o1,v0,v1) */
+    int32_t face_pair_count;
+    const int32_t* face_pairs;          /* [F,2] (triangle, shape) */
+    const int32_t* tri_indices;         /* [T,3] */
 } VbdCollideDesc;
 
 const char* vbd_last_error(void);
@@ -333,11 +358,18 @@ int32_t vbd_self_copy_lists(VbdContext* ctx, void* stream, int32_t* vt_list, int
  * than a detection can list (its contacts were not detected). */
 int32_t vbd_self_status(VbdContext* ctx, int32_t* grid_overflow);
 
-/* Soft-contact generation (CollisionPipeline.collide, particle part). */
+/* Soft-contact generation (CollisionPipeline.collide, soft part): particle records in pair order, then
+ * the full-surface edge and face records (soft_contacts_sdf.py:380-469, :277-377) in pair order;
+ * soft_contact_count counts every hit (it may exceed soft_contact_max, as the reference's counter does).
+ * vbd_collide_create reads desc.shape_body back once.  vbd_collide_generate is for static shapes only
+ * (VBD_ERR_ARG when a shape has a body); vbd_collide_generate_bodies places body-attached shapes with
+ * body_q ([body_count,7], the collided state's State.body_q, geometry/kernels.py:1065-1081). */
 int32_t vbd_collide_create(const VbdCollideDesc* desc, VbdCollide** out);
 int32_t vbd_collide_destroy(VbdCollide* col);
 int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* particle_q, float margin,
                              const VbdContactsOut* out);
+int32_t vbd_collide_generate_bodies(VbdCollide* col, void* stream, const float* particle_q, const float* body_q,
+                                    int32_t body_count, float margin, const VbdContactsOut* out);
 
 /* Largest pair_count vbd_collide_generate_noalloc accepts. */
 #define VBD_NOALLOC_MAX_PAIRS (16 * 1024 * 1024)
@@ -347,8 +379,9 @@ int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* particl
  * forbid allocation), and needs no VbdCollide: the descriptor is read per call.  Same records, same
  * order and same soft_contact_count as vbd_collide_generate.  Its device scratch (one int per
  * 1,024 pairs, 64 KiB) is a static array each GPU holds until the process ends; calls that overlap
- * on one GPU share it and race.  pair_count above VBD_NOALLOC_MAX_PAIRS is rejected with
- * VBD_ERR_ARG.  The first call on a device looks the scratch address up (not inside a capture). */
+ * on one GPU share it and race.  More than VBD_NOALLOC_MAX_PAIRS candidate pairs (particle, edge and
+ * face pairs together) are rejected with VBD_ERR_ARG, and so is a non-NULL desc.shape_body (shapes
+ * are static here).  The first call on a device looks the scratch address up (not inside a capture). */
 int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* desc, void* stream, const float* particle_q, float margin,
                                      const VbdContactsOut* out);
 

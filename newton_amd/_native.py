@@ -67,6 +67,8 @@ class VbdModelDesc(ctypes.Structure):
         ("spring_rest_length", c_void_p),
         ("spring_stiffness", c_void_p),
         ("spring_damping", c_void_p),
+        ("shape_body", c_void_p),
+        ("body_count", c_int32),
     ]
 
 
@@ -94,6 +96,7 @@ class VbdSolverParams(ctypes.Structure):
         ("vt_filter_offsets", c_void_p),
         ("ee_filter", c_void_p),
         ("ee_filter_offsets", c_void_p),
+        ("soft_contact_full_surface", c_int32),
     ]
 
 
@@ -106,6 +109,10 @@ class VbdContactsDesc(ctypes.Structure):
         ("soft_contact_body_pos", c_void_p),
         ("soft_contact_body_vel", c_void_p),
         ("soft_contact_normal", c_void_p),
+        ("soft_contact_barycentric", c_void_p),
+        ("body_q", c_void_p),
+        ("body_q_prev", c_void_p),
+        ("dt", c_float),
     ]
 
 
@@ -138,6 +145,13 @@ class VbdCollideDesc(ctypes.Structure):
         ("shape_transform", c_void_p),
         ("shape_scale", c_void_p),
         ("shape_margin", c_void_p),
+        ("shape_body", c_void_p),
+        ("edge_pair_count", c_int32),
+        ("edge_pairs", c_void_p),
+        ("edge_indices", c_void_p),
+        ("face_pair_count", c_int32),
+        ("face_pairs", c_void_p),
+        ("tri_indices", c_void_p),
     ]
 
 
@@ -193,6 +207,8 @@ HIP_EXPORTS = {
     "vbd_collide_create": (c_int32, [ctypes.POINTER(VbdCollideDesc), ctypes.POINTER(c_void_p)]),
     "vbd_collide_destroy": (c_int32, [c_void_p]),
     "vbd_collide_generate": (c_int32, [c_void_p, c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),
+    "vbd_collide_generate_bodies": (
+        c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_float, ctypes.POINTER(VbdContactsOut)]),
     "vbd_collide_generate_noalloc": (
         c_int32, [ctypes.POINTER(VbdCollideDesc), c_void_p, c_void_p, c_float, ctypes.POINTER(VbdContactsOut)]),
 }

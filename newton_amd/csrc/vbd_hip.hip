@@ -501,9 +501,10 @@ __device__ void eval_tet(const float* __restrict__ q, const float* __restrict__ 
     h_out = hess;
 }
 
-// Particle vs static shape contact: penalty + damping + IPC friction,
-// rigid_vbd_kernels.py:1331-1439, :1603-1645.
-__device__ void eval_contact(V3 x, V3 x_prev, V3 bx, V3 body_vel, V3 n, float radius, float margin, float ke, float kd,
+// Soft contact force law at the contact point x (anchor x_prev) against the world body point bx moving
+// at bv (computed per substep by contact_body_frame when the contacts are packed): penalty + damping +
+// IPC friction, rigid_vbd_kernels.py:1331-1439, :1603-1645.
+__device__ void eval_contact(V3 x, V3 x_prev, V3 bx, V3 bv, V3 n, float radius, float margin, float ke, float kd,
                              float mu, float friction_epsilon, float dt, V3& f_out, M3& h_out) {
     float pen = -(dot(n, sub(x, bx)) - radius - margin);
     if (!(pen > 0.0f)) {
@@ -513,7 +514,6 @@ __device__ void eval_contact(V3 x, V3 x_prev, V3 bx, V3 body_vel, V3 n, float ra
     }
     V3 dx = sub(x, x_prev);
     V3 zero = v3(0.0f, 0.0f, 0.0f);
-    V3 bv = add(add(zero, cross(zero, bx)), body_vel);
     V3 u = sub(dx, scl(bv, dt));
     float f_n = pen * ke;
     V3 force = scl(n, f_n);
@@ -680,15 +680,19 @@ struct KParams {
     // contacts packed per color position (k_contact_sort_pack)
     const int* pos_coff;
     const float* cpack;
+    const int* cpack_ef;  // per packed entry of an edge/face record: {c0, c1, c2, corner} {w0, w1, w2, 0}
     const float* shape_margin;
     int shape_count;
     float friction_epsilon;
     int fp64_ill;
 };
 
-// packed contact record: 4 x float4
-//   {particle, local slot, ke, kd} {mu, margin, radius, bx} {by, bz, vx, vy} {vz, nx, ny, nz}
+// packed contact record (one per corner entry): 4 x float4
+//   {particle, local slot | kContactEF, ke, kd} {mu, margin, radius, bx} {by, bz, bvx, bvy} {bvz, nx, ny, nz}
+// with bx the world body point and bv the body's velocity there; an edge/face record's entries carry
+// the largest corner radius and kContactEF, and their corners and weights in cpack_ef
 constexpr int kContactWords = 16;
+constexpr int kContactEF = 1 << 30;
 
 __device__ __forceinline__ int wave_align(int n) { return (n + 63) & ~63; }
 
@@ -732,13 +736,43 @@ __device__ __forceinline__ float row_component(V3 f, const M3& h, int c) {
     return v;
 }
 
+// Packed contact entry rec (its four float4 A-D) at the current positions: a particle record at its
+// particle; an edge/face record at x = sum_i w_i q[c_i] over its corners in corner order, its force and
+// Hessian scaled by w and w^2 of this entry's corner (particle_vbd_kernels.py:2379-2401,
+// rigid_vbd_kernels.py:1440-1536).
+__device__ __forceinline__ void eval_contact_rec(const KParams& P, const float* __restrict__ q, int rec, float4 A,
+                                                 float4 B, float4 C, float4 D, float dt, V3& f, M3& h) {
+    const int p = __float_as_int(A.x);
+    if (!(__float_as_int(A.y) & kContactEF)) {
+        eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z, B.y,
+                     A.z, A.w, B.x, P.friction_epsilon, dt, f, h);
+        return;
+    }
+    const int4* e = reinterpret_cast<const int4*>(P.cpack_ef) + 2 * (size_t)rec;
+    const int4 ci = e[0];
+    const float4 w = __builtin_bit_cast(float4, e[1]);
+    V3 x = scl(ld3(q, ci.x), w.x), xp = scl(ld3(P.q_prev, ci.x), w.x);
+    if (ci.y >= 0) {
+        x = add(x, scl(ld3(q, ci.y), w.y));
+        xp = add(xp, scl(ld3(P.q_prev, ci.y), w.y));
+    }
+    if (ci.z >= 0) {
+        x = add(x, scl(ld3(q, ci.z), w.z));
+        xp = add(xp, scl(ld3(P.q_prev, ci.z), w.z));
+    }
+    V3 fc;
+    M3 hc;
+    eval_contact(x, xp, v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z, B.y, A.z, A.w, B.x,
+                 P.friction_epsilon, dt, fc, hc);
+    const float wi = ci.w == 0 ? w.x : (ci.w == 1 ? w.y : w.z);
+    f = scl(fc, wi);
+    h = mscl(hc, wi * wi);
+}
+
 __device__ __forceinline__ void eval_packed_contact(const KParams& P, const float* __restrict__ q, int rec, float dt,
                                                     V3& f, M3& h) {
     const float4* c4 = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)rec;
-    float4 A = c4[0], B = c4[1], C = c4[2], D = c4[3];
-    int p = __float_as_int(A.x);
-    eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z, B.y,
-                 A.z, A.w, B.x, P.friction_epsilon, dt, f, h);
+    eval_contact_rec(P, q, rec, c4[0], c4[1], c4[2], c4[3], dt, f, h);
 }
 
 // evaluate_spring_force_and_hessian_both_vertices (particle_vbd_kernels.py:1685-1730), the
@@ -989,11 +1023,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
             row = r0.z & 0xffff;
             spring_item(P, q, r0.x, r0.y, r0.z >> 16, r0.w, pr.x, pr.y, pr.z, dt, fe, he);
         } else {
-            float4 A = __builtin_bit_cast(float4, r0), B = __builtin_bit_cast(float4, r1);
-            float4 C = __builtin_bit_cast(float4, r2), D = __builtin_bit_cast(float4, r3);
-            int p = r0.x;
-            eval_contact(ld3(q, p), ld3(P.q_prev, p), v3(B.w, C.x, C.y), v3(C.z, C.w, D.x), v3(D.y, D.z, D.w), B.z,
-                         B.y, A.z, A.w, B.x, P.friction_epsilon, dt, fe, he);
+            eval_contact_rec(P, q, cbeg + t - L4, __builtin_bit_cast(float4, r0), __builtin_bit_cast(float4, r1),
+                             __builtin_bit_cast(float4, r2), __builtin_bit_cast(float4, r3), dt, fe, he);
             row = nstatic + (t - L4);  // contact rows follow the static rows
         }
         if constexpr (FORCE_ONLY)
@@ -1175,18 +1206,12 @@ __global__ __launch_bounds__(kBlock) void k_solve_generic(KParams P, float* __re
             // the vertex's contacts as packed at its color position (contact-index order)
             const int vpos = P.slow_pos[slow_base + blockIdx.x];
             const int c0 = P.pos_coff[vpos], c1 = P.pos_coff[vpos + 1];
-            if (c1 > c0) {
-                V3 x = ld3(q, v), xp = ld3(P.q_prev, v);
-                for (int j = c0; j < c1; ++j) {
-                    const float4* r = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)j;
-                    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-                    V3 fc;
-                    M3 hc;
-                    eval_contact(x, xp, v3(r1.w, r2.x, r2.y), v3(r2.z, r2.w, r3.x), v3(r3.y, r3.z, r3.w), r1.z, r1.y,
-                                 r0.z, r0.w, r1.x, P.friction_epsilon, dt, fc, hc);
-                    cf = add(cf, fc);
-                    ch = madd(ch, hc);
-                }
+            for (int j = c0; j < c1; ++j) {
+                V3 fc;
+                M3 hc;
+                eval_packed_contact(P, q, j, dt, fc, hc);
+                cf = add(cf, fc);
+                ch = madd(ch, hc);
             }
             if (P.has_spring) {
                 const int si = slow_base + blockIdx.x;
@@ -1279,20 +1304,90 @@ __global__ void k_integrate_particles(int n, const float* __restrict__ x, const 
     st3(v_new, i, v1);
 }
 
+// ---- rigid transforms (p, q xyzw): warp's transform_point / transform_vector / transform_inverse /
+// transform_multiply and the Hamilton quaternion product, from their textbook definitions
+__device__ __forceinline__ V3 quat_rotate(float qx, float qy, float qz, float qw, V3 x) {
+    float c = 2.0f * qw * qw - 1.0f;
+    float d = 2.0f * (qx * x.x + qy * x.y + qz * x.z);
+    return v3(x.x * c + qx * d + (qy * x.z - qz * x.y) * qw * 2.0f, x.y * c + qy * d + (qz * x.x - qx * x.z) * qw * 2.0f,
+              x.z * c + qz * d + (qx * x.y - qy * x.x) * qw * 2.0f);
+}
+
+struct Xf {
+    V3 p;
+    float qx, qy, qz, qw;
+};
+
+__device__ __forceinline__ Xf xf_load(const float* __restrict__ t) {
+    return Xf{v3(t[0], t[1], t[2]), t[3], t[4], t[5], t[6]};
+}
+__device__ __forceinline__ V3 xf_point(const Xf& t, V3 x) { return add(quat_rotate(t.qx, t.qy, t.qz, t.qw, x), t.p); }
+__device__ __forceinline__ V3 xf_vector(const Xf& t, V3 x) { return quat_rotate(t.qx, t.qy, t.qz, t.qw, x); }
+__device__ __forceinline__ Xf xf_inverse(const Xf& t) {
+    Xf r;
+    r.qx = -t.qx;
+    r.qy = -t.qy;
+    r.qz = -t.qz;
+    r.qw = t.qw;
+    r.p = neg(quat_rotate(r.qx, r.qy, r.qz, r.qw, t.p));
+    return r;
+}
+// transform_multiply(a, b) = (quat_rotate(a.q, b.p) + a.p, a.q * b.q)
+__device__ __forceinline__ Xf xf_mul(const Xf& a, const Xf& b) {
+    Xf r;
+    r.p = add(quat_rotate(a.qx, a.qy, a.qz, a.qw, b.p), a.p);
+    r.qx = a.qw * b.qx + b.qw * a.qx + a.qy * b.qz - b.qy * a.qz;
+    r.qy = a.qw * b.qy + b.qw * a.qy + a.qz * b.qx - b.qz * a.qx;
+    r.qz = a.qw * b.qz + b.qw * a.qz + a.qx * b.qy - b.qx * a.qy;
+    r.qw = a.qw * b.qw - a.qx * b.qx - a.qy * b.qy - a.qz * b.qz;
+    return r;
+}
+
+// World contact point bx = X_wb * body_pos of a soft contact and the body's velocity there: for a shape
+// attached to a body moved by an external rigid solver (rigid_vbd_kernels.py:1365-1439 with body_q_prev,
+// solver_vbd.py:2523-2526) bv = (bx - X_wb_prev * body_pos) / dt + X_wb . body_vel; a static shape takes
+// the identity transform, bx = body_pos, bv = (0 + cross(0, bx)) + body_vel (the same values up to signed
+// zeros).
+struct BodyPoses {
+    const int* shape_body;  // null: every shape static
+    const float* body_q;
+    const float* body_q_prev;
+    int body_count;
+    int shape_count;
+};
+
+__device__ __forceinline__ void contact_body_frame(const BodyPoses& B, int shape, V3 body_pos, V3 body_vel, float dt,
+                                                   V3& bx, V3& bv) {
+    const int b = (B.shape_body && shape >= 0 && shape < B.shape_count) ? B.shape_body[shape] : -1;
+    if (b < 0 || !B.body_q || !B.body_q_prev || b >= B.body_count) {
+        const V3 zero = v3(0.0f, 0.0f, 0.0f);
+        bx = body_pos;
+        bv = add(add(zero, cross(zero, bx)), body_vel);
+        return;
+    }
+    const Xf X = xf_load(B.body_q + 7 * (size_t)b);
+    const Xf Xp = xf_load(B.body_q_prev + 7 * (size_t)b);
+    bx = xf_point(X, body_pos);
+    const V3 bxp = xf_point(Xp, body_pos);
+    bv = add(divs(sub(bx, bxp), dt), xf_vector(X, body_vel));
+}
+
 __global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
 }
 
-// init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + counts per color position of the
-// contact's particle (pos_cnt must be zero on entry; k_contact_place leaves it zero again)
+// init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + counts per color position: one entry
+// for a particle record's particle, one for each corner of an edge/face record (when ef is set; without
+// barycentric weights such records are skipped).  pos_cnt must be zero on entry; k_contact_place leaves it
+// zero again.
 __global__ void k_contact_init_pos(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
                                    const int* __restrict__ c_shape, float soft_ke, float soft_kd, float soft_mu,
                                    const float* __restrict__ shape_ke, const float* __restrict__ shape_kd,
                                    const float* __restrict__ shape_mu, float k_start, float* __restrict__ pk,
                                    float* __restrict__ mke, float* __restrict__ mkd, float* __restrict__ mmu,
                                    const int* __restrict__ pos_of, int* __restrict__ pos_cnt, int n_particles,
-                                   int n_shapes) {
+                                   int n_shapes, int ef) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
@@ -1312,38 +1407,60 @@ __global__ void k_contact_init_pos(const int* __restrict__ count_ptr, int cmax, 
     mkd[i] = kd;
     mmu[i] = mu;
     pk[i] = k_start < 0.0f ? ke : fminf(k_start, ke);
-    int p = c_indices[3 * i];
-    if (c_indices[3 * i + 1] < 0 && p >= 0 && p < n_particles) atomicAdd(&pos_cnt[pos_of[p]], 1);
+    const int* cr = c_indices + 3 * i;
+    if (cr[1] < 0) {
+        if (cr[0] >= 0 && cr[0] < n_particles) atomicAdd(&pos_cnt[pos_of[cr[0]]], 1);
+    } else if (ef) {
+        for (int k = 0; k < 3; ++k)
+            if (cr[k] >= 0 && cr[k] < n_particles) atomicAdd(&pos_cnt[pos_of[cr[k]]], 1);
+    }
 }
 
-// Places every counted contact id in its color position's range (any order within the range) by
-// counting the position's counter back down to zero, ready for the next substep.
+// Places every counted entry, key (contact id << 2 | corner), in its color position's range (any order
+// within the range) by counting the position's counter back down to zero, ready for the next substep.
 __global__ void k_contact_place(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
                                 const int* __restrict__ c_shape, const int* __restrict__ pos_of,
                                 const int* __restrict__ pos_coff, int* __restrict__ pos_cnt, int* __restrict__ ids,
-                                int n_particles, int n_shapes) {
+                                int n_particles, int n_shapes, int ef) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
-    if (c_indices[3 * i + 1] >= 0) return;
-    int p = c_indices[3 * i];
-    if (p < 0 || p >= n_particles) return;
     int sh = c_shape[i];
     if (sh < 0 || sh >= n_shapes) return;
-    const int pos = pos_of[p];
-    const int slot = atomicSub(&pos_cnt[pos], 1) - 1;
-    ids[pos_coff[pos] + slot] = i;
+    const int* cr = c_indices + 3 * i;
+    const int corners = cr[1] < 0 ? 1 : (ef ? 3 : 0);
+    for (int k = 0; k < corners; ++k) {
+        const int p = cr[k];
+        if (p < 0 || p >= n_particles) continue;
+        const int pos = pos_of[p];
+        const int slot = atomicSub(&pos_cnt[pos], 1) - 1;
+        ids[pos_coff[pos] + slot] = (i << 2) | k;
+    }
 }
 
-// Sorts each color position's contact ids ascending (the serial accumulation order) and packs the
-// records there, so a batch's contacts are one contiguous, vertex-ordered range.
+// Sorts each color position's entry keys ascending (contact-index order: the serial accumulation order)
+// and packs the entries there, so a batch's contacts are one contiguous, vertex-ordered range.  The body
+// point and its velocity are resolved here once per substep (the body poses are fixed during a step).
+struct PackArgs {
+    const float* pk;
+    const float* mkd;
+    const float* mmu;
+    const int* c_indices;
+    const float* c_bary;
+    const int* c_shape;
+    const float* body_pos;
+    const float* body_vel;
+    const float* normal;
+    const float* shape_margin;
+    int shape_count;
+    const float* radius;
+    BodyPoses bodies;
+    float dt;
+};
+
 __global__ void k_contact_sort_pack(int n, const int* __restrict__ color_particles, const int* __restrict__ vert_rec,
-                                    const int* __restrict__ pos_coff, int* __restrict__ ids,
-                                    const float* __restrict__ pk, const float* __restrict__ mkd,
-                                    const float* __restrict__ mmu, const int* __restrict__ c_shape,
-                                    const float* __restrict__ body_pos, const float* __restrict__ body_vel,
-                                    const float* __restrict__ normal, const float* __restrict__ shape_margin,
-                                    int shape_count, const float* __restrict__ radius, float* __restrict__ cpack,
+                                    const int* __restrict__ pos_coff, int* __restrict__ ids, PackArgs A,
+                                    float* __restrict__ cpack, int* __restrict__ cpack_ef,
                                     int* __restrict__ cid_to_pack) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -1360,16 +1477,31 @@ __global__ void k_contact_sort_pack(int n, const int* __restrict__ color_particl
     }
     const int v = color_particles[k];
     const int local = vert_rec[4 * k + 3] & 0xffff;
-    const float rad = radius[v];
     for (int j = b; j < e; ++j) {
-        const int cid = ids[j];
-        const int sh = c_shape[cid];
+        const int key = ids[j];
+        const int cid = key >> 2, corner = key & 3;
+        const int sh = A.c_shape[cid];
+        const int* cr = A.c_indices + 3 * cid;
+        const bool ef = cr[1] >= 0;
+        float rad = A.radius[v];
+        if (ef) {
+            rad = A.radius[cr[0]];
+            if (cr[1] >= 0) rad = fmaxf(rad, A.radius[cr[1]]);
+            if (cr[2] >= 0) rad = fmaxf(rad, A.radius[cr[2]]);
+            int4* o = reinterpret_cast<int4*>(cpack_ef) + 2 * (size_t)j;
+            o[0] = make_int4(cr[0], cr[1], cr[2], corner);
+            o[1] = make_int4(__float_as_int(A.c_bary[3 * cid]), __float_as_int(A.c_bary[3 * cid + 1]),
+                             __float_as_int(A.c_bary[3 * cid + 2]), 0);
+        }
+        V3 bx, bv;
+        contact_body_frame(A.bodies, sh, ld3(A.body_pos, cid), ld3(A.body_vel, cid), A.dt, bx, bv);
+        const V3 nrm = ld3(A.normal, cid);
         float4* o = reinterpret_cast<float4*>(cpack) + 4 * (size_t)j;
-        o[0] = make_float4(__int_as_float(v), __int_as_float(local), pk[cid], mkd[cid]);
-        o[1] = make_float4(mmu[cid], shape_count > 0 ? shape_margin[sh] : 0.0f, rad, body_pos[3 * cid]);
-        o[2] = make_float4(body_pos[3 * cid + 1], body_pos[3 * cid + 2], body_vel[3 * cid], body_vel[3 * cid + 1]);
-        o[3] = make_float4(body_vel[3 * cid + 2], normal[3 * cid], normal[3 * cid + 1], normal[3 * cid + 2]);
-        cid_to_pack[cid] = j;
+        o[0] = make_float4(__int_as_float(v), __int_as_float(local | (ef ? kContactEF : 0)), A.pk[cid], A.mkd[cid]);
+        o[1] = make_float4(A.mmu[cid], A.shape_count > 0 ? A.shape_margin[sh] : 0.0f, rad, bx.x);
+        o[2] = make_float4(bx.y, bx.z, bv.x, bv.y);
+        o[3] = make_float4(bv.z, nrm.x, nrm.y, nrm.z);
+        if (cid_to_pack) cid_to_pack[3 * (size_t)cid + corner] = j;
     }
 }
 
@@ -1388,29 +1520,56 @@ __global__ void k_batch_contacts(int nbatch, const int* __restrict__ blk, int bl
     cinfo[2 * b + 1] = pos_coff[vb + nv] - c0;
 }
 
-// update_duals_body_particle_contacts, rigid_vbd_kernels.py:4861-4926 (particle records)
+// update_duals_body_particle_contacts, rigid_vbd_kernels.py:4861-4926: the penetration of the particle,
+// or of an edge/face record's barycentric point with its largest corner radius (records skipped without
+// barycentric weights), at the world body point; the ramped penalty goes to every packed entry.
 __global__ void k_dual_update(const int* __restrict__ count_ptr, int cmax, const int* __restrict__ c_indices,
-                              const int* __restrict__ c_shape, const float* __restrict__ body_pos,
+                              const float* __restrict__ c_bary, const int* __restrict__ c_shape,
+                              const float* __restrict__ body_pos, const float* __restrict__ body_vel,
                               const float* __restrict__ normal, const float* __restrict__ q,
                               const float* __restrict__ radius, const float* __restrict__ shape_margin, int shape_count,
-                              const float* __restrict__ mke, float beta, float* __restrict__ pk, int n_particles,
-                              const int* __restrict__ cid_to_pack, float* __restrict__ cpack) {
+                              BodyPoses bodies, float dt, const float* __restrict__ mke, float beta,
+                              float* __restrict__ pk, int n_particles, const int* __restrict__ cid_to_pack,
+                              float* __restrict__ cpack) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = min(*count_ptr, cmax);
     if (i >= nc) return;
-    if (c_indices[3 * i + 1] >= 0) return;
-    int p = c_indices[3 * i];
+    const int* cr = c_indices + 3 * i;
+    const bool ef = cr[1] >= 0;
+    if (ef && !c_bary) return;
+    int p = cr[0];
     if (p < 0 || p >= n_particles) return;
     int s = c_shape[i];
     if (s < 0 || s >= shape_count) return;
     float margin = shape_count > 0 ? shape_margin[s] : 0.0f;
     V3 n = ld3(normal, i);
-    float pen = -(dot(n, sub(ld3(q, p), ld3(body_pos, i))) - radius[p] - margin);
+    V3 bx, bv;
+    contact_body_frame(bodies, s, ld3(body_pos, i), ld3(body_vel, i), dt, bx, bv);
+    V3 x;
+    float rad;
+    if (!ef) {
+        x = ld3(q, p);
+        rad = radius[p];
+    } else {
+        x = scl(ld3(q, cr[0]), c_bary[3 * i]);
+        rad = radius[cr[0]];
+        if (cr[1] >= 0) {
+            x = add(x, scl(ld3(q, cr[1]), c_bary[3 * i + 1]));
+            rad = fmaxf(rad, radius[cr[1]]);
+        }
+        if (cr[2] >= 0) {
+            x = add(x, scl(ld3(q, cr[2]), c_bary[3 * i + 2]));
+            rad = fmaxf(rad, radius[cr[2]]);
+        }
+    }
+    float pen = -(dot(n, sub(x, bx)) - rad - margin);
     pen = fmaxf(0.0f, pen);
     float k = fminf(pk[i] + beta * pen, mke[i]);
     pk[i] = k;
-    int slot = cid_to_pack[i];
-    if (slot >= 0) cpack[16 * (size_t)slot + 2] = k;
+    for (int c = 0; c < 3; ++c) {
+        int slot = cid_to_pack[3 * (size_t)i + c];
+        if (slot >= 0) cpack[16 * (size_t)slot + 2] = k;
+    }
 }
 
 // Squared residual norm in two deterministic stages: per-workgroup partial
@@ -1489,53 +1648,149 @@ __global__ void k_set_iterate(KParams P, float* __restrict__ q_dst, const float*
     st3(q_dst, i, add(x0, scl(d, 1.0f)));
 }
 
-// ---- soft-contact generation (geometry/kernels.py:1020-1184) -------------
+// ---- soft-contact generation (geometry/kernels.py:1020-1184) and the full-surface edge / face passes
+// (geometry/soft_contacts_sdf.py:240-551), plane and sphere shapes, static or attached to a body -------
+// One index space of candidate pairs: [0, P) (particle, shape), [P, P + E) (edge, shape),
+// [P + E, P + E + F) (triangle, shape); records are written in that order.
 struct ColParams {
-    int P;
+    int P, E, F;
     const int* pairs;
+    const int* epairs;
+    const int* fpairs;
+    const int* edge_idx;  // [edges, 4] (o0, o1, v0, v1)
+    const int* tri_idx;   // [T, 3]
     const float* radius;
     const int* pflags;
     const int* pworld;
     const int* stype;
     const int* sflags;
     const int* sworld;
+    const int* sbody;     // null: every shape static
     const float* sxf;
     const float* sscale;
     const float* smargin;
+    const float* body_q;  // this call's body poses (null without bodies)
+    int body_count;
 };
-
-__device__ __forceinline__ V3 quat_rotate(float qx, float qy, float qz, float qw, V3 x) {
-    float c = 2.0f * qw * qw - 1.0f;
-    float d = 2.0f * (qx * x.x + qy * x.y + qz * x.z);
-    return v3(x.x * c + qx * d + (qy * x.z - qz * x.y) * qw * 2.0f, x.y * c + qy * d + (qz * x.x - qx * x.z) * qw * 2.0f,
-              x.z * c + qz * d + (qx * x.y - qy * x.x) * qw * 2.0f);
-}
 
 struct ContactGeo {
     bool hit;
     int particle, shape;
+    int i0, i1, i2;
+    V3 bary;
     V3 body_pos, body_vel, normal;
 };
 
-__device__ ContactGeo soft_contact_geometry(const ColParams& C, const float* __restrict__ q, float margin, int tid) {
+// shape-local -> body (X_bs), -> world (X_ws) and world -> shape-local (X_sw); a static shape has
+// X_ws = X_bs (transform_multiply with the identity gives X_bs up to signed zeros)
+__device__ __forceinline__ void shape_frames(const ColParams& C, int s, Xf& X_bs, Xf& X_ws, Xf& X_sw) {
+    X_bs = xf_load(C.sxf + 7 * (size_t)s);
+    const int b = C.sbody ? C.sbody[s] : -1;
+    X_ws = (b >= 0 && C.body_q && b < C.body_count) ? xf_mul(xf_load(C.body_q + 7 * (size_t)b), X_bs) : X_bs;
+    X_sw = xf_inverse(X_ws);
+}
+
+// eval_shape_sdf (soft_contacts_sdf.py:61-96) for this path's analytic shapes: sphere |x| - r with the
+// gradient x / |x| ((0, 0, 1) at the centre), infinite plane x.z with gradient +z
+__device__ __forceinline__ float shape_sdf(int type, const float* __restrict__ sc, V3 x, V3& grad) {
+    if (type == 3) {
+        const float d = length(x) - sc[0];
+        const float l = length(x);
+        grad = l > 1.0e-8f ? divs(x, l) : v3(0.0f, 0.0f, 1.0f);
+        return d;
+    }
+    grad = v3(0.0f, 0.0f, 1.0f);
+    return x.z;
+}
+
+__device__ __forceinline__ V3 seg_point(V3 p, V3 q, float u) { return add(scl(p, 1.0f - u), scl(q, u)); }
+
+// optimize_edge_sdf (soft_contacts_sdf.py:136-175): golden-section search along p -> q, fixed iterations
+__device__ float opt_edge_sdf(int type, const float* __restrict__ sc, V3 p, V3 q, int n_iter, V3& x_out,
+                              float& phi_out, V3& g_out) {
+    const float inv_phi = 0.6180339887498949f;
+    float lo = 0.0f, hi = 1.0f;
+    float c = hi - (hi - lo) * inv_phi;
+    float d = lo + (hi - lo) * inv_phi;
+    V3 g;
+    float fc = shape_sdf(type, sc, seg_point(p, q, c), g);
+    float fd = shape_sdf(type, sc, seg_point(p, q, d), g);
+    for (int i = 0; i < n_iter; ++i) {
+        if (fc < fd) {
+            hi = d;
+            d = c;
+            fd = fc;
+            c = hi - (hi - lo) * inv_phi;
+            fc = shape_sdf(type, sc, seg_point(p, q, c), g);
+        } else {
+            lo = c;
+            c = d;
+            fc = fd;
+            d = lo + (hi - lo) * inv_phi;
+            fd = shape_sdf(type, sc, seg_point(p, q, d), g);
+        }
+    }
+    const float u = 0.5f * (lo + hi);
+    x_out = seg_point(p, q, u);
+    phi_out = shape_sdf(type, sc, x_out, g_out);
+    return u;
+}
+
+__device__ __forceinline__ V3 bary_point(V3 a, V3 b, V3 c, V3 w) {
+    return add(add(scl(a, w.x), scl(b, w.y)), scl(c, w.z));
+}
+
+// optimize_face_sdf (soft_contacts_sdf.py:178-219): Frank-Wolfe on the barycentric simplex from the
+// centroid; each step line-searches toward the corner minimising grad . corner
+__device__ V3 opt_face_sdf(int type, const float* __restrict__ sc, V3 a, V3 b, V3 c, int n_iter, int ls_iter,
+                           V3& x_out, float& phi_out, V3& g_out) {
+    const float third = 1.0f / 3.0f;
+    V3 bary = v3(third, third, third);
+    for (int i = 0; i < n_iter; ++i) {
+        const V3 x = bary_point(a, b, c, bary);
+        V3 g;
+        shape_sdf(type, sc, x, g);
+        const float da = dot(g, a), db = dot(g, b), dc = dot(g, c);
+        V3 s = v3(1.0f, 0.0f, 0.0f);
+        if (db <= da && db <= dc)
+            s = v3(0.0f, 1.0f, 0.0f);
+        else if (dc <= da && dc <= db)
+            s = v3(0.0f, 0.0f, 1.0f);
+        const V3 target = bary_point(a, b, c, s);
+        V3 lx, lg;
+        float lphi;
+        const float gamma = opt_edge_sdf(type, sc, x, target, ls_iter, lx, lphi, lg);
+        bary = add(scl(bary, 1.0f - gamma), scl(s, gamma));
+    }
+    x_out = bary_point(a, b, c, bary);
+    phi_out = shape_sdf(type, sc, x_out, g_out);
+    return bary;
+}
+
+constexpr int kSdfEdgeIters = 24;  // soft_contacts_sdf.py:43-45
+constexpr int kSdfFaceIters = 24;
+constexpr int kSdfLsIters = 16;
+
+__device__ ContactGeo particle_contact_geometry(const ColParams& C, const float* __restrict__ q, float margin,
+                                                int tid) {
     ContactGeo g;
     g.hit = false;
     int p = C.pairs[2 * tid], s = C.pairs[2 * tid + 1];
     g.particle = p;
     g.shape = s;
+    g.i0 = p;
+    g.i1 = -1;
+    g.i2 = -1;
+    g.bary = v3(1.0f, 0.0f, 0.0f);
     if ((C.pflags[p] & 1) == 0) return g;
     if ((C.sflags[s] & 4) == 0) return g;
     int pw = C.pworld[p], sw = C.sworld[s];
     if (pw != -1 && sw != -1 && pw != sw) return g;
     V3 px = ld3(q, p);
     float radius = C.radius[p];
-    const float* tf = C.sxf + 7 * s;
-    V3 bp = v3(tf[0], tf[1], tf[2]);
-    float qx = tf[3], qy = tf[4], qz = tf[5], qw = tf[6];
-    // X_ws == X_bs for static shapes; X_sw = inverse(X_ws)
-    float iqx = -qx, iqy = -qy, iqz = -qz, iqw = qw;
-    V3 ip = neg(quat_rotate(iqx, iqy, iqz, iqw, bp));
-    V3 xl = add(quat_rotate(iqx, iqy, iqz, iqw, px), ip);
+    Xf X_bs, X_ws, X_sw;
+    shape_frames(C, s, X_bs, X_ws, X_sw);
+    V3 xl = xf_point(X_sw, px);
     const float* sc = C.sscale + 3 * s;
     float s_margin = C.smargin[s];
     float dd = 1.0e6f;
@@ -1559,11 +1814,95 @@ __device__ ContactGeo soft_contact_geometry(const ColParams& C, const float* __r
     }
     if (dd < margin + s_margin + radius) {
         g.hit = true;
-        g.body_pos = add(quat_rotate(qx, qy, qz, qw, sub(xl, scl(n, dd))), bp);
-        g.body_vel = quat_rotate(qx, qy, qz, qw, v3(0.0f, 0.0f, 0.0f));
-        g.normal = quat_rotate(qx, qy, qz, qw, n);
+        g.body_pos = xf_point(X_bs, sub(xl, scl(n, dd)));
+        g.body_vel = xf_vector(X_bs, v3(0.0f, 0.0f, 0.0f));
+        g.normal = xf_vector(X_ws, n);
     }
     return g;
+}
+
+// create_soft_edge_contacts (soft_contacts_sdf.py:380-469)
+__device__ ContactGeo edge_contact_geometry(const ColParams& C, const float* __restrict__ q, float margin, int k) {
+    ContactGeo g;
+    g.hit = false;
+    g.particle = -1;
+    const int e = C.epairs[2 * k], s = C.epairs[2 * k + 1];
+    g.shape = s;
+    if ((C.sflags[s] & 4) == 0) return g;
+    const int type = C.stype[s];
+    if (type != 1 && type != 3) return g;
+    const int v0 = C.edge_idx[4 * e + 2], v1 = C.edge_idx[4 * e + 3];
+    const float radius = fmaxf(C.radius[v0], C.radius[v1]);
+    Xf X_bs, X_ws, X_sw;
+    shape_frames(C, s, X_bs, X_ws, X_sw);
+    const V3 p_s = xf_point(X_sw, ld3(q, v0));
+    const V3 q_s = xf_point(X_sw, ld3(q, v1));
+    const float* sc = C.sscale + 3 * s;
+    const float threshold = margin + C.smargin[s] + radius;
+    V3 gr;
+    const float phi_m = shape_sdf(type, sc, scl(add(p_s, q_s), 0.5f), gr);
+    if (phi_m > threshold + 0.5f * length(sub(q_s, p_s))) return g;
+    V3 x;
+    float phi;
+    const float u = opt_edge_sdf(type, sc, p_s, q_s, kSdfEdgeIters, x, phi, gr);
+    if (phi < threshold) {
+        g.hit = true;
+        g.i0 = v0;
+        g.i1 = v1;
+        g.i2 = -1;
+        g.bary = v3(1.0f - u, u, 0.0f);
+        g.body_pos = xf_point(X_bs, sub(x, scl(gr, phi)));
+        g.body_vel = v3(0.0f, 0.0f, 0.0f);
+        g.normal = xf_vector(X_ws, gr);
+    }
+    return g;
+}
+
+// create_soft_face_contacts (soft_contacts_sdf.py:277-377)
+__device__ ContactGeo face_contact_geometry(const ColParams& C, const float* __restrict__ q, float margin, int k) {
+    ContactGeo g;
+    g.hit = false;
+    g.particle = -1;
+    const int t = C.fpairs[2 * k], s = C.fpairs[2 * k + 1];
+    g.shape = s;
+    if ((C.sflags[s] & 4) == 0) return g;
+    const int type = C.stype[s];
+    if (type != 1 && type != 3) return g;
+    const int ia = C.tri_idx[3 * t], ib = C.tri_idx[3 * t + 1], ic = C.tri_idx[3 * t + 2];
+    const float radius = fmaxf(C.radius[ia], fmaxf(C.radius[ib], C.radius[ic]));
+    Xf X_bs, X_ws, X_sw;
+    shape_frames(C, s, X_bs, X_ws, X_sw);
+    const V3 a_s = xf_point(X_sw, ld3(q, ia));
+    const V3 b_s = xf_point(X_sw, ld3(q, ib));
+    const V3 c_s = xf_point(X_sw, ld3(q, ic));
+    const float* sc = C.sscale + 3 * s;
+    const float threshold = margin + C.smargin[s] + radius;
+    const V3 cen = divs(add(add(a_s, b_s), c_s), 3.0f);
+    V3 gr;
+    const float phi_c = shape_sdf(type, sc, cen, gr);
+    const float reach = fmaxf(length(sub(a_s, cen)), fmaxf(length(sub(b_s, cen)), length(sub(c_s, cen))));
+    if (phi_c > threshold + reach) return g;
+    V3 x;
+    float phi;
+    const V3 bary = opt_face_sdf(type, sc, a_s, b_s, c_s, kSdfFaceIters, kSdfLsIters, x, phi, gr);
+    if (phi < threshold) {
+        g.hit = true;
+        g.i0 = ia;
+        g.i1 = ib;
+        g.i2 = ic;
+        g.bary = bary;
+        g.body_pos = xf_point(X_bs, sub(x, scl(gr, phi)));
+        g.body_vel = v3(0.0f, 0.0f, 0.0f);
+        g.normal = xf_vector(X_ws, gr);
+    }
+    return g;
+}
+
+__device__ __forceinline__ ContactGeo soft_contact_geometry(const ColParams& C, const float* __restrict__ q,
+                                                            float margin, int tid) {
+    if (tid < C.P) return particle_contact_geometry(C, q, margin, tid);
+    if (tid < C.P + C.E) return edge_contact_geometry(C, q, margin, tid - C.P);
+    return face_contact_geometry(C, q, margin, tid - C.P - C.E);
 }
 
 // ---- soft-contact generation from per-tile hit counts ------------------------
@@ -1587,7 +1926,7 @@ __global__ __launch_bounds__(256) void k_collide_tile_count(ColParams C, const f
     int count = 0;
     for (int r = 0; r < kColRounds; ++r) {
         int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
-        bool hit = tid < C.P && soft_contact_geometry(C, q, margin, tid).hit;
+        bool hit = tid < C.P + C.E + C.F && soft_contact_geometry(C, q, margin, tid).hit;
         int t;
         wave_rank(hit, lane, &t);
         count += t;
@@ -1633,7 +1972,7 @@ __global__ __launch_bounds__(256) void k_collide_tile_write(ColParams C, const f
         int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
         ContactGeo g;
         g.hit = false;
-        if (tid < C.P) g = soft_contact_geometry(C, q, margin, tid);
+        if (tid < C.P + C.E + C.F) g = soft_contact_geometry(C, q, margin, tid);
         int t;
         int rank = wave_rank(g.hit, lane, &t);
         if (lane == 0) wsum[r][wave] = t;
@@ -1647,10 +1986,10 @@ __global__ __launch_bounds__(256) void k_collide_tile_write(ColParams C, const f
             st3(out.soft_contact_body_pos, k, g.body_pos);
             st3(out.soft_contact_body_vel, k, g.body_vel);
             if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
-            out.soft_contact_indices[3 * k] = g.particle;
-            out.soft_contact_indices[3 * k + 1] = -1;
-            out.soft_contact_indices[3 * k + 2] = -1;
-            if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, v3(1.0f, 0.0f, 0.0f));
+            out.soft_contact_indices[3 * k] = g.i0;
+            out.soft_contact_indices[3 * k + 1] = g.i1;
+            out.soft_contact_indices[3 * k + 2] = g.i2;
+            if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, g.bary);
             st3(out.soft_contact_normal, k, g.normal);
         }
     }
@@ -1683,11 +2022,12 @@ struct VbdContext {
     KParams kp;
     float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
     int* c_cnt = nullptr;       // [N+1] contacts per color position; zero between substeps
-    int* c_ids = nullptr;       // [cap] contact ids in color-position order
+    int* c_ids = nullptr;       // [entry_cap] entry keys (contact id << 2 | corner) in color-position order
     float* c_state = nullptr;   // pk | mke | mkd | mmu, each [cap]
     int* pos_coff = nullptr;    // [N+1]
-    float* cpack = nullptr;     // [cap * 16]
-    int* cid_to_pack = nullptr; // [cap]
+    float* cpack = nullptr;     // [entry_cap * 16]
+    int* cpack_ef = nullptr;    // [entry_cap * 8] corners and weights of edge/face entries (full surface only)
+    int* cid_to_pack = nullptr; // [cap * 3] packed entry of each (contact, corner), -1 when none
     float* g_tmp = nullptr;     // [N,3] residual scratch
     double* r_part = nullptr;   // residual partial sums, one per first-stage workgroup
     double* r_dev = nullptr;    // [1] squared residual norm (iterate-to-tolerance driver)
@@ -1698,6 +2038,8 @@ struct VbdContext {
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int capacity = 0;
+    size_t entry_cap = 0;      // packed entries: capacity, or 3 x capacity with full-surface records
+    int32_t max_body = -1;     // largest Model.shape_body entry (-1: every shape static)
     // pass layout on the device
     int* lay_batch = nullptr;
     int* lay_vert = nullptr;
@@ -1741,6 +2083,8 @@ struct VbdCollide {
     VbdCollideDesc desc;
     ColParams cp;
     int* tiles = nullptr;  // hit count per tile of kColTile pairs, then its exclusive scan
+    bool has_bodies = false;  // some shape is attached to a body (its poses come with each call)
+    int32_t max_body = -1;
 };
 
 namespace {
@@ -2221,7 +2565,7 @@ std::string noalloc_scratch(int32_t** out) {
 // Soft contacts in pair order from per-tile hit counts (count, scan of the tile counts, write).
 int32_t launch_collide_tiles(const ColParams& cp, hipStream_t s, const float* q, float margin, int* tiles_dev,
                              const VbdContactsOut& out) {
-    const int tiles = blocks_for(cp.P, kColTile);
+    const int tiles = blocks_for((long)cp.P + cp.E + cp.F, kColTile);
     hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev);
     VBD_LAUNCH_CHECK("k_collide_tile_count");
     hipLaunchKernelGGL(k_collide_tile_scan, dim3(1), dim3(1024), 0, s, tiles, tiles_dev, out.soft_contact_count);
@@ -2262,6 +2606,9 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
         return fail(VBD_ERR_ARG, "vbd_create: spring arrays missing");
     if (prm->iterations < 0) return fail(VBD_ERR_ARG, "vbd_create: iterations must be >= 0");
     if (prm->soft_contact_capacity < 0) return fail(VBD_ERR_ARG, "vbd_create: negative soft_contact_capacity");
+    if (prm->soft_contact_capacity >= (1 << 29))
+        return fail(VBD_ERR_ARG, "vbd_create: soft_contact_capacity must be below 2^29");
+    if (m->body_count < 0) return fail(VBD_ERR_ARG, "vbd_create: negative body_count");
 
     VbdContext* c = new VbdContext();
     c->model = *m;
@@ -2279,13 +2626,32 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     };
     hipError_t e;
     const size_t cap = (size_t)c->capacity;
+    c->entry_cap = prm->soft_contact_full_surface ? 3 * cap : cap;
+    if (m->shape_body && m->shape_count > 0) {
+        std::vector<int32_t> body;
+        std::string err = d2h(body, m->shape_body, (size_t)m->shape_count);
+        if (!err.empty()) {
+            vbd_destroy(c);
+            return fail(VBD_ERR_HIP, "vbd_create: shape_body: " + err);
+        }
+        for (int32_t b : body) c->max_body = std::max(c->max_body, b);
+        if (c->max_body >= m->body_count) {
+            vbd_destroy(c);
+            return fail(VBD_ERR_ARG, "vbd_create: shape_body names body " + std::to_string(c->max_body) +
+                                         " of a model with body_count " + std::to_string(m->body_count));
+        }
+    }
     if ((e = hipMalloc(&c->scratch3, sizeof(float) * 9 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_cnt, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->pos_coff, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->c_ids, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->cid_to_pack, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_ids, sizeof(int) * c->entry_cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->cid_to_pack, sizeof(int) * 3 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
-    if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * c->entry_cap)) != hipSuccess)
+        return cleanup("hipMalloc", e);
+    if (prm->soft_contact_full_surface &&
+        (e = hipMalloc(&c->cpack_ef, sizeof(int) * 8 * c->entry_cap)) != hipSuccess)
+        return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->g_tmp, sizeof(float) * 3 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->r_part, sizeof(double) * (size_t)norm_parts(N))) != hipSuccess)
         return cleanup("hipMalloc", e);
@@ -2306,7 +2672,7 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     // up and back down to zero)
     if ((e = hipMemset(c->c_cnt, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipMemset(c->pos_coff, 0, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMemset", e);
-    if ((e = hipMemset(c->cid_to_pack, 0xff, sizeof(int) * cap)) != hipSuccess) return cleanup("hipMemset", e);
+    if ((e = hipMemset(c->cid_to_pack, 0xff, sizeof(int) * 3 * cap)) != hipSuccess) return cleanup("hipMemset", e);
     if ((e = hipDeviceSynchronize()) != hipSuccess) return cleanup("sync", e);
 
     KParams& k = c->kp;
@@ -2346,6 +2712,7 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.c_mu = c->c_state + 3 * cap;
     k.pos_coff = c->pos_coff;
     k.cpack = c->cpack;
+    k.cpack_ef = c->cpack_ef;
     k.shape_margin = m->shape_margin;
     k.shape_count = m->shape_count;
     k.friction_epsilon = prm->friction_epsilon;
@@ -2378,6 +2745,7 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->cid_to_pack);
     hipFree(c->c_state);
     hipFree(c->cpack);
+    hipFree(c->cpack_ef);
     hipFree(c->g_tmp);
     hipFree(c->r_part);
     hipFree(c->r_dev);
@@ -2452,6 +2820,19 @@ int32_t vbd_layout_stats(VbdContext* c, int64_t* out4) {
     return VBD_OK;
 }
 
+// Body poses of the current contacts (static shapes when the model has no body-attached shape).
+static BodyPoses body_poses(const VbdContext* c) {
+    BodyPoses B{};
+    if (c->max_body >= 0) {
+        B.shape_body = c->model.shape_body;
+        B.body_q = c->contacts.body_q;
+        B.body_q_prev = c->contacts.body_q_prev;
+        B.body_count = c->model.body_count;
+    }
+    B.shape_count = c->model.shape_count;
+    return B;
+}
+
 static int32_t launch_batch_contacts(VbdContext* c, hipStream_t s) {
     const int nbat = c->fast_color_off.back();
     if (nbat <= 0) return VBD_OK;
@@ -2480,9 +2861,19 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
         return fail(VBD_ERR_ARG, "vbd_contact_prepare: missing contact arrays");
     if (c->model.shape_count <= 0 || !c->model.shape_material_ke)
         return fail(VBD_ERR_ARG, "vbd_contact_prepare: contacts given but the model has no shapes");
+    if (cd->soft_contact_barycentric && !c->params.soft_contact_full_surface)
+        return fail(VBD_ERR_STATE, "vbd_contact_prepare: barycentric weights given, but the solver was created "
+                                   "without soft_contact_full_surface (no room for edge/face entries)");
+    if (c->max_body >= 0) {
+        if (!cd->body_q || !cd->body_q_prev)
+            return fail(VBD_ERR_ARG, "vbd_contact_prepare: shapes are attached to bodies; body_q and body_q_prev "
+                                     "(the external rigid solver's poses at the step's end and start) are required");
+        if (!(cd->dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_contact_prepare: dt must be positive with body poses");
+    }
     c->contacts = *cd;
     c->have_contacts = true;
     const int cap = c->capacity;
+    const int ef = cd->soft_contact_barycentric != nullptr;
     float* pk = c->c_state;
     float* mke = c->c_state + cap;
     float* mkd = c->c_state + 2 * (size_t)cap;
@@ -2493,23 +2884,37 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
                        cd->soft_contact_indices, cd->soft_contact_shape, c->model.soft_contact_ke,
                        c->model.soft_contact_kd, c->model.soft_contact_mu, c->model.shape_material_ke,
                        c->model.shape_material_kd, c->model.shape_material_mu, c->params.k_start, pk, mke, mkd, mmu,
-                       c->pos_of, c->c_cnt, N, c->model.shape_count);
+                       c->pos_of, c->c_cnt, N, c->model.shape_count, ef);
     VBD_LAUNCH_CHECK("k_contact_init_pos");
     size_t bytes = c->cub_bytes;
     VBD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, c->c_cnt, c->pos_coff, N + 1, s));
     hipLaunchKernelGGL(k_contact_place, dim3(cb), dim3(256), 0, s, cd->soft_contact_count, cd->soft_contact_max,
                        cd->soft_contact_indices, cd->soft_contact_shape, c->pos_of, c->pos_coff, c->c_cnt, c->c_ids,
-                       N, c->model.shape_count);
+                       N, c->model.shape_count, ef);
     VBD_LAUNCH_CHECK("k_contact_place");
     if (c->params.avbd_beta != 0.0f) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(cap, 256)), dim3(256), 0, s, c->cid_to_pack, -1, cap);
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(3 * (long)cap, 256)), dim3(256), 0, s, c->cid_to_pack, -1,
+                           3 * cap);
         VBD_LAUNCH_CHECK("k_fill_i32");
     }
+    PackArgs pa;
+    pa.pk = pk;
+    pa.mkd = mkd;
+    pa.mmu = mmu;
+    pa.c_indices = cd->soft_contact_indices;
+    pa.c_bary = cd->soft_contact_barycentric;
+    pa.c_shape = cd->soft_contact_shape;
+    pa.body_pos = cd->soft_contact_body_pos;
+    pa.body_vel = cd->soft_contact_body_vel;
+    pa.normal = cd->soft_contact_normal;
+    pa.shape_margin = c->model.shape_margin;
+    pa.shape_count = c->model.shape_count;
+    pa.radius = c->model.particle_radius;
+    pa.bodies = body_poses(c);
+    pa.dt = cd->dt;
     hipLaunchKernelGGL(k_contact_sort_pack, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->model.color_particles,
-                       c->lay_vert, c->pos_coff, c->c_ids, pk, mkd, mmu, cd->soft_contact_shape,
-                       cd->soft_contact_body_pos, cd->soft_contact_body_vel, cd->soft_contact_normal,
-                       c->model.shape_margin, c->model.shape_count, c->model.particle_radius, c->cpack,
-                       c->cid_to_pack);
+                       c->lay_vert, c->pos_coff, c->c_ids, pa, c->cpack, c->cpack_ef,
+                       c->params.avbd_beta != 0.0f ? c->cid_to_pack : nullptr);
     VBD_LAUNCH_CHECK("k_contact_sort_pack");
     int32_t rc = launch_batch_contacts(c, s);
     if (rc) return rc;
@@ -2645,10 +3050,12 @@ int32_t vbd_iteration(VbdContext* c, void* stream, float* q, float dt) {
     if (c->have_contacts && c->params.avbd_beta != 0.0f) {
         const VbdContactsDesc& cd = c->contacts;
         hipLaunchKernelGGL(k_dual_update, dim3(blocks_for(cd.soft_contact_max, 256)), dim3(256), 0, s,
-                           cd.soft_contact_count, cd.soft_contact_max, cd.soft_contact_indices, cd.soft_contact_shape,
-                           cd.soft_contact_body_pos, cd.soft_contact_normal, q, c->model.particle_radius,
-                           c->model.shape_margin, c->model.shape_count, c->c_state + c->capacity,
-                           c->params.avbd_beta, c->c_state, c->model.particle_count, c->cid_to_pack, c->cpack);
+                           cd.soft_contact_count, cd.soft_contact_max, cd.soft_contact_indices,
+                           cd.soft_contact_barycentric, cd.soft_contact_shape, cd.soft_contact_body_pos,
+                           cd.soft_contact_body_vel, cd.soft_contact_normal, q, c->model.particle_radius,
+                           c->model.shape_margin, c->model.shape_count, body_poses(c), cd.dt,
+                           c->c_state + c->capacity, c->params.avbd_beta, c->c_state, c->model.particle_count,
+                           c->cid_to_pack, c->cpack);
         VBD_LAUNCH_CHECK("k_dual_update");
     }
     for (int color = 0; color < c->model.color_count; ++color) {
@@ -2689,7 +3096,12 @@ int32_t vbd_step(VbdContext* c, void* stream, float* q_in, const float* qd_in, c
                  float* qd_out, const VbdContactsDesc* contacts, float dt) {
     if (!c) return fail(VBD_ERR_ARG, "vbd_step: null context");
     if (!(dt > 0.0f)) return fail(VBD_ERR_ARG, "vbd_step: dt must be positive");
-    int32_t rc = vbd_contact_prepare(c, stream, contacts);
+    VbdContactsDesc cd{};
+    if (contacts) {
+        cd = *contacts;
+        cd.dt = dt;
+    }
+    int32_t rc = vbd_contact_prepare(c, stream, contacts ? &cd : nullptr);
     if (rc) return rc;
     rc = vbd_forward(c, stream, q_in, qd_in, f_ext, dt);
     if (rc) return rc;
@@ -3167,29 +3579,91 @@ int32_t vbd_iterate_to_residual(VbdContext* c, void* stream, float* q, float dt,
     return VBD_OK;
 }
 
-int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
-    if (!d || !out) return fail(VBD_ERR_ARG, "vbd_collide_create: null argument");
-    *out = nullptr;
-    if (d->pair_count < 0) return fail(VBD_ERR_ARG, "vbd_collide_create: negative pair_count");
-    if (d->pair_count > 0 && (!d->pairs || !d->particle_radius || !d->particle_flags || !d->particle_world ||
-                              !d->shape_type || !d->shape_flags || !d->shape_world || !d->shape_transform ||
-                              !d->shape_scale || !d->shape_margin))
-        return fail(VBD_ERR_ARG, "vbd_collide_create: missing arrays");
-    VbdCollide* col = new VbdCollide();
-    col->desc = *d;
-    ColParams& cp = col->cp;
+namespace {
+
+// Checks a collision descriptor and fills the kernels' parameters (body poses are set per call).
+int32_t collide_params(const VbdCollideDesc* d, const char* who, ColParams& cp) {
+    if (d->pair_count < 0 || d->edge_pair_count < 0 || d->face_pair_count < 0)
+        return fail(VBD_ERR_ARG, std::string(who) + ": negative pair count");
+    const long total = (long)d->pair_count + d->edge_pair_count + d->face_pair_count;
+    if (total > INT32_MAX) return fail(VBD_ERR_ARG, std::string(who) + ": more than 2^31 - 1 candidate pairs");
+    if (total > 0 && (!d->particle_radius || !d->shape_type || !d->shape_flags || !d->shape_world ||
+                      !d->shape_transform || !d->shape_scale || !d->shape_margin))
+        return fail(VBD_ERR_ARG, std::string(who) + ": missing arrays");
+    if (d->pair_count > 0 && (!d->pairs || !d->particle_flags || !d->particle_world))
+        return fail(VBD_ERR_ARG, std::string(who) + ": missing particle-pair arrays");
+    if (d->edge_pair_count > 0 && (!d->edge_pairs || !d->edge_indices))
+        return fail(VBD_ERR_ARG, std::string(who) + ": edge pairs given without edge_pairs / edge_indices");
+    if (d->face_pair_count > 0 && (!d->face_pairs || !d->tri_indices))
+        return fail(VBD_ERR_ARG, std::string(who) + ": face pairs given without face_pairs / tri_indices");
+    cp = ColParams{};
     cp.P = d->pair_count;
+    cp.E = d->edge_pair_count;
+    cp.F = d->face_pair_count;
     cp.pairs = d->pairs;
+    cp.epairs = d->edge_pairs;
+    cp.fpairs = d->face_pairs;
+    cp.edge_idx = d->edge_indices;
+    cp.tri_idx = d->tri_indices;
     cp.radius = d->particle_radius;
     cp.pflags = d->particle_flags;
     cp.pworld = d->particle_world;
     cp.stype = d->shape_type;
     cp.sflags = d->shape_flags;
     cp.sworld = d->shape_world;
+    cp.sbody = d->shape_body;
     cp.sxf = d->shape_transform;
     cp.sscale = d->shape_scale;
     cp.smargin = d->shape_margin;
-    const int ntiles = std::max(blocks_for(d->pair_count, kColTile), 1);
+    return VBD_OK;
+}
+
+bool contacts_out_ok(const VbdContactsOut* out) {
+    return out->soft_contact_indices && out->soft_contact_shape && out->soft_contact_body_pos &&
+           out->soft_contact_body_vel && out->soft_contact_normal;
+}
+
+int32_t collide_run(VbdCollide* col, hipStream_t s, const float* q, const float* body_q, int32_t body_count,
+                    float margin, const VbdContactsOut* out, const char* who) {
+    const long total = (long)col->cp.P + col->cp.E + col->cp.F;
+    if (total > 0 && !contacts_out_ok(out)) return fail(VBD_ERR_ARG, std::string(who) + ": missing output arrays");
+    if (col->has_bodies && (!body_q || body_count < col->max_body + 1))
+        return fail(VBD_ERR_ARG, std::string(who) + ": shapes are attached to bodies; body_q must hold at least " +
+                                     std::to_string(col->max_body + 1) + " poses");
+    if (total == 0 || out->soft_contact_max <= 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out->soft_contact_count, 0, 1);
+        VBD_LAUNCH_CHECK("k_fill_i32");
+        return VBD_OK;
+    }
+    ColParams cp = col->cp;
+    cp.body_q = body_q;
+    cp.body_count = body_count;
+    return launch_collide_tiles(cp, s, q, margin, col->tiles, *out);
+}
+
+}  // namespace
+
+int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
+    if (!d || !out) return fail(VBD_ERR_ARG, "vbd_collide_create: null argument");
+    *out = nullptr;
+    ColParams cp;
+    int32_t rc = collide_params(d, "vbd_collide_create", cp);
+    if (rc) return rc;
+    VbdCollide* col = new VbdCollide();
+    col->desc = *d;
+    col->cp = cp;
+    if (d->shape_body && d->shape_count > 0) {
+        std::vector<int32_t> body;
+        std::string err = d2h(body, d->shape_body, (size_t)d->shape_count);
+        if (!err.empty()) {
+            vbd_collide_destroy(col);
+            return fail(VBD_ERR_HIP, "vbd_collide_create: shape_body: " + err);
+        }
+        for (int32_t b : body) col->max_body = std::max(col->max_body, b);
+        col->has_bodies = col->max_body >= 0;
+    }
+    const long total = (long)cp.P + cp.E + cp.F;
+    const int ntiles = std::max(blocks_for(total, kColTile), 1);
     hipError_t e = hipMalloc(&col->tiles, sizeof(int) * (size_t)ntiles);
     if (e != hipSuccess) {
         std::string msg = std::string("vbd_collide_create: ") + hipGetErrorString(e);
@@ -3209,36 +3683,37 @@ int32_t vbd_collide_destroy(VbdCollide* col) {
 
 int32_t vbd_collide_generate(VbdCollide* col, void* stream, const float* q, float margin, const VbdContactsOut* out) {
     if (!col || !q || !out || !out->soft_contact_count) return fail(VBD_ERR_ARG, "vbd_collide_generate: null argument");
-    if (col->cp.P > 0 && (!out->soft_contact_indices || !out->soft_contact_shape || !out->soft_contact_body_pos ||
-                          !out->soft_contact_body_vel || !out->soft_contact_normal))
-        return fail(VBD_ERR_ARG, "vbd_collide_generate: missing output arrays");
-    hipStream_t s = (hipStream_t)stream;
-    if (col->cp.P == 0 || out->soft_contact_max <= 0) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out->soft_contact_count, 0, 1);
-        VBD_LAUNCH_CHECK("k_fill_i32");
-        return VBD_OK;
-    }
-    return launch_collide_tiles(col->cp, s, q, margin, col->tiles, *out);
+    return collide_run(col, (hipStream_t)stream, q, nullptr, 0, margin, out, "vbd_collide_generate");
+}
+
+int32_t vbd_collide_generate_bodies(VbdCollide* col, void* stream, const float* q, const float* body_q,
+                                    int32_t body_count, float margin, const VbdContactsOut* out) {
+    if (!col || !q || !out || !out->soft_contact_count)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_bodies: null argument");
+    if (body_count < 0) return fail(VBD_ERR_ARG, "vbd_collide_generate_bodies: negative body_count");
+    return collide_run(col, (hipStream_t)stream, q, body_count > 0 ? body_q : nullptr, body_count, margin, out,
+                       "vbd_collide_generate_bodies");
 }
 
 int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, const float* q, float margin,
                                      const VbdContactsOut* out) {
     if (!d || !q || !out || !out->soft_contact_count)
         return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: null argument");
-    if (d->pair_count < 0) return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: negative pair_count");
-    if (d->pair_count > VBD_NOALLOC_MAX_PAIRS)
-        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: pair_count " + std::to_string(d->pair_count) +
-                                     " exceeds the maximum of " + std::to_string(VBD_NOALLOC_MAX_PAIRS) +
+    ColParams cp;
+    int32_t rc = collide_params(d, "vbd_collide_generate_noalloc", cp);
+    if (rc) return rc;
+    const long total = (long)cp.P + cp.E + cp.F;
+    if (total > VBD_NOALLOC_MAX_PAIRS)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: " + std::to_string(total) +
+                                     " candidate pairs exceed the maximum of " + std::to_string(VBD_NOALLOC_MAX_PAIRS) +
                                      " (use vbd_collide_create / vbd_collide_generate)");
-    if (d->pair_count > 0 && (!d->pairs || !d->particle_radius || !d->particle_flags || !d->particle_world ||
-                              !d->shape_type || !d->shape_flags || !d->shape_world || !d->shape_transform ||
-                              !d->shape_scale || !d->shape_margin))
-        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: missing arrays");
-    if (d->pair_count > 0 && (!out->soft_contact_indices || !out->soft_contact_shape || !out->soft_contact_body_pos ||
-                              !out->soft_contact_body_vel || !out->soft_contact_normal))
+    if (d->shape_body)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: shapes are treated as static here (shape_body must be "
+                                 "NULL; use vbd_collide_generate_bodies for body-attached shapes)");
+    if (total > 0 && !contacts_out_ok(out))
         return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: missing output arrays");
     hipStream_t s = (hipStream_t)stream;
-    if (d->pair_count == 0 || out->soft_contact_max <= 0) {
+    if (total == 0 || out->soft_contact_max <= 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out->soft_contact_count, 0, 1);
         VBD_LAUNCH_CHECK("k_fill_i32");
         return VBD_OK;
@@ -3246,18 +3721,6 @@ int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, cons
     int32_t* tiles_dev = nullptr;
     std::string err = noalloc_scratch(&tiles_dev);
     if (!err.empty()) return fail(VBD_ERR_HIP, "vbd_collide_generate_noalloc: " + err);
-    ColParams cp;
-    cp.P = d->pair_count;
-    cp.pairs = d->pairs;
-    cp.radius = d->particle_radius;
-    cp.pflags = d->particle_flags;
-    cp.pworld = d->particle_world;
-    cp.stype = d->shape_type;
-    cp.sflags = d->shape_flags;
-    cp.sworld = d->shape_world;
-    cp.sxf = d->shape_transform;
-    cp.sscale = d->shape_scale;
-    cp.smargin = d->shape_margin;
     return launch_collide_tiles(cp, s, q, margin, tiles_dev, *out);
 }
 

@@ -211,8 +211,20 @@ class ModelBuilder:
         # materialised on first read (merging W worlds one by one stays O(total))
         self._color_plan: list = []
         self._color_groups_cache: list[np.ndarray] | None = []
-        self.body_count = 0
+        # rigid bodies: poses only (this package does not integrate them; an external rigid solver or the
+        # caller moves them and the particle solver reads their poses, one-way coupling)
+        self.body_q: list[tuple] = []
+        self.body_com: list[tuple] = []
+        self.body_mass: list[float] = []
+        self.body_inertia: list[np.ndarray] = []
+        self.body_flags: list[int] = []
+        self.body_world: list[int] = []
+        self.body_label: list[str] = []
         self.joint_count = 0
+
+    @property
+    def body_count(self) -> int:
+        return len(self.body_q)
 
     # ------------------------------------------------------------------ counts
     @property
@@ -714,14 +726,36 @@ class ModelBuilder:
         if add_surface_mesh_edges and end_tri > start_tri:
             self._add_soft_mesh_edges_from_triangles(start_tri, end_tri, edge_ke=edge_ke, edge_kd=edge_kd)
 
+    # ----------------------------------------------------------------- bodies
+    def add_body(self, *, xform=None, com=None, inertia=None, mass: float = 0.0, label: str | None = None,
+                 lock_inertia: bool = False, is_kinematic: bool = False, custom_attributes: dict | None = None) -> int:
+        """A rigid body whose pose the caller (or an external rigid solver) drives (``builder.py:4430-4500``).
+
+        Shapes attached to it (``add_shape_*(body=...)``) follow ``State.body_q``; ``SolverVBD`` with
+        ``integrate_with_external_rigid_solver=True`` reads the poses for particle contact and never moves
+        the body.  The joint and articulation the reference adds with a free body are not built here."""
+        if xform is None:
+            xform = (np.zeros(3, np.float32), np.array([0.0, 0.0, 0.0, 1.0], np.float32))
+        p, q = xform
+        self.body_q.append(tuple(float(v) for v in np.asarray(p, np.float32)) +
+                           tuple(float(v) for v in np.asarray(q, np.float32)))
+        self.body_com.append(tuple(float(v) for v in np.asarray(com if com is not None else (0.0, 0.0, 0.0),
+                                                                np.float32)))
+        self.body_mass.append(float(mass))
+        self.body_inertia.append(np.asarray(inertia if inertia is not None else np.zeros((3, 3)), np.float32))
+        self.body_flags.append(1 if is_kinematic else 0)
+        self.body_world.append(self.current_world)
+        self.body_label.append(label or f"body_{len(self.body_q) - 1}")
+        return len(self.body_q) - 1
+
     # ----------------------------------------------------------------- shapes
     def _add_shape(self, *, type: int, xform, cfg, scale, body: int, label: str | None) -> int:
-        if body != -1:
-            raise NotImplementedError("only static shapes (body=-1) collide with particles on this path")
+        if body < -1 or body >= self.body_count:
+            raise ValueError(f"body {body} does not exist (body_count {self.body_count})")
         cfg = self.default_shape_cfg if cfg is None else cfg
         p, q = xform
         self.shape_type.append(int(type))
-        self.shape_body.append(-1)
+        self.shape_body.append(int(body))
         self.shape_transform.append(tuple(float(v) for v in np.asarray(p, np.float32)) +
                                     tuple(float(v) for v in np.asarray(q, np.float32)))
         self.shape_scale.append(tuple(float(v) for v in np.asarray(scale, np.float32)))
@@ -838,15 +872,31 @@ class ModelBuilder:
             self._tet.extend(builder.tet_indices + off)
             self._tet_pose.extend(builder._tet_pose.array())
             self._tet_mat.extend(builder._tet_mat.array())
+        body_off = self.body_count
+        for b in range(builder.body_count):
+            bq = builder.body_q[b]
+            if xform is not None:
+                p, q = xform
+                bp = quat_rotate(q, np.asarray(bq[:3], np.float32)) + np.asarray(p, np.float32)
+                br = _quat_mul(np.asarray(q, np.float32), np.asarray(bq[3:], np.float32))
+                bq = tuple(float(v) for v in bp) + tuple(float(v) for v in br)
+            self.body_q.append(bq)
+            self.body_com.append(builder.body_com[b])
+            self.body_mass.append(builder.body_mass[b])
+            self.body_inertia.append(builder.body_inertia[b])
+            self.body_flags.append(builder.body_flags[b])
+            self.body_world.append(w)
+            self.body_label.append(builder.body_label[b])
         for s in range(builder.shape_count):
             tf = builder.shape_transform[s]
-            if xform is not None:
+            body = builder.shape_body[s]
+            if xform is not None and body < 0:  # a body's shapes stay in its frame
                 p, q = xform
                 sp = quat_rotate(q, np.asarray(tf[:3], np.float32)) + np.asarray(p, np.float32)
                 sq = _quat_mul(np.asarray(q, np.float32), np.asarray(tf[3:], np.float32))
                 tf = tuple(float(v) for v in sp) + tuple(float(v) for v in sq)
             self.shape_type.append(builder.shape_type[s])
-            self.shape_body.append(-1)
+            self.shape_body.append(body + body_off if body >= 0 else -1)
             self.shape_transform.append(tf)
             self.shape_scale.append(builder.shape_scale[s])
             self.shape_flags.append(builder.shape_flags[s])
@@ -954,6 +1004,18 @@ class ModelBuilder:
         m.shape_material_kd = f32(self.shape_material_kd)
         m.shape_material_mu = f32(self.shape_material_mu)
         m.shape_label = list(self.shape_label)
+        m.body_count = self.body_count
+        nb = self.body_count
+        m.body_q = f32(self.body_q if nb else np.zeros((0, 7)), (-1, 7))
+        m.body_qd = torch.zeros((nb, 6), dtype=torch.float32, device=dev)
+        m.body_com = f32(self.body_com if nb else np.zeros((0, 3)), (-1, 3))
+        bm = np.asarray(self.body_mass, dtype=np.float32)
+        m.body_mass = f32(bm)
+        m.body_inv_mass = f32(np.divide(np.float32(1.0), bm, out=np.zeros_like(bm), where=bm != 0.0))
+        m.body_inertia = f32(np.stack(self.body_inertia) if nb else np.zeros((0, 3, 3)), (-1, 3, 3))
+        m.body_flags = i32(self.body_flags)
+        m.body_world = i32(self.body_world)
+        m.body_label = list(self.body_label)
         return m
 
 

@@ -1,17 +1,20 @@
-"""Soft-contact generation between particles and static shapes.
+"""Soft-contact generation between particles / mesh features and plane or sphere shapes.
 
-``CollisionPipeline`` mirrors the particle part of the reference's
+``CollisionPipeline`` mirrors the soft part of the reference's
 ``newton/_src/sim/collide.py`` (``CollisionPipeline`` :710, soft-contact sizing
-:1110-1136, ``collide`` :1236, launch :1553-1592).  Candidate pairs are built on
-the host once, in the order of ``_world_compatible_pairs`` (:480-546); the
-contacts are generated on the GPU by ``libvbd_hip`` (``vbd_collide_generate``),
-which writes records in pair order (a stable compaction), where the reference
-takes slots from an atomic counter.
+:1110-1136, ``collide`` :1236, launches :1553-1610, the full-surface edge / face
+passes of ``geometry/soft_contacts_sdf.py``).  Candidate pairs are built on the
+host once, in the order of ``_world_compatible_pairs`` (:480-546); the contacts
+are generated on the GPU by ``libvbd_hip`` (``vbd_collide_generate`` /
+``vbd_collide_generate_bodies``), which writes records in pair order (a stable
+compaction: particle records, then edge records, then face records), where the
+reference takes slots from an atomic counter.
 """
 
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -20,8 +23,27 @@ from .. import _native
 from .model import Contacts, Model, State
 
 
-def world_compatible_pairs(feature_world: np.ndarray, shape_world: np.ndarray, world_count: int) -> np.ndarray:
-    """``(feature, shape)`` pairs whose worlds are compatible (``collide.py:480-546``)."""
+def world_compatible_pairs(feature_world: np.ndarray, shape_world: np.ndarray, world_count: int,
+                           shape_ok: np.ndarray | None = None) -> np.ndarray:
+    """``(feature, shape)`` pairs whose worlds are compatible (``collide.py:480-546``); ``shape_ok`` (per
+    shape) drops the pairs of shapes that cannot take part, keeping the order of the others."""
+    pairs = _world_pairs(feature_world, shape_world, world_count)
+    if shape_ok is not None and len(pairs):
+        pairs = pairs[np.asarray(shape_ok, dtype=bool)[pairs[:, 1]]]
+    return np.ascontiguousarray(pairs, dtype=np.int32)
+
+
+def full_surface_capable_shapes(model: Model) -> np.ndarray:
+    """Shapes that can generate edge/face contacts (``collide.py:607-640``): spheres and infinite planes
+    (width = length = 0) among this path's shape types."""
+    if not model.shape_count:
+        return np.zeros(0, dtype=bool)
+    types = model.shape_type.cpu().numpy()
+    scale = model.shape_scale.cpu().numpy()
+    return (types == 3) | ((types == 1) & (scale[:, 0] == 0.0) & (scale[:, 1] == 0.0))
+
+
+def _world_pairs(feature_world: np.ndarray, shape_world: np.ndarray, world_count: int) -> np.ndarray:
     feature_world = np.asarray(feature_world)
     shape_world = np.asarray(shape_world)
     n_features, n_shapes = len(feature_world), len(shape_world)
@@ -61,33 +83,58 @@ def current_stream_handle(device) -> int:
 
 
 class CollisionPipeline:
-    """Particle-vs-static-shape soft contacts (plane, sphere)."""
+    """Soft contacts between particles (and, with ``enable_rigid_soft_full_surface_contact``, mesh edges
+    and triangles) and plane / sphere shapes, static or attached to bodies (``collide.py:710-1592``)."""
 
-    def __init__(self, model: Model, *, soft_contact_max: int | None = None, soft_contact_margin: float = 0.01):
+    def __init__(self, model: Model, *, soft_contact_max: int | None = None, soft_contact_margin: float = 0.01,
+                 enable_rigid_soft_full_surface_contact: bool = False):
         self.model = model
         self.device = model.device
-        pairs = world_compatible_pairs(
-            model.particle_world.cpu().numpy() if model.particle_count else np.empty(0, np.int32),
-            model.shape_world.cpu().numpy() if model.shape_count else np.empty(0, np.int32),
-            model.world_count,
-        )
-        self.soft_rigid_contact_pairs_host = pairs
-        self.soft_rigid_contact_pairs = torch.as_tensor(np.ascontiguousarray(pairs), device=self.device)
-        self.soft_rigid_contact_pair_count = int(pairs.shape[0])
-        if soft_contact_max is None:
-            soft_contact_max = self.soft_rigid_contact_pair_count
-        self.soft_contact_max = int(soft_contact_max)
-        self.soft_contact_margin = float(soft_contact_margin)
+        pworld = model.particle_world.cpu().numpy() if model.particle_count else np.empty(0, np.int32)
+        sworld = model.shape_world.cpu().numpy() if model.shape_count else np.empty(0, np.int32)
+        pairs = world_compatible_pairs(pworld, sworld, model.world_count)
         if model.shape_count:
-            body = model.shape_body.cpu().numpy()
-            if np.any(body >= 0):
-                raise NotImplementedError("only static shapes (body=-1) are supported on this path")
             types = model.shape_type.cpu().numpy()
             if np.any((types != 1) & (types != 3)):
                 raise NotImplementedError("particle contact supports plane and sphere shapes only")
+            body = model.shape_body.cpu().numpy()
+            if np.any(body >= model.body_count):
+                raise ValueError("shape_body names a body the model does not have")
+        self.has_bodies = bool(model.shape_count and np.any(model.shape_body.cpu().numpy() >= 0))
+        self.enable_rigid_soft_full_surface_contact = bool(enable_rigid_soft_full_surface_contact)
+        edge_pairs = face_pairs = np.empty((0, 2), dtype=np.int32)
+        if self.enable_rigid_soft_full_surface_contact and model.shape_count:
+            capable = full_surface_capable_shapes(model)
+            fallback = np.where(((model.shape_flags.cpu().numpy() & 4) != 0) & ~capable)[0]
+            if fallback.size:
+                warnings.warn(
+                    "enable_rigid_soft_full_surface_contact=True: these participating shapes cannot generate "
+                    f"edge/face contacts and fall back to per-particle soft contact only -- finite planes "
+                    f"{[model.shape_label[i] if getattr(model, 'shape_label', None) else i for i in fallback]} "
+                    "(only infinite planes are supported). Full-surface contacts still apply to the rest of the "
+                    "scene.", stacklevel=2)
+            if model.edge_count:
+                ew = pworld[model.edge_indices.cpu().numpy()[:, 2]]
+                edge_pairs = world_compatible_pairs(ew, sworld, model.world_count, capable)
+            if model.tri_count:
+                fw = pworld[model.tri_indices.cpu().numpy()[:, 0]]
+                face_pairs = world_compatible_pairs(fw, sworld, model.world_count, capable)
+        self.soft_rigid_contact_pairs_host = pairs
+        self.soft_rigid_contact_pairs = torch.as_tensor(np.ascontiguousarray(pairs), device=self.device)
+        self.soft_rigid_contact_pair_count = int(pairs.shape[0])
+        self.soft_edge_rigid_pairs_host = edge_pairs
+        self.soft_face_rigid_pairs_host = face_pairs
+        self.soft_edge_rigid_pairs = torch.as_tensor(np.ascontiguousarray(edge_pairs), device=self.device)
+        self.soft_face_rigid_pairs = torch.as_tensor(np.ascontiguousarray(face_pairs), device=self.device)
+        if soft_contact_max is None:
+            # one record per candidate pair (collide.py:1131-1134)
+            soft_contact_max = self.soft_rigid_contact_pair_count + len(edge_pairs) + len(face_pairs)
+        self.soft_contact_max = int(soft_contact_max)
+        self.soft_contact_margin = float(soft_contact_margin)
         self._handle = None
         self.desc = None
-        if self.device.type == "cuda" and self.soft_rigid_contact_pair_count > 0:
+        total = self.soft_rigid_contact_pair_count + len(edge_pairs) + len(face_pairs)
+        if self.device.type == "cuda" and total > 0:
             lib = _native.hip_lib()
             d = _native.VbdCollideDesc()
             d.particle_count = model.particle_count
@@ -103,6 +150,13 @@ class CollisionPipeline:
             d.shape_transform = _native.ptr(model.shape_transform)
             d.shape_scale = _native.ptr(model.shape_scale)
             d.shape_margin = _native.ptr(model.shape_margin)
+            d.shape_body = _native.ptr(model.shape_body) if self.has_bodies else None
+            d.edge_pair_count = len(edge_pairs)
+            d.edge_pairs = _native.ptr(self.soft_edge_rigid_pairs) if len(edge_pairs) else None
+            d.edge_indices = _native.ptr(model.edge_indices) if len(edge_pairs) else None
+            d.face_pair_count = len(face_pairs)
+            d.face_pairs = _native.ptr(self.soft_face_rigid_pairs) if len(face_pairs) else None
+            d.tri_indices = _native.ptr(model.tri_indices) if len(face_pairs) else None
             h = ctypes.c_void_p()
             _native.check(lib.vbd_collide_create(ctypes.byref(d), ctypes.byref(h)), "vbd_collide_create")
             self._handle = h
@@ -119,22 +173,33 @@ class CollisionPipeline:
 
     def contacts(self) -> Contacts:
         """Allocate a :class:`Contacts` buffer sized for this pipeline."""
-        return Contacts(self.soft_contact_max, self.device)
+        c = Contacts(self.soft_contact_max, self.device)
+        c._enable_rigid_soft_full_surface_contact = self.enable_rigid_soft_full_surface_contact
+        return c
 
     def collide(self, state: State, contacts: Contacts, *, soft_contact_margin: float | None = None) -> None:
-        """Generate soft contacts for ``state.particle_q`` into ``contacts``."""
+        """Generate soft contacts for ``state.particle_q`` (shapes on bodies placed by ``state.body_q``)."""
         if self.device.type != "cuda":
             raise RuntimeError("CollisionPipeline.collide runs on the GPU (libvbd_hip); the model is on the CPU")
         margin = self.soft_contact_margin if soft_contact_margin is None else float(soft_contact_margin)
         if contacts.soft_contact_max > self.soft_contact_max:
             raise ValueError("contacts buffer is larger than the pipeline's soft_contact_max")
+        if self.enable_rigid_soft_full_surface_contact:
+            contacts._enable_rigid_soft_full_surface_contact = True
         if self._handle is None:
             contacts.soft_contact_count.zero_()
             return
-        rc = _native.hip_lib().vbd_collide_generate(
-            self._handle, current_stream_handle(self.device), _native.ptr(state.particle_q), margin,
-            ctypes.byref(contacts_out(contacts))
-        )
+        lib = _native.hip_lib()
+        s = current_stream_handle(self.device)
+        if self.has_bodies:
+            if state.body_q is None:
+                raise ValueError("shapes are attached to bodies: state.body_q is required")
+            rc = lib.vbd_collide_generate_bodies(self._handle, s, _native.ptr(state.particle_q),
+                                                 _native.ptr(state.body_q), int(state.body_q.shape[0]), margin,
+                                                 ctypes.byref(contacts_out(contacts)))
+        else:
+            rc = lib.vbd_collide_generate(self._handle, s, _native.ptr(state.particle_q), margin,
+                                          ctypes.byref(contacts_out(contacts)))
         _native.check(rc, "vbd_collide_generate")
 
 
@@ -153,4 +218,4 @@ def contacts_out(contacts: Contacts) -> _native.VbdContactsOut:
     return out
 
 
-__all__ = ["CollisionPipeline", "contacts_out", "world_compatible_pairs"]
+__all__ = ["CollisionPipeline", "contacts_out", "full_surface_capable_shapes", "world_compatible_pairs"]

@@ -33,7 +33,7 @@ class State:
             self.particle_f.zero_()
 
     def assign(self, other: "State") -> None:
-        for name in ("particle_q", "particle_qd", "particle_f"):
+        for name in ("particle_q", "particle_qd", "particle_f", "body_q", "body_qd", "body_f"):
             a, b = getattr(self, name), getattr(other, name)
             if a is not None and b is not None:
                 a.copy_(b)
@@ -63,6 +63,8 @@ class Contacts:
         self.soft_contact_normal = torch.zeros((n, 3), dtype=torch.float32, device=device)
         self.soft_contact_tids = torch.zeros(0, dtype=torch.int32, device=device)
         self.device = torch.device(device)
+        # set by the CollisionPipeline that allocates the buffer: edge/face records may appear
+        self._enable_rigid_soft_full_surface_contact = False
 
     def clear(self) -> None:
         self.soft_contact_count.zero_()
@@ -129,15 +131,27 @@ class Model:
         self.shape_material_kd = None
         self.shape_material_mu = None
         self.body_count = 0
+        self.body_q = None
+        self.body_qd = None
+        self.body_com = None
+        self.body_mass = None
+        self.body_inv_mass = None
+        self.body_inertia = None
+        self.body_flags = None
+        self.body_world = None
         self.joint_count = 0
 
     def state(self, requires_grad: bool | None = None) -> State:
-        """New state holding clones of the initial particle configuration (``model.py:1828-1870``)."""
+        """New state holding clones of the initial particle and body configuration (``model.py:1828-1870``)."""
         s = State()
         if self.particle_count:
             s.particle_q = self.particle_q.clone()
             s.particle_qd = self.particle_qd.clone()
             s.particle_f = torch.zeros_like(self.particle_qd)
+        if self.body_count:
+            s.body_q = self.body_q.clone()
+            s.body_qd = self.body_qd.clone()
+            s.body_f = torch.zeros_like(self.body_qd)
         return s
 
     def control(self, clone_variables: bool = True) -> Control:

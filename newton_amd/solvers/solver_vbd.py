@@ -117,8 +117,12 @@ class SolverVBD(SolverBase):
                 stacklevel=3,
             )
         super().__init__(model)
-        if model.body_count > 0:
-            raise NotImplementedError("rigid bodies are not part of this VBD path")
+        if model.body_count > 0 and not integrate_with_external_rigid_solver:
+            raise NotImplementedError(
+                "SolverVBD here solves particles only: rigid bodies are not integrated (AVBD is not part of this "
+                "path). Pass integrate_with_external_rigid_solver=True and move the bodies with another solver "
+                "(state_in.body_q / state_out.body_q are then read for particle contact)."
+            )
         self.iterations = int(iterations)
         self.friction_epsilon = float(friction_epsilon)
         self.integrate_with_external_rigid_solver = integrate_with_external_rigid_solver
@@ -137,6 +141,7 @@ class SolverVBD(SolverBase):
         self.particle_enable_tile_solve = particle_enable_tile_solve
         self.deterministic = deterministic  # every reduction here has a fixed order already
         self.fp64_ill_conditioned = bool(fp64_ill_conditioned)
+        self._full_surface = False  # per-corner contact buffers for edge/face records (set on first use)
         self._ctx = None
         self._pass_shape = None  # (block, waves, aligned) chosen by autotune() / set_*(), kept across rebuilds
         if model.particle_count > 0:
@@ -265,7 +270,11 @@ class SolverVBD(SolverBase):
             d.spring_rest_length = p(m.spring_rest_length)
             d.spring_stiffness = p(m.spring_stiffness)
             d.spring_damping = p(m.spring_damping)
+        if m.shape_count and m.body_count and bool((m.shape_body >= 0).any()):
+            d.shape_body = p(m.shape_body)
+        d.body_count = m.body_count
         prm = _native.VbdSolverParams()
+        prm.soft_contact_full_surface = int(self._full_surface)
         prm.iterations = self.iterations
         prm.friction_epsilon = self.friction_epsilon
         prm.avbd_beta = self.rigid_linear_beta
@@ -330,22 +339,26 @@ class SolverVBD(SolverBase):
 
     def _ensure_context(self, contacts: Contacts | None):
         need = contacts.soft_contact_max if contacts is not None else 0
-        if need > self._capacity:
+        full = bool(contacts is not None and getattr(contacts, "_enable_rigid_soft_full_surface_contact", False))
+        if need > self._capacity or (full and not self._full_surface):
             if torch.cuda.is_current_stream_capturing():
                 raise RuntimeError(
-                    "SolverVBD: the soft-contact buffers would need to grow (larger soft_contact_max) during graph "
-                    "capture; run one step before capturing"
+                    "SolverVBD: the soft-contact buffers would need to grow (larger soft_contact_max, or edge/face "
+                    "records) during graph capture; run one step before capturing"
                 )
             torch.cuda.synchronize(self.model.device)
-            self._capacity = need
+            self._capacity = max(need, self._capacity)
+            self._full_surface = self._full_surface or full
             self._destroy_context()
             self._create_context()
             self._apply_pass_shape()
         elif self._contact_material_key() != self._material_key:
             self._push_scalars()
 
-    @staticmethod
-    def _contacts_desc(contacts: Contacts | None):
+    def _contacts_desc(self, contacts: Contacts | None, state_in: State | None = None,
+                       state_out: State | None = None, dt: float = 0.0):
+        """The contacts and, for shapes on bodies, the external rigid solver's poses: ``state_out.body_q`` at
+        the step's end and ``state_in.body_q`` at its start (``solver_vbd.py:2523-2526``)."""
         if contacts is None or contacts.soft_contact_max <= 0:
             return None
         c = _native.VbdContactsDesc()
@@ -356,6 +369,15 @@ class SolverVBD(SolverBase):
         c.soft_contact_body_pos = _native.ptr(contacts.soft_contact_body_pos)
         c.soft_contact_body_vel = _native.ptr(contacts.soft_contact_body_vel)
         c.soft_contact_normal = _native.ptr(contacts.soft_contact_normal)
+        if getattr(contacts, "_enable_rigid_soft_full_surface_contact", False):
+            c.soft_contact_barycentric = _native.ptr(contacts.soft_contact_barycentric)
+        if self.model.body_count:
+            cur = state_out if state_out is not None and state_out.body_q is not None else state_in
+            if cur is None or cur.body_q is None or state_in is None or state_in.body_q is None:
+                raise ValueError("the model has bodies: state_in.body_q and state_out.body_q are required")
+            c.body_q = _native.ptr(cur.body_q)
+            c.body_q_prev = _native.ptr(state_in.body_q)
+        c.dt = float(dt)
         return c
 
     # ------------------------------------------------------------ public API
@@ -369,7 +391,7 @@ class SolverVBD(SolverBase):
         if self.model.particle_count == 0:
             return
         self._ensure_context(contacts)
-        cd = self._contacts_desc(contacts)
+        cd = self._contacts_desc(contacts, state_in, state_out, dt)
         f_ext = state_in.particle_f
         rc = _native.hip_lib().vbd_step(
             self._ctx,
@@ -385,10 +407,11 @@ class SolverVBD(SolverBase):
         _native.check(rc, "SolverVBD.step")
 
     # pieces of step(), for drivers that interleave residual checks
-    def begin_step(self, state_in: State, contacts: Contacts | None, dt: float) -> None:
-        """Contact preparation and the inertial forward step (``solver_vbd.py:2065-2100``, ``:2493-2514``)."""
+    def begin_step(self, state_in: State, contacts: Contacts | None, dt: float, state_out: State | None = None) -> None:
+        """Contact preparation and the inertial forward step (``solver_vbd.py:2065-2100``, ``:2493-2514``);
+        ``state_out.body_q`` (default: ``state_in.body_q``) poses the bodies at the step's end."""
         self._ensure_context(contacts)
-        cd = self._contacts_desc(contacts)
+        cd = self._contacts_desc(contacts, state_in, state_out, dt)
         lib = _native.hip_lib()
         s = current_stream_handle(self.model.device)
         _native.check(lib.vbd_contact_prepare(self._ctx, s, ctypes.byref(cd) if cd is not None else None),
@@ -618,7 +641,7 @@ class SolverVBD(SolverBase):
         body) stops there instead of chasing a relative tolerance its FP32 residual cannot reach.
         """
         self.world_offsets()
-        self.begin_step(state_in, contacts, dt)
+        self.begin_step(state_in, contacts, dt, state_out)
         r0 = self.residual_worlds(state_in, dt).cpu().numpy()
         nw = r0.size
         target = np.maximum((rel_tol * rel_tol) * r0, np.square(np.broadcast_to(np.asarray(abs_tol, np.float64),
@@ -717,7 +740,7 @@ class SolverVBD(SolverBase):
         remaining iterations.  The winning start's iterate continues (a candidate must beat the
         inertial start strictly); every trial iteration is counted in ``iterations``.
         """
-        self.begin_step(state_in, contacts, dt)
+        self.begin_step(state_in, contacts, dt, state_out)
         r0_dev = self.residual(state_in, dt)
         chosen = None
         r_ws = None
@@ -820,6 +843,7 @@ class SolverVBD(SolverBase):
         trial.particle_q = state_in.particle_q.clone()
         trial.particle_qd = state_in.particle_qd.clone()
         trial.particle_f = state_in.particle_f.clone()
+        trial.body_q = state_in.body_q
         self.begin_step(trial, contacts, dt)
         shape = (ctypes.c_int32 * 3)()
         ms = (ctypes.c_float * 7)()

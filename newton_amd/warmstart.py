@@ -175,11 +175,12 @@ class LearnedWarmStart:
     """
 
     def __init__(self, model: Model, dt: float, mlp: WarmStartMLP | None = None, seed: int = 0,
-                 offer_reuse: bool = True, blend: float = 0.5, smooth_sweeps: int = 0):
+                 offer_reuse: bool = True, blend: float = 0.5, smooth_sweeps: int = 0, delta_sweeps: int = 0):
         self.model = model
         self.offer_reuse = offer_reuse
         self.blend = float(blend)
         self.smooth_sweeps = int(smooth_sweeps)
+        self.delta_sweeps = int(delta_sweeps)
         self.device = model.device
         self.dt = float(dt)
         n = model.particle_count
@@ -275,10 +276,14 @@ class LearnedWarmStart:
         picks among them.  S is ``smooth_sweeps`` sweeps of the 1-ring mean (identity at 0): the
         per-particle network leaves rough, neighbour-to-neighbour error that the stiff membrane turns
         into large forces, and the low-pass removes it while keeping the smooth correction the
-        Gauss-Seidel sweeps are slow to find."""
+        Gauss-Seidel sweeps are slow to find.  ``delta_sweeps`` low-passes the network's change alone
+        (before it is added to c_prev), which keeps the carried correction's detail."""
         feat = self.features(solver, state_in)
         self.mlp.forward(feat, self.pred)
-        learned = self.corr + self.pred * (self.blend * self.scale_corr)
+        delta = self.pred * (self.blend * self.scale_corr)
+        if self.delta_sweeps > 0:
+            self.smooth(delta, self.delta_sweeps)
+        learned = self.corr + delta
         if not self.offer_reuse:
             return self._y + self.smooth(learned)
         both = self.smooth(torch.cat([self.corr, learned], dim=1))  # one set of sweeps for both fields

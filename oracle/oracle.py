@@ -90,6 +90,9 @@ class _OracleParams(ctypes.Structure):
         ("num_threads", i32),
         ("self_contact", vp),
         ("residual_stop", ctypes.c_double),
+        ("body_q", vp),
+        ("body_q_prev", vp),
+        ("body_count", i32),
     ]
 
 
@@ -120,7 +123,8 @@ class _OracleSelf(ctypes.Structure):
 
 
 class _OraclePairs(ctypes.Structure):
-    _fields_ = [("pair_count", i32), ("pairs", vp)]
+    _fields_ = [("pair_count", i32), ("pairs", vp), ("edge_pair_count", i32), ("edge_pairs", vp),
+                ("face_pair_count", i32), ("face_pairs", vp)]
 
 
 _lib = None
@@ -286,21 +290,47 @@ class OracleModel:
 
         pairs = world_compatible_pairs(a["world"], a["shape_world"], model.world_count)
         self.pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
-        self.pairs_struct = _OraclePairs(int(pairs.shape[0]), _addr(self.pairs))
+        self.pairs_struct = _OraclePairs(int(pairs.shape[0]), _addr(self.pairs), 0, None, 0, None)
+
+    def set_full_surface_pairs(self, edge_pairs, face_pairs) -> None:
+        """Candidate (edge, shape) and (triangle, shape) pairs of the full-surface passes (empty: off)."""
+        self.edge_pairs = np.ascontiguousarray(np.asarray(edge_pairs, dtype=np.int32).reshape(-1))
+        self.face_pairs = np.ascontiguousarray(np.asarray(face_pairs, dtype=np.int32).reshape(-1))
+        self.pairs_struct.edge_pair_count = self.edge_pairs.size // 2
+        self.pairs_struct.edge_pairs = _addr(self.edge_pairs) if self.edge_pairs.size else None
+        self.pairs_struct.face_pair_count = self.face_pairs.size // 2
+        self.pairs_struct.face_pairs = _addr(self.face_pairs) if self.face_pairs.size else None
 
 
 def params(iterations: int, friction_epsilon: float = 1e-2, avbd_beta: float = 0.0, k_start: float = -1.0,
            use_libm_atan2: bool = False, fp64_ill_conditioned: bool = True, num_threads: int = 1,
-           self_contact: "SelfContact | None" = None, residual_stop: float = 0.0) -> _OracleParams:
+           self_contact: "SelfContact | None" = None, residual_stop: float = 0.0, body_q=None,
+           body_q_prev=None) -> _OracleParams:
     """Solver settings of the oracle.  ``residual_stop`` > 0 ends a step with a residual history once
     the residual norm is <= ``residual_stop`` x its value at the inertial guess (iterate to a relative
-    tolerance, checked after every iteration); later history entries stay 0."""
+    tolerance, checked after every iteration); later history entries stay 0.  ``body_q`` /
+    ``body_q_prev`` ([B,7] float32, p then q xyzw) are the poses of the bodies shapes are attached to at
+    the step's end and start (an external rigid solver's state_out / state_in)."""
     prm = _OracleParams(int(iterations), float(friction_epsilon), float(avbd_beta), float(k_start),
                         int(use_libm_atan2), int(fp64_ill_conditioned), int(num_threads),
                         None if self_contact is None else ctypes.addressof(self_contact.struct),
                         float(residual_stop))
     prm._self_contact = self_contact  # keeps the settings and their arrays alive
+    set_bodies(prm, body_q, body_q_prev)
     return prm
+
+
+def set_bodies(prm: _OracleParams, body_q, body_q_prev=None) -> None:
+    """Body poses of the next oracle calls ([B,7] float32 each; ``body_q_prev`` defaults to ``body_q``)."""
+    if body_q is None:
+        prm.body_q = prm.body_q_prev = None
+        prm.body_count = 0
+        prm._bodies = None
+        return
+    bq = np.ascontiguousarray(np.asarray(body_q, dtype=np.float32).reshape(-1, 7))
+    bp = bq if body_q_prev is None else np.ascontiguousarray(np.asarray(body_q_prev, dtype=np.float32).reshape(-1, 7))
+    prm._bodies = (bq, bp)
+    prm.body_q, prm.body_q_prev, prm.body_count = _addr(bq), _addr(bp), int(bq.shape[0])
 
 
 class SelfContact:
@@ -368,6 +398,7 @@ class Contacts:
         self.body_pos = np.zeros((n, 3), dtype=dt)
         self.body_vel = np.zeros((n, 3), dtype=dt)
         self.normal = np.zeros((n, 3), dtype=dt)
+        self.bary = np.zeros((n, 3), dtype=dt)
 
     @classmethod
     def from_device(cls, contacts, precision: str = "f32"):
@@ -382,15 +413,21 @@ class Contacts:
             c.body_pos[:n] = contacts.soft_contact_body_pos.cpu().numpy().astype(dt)
             c.body_vel[:n] = contacts.soft_contact_body_vel.cpu().numpy().astype(dt)
             c.normal[:n] = contacts.soft_contact_normal.cpu().numpy().astype(dt)
+            c.bary[:n] = contacts.soft_contact_barycentric.cpu().numpy().astype(dt)
         return c
 
 
-def create_soft_contacts(om: OracleModel, q: np.ndarray, margin: float, contacts: Contacts, precision: str = "f32"):
+def create_soft_contacts(om: OracleModel, q: np.ndarray, margin: float, contacts: Contacts, precision: str = "f32",
+                         body_q=None):
+    """Particle records (and, with full-surface pairs set on ``om``, edge and face records after them)
+    in pair order; ``body_q`` ([B,7]) places shapes attached to bodies."""
     dt, ct = _real(precision)
     q = np.ascontiguousarray(q, dtype=dt)
+    bq = None if body_q is None else np.ascontiguousarray(np.asarray(body_q, dtype=np.float32).reshape(-1, 7))
     fn = getattr(lib(), f"vbd_oracle_create_soft_contacts_{precision}")
-    rc = fn(ctypes.byref(om.struct), ctypes.byref(om.pairs_struct), _p(q), ct(margin), i32(contacts.max),
-            _p(contacts.count), _p(contacts.particle), _p(contacts.indices), _p(contacts.shape),
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(om.pairs_struct), _p(q), None if bq is None else _p(bq),
+            i32(0 if bq is None else bq.shape[0]), ct(margin), i32(contacts.max), _p(contacts.count),
+            _p(contacts.particle), _p(contacts.indices), _p(contacts.bary), _p(contacts.shape),
             _p(contacts.body_pos), _p(contacts.body_vel), _p(contacts.normal))
     if rc != 0:
         raise RuntimeError(f"oracle create_soft_contacts failed ({rc})")
@@ -408,15 +445,16 @@ def step(om: OracleModel, prm: _OracleParams, q, qd, f_ext, contacts: Contacts |
     qd_out = np.zeros_like(q)
     hist = np.zeros(prm.iterations + 1, dtype=np.float64) if residual_history else None
     if contacts is None:
-        count, cmax, ci, cs, cbp, cbv, cn = 0, 0, None, None, None, None, None
+        count, cmax, ci, cs, cbp, cbv, cn, cb = 0, 0, None, None, None, None, None, None
     else:
         count, cmax = int(contacts.count[0]), contacts.max
         ci, cs = _p(contacts.indices), _p(contacts.shape)
-        keep = [np.ascontiguousarray(x, dtype=rt) for x in (contacts.body_pos, contacts.body_vel, contacts.normal)]
-        cbp, cbv, cn = (_p(x) for x in keep)
+        keep = [np.ascontiguousarray(x, dtype=rt)
+                for x in (contacts.body_pos, contacts.body_vel, contacts.normal, contacts.bary)]
+        cbp, cbv, cn, cb = (_p(x) for x in keep)
     fn = getattr(lib(), f"vbd_oracle_step_{precision}")
-    rc = fn(ctypes.byref(om.struct), ctypes.byref(prm), i32(count), i32(cmax), ci, cs, cbp, cbv, cn, _p(q), _p(qd),
-            None if fe is None else _p(fe), _p(q_out), _p(qd_out), ct(dt), None if hist is None else _p(hist))
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(prm), i32(count), i32(cmax), ci, cs, cbp, cbv, cn, cb, _p(q),
+            _p(qd), None if fe is None else _p(fe), _p(q_out), _p(qd_out), ct(dt), None if hist is None else _p(hist))
     if rc != 0:
         raise RuntimeError(f"oracle step failed ({rc})")
     if residual_history:
@@ -449,13 +487,17 @@ def residual(om: OracleModel, prm: _OracleParams, contacts: Contacts | None, q, 
     g = np.zeros_like(q)
     norm = np.zeros(1, dtype=np.float64)
     if contacts is None:
-        args = (i32(0), i32(0), None, None, None, None, None)
+        args = (i32(0), i32(0), None, None, None, None, None, None)
     else:
-        keep = [np.ascontiguousarray(x, dtype=rt) for x in (contacts.body_pos, contacts.body_vel, contacts.normal)]
+        keep = [np.ascontiguousarray(x, dtype=rt)
+                for x in (contacts.body_pos, contacts.body_vel, contacts.normal, contacts.bary)]
         args = (i32(int(contacts.count[0])), i32(contacts.max), _p(contacts.indices), _p(contacts.shape),
                 *(_p(x) for x in keep))
     fn = getattr(lib(), f"vbd_oracle_residual_{precision}")
-    fn(ctypes.byref(om.struct), ctypes.byref(prm), *args, _p(q), _p(q_prev), _p(inertia), ct(dt), _p(g), _p(norm))
+    rc = fn(ctypes.byref(om.struct), ctypes.byref(prm), *args, _p(q), _p(q_prev), _p(inertia), ct(dt), _p(g),
+            _p(norm))
+    if rc != 0:
+        raise RuntimeError(f"oracle residual failed ({rc})")
     return g, float(norm[0])
 
 

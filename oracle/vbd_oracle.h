@@ -87,6 +87,12 @@ typedef struct OracleParams {
     int32_t num_threads;      /* OpenMP threads for the per-color vertex loop (<=1: serial) */
     const struct OracleSelf* self_contact; /* NULL: particle_enable_self_contact = False */
     double residual_stop;     /* > 0 with a residual history: stop once residual <= residual_stop * initial residual */
+    /* poses of the bodies that shapes are attached to (one-way coupling to an external rigid solver,
+     * solver_vbd.py:2523-2526): body_q = this step's (state_out.body_q), body_q_prev = the step's start
+     * (state_in.body_q); [body_count * 7] (p, q xyzw); NULL when every shape is static */
+    const float* body_q;
+    const float* body_q_prev;
+    int32_t body_count;
 } OracleParams;
 
 /* Cloth self-contact settings (SolverVBD particle_* arguments, solver_vbd.py:231-242) and the
@@ -118,10 +124,15 @@ typedef struct OracleSelf {
     double* ee_min_dist;             /* [E] */
 } OracleSelf;
 
-/* Soft-contact pair list for create_soft_contacts: [P*2] (particle, shape). */
+/* Soft-contact candidate pairs: [P*2] (particle, shape) for create_soft_contacts, and for the
+ * full-surface passes [E*2] (edge, shape) and [F*2] (triangle, shape) (counts 0 when off). */
 typedef struct OraclePairs {
     int32_t pair_count;
     const int32_t* pairs;
+    int32_t edge_pair_count;
+    const int32_t* edge_pairs;
+    int32_t face_pair_count;
+    const int32_t* face_pairs;
 } OraclePairs;
 
 #define ORACLE_DECLARE(SFX, REAL)                                                              \
@@ -129,13 +140,14 @@ typedef struct OraclePairs {
                                   int32_t contact_count, int32_t contact_max,                  \
                                   const int32_t* c_indices, const int32_t* c_shape,            \
                                   const REAL* c_body_pos, const REAL* c_body_vel,              \
-                                  const REAL* c_normal, REAL* q_in, const REAL* qd_in,         \
-                                  const REAL* f_ext, REAL* q_out, REAL* qd_out, REAL dt,       \
-                                  double* residual_hist);                                      \
+                                  const REAL* c_normal, const REAL* c_bary, REAL* q_in,        \
+                                  const REAL* qd_in, const REAL* f_ext, REAL* q_out,           \
+                                  REAL* qd_out, REAL dt, double* residual_hist);               \
     int32_t vbd_oracle_create_soft_contacts_##SFX(                                             \
-        const OracleModel* m, const OraclePairs* pairs, const REAL* q, REAL margin,            \
-        int32_t contact_max, int32_t* c_count, int32_t* c_particle, int32_t* c_indices,        \
-        int32_t* c_shape, REAL* c_body_pos, REAL* c_body_vel, REAL* c_normal);                 \
+        const OracleModel* m, const OraclePairs* pairs, const REAL* q, const float* body_q,    \
+        int32_t body_count, REAL margin, int32_t contact_max, int32_t* c_count,                \
+        int32_t* c_particle, int32_t* c_indices, REAL* c_bary, int32_t* c_shape,               \
+        REAL* c_body_pos, REAL* c_body_vel, REAL* c_normal);                                   \
     int32_t vbd_oracle_simulate_##SFX(const OracleModel* m, const OracleParams* p,             \
                                       const OraclePairs* pairs, REAL* q, REAL* qd,             \
                                       const REAL* f_ext, int32_t substeps, REAL dt,            \
@@ -182,9 +194,9 @@ typedef struct OraclePairs {
                                       int32_t contact_count, int32_t contact_max,              \
                                       const int32_t* c_indices, const int32_t* c_shape,        \
                                       const REAL* c_body_pos, const REAL* c_body_vel,          \
-                                      const REAL* c_normal, const REAL* q, const REAL* q_prev, \
-                                      const REAL* inertia, REAL dt, REAL* g_out,               \
-                                      double* norm_out);
+                                      const REAL* c_normal, const REAL* c_bary, const REAL* q, \
+                                      const REAL* q_prev, const REAL* inertia, REAL dt,        \
+                                      REAL* g_out, double* norm_out);
 
 ORACLE_DECLARE(f32, float)
 ORACLE_DECLARE(f64, double)

@@ -436,11 +436,69 @@ static void tet_eval(const V3* X, const V3* P, const float* dm_inv, REAL mu, REA
     *h_out = hess;
 }
 
-/* ---- body-particle contact: R:1331-1439, R:1603-1645 (static shape) ----- */
-static void body_contact_eval(V3 x, V3 x_prev, V3 body_pos, V3 body_vel, V3 n, REAL radius, REAL margin,
-                              REAL ke, REAL kd, REAL mu, REAL friction_epsilon, REAL dt, V3* f_out,
-                              M3* h_out) {
-    V3 bx = body_pos; /* transform_point(identity, body_pos) */
+/* ---- rigid transforms (p, q xyzw): warp transform_point / transform_vector / transform_inverse /
+ * transform_multiply and the Hamilton quaternion product, restated from their textbook definitions */
+typedef struct { V3 p; REAL qx, qy, qz, qw; } Xf;
+
+static V3 quat_rotate(REAL qx, REAL qy, REAL qz, REAL qw, V3 x) {
+    REAL c = R(2.0) * qw * qw - R(1.0);
+    REAL d = R(2.0) * (qx * x.x + qy * x.y + qz * x.z);
+    return v3(x.x * c + qx * d + (qy * x.z - qz * x.y) * qw * R(2.0),
+              x.y * c + qy * d + (qz * x.x - qx * x.z) * qw * R(2.0),
+              x.z * c + qz * d + (qx * x.y - qy * x.x) * qw * R(2.0));
+}
+static V3 xf_point(Xf t, V3 x) { return vadd(quat_rotate(t.qx, t.qy, t.qz, t.qw, x), t.p); }
+static V3 xf_vector(Xf t, V3 x) { return quat_rotate(t.qx, t.qy, t.qz, t.qw, x); }
+static Xf xf_inverse(Xf t) {
+    Xf r;
+    r.qx = -t.qx; r.qy = -t.qy; r.qz = -t.qz; r.qw = t.qw;
+    r.p = vneg(quat_rotate(r.qx, r.qy, r.qz, r.qw, t.p));
+    return r;
+}
+static Xf xf_load(const float* t) {
+    Xf r;
+    r.p = v3(R(t[0]), R(t[1]), R(t[2]));
+    r.qx = R(t[3]); r.qy = R(t[4]); r.qz = R(t[5]); r.qw = R(t[6]);
+    return r;
+}
+/* transform_multiply(a, b) = (quat_rotate(a.q, b.p) + a.p, a.q * b.q) */
+static Xf xf_mul(Xf a, Xf b) {
+    Xf r;
+    r.p = vadd(quat_rotate(a.qx, a.qy, a.qz, a.qw, b.p), a.p);
+    r.qx = a.qw * b.qx + b.qw * a.qx + a.qy * b.qz - b.qy * a.qz;
+    r.qy = a.qw * b.qy + b.qw * a.qy + a.qz * b.qx - b.qz * a.qx;
+    r.qz = a.qw * b.qz + b.qw * a.qz + a.qx * b.qy - b.qx * a.qy;
+    r.qw = a.qw * b.qw - a.qx * b.qx - a.qy * b.qy - a.qz * b.qz;
+    return r;
+}
+
+/* World contact point bx = X_wb * body_pos of a soft contact and the body's velocity there
+ * (R:1365-1439 with body_q_prev, the one-way coupling to an external rigid solver, solver_vbd.py:2523-2526):
+ * bv = (bx - X_wb_prev * body_pos) / dt + X_wb . body_vel.  A static shape (body < 0) takes the identity
+ * transform: bx = body_pos, bv = (0 + cross(0, bx)) + body_vel (the same values up to signed zeros).
+ * Returns 0, or -4 when the shape has a body but no body poses were given. */
+static int contact_body_frame(const OracleModel* m, const OracleParams* p, int shape, V3 body_pos, V3 body_vel,
+                              REAL dt, V3* bx, V3* bv) {
+    int b = (shape >= 0 && shape < m->shape_count && m->shape_body) ? m->shape_body[shape] : -1;
+    V3 zero = v3(R(0.0), R(0.0), R(0.0));
+    if (b < 0) {
+        *bx = body_pos;
+        *bv = vadd(vadd(zero, vcross(zero, *bx)), body_vel);
+        return 0;
+    }
+    if (!p->body_q || !p->body_q_prev || b >= p->body_count) return -4;
+    Xf X = xf_load(p->body_q + 7 * (size_t)b);
+    Xf Xp = xf_load(p->body_q_prev + 7 * (size_t)b);
+    *bx = xf_point(X, body_pos);
+    V3 bxp = xf_point(Xp, body_pos);
+    *bv = vadd(vdivs(vsub(*bx, bxp), dt), xf_vector(X, body_vel));
+    return 0;
+}
+
+/* ---- soft contact force law: R:1331-1363 (penalty + damping + projected isotropic friction), at the
+ * contact point x (anchor x_prev) against the world body point bx moving at bv ---- */
+static void contact_force_law(V3 x, V3 x_prev, V3 bx, V3 bv, V3 n, REAL radius, REAL margin, REAL ke, REAL kd,
+                              REAL mu, REAL friction_epsilon, REAL dt, V3* f_out, M3* h_out) {
     REAL pen = -(vdot(n, vsub(x, bx)) - radius - margin);
     if (!(pen > R(0.0))) {
         *f_out = v3(R(0.0), R(0.0), R(0.0));
@@ -448,9 +506,7 @@ static void body_contact_eval(V3 x, V3 x_prev, V3 body_pos, V3 body_vel, V3 n, R
         return;
     }
     V3 dx = vsub(x, x_prev);
-    /* static shape, no body velocity history: bv = (0 + cross(0, r)) + body_vel */
     V3 zero = v3(R(0.0), R(0.0), R(0.0));
-    V3 bv = vadd(vadd(zero, vcross(zero, bx)), body_vel);
     V3 u = vsub(dx, vscale(bv, dt));
     REAL f_n = pen * ke;
     V3 force = vscale(n, f_n);
@@ -478,6 +534,16 @@ static void body_contact_eval(V3 x, V3 x_prev, V3 body_pos, V3 body_vel, V3 n, R
     }
     *f_out = vadd(force, ff);
     *h_out = madd(hess, fh);
+}
+
+/* particle vs static shape (R:1365-1439 with the identity body transform) */
+static void body_contact_eval(V3 x, V3 x_prev, V3 body_pos, V3 body_vel, V3 n, REAL radius, REAL margin,
+                              REAL ke, REAL kd, REAL mu, REAL friction_epsilon, REAL dt, V3* f_out,
+                              M3* h_out) {
+    V3 zero = v3(R(0.0), R(0.0), R(0.0));
+    V3 bx = body_pos;
+    V3 bv = vadd(vadd(zero, vcross(zero, bx)), body_vel);
+    contact_force_law(x, x_prev, bx, bv, n, radius, margin, ke, kd, mu, friction_epsilon, dt, f_out, h_out);
 }
 
 /* Guarded 3x3 solve (K:2701-2706): no update when |det H| <= 1e-8.  With fp64_ill, the Frobenius
@@ -695,24 +761,71 @@ static REAL shape_margin_of(const OracleModel* m, int s) {
     return (m->shape_count > 0 && m->shape_margin) ? R(m->shape_margin[s]) : R(0.0);
 }
 
-/* per-vertex contact sum in contact-index order (the serial atomic_add order of K:2289-2401) */
-static void accumulate_contacts(const OracleModel* m, const OracleParams* p, int ncontact, const int32_t* c_indices,
-                                const int32_t* c_shape, const REAL* c_body_pos, const REAL* c_body_vel,
-                                const REAL* c_normal, const REAL* pk, const REAL* mkd, const REAL* mmu,
-                                const REAL* q, const REAL* q_prev, REAL dt, int color, REAL* pf, REAL* ph) {
+/* Soft-contact point of an edge/face record (corners (v0, v1, -1) / (v0, v1, v2), R:1466-1478,
+ * R:4906-4919): x = sum_i bary[i] * pos[c_i] over the valid corners in corner order, the anchor likewise
+ * from pos_prev (may be NULL), and the largest corner radius. */
+static void ef_contact_point(const OracleModel* m, const int32_t* cr, V3 bary, const REAL* q, const REAL* q_prev,
+                             V3* x, V3* x_prev, REAL* radius) {
+    *x = vscale(vld(q, cr[0]), bary.x);
+    if (q_prev) *x_prev = vscale(vld(q_prev, cr[0]), bary.x);
+    *radius = R(m->particle_radius[cr[0]]);
+    if (cr[1] >= 0) {
+        *x = vadd(*x, vscale(vld(q, cr[1]), bary.y));
+        if (q_prev) *x_prev = vadd(*x_prev, vscale(vld(q_prev, cr[1]), bary.y));
+        *radius = rmax(*radius, R(m->particle_radius[cr[1]]));
+    }
+    if (cr[2] >= 0) {
+        *x = vadd(*x, vscale(vld(q, cr[2]), bary.z));
+        if (q_prev) *x_prev = vadd(*x_prev, vscale(vld(q_prev, cr[2]), bary.z));
+        *radius = rmax(*radius, R(m->particle_radius[cr[2]]));
+    }
+}
+
+/* per-vertex contact sum in contact-index order (the serial atomic_add order of K:2289-2401): a particle
+ * record (p, -1, -1) adds its force to p; an edge/face record evaluates the force law at its barycentric
+ * point and adds bary[i] * f and bary[i]^2 * H to each corner of the color (R:1440-1536).  Returns -4 for
+ * a contact against a body whose poses were not given. */
+static int accumulate_contacts(const OracleModel* m, const OracleParams* p, int ncontact, const int32_t* c_indices,
+                               const int32_t* c_shape, const REAL* c_body_pos, const REAL* c_body_vel,
+                               const REAL* c_normal, const REAL* c_bary, const REAL* pk, const REAL* mkd,
+                               const REAL* mmu, const REAL* q, const REAL* q_prev, REAL dt, int color, REAL* pf,
+                               REAL* ph) {
     for (int t = 0; t < ncontact; ++t) {
         const int32_t* cr = c_indices + 3 * t;
-        if (cr[1] >= 0) continue; /* edge/face records: not generated by this path */
-        int pi = cr[0];
-        if (color >= 0 && m->particle_colors[pi] != color) continue;
-        V3 fc;
+        V3 fc, bx, bv;
         M3 hc;
-        body_contact_eval(vld(q, pi), vld(q_prev, pi), vld(c_body_pos, t), vld(c_body_vel, t), vld(c_normal, t),
-                          R(m->particle_radius[pi]), shape_margin_of(m, c_shape[t]), pk[t], mkd[t], mmu[t],
-                          R(p->friction_epsilon), dt, &fc, &hc);
-        vst(pf, pi, vadd(vld(pf, pi), fc));
-        mstore(ph + 9 * pi, madd(mload(ph + 9 * pi), hc));
+        if (cr[1] < 0) {
+            int pi = cr[0];
+            if (color >= 0 && m->particle_colors[pi] != color) continue;
+            if (contact_body_frame(m, p, c_shape[t], vld(c_body_pos, t), vld(c_body_vel, t), dt, &bx, &bv)) return -4;
+            contact_force_law(vld(q, pi), vld(q_prev, pi), bx, bv, vld(c_normal, t), R(m->particle_radius[pi]),
+                              shape_margin_of(m, c_shape[t]), pk[t], mkd[t], mmu[t], R(p->friction_epsilon), dt,
+                              &fc, &hc);
+            vst(pf, pi, vadd(vld(pf, pi), fc));
+            mstore(ph + 9 * pi, madd(mload(ph + 9 * pi), hc));
+            continue;
+        }
+        if (!c_bary) return -5; /* edge/face records need their barycentric weights */
+        V3 w = vld(c_bary, t);
+        int any = color < 0;
+        for (int i = 0; i < 3; ++i)
+            if (cr[i] >= 0 && m->particle_colors[cr[i]] == color) any = 1;
+        if (!any) continue; /* the force would be added to no corner */
+        V3 x, xp;
+        REAL radius;
+        ef_contact_point(m, cr, w, q, q_prev, &x, &xp, &radius);
+        if (contact_body_frame(m, p, c_shape[t], vld(c_body_pos, t), vld(c_body_vel, t), dt, &bx, &bv)) return -4;
+        contact_force_law(x, xp, bx, bv, vld(c_normal, t), radius, shape_margin_of(m, c_shape[t]), pk[t], mkd[t],
+                          mmu[t], R(p->friction_epsilon), dt, &fc, &hc);
+        for (int i = 0; i < 3; ++i) {
+            int ci = cr[i];
+            if (ci < 0 || (color >= 0 && m->particle_colors[ci] != color)) continue;
+            REAL wi = vcomp(w, i);
+            vst(pf, ci, vadd(vld(pf, ci), vscale(fc, wi)));
+            mstore(ph + 9 * ci, madd(mload(ph + 9 * ci), mscale(hc, wi * wi)));
+        }
     }
+    return 0;
 }
 
 /* evaluate_spring_force_and_hessian_both_vertices (K:1685-1735): the forces on v0 and v1 and the
@@ -795,8 +908,8 @@ static void accumulate_springs(const OracleModel* m, const REAL* q, const REAL* 
 int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, int32_t contact_count,
                                  int32_t contact_max, const int32_t* c_indices, const int32_t* c_shape,
                                  const REAL* c_body_pos, const REAL* c_body_vel, const REAL* c_normal,
-                                 const REAL* q, const REAL* q_prev, const REAL* inertia, REAL dt, REAL* g_out,
-                                 double* norm_out) {
+                                 const REAL* c_bary, const REAL* q, const REAL* q_prev, const REAL* inertia, REAL dt,
+                                 REAL* g_out, double* norm_out) {
 #ifdef ORACLE_IS_F32
     g_use_libm_atan2 = p->use_libm_atan2;
 #endif
@@ -807,8 +920,14 @@ int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, in
     REAL* pf = (REAL*)calloc((size_t)N * 3, sizeof(REAL));
     REAL* ph = (REAL*)calloc((size_t)N * 9, sizeof(REAL));
     init_contacts(m, p, nc, c_shape, pk, mke, mkd, mmu);
-    accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q, q_prev,
-                        dt, -1, pf, ph);
+    int crc = accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, c_bary, pk, mkd,
+                                  mmu, q, q_prev, dt, -1, pf, ph);
+    if (crc) {
+        free(pk);
+        free(pf);
+        free(ph);
+        return crc;
+    }
     accumulate_springs(m, q, q_prev, dt, -1, pf, ph);
     if (p->self_contact) {
         /* self-contact terms of the contact set detected at q */
@@ -847,9 +966,9 @@ int32_t API(vbd_oracle_residual)(const OracleModel* m, const OracleParams* p, in
 /* ---- SolverVBD.step, particle-only: S:1773-1825 -------------------------- */
 int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_t contact_count,
                              int32_t contact_max, const int32_t* c_indices, const int32_t* c_shape,
-                             const REAL* c_body_pos, const REAL* c_body_vel, const REAL* c_normal, REAL* q,
-                             const REAL* qd, const REAL* f_ext, REAL* q_out, REAL* qd_out, REAL dt,
-                             double* residual_hist) {
+                             const REAL* c_body_pos, const REAL* c_body_vel, const REAL* c_normal,
+                             const REAL* c_bary, REAL* q, const REAL* qd, const REAL* f_ext, REAL* q_out,
+                             REAL* qd_out, REAL dt, double* residual_hist) {
 #ifdef ORACLE_IS_F32
     g_use_libm_atan2 = p->use_libm_atan2;
 #endif
@@ -906,9 +1025,10 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
 
     Ctx c = {m, q, q_prev, dt};
     int run = p->iterations;
+    int rc = 0;
     if (residual_hist) {
         API(vbd_oracle_residual)(m, p, contact_count, contact_max, c_indices, c_shape, c_body_pos, c_body_vel,
-                                 c_normal, q, q_prev, inertia, dt, NULL, &residual_hist[0]);
+                                 c_normal, c_bary, q, q_prev, inertia, dt, NULL, &residual_hist[0]);
         if (p->residual_stop > 0.0 && residual_hist[0] <= 0.0) run = 0;
     }
     for (int it = 0; it < run; ++it) {
@@ -920,22 +1040,38 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
             memcpy(pos_cd, q, sizeof(REAL) * 3 * (size_t)N);
             memset(disp, 0, sizeof(REAL) * 3 * (size_t)N);
         }
-        /* update_duals_body_particle_contacts (R:4861-4926) */
+        /* update_duals_body_particle_contacts (R:4861-4926): the penetration of the particle, or of an
+         * edge/face record's barycentric point with its largest corner radius, at the body point */
         for (int t = 0; t < nc; ++t) {
             const int32_t* cr = c_indices + 3 * t;
-            if (cr[1] >= 0) continue;
-            V3 n = vld(c_normal, t);
-            REAL pen = -(vdot(n, vsub(vld(q, cr[0]), vld(c_body_pos, t))) - R(m->particle_radius[cr[0]]) -
-                         shape_margin_of(m, c_shape[t]));
+            V3 n = vld(c_normal, t), bx, bv, xc, xp;
+            REAL radius;
+            if (contact_body_frame(m, p, c_shape[t], vld(c_body_pos, t), vld(c_body_vel, t), dt, &bx, &bv)) {
+                rc = -4;
+                break;
+            }
+            if (cr[1] < 0) {
+                xc = vld(q, cr[0]);
+                radius = R(m->particle_radius[cr[0]]);
+            } else {
+                if (!c_bary) {
+                    rc = -5;
+                    break;
+                }
+                ef_contact_point(m, cr, vld(c_bary, t), q, NULL, &xc, &xp, &radius);
+            }
+            REAL pen = -(vdot(n, vsub(xc, bx)) - radius - shape_margin_of(m, c_shape[t]));
             pen = rmax(R(0.0), pen);
             pk[t] = rmin(pk[t] + R(p->avbd_beta) * pen, mke[t]);
         }
+        if (rc) break;
         /* _solve_particle_iteration (S:2516-2704) */
         memset(pf, 0, sizeof(REAL) * 3 * (size_t)N);
         memset(ph, 0, sizeof(REAL) * 9 * (size_t)N);
         for (int color = 0; color < m->color_count; ++color) {
-            accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, pk, mkd, mmu, q,
-                                q_prev, dt, color, pf, ph);
+            rc = accumulate_contacts(m, p, nc, c_indices, c_shape, c_body_pos, c_body_vel, c_normal, c_bary, pk, mkd,
+                                     mmu, q, q_prev, dt, color, pf, ph);
+            if (rc) break;
             accumulate_springs(m, q, q_prev, dt, color, pf, ph);
             if (sc) accumulate_self_contacts(m, p, &SL, q, q_prev, dt, color, pf, ph);
             int cb = m->color_offsets[color], ce = m->color_offsets[color + 1];
@@ -961,10 +1097,12 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
             else
                 for (int i = 0; i < N; ++i) vst(q, i, vadd(vld(x0, i), vscale(vld(disp, i), R(1.0))));
         }
+        if (rc) break;
         memcpy(q_out, q, sizeof(REAL) * 3 * (size_t)N);
         if (residual_hist) {
             API(vbd_oracle_residual)(m, p, contact_count, contact_max, c_indices, c_shape, c_body_pos,
-                                     c_body_vel, c_normal, q, q_prev, inertia, dt, NULL, &residual_hist[it + 1]);
+                                     c_body_vel, c_normal, c_bary, q, q_prev, inertia, dt, NULL,
+                                     &residual_hist[it + 1]);
             if (p->residual_stop > 0.0 && residual_hist[it + 1] <= p->residual_stop * residual_hist[0]) break;
         }
     }
@@ -978,54 +1116,144 @@ int32_t API(vbd_oracle_step)(const OracleModel* m, const OracleParams* p, int32_
     }
     free(buf);
     free(cbuf);
+    return rc;
+}
+
+/* ---- soft-contact generation: create_soft_contacts (G:1020-1184) and the full-surface edge / face
+ * passes (soft_contacts_sdf.py:240-551), plane and sphere shapes, static or attached to a body ---- */
+
+/* shape-local -> world (X_ws) and its inverse; X_bs alone for a static shape (transform_multiply of the
+ * identity and X_bs is X_bs up to signed zeros) */
+static int shape_frames(const OracleModel* m, const float* body_q, int32_t body_count, int s, Xf* X_bs, Xf* X_ws,
+                        Xf* X_sw) {
+    *X_bs = xf_load(m->shape_transform + 7 * (size_t)s);
+    int b = m->shape_body ? m->shape_body[s] : -1;
+    if (b >= 0) {
+        if (!body_q || b >= body_count) return -2;
+        *X_ws = xf_mul(xf_load(body_q + 7 * (size_t)b), *X_bs);
+    } else {
+        *X_ws = *X_bs;
+    }
+    *X_sw = xf_inverse(*X_ws);
     return 0;
 }
 
-/* ---- create_soft_contacts (G:1020-1184), static plane/sphere shapes ----- */
-typedef struct { V3 p; REAL qx, qy, qz, qw; } Xf;
-
-static V3 quat_rotate(REAL qx, REAL qy, REAL qz, REAL qw, V3 x) {
-    REAL c = R(2.0) * qw * qw - R(1.0);
-    REAL d = R(2.0) * (qx * x.x + qy * x.y + qz * x.z);
-    return v3(x.x * c + qx * d + (qy * x.z - qz * x.y) * qw * R(2.0),
-              x.y * c + qy * d + (qz * x.x - qx * x.z) * qw * R(2.0),
-              x.z * c + qz * d + (qx * x.y - qy * x.x) * qw * R(2.0));
+/* eval_shape_sdf (soft_contacts_sdf.py:61-96) for the analytic shapes of this path: sphere |x| - r with the
+ * gradient x / |x| ((0, 0, 1) at the centre, G:181-211), infinite plane x.z with gradient +z (G:584-620) */
+static REAL shape_sdf(int type, const float* sc, V3 x, V3* grad) {
+    if (type == 3) {
+        REAL d = vlength(x) - R(sc[0]);
+        REAL l = vlength(x);
+        *grad = l > R(1.0e-8) ? vdivs(x, l) : v3(R(0.0), R(0.0), R(1.0));
+        return d;
+    }
+    *grad = v3(R(0.0), R(0.0), R(1.0));
+    return x.z;
 }
-static V3 xf_point(Xf t, V3 x) { return vadd(quat_rotate(t.qx, t.qy, t.qz, t.qw, x), t.p); }
-static V3 xf_vector(Xf t, V3 x) { return quat_rotate(t.qx, t.qy, t.qz, t.qw, x); }
-static Xf xf_inverse(Xf t) {
-    Xf r;
-    r.qx = -t.qx; r.qy = -t.qy; r.qz = -t.qz; r.qw = t.qw;
-    r.p = vneg(quat_rotate(r.qx, r.qy, r.qz, r.qw, t.p));
-    return r;
+
+/* optimize_edge_sdf (soft_contacts_sdf.py:136-175): golden-section search of the SDF along p -> q with a
+ * fixed iteration count; returns u, the point, its distance and gradient */
+static REAL opt_edge_sdf(int type, const float* sc, V3 p, V3 q, int n_iter, V3* x_out, REAL* phi_out, V3* g_out) {
+    const REAL inv_phi = R(0.6180339887498949);
+    REAL lo = R(0.0), hi = R(1.0);
+    REAL c = hi - (hi - lo) * inv_phi;
+    REAL d = lo + (hi - lo) * inv_phi;
+    V3 g;
+    REAL fc = shape_sdf(type, sc, vadd(vscale(p, R(1.0) - c), vscale(q, c)), &g);
+    REAL fd = shape_sdf(type, sc, vadd(vscale(p, R(1.0) - d), vscale(q, d)), &g);
+    for (int i = 0; i < n_iter; ++i) {
+        if (fc < fd) {
+            hi = d;
+            d = c;
+            fd = fc;
+            c = hi - (hi - lo) * inv_phi;
+            fc = shape_sdf(type, sc, vadd(vscale(p, R(1.0) - c), vscale(q, c)), &g);
+        } else {
+            lo = c;
+            c = d;
+            fc = fd;
+            d = lo + (hi - lo) * inv_phi;
+            fd = shape_sdf(type, sc, vadd(vscale(p, R(1.0) - d), vscale(q, d)), &g);
+        }
+    }
+    REAL u = R(0.5) * (lo + hi);
+    *x_out = vadd(vscale(p, R(1.0) - u), vscale(q, u));
+    *phi_out = shape_sdf(type, sc, *x_out, g_out);
+    return u;
+}
+
+static V3 bary_point(V3 a, V3 b, V3 c, V3 w) {
+    return vadd(vadd(vscale(a, w.x), vscale(b, w.y)), vscale(c, w.z));
+}
+
+/* optimize_face_sdf (soft_contacts_sdf.py:178-219): Frank-Wolfe on the barycentric simplex from the
+ * centroid, each step a golden-section line search toward the simplex vertex minimising grad . corner */
+static V3 opt_face_sdf(int type, const float* sc, V3 a, V3 b, V3 c, int n_iter, int ls_iter, V3* x_out,
+                       REAL* phi_out, V3* g_out) {
+    REAL third = R(1.0) / R(3.0);
+    V3 bary = v3(third, third, third);
+    for (int i = 0; i < n_iter; ++i) {
+        V3 x = bary_point(a, b, c, bary);
+        V3 g;
+        shape_sdf(type, sc, x, &g);
+        REAL da = vdot(g, a), db = vdot(g, b), dc = vdot(g, c);
+        V3 s = v3(R(1.0), R(0.0), R(0.0));
+        if (db <= da && db <= dc)
+            s = v3(R(0.0), R(1.0), R(0.0));
+        else if (dc <= da && dc <= db)
+            s = v3(R(0.0), R(0.0), R(1.0));
+        V3 target = bary_point(a, b, c, s);
+        V3 lx, lg;
+        REAL lphi;
+        REAL gamma = opt_edge_sdf(type, sc, x, target, ls_iter, &lx, &lphi, &lg);
+        bary = vadd(vscale(bary, R(1.0) - gamma), vscale(s, gamma));
+    }
+    *x_out = bary_point(a, b, c, bary);
+    *phi_out = shape_sdf(type, sc, *x_out, g_out);
+    return bary;
+}
+
+/* the soft-contact counter: every hit counts, records past contact_max are dropped (G:986-1010) */
+static void emit_record(int* count, int32_t contact_max, int32_t* c_particle, int32_t* c_indices, REAL* c_bary,
+                        int32_t* c_shape, REAL* c_body_pos, REAL* c_body_vel, REAL* c_normal, int particle, int i0,
+                        int i1, int i2, V3 bary, int s, V3 bp, V3 bv, V3 wn) {
+    int index = (*count)++;
+    if (index >= contact_max) return;
+    c_shape[index] = s;
+    vst(c_body_pos, index, bp);
+    vst(c_body_vel, index, bv);
+    if (c_particle) c_particle[index] = particle;
+    c_indices[3 * index] = i0;
+    c_indices[3 * index + 1] = i1;
+    c_indices[3 * index + 2] = i2;
+    if (c_bary) vst(c_bary, index, bary);
+    vst(c_normal, index, wn);
 }
 
 int32_t API(vbd_oracle_create_soft_contacts)(const OracleModel* m, const OraclePairs* pairs, const REAL* q,
-                                             REAL margin, int32_t contact_max, int32_t* c_count,
-                                             int32_t* c_particle, int32_t* c_indices, int32_t* c_shape,
-                                             REAL* c_body_pos, REAL* c_body_vel, REAL* c_normal) {
+                                             const float* body_q, int32_t body_count, REAL margin,
+                                             int32_t contact_max, int32_t* c_count, int32_t* c_particle,
+                                             int32_t* c_indices, REAL* c_bary, int32_t* c_shape, REAL* c_body_pos,
+                                             REAL* c_body_vel, REAL* c_normal) {
     int count = 0;
+    V3 zero = v3(R(0.0), R(0.0), R(0.0));
+    /* particle pass (G:1020-1184), pair order */
     for (int tid = 0; tid < pairs->pair_count; ++tid) {
         int pi = pairs->pairs[2 * tid], s = pairs->pairs[2 * tid + 1];
         if ((m->particle_flags[pi] & 1) == 0) continue;
         if ((m->shape_flags[s] & 4) == 0) continue;
         int pw = m->particle_world[pi], sw = m->shape_world[s];
         if (pw != -1 && sw != -1 && pw != sw) continue;
-        if (m->shape_body[s] >= 0) return -2; /* body-attached shapes: not on this path */
         V3 px = vld(q, pi);
         REAL radius = R(m->particle_radius[pi]);
-        const float* tf = m->shape_transform + 7 * s;
-        Xf X_bs;
-        X_bs.p = v3(R(tf[0]), R(tf[1]), R(tf[2]));
-        X_bs.qx = R(tf[3]); X_bs.qy = R(tf[4]); X_bs.qz = R(tf[5]); X_bs.qw = R(tf[6]);
-        Xf X_ws = X_bs; /* transform_multiply(identity, X_bs) is exactly X_bs */
-        Xf X_sw = xf_inverse(X_ws);
+        Xf X_bs, X_ws, X_sw;
+        if (shape_frames(m, body_q, body_count, s, &X_bs, &X_ws, &X_sw)) return -2;
         V3 xl = xf_point(X_sw, px);
         const float* sc = m->shape_scale + 3 * s;
         REAL s_margin = shape_margin_of(m, s);
         REAL d = R(1.0e6);
-        V3 n = v3(R(0.0), R(0.0), R(0.0));
-        V3 v = v3(R(0.0), R(0.0), R(0.0));
+        V3 n = zero;
+        V3 v = zero;
         int type = m->shape_type[s];
         if (type == 3) {
             REAL pl = vlength(xl);
@@ -1044,21 +1272,65 @@ int32_t API(vbd_oracle_create_soft_contacts)(const OracleModel* m, const OracleP
         } else {
             return -3; /* other shape types: not on this path */
         }
-        if (d < margin + s_margin + radius) {
-            int index = count++;
-            if (index < contact_max) {
-                V3 bp = xf_point(X_bs, vsub(xl, vscale(n, d)));
-                V3 bv = xf_vector(X_bs, v);
-                V3 wn = xf_vector(X_ws, n);
-                c_shape[index] = s;
-                vst(c_body_pos, index, bp);
-                vst(c_body_vel, index, bv);
-                if (c_particle) c_particle[index] = pi;
-                c_indices[3 * index] = pi;
-                c_indices[3 * index + 1] = -1;
-                c_indices[3 * index + 2] = -1;
-                vst(c_normal, index, wn);
-            }
+        if (d < margin + s_margin + radius)
+            emit_record(&count, contact_max, c_particle, c_indices, c_bary, c_shape, c_body_pos, c_body_vel, c_normal,
+                        pi, pi, -1, -1, v3(R(1.0), R(0.0), R(0.0)), s, xf_point(X_bs, vsub(xl, vscale(n, d))),
+                        xf_vector(X_bs, v), xf_vector(X_ws, n));
+    }
+    /* edge pass (soft_contacts_sdf.py:380-469), edge-pair order */
+    for (int tid = 0; tid < pairs->edge_pair_count; ++tid) {
+        int e = pairs->edge_pairs[2 * tid], s = pairs->edge_pairs[2 * tid + 1];
+        if ((m->shape_flags[s] & 4) == 0) continue;
+        int type = m->shape_type[s];
+        if (type != 1 && type != 3) return -3;
+        int v0 = m->edge_indices[4 * e + 2], v1 = m->edge_indices[4 * e + 3];
+        REAL radius = rmax(R(m->particle_radius[v0]), R(m->particle_radius[v1]));
+        Xf X_bs, X_ws, X_sw;
+        if (shape_frames(m, body_q, body_count, s, &X_bs, &X_ws, &X_sw)) return -2;
+        V3 p_s = xf_point(X_sw, vld(q, v0));
+        V3 q_s = xf_point(X_sw, vld(q, v1));
+        const float* sc = m->shape_scale + 3 * s;
+        REAL threshold = margin + shape_margin_of(m, s) + radius;
+        V3 mid = vscale(vadd(p_s, q_s), R(0.5));
+        V3 g;
+        REAL phi_m = shape_sdf(type, sc, mid, &g);
+        if (phi_m > threshold + R(0.5) * vlength(vsub(q_s, p_s))) continue;
+        V3 x;
+        REAL phi;
+        REAL u = opt_edge_sdf(type, sc, p_s, q_s, 24, &x, &phi, &g);
+        if (phi < threshold) {
+            V3 y = vsub(x, vscale(g, phi));
+            emit_record(&count, contact_max, c_particle, c_indices, c_bary, c_shape, c_body_pos, c_body_vel, c_normal,
+                        -1, v0, v1, -1, v3(R(1.0) - u, u, R(0.0)), s, xf_point(X_bs, y), zero, xf_vector(X_ws, g));
+        }
+    }
+    /* face pass (soft_contacts_sdf.py:277-377), face-pair order */
+    for (int tid = 0; tid < pairs->face_pair_count; ++tid) {
+        int t = pairs->face_pairs[2 * tid], s = pairs->face_pairs[2 * tid + 1];
+        if ((m->shape_flags[s] & 4) == 0) continue;
+        int type = m->shape_type[s];
+        if (type != 1 && type != 3) return -3;
+        int ia = m->tri_indices[3 * t], ib = m->tri_indices[3 * t + 1], ic = m->tri_indices[3 * t + 2];
+        REAL radius = rmax(R(m->particle_radius[ia]), rmax(R(m->particle_radius[ib]), R(m->particle_radius[ic])));
+        Xf X_bs, X_ws, X_sw;
+        if (shape_frames(m, body_q, body_count, s, &X_bs, &X_ws, &X_sw)) return -2;
+        V3 a_s = xf_point(X_sw, vld(q, ia));
+        V3 b_s = xf_point(X_sw, vld(q, ib));
+        V3 c_s = xf_point(X_sw, vld(q, ic));
+        const float* sc = m->shape_scale + 3 * s;
+        REAL threshold = margin + shape_margin_of(m, s) + radius;
+        V3 cen = vdivs(vadd(vadd(a_s, b_s), c_s), R(3.0));
+        V3 g;
+        REAL phi_c = shape_sdf(type, sc, cen, &g);
+        REAL reach = rmax(vlength(vsub(a_s, cen)), rmax(vlength(vsub(b_s, cen)), vlength(vsub(c_s, cen))));
+        if (phi_c > threshold + reach) continue;
+        V3 x;
+        REAL phi;
+        V3 bary = opt_face_sdf(type, sc, a_s, b_s, c_s, 24, 16, &x, &phi, &g);
+        if (phi < threshold) {
+            V3 y = vsub(x, vscale(g, phi));
+            emit_record(&count, contact_max, c_particle, c_indices, c_bary, c_shape, c_body_pos, c_body_vel, c_normal,
+                        -1, ia, ib, ic, bary, s, xf_point(X_bs, y), zero, xf_vector(X_ws, g));
         }
     }
     *c_count = count;
@@ -1075,17 +1347,18 @@ int32_t API(vbd_oracle_simulate)(const OracleModel* m, const OracleParams* p, co
     REAL* qd1 = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)N);
     int32_t* ci = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)S);
     int32_t* cs = ci + 3 * S;
-    REAL* cf = (REAL*)malloc(sizeof(REAL) * 9 * (size_t)S);
-    REAL *cbp = cf, *cbv = cf + 3 * S, *cn = cf + 6 * S;
+    REAL* cf = (REAL*)malloc(sizeof(REAL) * 12 * (size_t)S);
+    REAL *cbp = cf, *cbv = cf + 3 * S, *cn = cf + 6 * S, *cb = cf + 9 * S;
     int rc = 0;
     for (int s = 0; s < substeps && rc == 0; ++s) {
         int32_t count = 0;
-        if (pairs && pairs->pair_count > 0 && contact_max > 0) {
-            rc = API(vbd_oracle_create_soft_contacts)(m, pairs, q, margin, contact_max, &count, NULL, ci, cs, cbp,
-                                                      cbv, cn);
+        if (pairs && pairs->pair_count + pairs->edge_pair_count + pairs->face_pair_count > 0 && contact_max > 0) {
+            rc = API(vbd_oracle_create_soft_contacts)(m, pairs, q, p->body_q, p->body_count, margin, contact_max,
+                                                      &count, NULL, ci, cb, cs, cbp, cbv, cn);
             if (rc) break;
         }
-        rc = API(vbd_oracle_step)(m, p, count, contact_max, ci, cs, cbp, cbv, cn, q, qd, f_ext, q1, qd1, dt, NULL);
+        rc = API(vbd_oracle_step)(m, p, count, contact_max, ci, cs, cbp, cbv, cn, cb, q, qd, f_ext, q1, qd1, dt,
+                                  NULL);
         memcpy(q, q1, sizeof(REAL) * 3 * (size_t)N);
         memcpy(qd, qd1, sizeof(REAL) * 3 * (size_t)N);
         if (record_every > 0 && q_record && (s + 1) % record_every == 0)
