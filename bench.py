@@ -61,8 +61,10 @@ def parse():
     ap.add_argument("--residual-worlds", type=int, default=64, help="worlds of the headline batch")
     ap.add_argument("--single-world-stride", type=int, default=4,
                     help="headline: every k-th world is also solved alone (the plain single-world baseline)")
-    ap.add_argument("--trial-iterations", type=int, default=4,
+    ap.add_argument("--trial-iterations", type=int, default=1,
                     help="iterations each start runs before the residual judges it (the trial-judged variants)")
+    ap.add_argument("--delta-sweeps", type=int, default=32,
+                    help="1-ring low-pass sweeps of the network's change of the carried correction (deployed)")
     ap.add_argument("--smooth-sweeps", type=int, default=64,
                     help="1-ring low-pass sweeps of the deployed warm-start candidates")
     ap.add_argument("--blend", type=float, default=1.0, help="weight of the network's correction (deployed)")
@@ -886,11 +888,13 @@ def headline_leg(args, s_single, ws):
     s0.particle_qd.copy_(s_single.particle_qd.repeat(W, 1))
     solver = SolverVBD(model, iterations=10)
     tune = solver.autotune(s0, None, dt)
-    # deployed: y + c_prev and y + c_prev + blend * MLP, judged against the inertial start after
-    # trial_iterations VBD iterations each
-    ws_dep = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.5, smooth_sweeps=0)
-    # alternative: both candidates low-passed (smooth_sweeps of the 1-ring mean), judged at the start
-    ws_alt = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=args.blend, smooth_sweeps=args.smooth_sweeps)
+    # deployed: y + c_prev and y + c_prev + S(blend * MLP) (the network's change low-passed by
+    # delta_sweeps 1-ring sweeps), judged against the inertial start after trial_iterations VBD
+    # iterations each
+    ws_dep = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.5, smooth_sweeps=0,
+                              delta_sweeps=args.delta_sweeps)
+    # ablation: the same protocol without the network (y + c_prev against the inertial start)
+    ws_reuse = LearnedWarmStart(model, dt, mlp=ws.mlp, seed=0, blend=0.0, offer_reuse=False)
     one_model, one_state, _ = scenes.cloth_wind(dev)
     one_solver = SolverVBD(one_model, iterations=10)
     one_solver.autotune(one_state, None, dt)
@@ -899,10 +903,10 @@ def headline_leg(args, s_single, ws):
            "one_state": one_state, "one_out": one_model.state(), "one_solver": one_solver,
            "per": one_model.particle_count, "W": W, "candidate_names": ("reuse", "network"),
            "deployed": "batched_learned",
-           "variants": ("plain_batched", "batched_learned", "batched_learned_smoothed"),
+           "variants": ("plain_batched", "batched_learned", "batched_reuse_only"),
            "protocol": {"plain_batched": {},
                         "batched_learned": {"warm_start": ws_dep, "trial_iterations": args.trial_iterations},
-                        "batched_learned_smoothed": {"warm_start": ws_alt}}}
+                        "batched_reuse_only": {"warm_start": ws_reuse, "trial_iterations": args.trial_iterations}}}
     frame = args.train_frames + args.residual_frames
     mass = model.particle_mass
     _progress(f"headline: {W} worlds x {one_model.particle_count} vertices, {args.warmup} warm-up substeps")
@@ -1257,9 +1261,10 @@ def main():
             "step": "one substep (dt 1/600 s) of every world, each world iterated until r_w <= rel_tol * r0_w",
             "check_every": args.check_every,
             "solver": "batched + learned warm start: per world the inertial start y, y + c_prev and "
-                      "y + c_prev + 0.5 MLP (the BF16 network trained in this run), judged by their force "
-                      f"residuals after {args.trial_iterations} VBD iterations each; every solver on its own "
-                      "trajectory from one common state",
+                      f"y + c_prev + S(0.5 MLP) (the BF16 network trained in this run; S = {args.delta_sweeps} "
+                      "sweeps of the 1-ring mean on the network's change), judged by their force residuals after "
+                      f"{args.trial_iterations} VBD iteration(s) each; every solver on its own trajectory from one "
+                      "common state; batched_reuse_only in the comparison is the same protocol without the network",
             "pass_shape": head["pass_shape"],
         },
         "comparison": main_rec,

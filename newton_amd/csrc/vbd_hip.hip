@@ -3699,14 +3699,15 @@ int32_t vbd_collide_generate_noalloc(const VbdCollideDesc* d, void* stream, cons
                                      const VbdContactsOut* out) {
     if (!d || !q || !out || !out->soft_contact_count)
         return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: null argument");
+    // the size limit first: it is the documented rejection, whatever else the descriptor holds
+    const long total = (long)d->pair_count + d->edge_pair_count + d->face_pair_count;
+    if (total > VBD_NOALLOC_MAX_PAIRS)
+        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: the candidate pair count " + std::to_string(total) +
+                                     " exceeds the maximum of " + std::to_string(VBD_NOALLOC_MAX_PAIRS) +
+                                     " (use vbd_collide_create / vbd_collide_generate)");
     ColParams cp;
     int32_t rc = collide_params(d, "vbd_collide_generate_noalloc", cp);
     if (rc) return rc;
-    const long total = (long)cp.P + cp.E + cp.F;
-    if (total > VBD_NOALLOC_MAX_PAIRS)
-        return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: " + std::to_string(total) +
-                                     " candidate pairs exceed the maximum of " + std::to_string(VBD_NOALLOC_MAX_PAIRS) +
-                                     " (use vbd_collide_create / vbd_collide_generate)");
     if (d->shape_body)
         return fail(VBD_ERR_ARG, "vbd_collide_generate_noalloc: shapes are treated as static here (shape_body must be "
                                  "NULL; use vbd_collide_generate_bodies for body-attached shapes)");
