@@ -100,6 +100,9 @@ def parse():
     ap.add_argument("--pass-shape", default="auto",
                     help="color-pass shape of the fixed-iteration and --pmc-mode runs: 'auto' (autotuner) or "
                          "lanes:waves[:aligned], e.g. 256:7:1")
+    ap.add_argument("--moving-steps", type=int, default=20,
+                    help="timed frames of the moving-body, full-surface contact leg (0 disables)")
+    ap.add_argument("--moving-frame", type=int, default=45, help="first timed frame of that leg (cloth landed)")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube legs (0 disables)")
     ap.add_argument("--cube-train-frames", type=int, default=6,
                     help="frames of plain-VBD tet-cube solves that train the cube warm start (0 disables the leg)")
@@ -436,6 +439,82 @@ def fixed_iterations_leg(args):
         out["falling"]["roofline"]["traffic"] = None if traffic is None else round(traffic, 1)
         out["falling"]["roofline"]["traffic_detail"] = detail
     return out, model, info
+
+
+def moving_body_leg(args):
+    """The 256x256 cloth falling onto a sphere that an external rigid solver moves (sweeping, rising,
+    spinning; SolverVBD(integrate_with_external_rigid_solver=True) reads the poses of the step's end and
+    start), with the full-surface edge/face contacts on: 10 x (pose update, collide, step with 10
+    iterations) per frame, the pose update computed on the device from a device time so the frame is
+    captured in one HIP graph; timed after the cloth has landed."""
+    import torch
+
+    from newton_amd import CollisionPipeline, scenes
+    from newton_amd.solvers import SolverVBD
+
+    dev = "cuda:0"
+    model, state0, info = scenes.cloth_on_moving_sphere(dev, verts_per_side=args.verts, iterations=args.iterations)
+    n, sub, dt = model.particle_count, info["substeps"], info["dt"]
+    pose = info["body_pose"]
+    solver = SolverVBD(model, iterations=args.iterations, integrate_with_external_rigid_solver=True)
+    pipe = CollisionPipeline(model, soft_contact_margin=0.01, enable_rigid_soft_full_surface_contact=True)
+    contacts = pipe.contacts()
+    states = [state0, model.state()]
+    t = torch.zeros((), dtype=torch.float64, device=dev)
+
+    def frame():
+        a, b = states
+        for _ in range(sub):
+            a.body_q.copy_(pose(t))
+            t.add_(dt)
+            b.body_q.copy_(pose(t))
+            a.clear_forces()
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, dt)
+            a, b = b, a
+
+    warm = max(args.moving_frame - 2, 1)
+    _progress(f"moving body, full surface: frames {warm + 2}..{warm + 2 + args.moving_steps}")
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        for _ in range(warm):
+            frame()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            frame()
+        for _ in range(2):
+            graph.replay()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.moving_steps):
+            graph.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        gpu_ms = ev0.elapsed_time(ev1)
+    final = states[0]
+    pipe.collide(final, contacts)
+    torch.cuda.synchronize()
+    cnt = min(int(contacts.soft_contact_count.item()), contacts.soft_contact_max)
+    idx = contacts.soft_contact_indices[:cnt].cpu().numpy()
+    kinds = {"particle": int((idx[:, 1] < 0).sum()), "edge": int(((idx[:, 1] >= 0) & (idx[:, 2] < 0)).sum()),
+             "face": int((idx[:, 2] >= 0).sum())}
+    q = final.particle_q
+    centre = final.body_q[0, :3]
+    gap = float((torch.linalg.norm(q - centre, dim=1) - 0.4).min().item())
+    del graph
+    return {"workload": info["workload"], "vertices": n, "triangles": model.tri_count, "edges": model.edge_count,
+            "bodies": model.body_count, "iterations": args.iterations, "substeps_per_step": sub,
+            "full_surface": True, "candidate_pairs": {"particle": pipe.soft_rigid_contact_pair_count,
+                                                      "edge": len(pipe.soft_edge_rigid_pairs_host),
+                                                      "face": len(pipe.soft_face_rigid_pairs_host)},
+            "hip_graph": True, "frames": [warm + 2, warm + 2 + args.moving_steps],
+            "ms_per_step": round(gpu_ms / args.moving_steps, 3),
+            "vertex_steps_per_s": round(n * sub * args.moving_steps / (gpu_ms / 1e3), 1),
+            "records_at_end": kinds, "min_vertex_gap_to_sphere_m": round(gap, 5),
+            "finite": bool(torch.isfinite(q).all().item())}
 
 
 def batched_leg(args, name="batched", steps=None):
@@ -1219,6 +1298,11 @@ def main():
             legs[name] = batched_leg(args, {"batched_tets": "cubes"}.get(name, name), steps)
             _progress(f"{name}: {legs[name]['vertex_steps_per_s']:.4g} vertex-steps/s")
             torch.cuda.empty_cache()
+    if args.moving_steps > 0:
+        legs["moving_body_full_surface"] = moving_body_leg(args)
+        _progress(f"moving body, full surface: {legs['moving_body_full_surface']['vertex_steps_per_s']:.4g} "
+                  "vertex-steps/s")
+        torch.cuda.empty_cache()
     if args.cube_worlds > 0 and args.cube_train_frames > 0:
         _progress("tet cubes with the learned warm start")
         legs["batched_tets_residual"] = cubes_residual_leg(args, ws)

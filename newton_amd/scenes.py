@@ -96,6 +96,43 @@ def cloth_drop_on_sphere(device="cuda:0", verts_per_side: int = 256, size: float
     return model, state, info
 
 
+def cloth_on_moving_sphere(device="cuda:0", verts_per_side: int = 256, size: float = 2.0, height: float = 1.5,
+                           seed: int = 1, iterations: int = 10, sweep: float = 0.3, rise: float = 0.1,
+                           spin: float = 1.5):
+    """The 256x256 cloth of :func:`cloth_drop_on_sphere` falling onto a sphere carried by body 0, which an
+    external rigid solver moves: centre (sweep sin(1.5 t), rise t, 0) and a spin of ``spin`` rad/s about
+    y (``info["body_pose"](t)`` gives the [1, 7] pose on the device for a device time scalar ``t``), and
+    the static ground.  For ``SolverVBD(integrate_with_external_rigid_solver=True)`` and, optionally, the
+    full-surface edge/face contacts."""
+    n = verts_per_side - 1
+    b = ModelBuilder(up_axis="Y", gravity=-9.81)
+    b.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.5))
+    body = b.add_body(xform=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)), label="mover")
+    b.add_shape_sphere(body, radius=0.4, cfg=ModelBuilder.ShapeConfig(ke=1.0e4, kd=0.0, mu=0.18))
+    verts, tris = cloth_grid_mesh(n, n, size / n, size / n)
+    rot = quat_from_axis_angle((1.0, 0.0, 0.0), math.pi / 2)
+    b.add_cloth_mesh(pos=(-0.5 * size, height, -0.5 * size), rot=rot, scale=1.0, vel=(0.0, 0.0, 0.0),
+                     vertices=verts, indices=tris, density=0.1, tri_ke=1.0e4, tri_ka=1.0e4, tri_kd=1.0e-5,
+                     edge_ke=1.0, edge_kd=0.0)
+    rng = np.random.default_rng(seed)
+    ke = np.exp(rng.uniform(math.log(5.0e3), math.log(2.0e4), size=b.tri_count))
+    b.tri_materials[:, 0] = ke
+    b.color(include_bending=True)
+    model, state = _finish(b, device, 0.0, seed, 1.0e4, 0.0, 0.5)
+
+    def body_pose(t):
+        import torch
+
+        half = 0.5 * spin * t
+        zero = torch.zeros_like(t)
+        return torch.stack([sweep * torch.sin(1.5 * t), rise * t, zero, zero, torch.sin(half), zero,
+                            torch.cos(half)]).reshape(1, 7).to(torch.float32)
+
+    info = {"workload": f"cloth_on_moving_sphere_{verts_per_side}x{verts_per_side}", "dt": FRAME_DT / SUBSTEPS,
+            "substeps": SUBSTEPS, "iterations": iterations, "body_pose": body_pose}
+    return model, state, info
+
+
 def cloth_strip_pinned(device="cuda:0", nx: int = 6, ny: int = 4, seed: int = 0, tri_kd: float = 1e-2,
                        edge_kd: float = 1e-3, iterations: int = 3):
     """Tiny cloth (default 6x4 vertices) with membrane and bending damping on, one corner pinned.
@@ -410,6 +447,7 @@ def soft_cube(device="cuda:0", cells: int = 3, seed: int = 0, iterations: int = 
 
 
 __all__ = ["FRAME_DT", "SUBSTEPS", "FourierWind", "cloth_batched", "cloth_drop_on_sphere", "cloth_hanging",
+           "cloth_on_moving_sphere",
            "cloth_strip_pinned", "cloth_wind", "soft_cube", "soft_cubes_batched", "cloth_hanging_self_contact",
            "cloth_layers_on_sphere"]
 
