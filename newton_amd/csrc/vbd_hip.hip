@@ -681,6 +681,8 @@ struct KParams {
     const int* pos_coff;
     const float* cpack;
     const int* cpack_ef;  // per packed entry of an edge/face record: {c0, c1, c2, corner} {w0, w1, w2, 0}
+    float* crows;         // full-surface contexts: every packed entry's row (force 3 + Hessian 9), evaluated
+                          // by k_contact_rows at the pass start and read back in the serial sum
     const float* shape_margin;
     int shape_count;
     float friction_epsilon;
@@ -773,6 +775,23 @@ __device__ __forceinline__ void eval_packed_contact(const KParams& P, const floa
                                                     V3& f, M3& h) {
     const float4* c4 = reinterpret_cast<const float4*>(P.cpack) + 4 * (size_t)rec;
     eval_contact_rec(P, q, rec, c4[0], c4[1], c4[2], c4[3], dt, f, h);
+}
+
+// The rows of every packed contact entry of the color positions [p_lo, p_hi) at the positions q of
+// the pass start (a grid-stride loop over the device-sized entry range): with full-surface contacts a
+// vertex can carry a dozen entries and a batch's entries outnumber its LDS rows, so they are evaluated
+// here, one lane each, and the pass sums the stored rows in order.  An entry's force depends on its
+// vertex and on corners of other colors only, none of which moves before the vertex is solved, so
+// the rows hold the values the pass would evaluate itself.
+__global__ __launch_bounds__(256) void k_contact_rows(KParams P, const float* __restrict__ q, int p_lo, int p_hi,
+                                                      float dt) {
+    const int j0 = P.pos_coff[p_lo], j1 = P.pos_coff[p_hi];
+    for (int j = j0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); j < j1; j += (int)(gridDim.x * blockDim.x)) {
+        V3 f;
+        M3 h;
+        eval_packed_contact(P, q, j, dt, f, h);
+        store_row(P.crows + (size_t)kComp * j, f, h);
+    }
 }
 
 // evaluate_spring_force_and_hessian_both_vertices (particle_vbd_kernels.py:1685-1730), the
@@ -926,7 +945,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
     const int spr_b = b2.x, nspr = b2.y;
     const int nstatic = ntri + nedge + ntet + nspr;
     const int cbeg = ci.x, ncont = ci.y;
-    const bool par_contacts = nstatic + ncont <= ROWS;
+    const bool par_contacts = !P.crows && nstatic + ncont <= ROWS;
     const float dtr = 1.0f / (dt * dt);
 
     // ---- round trip 2: item record of this lane (items 0..255), own-vertex state
@@ -1069,6 +1088,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         float ca = 0.0f;
         if (par_contacts) {
             for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
+        } else if (P.crows) {
+            for (int r = c0; r < c1; ++r) ca = ca + P.crows[(size_t)kComp * (cbeg + r) + cc];
         } else {
             for (int r = c0; r < c1; ++r) {
                 V3 fc;
@@ -1671,6 +1692,8 @@ struct ColParams {
     const float* smargin;
     const float* body_q;  // this call's body poses (null without bodies)
     int body_count;
+    float4* ef_cache;     // [E + F][3] edge/face results of the count kernel, read by the write kernel
+                          // (null: the write kernel evaluates them again)
 };
 
 struct ContactGeo {
@@ -1905,6 +1928,47 @@ __device__ __forceinline__ ContactGeo soft_contact_geometry(const ColParams& C, 
     return face_contact_geometry(C, q, margin, tid - C.P - C.E);
 }
 
+// An edge/face result as the count kernel leaves it for the write kernel: {hit, body_pos} {bary, nx} {ny, nz}.
+__device__ __forceinline__ void ef_store(const ColParams& C, int tid, const ContactGeo& g) {
+    float4* o = C.ef_cache + 3 * (size_t)(tid - C.P);
+    if (!g.hit) {
+        o[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    o[0] = make_float4(1.0f, g.body_pos.x, g.body_pos.y, g.body_pos.z);
+    o[1] = make_float4(g.bary.x, g.bary.y, g.bary.z, g.normal.x);
+    o[2] = make_float4(g.normal.y, g.normal.z, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ ContactGeo ef_load(const ColParams& C, int tid) {
+    ContactGeo g;
+    const float4* o = C.ef_cache + 3 * (size_t)(tid - C.P);
+    const float4 a = o[0];
+    g.hit = a.x != 0.0f;
+    g.particle = -1;
+    if (!g.hit) return g;
+    const float4 b = o[1], c = o[2];
+    const int k = tid - C.P;
+    if (k < C.E) {
+        const int e = C.epairs[2 * k];
+        g.shape = C.epairs[2 * k + 1];
+        g.i0 = C.edge_idx[4 * e + 2];
+        g.i1 = C.edge_idx[4 * e + 3];
+        g.i2 = -1;
+    } else {
+        const int t = C.fpairs[2 * (k - C.E)];
+        g.shape = C.fpairs[2 * (k - C.E) + 1];
+        g.i0 = C.tri_idx[3 * t];
+        g.i1 = C.tri_idx[3 * t + 1];
+        g.i2 = C.tri_idx[3 * t + 2];
+    }
+    g.body_pos = v3(a.y, a.z, a.w);
+    g.bary = v3(b.x, b.y, b.z);
+    g.normal = v3(b.w, c.x, c.y);
+    g.body_vel = v3(0.0f, 0.0f, 0.0f);
+    return g;
+}
+
 // ---- soft-contact generation from per-tile hit counts ------------------------
 // Records in pair order (a stable compaction of the hitting pairs; the reference's atomic counter
 // gives an arbitrary order), with no per-pair flag or index array: a tile of kColTile pairs is one
@@ -1926,7 +1990,12 @@ __global__ __launch_bounds__(256) void k_collide_tile_count(ColParams C, const f
     int count = 0;
     for (int r = 0; r < kColRounds; ++r) {
         int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
-        bool hit = tid < C.P + C.E + C.F && soft_contact_geometry(C, q, margin, tid).hit;
+        bool hit = false;
+        if (tid < C.P + C.E + C.F) {
+            const ContactGeo g = soft_contact_geometry(C, q, margin, tid);
+            hit = g.hit;
+            if (C.ef_cache && tid >= C.P) ef_store(C, tid, g);
+        }
         int t;
         wave_rank(hit, lane, &t);
         count += t;
@@ -1972,7 +2041,8 @@ __global__ __launch_bounds__(256) void k_collide_tile_write(ColParams C, const f
         int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
         ContactGeo g;
         g.hit = false;
-        if (tid < C.P + C.E + C.F) g = soft_contact_geometry(C, q, margin, tid);
+        if (tid < C.P + C.E + C.F)
+            g = (C.ef_cache && tid >= C.P) ? ef_load(C, tid) : soft_contact_geometry(C, q, margin, tid);
         int t;
         int rank = wave_rank(g.hit, lane, &t);
         if (lane == 0) wsum[r][wave] = t;
@@ -2027,6 +2097,7 @@ struct VbdContext {
     int* pos_coff = nullptr;    // [N+1]
     float* cpack = nullptr;     // [entry_cap * 16]
     int* cpack_ef = nullptr;    // [entry_cap * 8] corners and weights of edge/face entries (full surface only)
+    float* crows = nullptr;     // [entry_cap * 12] contact rows of a pass (full surface only)
     int* cid_to_pack = nullptr; // [cap * 3] packed entry of each (contact, corner), -1 when none
     float* g_tmp = nullptr;     // [N,3] residual scratch
     double* r_part = nullptr;   // residual partial sums, one per first-stage workgroup
@@ -2083,6 +2154,7 @@ struct VbdCollide {
     VbdCollideDesc desc;
     ColParams cp;
     int* tiles = nullptr;  // hit count per tile of kColTile pairs, then its exclusive scan
+    float4* ef_cache = nullptr;  // edge/face results between the count and the write kernel
     bool has_bodies = false;  // some shape is attached to a body (its poses come with each call)
     int32_t max_body = -1;
 };
@@ -2650,7 +2722,8 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * c->entry_cap)) != hipSuccess)
         return cleanup("hipMalloc", e);
     if (prm->soft_contact_full_surface &&
-        (e = hipMalloc(&c->cpack_ef, sizeof(int) * 8 * c->entry_cap)) != hipSuccess)
+        ((e = hipMalloc(&c->cpack_ef, sizeof(int) * 8 * c->entry_cap)) != hipSuccess ||
+         (e = hipMalloc(&c->crows, sizeof(float) * kComp * c->entry_cap)) != hipSuccess))
         return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->g_tmp, sizeof(float) * 3 * (size_t)N)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->r_part, sizeof(double) * (size_t)norm_parts(N))) != hipSuccess)
@@ -2713,6 +2786,7 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     k.pos_coff = c->pos_coff;
     k.cpack = c->cpack;
     k.cpack_ef = c->cpack_ef;
+    k.crows = c->crows;
     k.shape_margin = m->shape_margin;
     k.shape_count = m->shape_count;
     k.friction_epsilon = prm->friction_epsilon;
@@ -2746,6 +2820,7 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->c_state);
     hipFree(c->cpack);
     hipFree(c->cpack_ef);
+    hipFree(c->crows);
     hipFree(c->g_tmp);
     hipFree(c->r_part);
     hipFree(c->r_dev);
@@ -2980,8 +3055,17 @@ int32_t launch_spring_rows(VbdContext* c, hipStream_t s, int j0, int j1, const f
     return VBD_OK;
 }
 
+int32_t launch_contact_rows(VbdContext* c, hipStream_t s, int p_lo, int p_hi, const float* q, float dt) {
+    if (!c->crows || !c->have_contacts || p_hi <= p_lo) return VBD_OK;
+    hipLaunchKernelGGL(k_contact_rows, dim3(kSelfGrid), dim3(256), 0, s, c->kp, q, p_lo, p_hi, dt);
+    VBD_LAUNCH_CHECK("k_contact_rows");
+    return VBD_OK;
+}
+
 int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt, int mode, float* g) {
     int32_t rc0 = launch_spring_rows(c, s, c->pre_color_off[color], c->pre_color_off[color + 1], q, dt);
+    if (rc0) return rc0;
+    rc0 = launch_contact_rows(c, s, c->color_offsets[color], c->color_offsets[color + 1], q, dt);
     if (rc0) return rc0;
     if (c->self_on) {
         hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev,
@@ -3008,6 +3092,8 @@ int32_t launch_color(VbdContext* c, hipStream_t s, int color, float* q, float dt
 // over every batch) and, for vertices with more than 256 items, the generic kernel.
 int32_t residual_by_passes(VbdContext* c, hipStream_t s, const float* q, float dt, float* g) {
     int32_t rc0 = launch_spring_rows(c, s, 0, c->pre_color_off.back(), q, dt);
+    if (rc0) return rc0;
+    rc0 = launch_contact_rows(c, s, 0, c->model.particle_count, q, dt);
     if (rc0) return rc0;
     if (c->self_on) {
         hipLaunchKernelGGL(k_sc_rows, dim3(kSelfGrid), dim3(256), 0, s, c->sp, q, c->kp.q_prev, 0,
@@ -3665,6 +3751,10 @@ int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
     const long total = (long)cp.P + cp.E + cp.F;
     const int ntiles = std::max(blocks_for(total, kColTile), 1);
     hipError_t e = hipMalloc(&col->tiles, sizeof(int) * (size_t)ntiles);
+    if (e == hipSuccess && cp.E + cp.F > 0) {
+        e = hipMalloc(&col->ef_cache, sizeof(float4) * 3 * ((size_t)cp.E + cp.F));
+        col->cp.ef_cache = col->ef_cache;
+    }
     if (e != hipSuccess) {
         std::string msg = std::string("vbd_collide_create: ") + hipGetErrorString(e);
         vbd_collide_destroy(col);
@@ -3677,6 +3767,7 @@ int32_t vbd_collide_create(const VbdCollideDesc* d, VbdCollide** out) {
 int32_t vbd_collide_destroy(VbdCollide* col) {
     if (!col) return VBD_OK;
     hipFree(col->tiles);
+    hipFree(col->ef_cache);
     delete col;
     return VBD_OK;
 }
