@@ -103,6 +103,8 @@ def parse():
     ap.add_argument("--moving-steps", type=int, default=20,
                     help="timed frames of the moving-body, full-surface contact leg (0 disables)")
     ap.add_argument("--moving-frame", type=int, default=45, help="first timed frame of that leg (cloth landed)")
+    ap.add_argument("--only-leg", choices=("moving",), default=None,
+                    help="run one secondary leg alone and print its record (for kernel-trace profiles)")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube legs (0 disables)")
     ap.add_argument("--cube-train-frames", type=int, default=6,
                     help="frames of plain-VBD tet-cube solves that train the cube warm start (0 disables the leg)")
@@ -1264,6 +1266,12 @@ def main():
         return
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.only_leg == "moving":
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps({"only_leg": "moving_body_full_surface", **moving_body_leg(args)}))
         return
     import torch
 
