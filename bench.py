@@ -103,7 +103,7 @@ def parse():
     ap.add_argument("--moving-steps", type=int, default=20,
                     help="timed frames of the moving-body, full-surface contact leg (0 disables)")
     ap.add_argument("--moving-frame", type=int, default=45, help="first timed frame of that leg (cloth landed)")
-    ap.add_argument("--only-leg", choices=("moving",), default=None,
+    ap.add_argument("--only-leg", choices=("moving", "self"), default=None,
                     help="run one secondary leg alone and print its record (for kernel-trace profiles)")
     ap.add_argument("--cube-worlds", type=int, default=512, help="worlds of the batched tet-cube legs (0 disables)")
     ap.add_argument("--cube-train-frames", type=int, default=6,
@@ -1272,6 +1272,12 @@ def main():
 
         torch.cuda.set_device(0)
         print(json.dumps({"only_leg": "moving_body_full_surface", **moving_body_leg(args)}))
+        return
+    if args.only_leg == "self":
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps({"only_leg": "self_contact", **self_contact_leg(args)}))
         return
     import torch
 

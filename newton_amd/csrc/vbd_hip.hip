@@ -783,6 +783,7 @@ __device__ __forceinline__ void eval_packed_contact(const KParams& P, const floa
 // here, one lane each, and the pass sums the stored rows in order.  An entry's force depends on its
 // vertex and on corners of other colors only, none of which moves before the vertex is solved, so
 // the rows hold the values the pass would evaluate itself.
+constexpr int kContactRowsGrid = 1024;  // work-groups of its grid-stride loop
 __global__ __launch_bounds__(256) void k_contact_rows(KParams P, const float* __restrict__ q, int p_lo, int p_hi,
                                                       float dt) {
     const int j0 = P.pos_coff[p_lo], j1 = P.pos_coff[p_hi];
@@ -887,6 +888,20 @@ __device__ __forceinline__ void finish_vertex(const KParams& P, float* __restric
     }
 }
 
+// ca + component cc of the global rows [r, re), added one at a time in row order; eight loads are in
+// flight before their additions (the rows' latency is paid once per eight rather than per row)
+__device__ __forceinline__ float add_rows_in_order(float ca, const float* __restrict__ rows, int r, int re, int cc) {
+    for (; r + 8 <= re; r += 8) {
+        float a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = rows[(size_t)kComp * (r + k) + cc];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ca = ca + a[k];
+    }
+    for (; r < re; ++r) ca = ca + rows[(size_t)kComp * r + cc];
+    return ca;
+}
+
 // One color pass over packed batches (mode 0: solve and update; mode 1: write the assembled force g).
 // The work-group's global reads form three dependent round trips:
 //   1  the batch block (header + vertex records) and the batch's contact ranges (indexed by batch)
@@ -918,6 +933,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
     __shared__ int s_nrow[MAXV];
     __shared__ int s_nspr[MAXV];
     __shared__ int s_cb[MAXV + 1];
+    __shared__ int s_so[MAXV + 1];
     __shared__ __align__(16) float s_rows[ROWS * kComp];
     __shared__ float s_acc[MAXV * kComp];
 
@@ -1011,7 +1027,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         s_nrow[t] = vr.z;
         s_nspr[t] = vr.w >> 17;
     }
-    if (t <= nv) s_cb[t] = cbt;
+    if (t <= nv) {
+        s_cb[t] = cbt;
+        // self-contact row bounds of the batch's color positions (consecutive positions)
+        if (P.has_self) s_so[t] = P.so_off[b0.x + t];
+    }
 
     // ---- round trip 3 + phase a: evaluate the lane's item
     if (kind != 0) {
@@ -1089,7 +1109,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         if (par_contacts) {
             for (int r = c0; r < c1; ++r) ca = ca + s_rows[(nstatic + r) * kComp + cc];
         } else if (P.crows) {
-            for (int r = c0; r < c1; ++r) ca = ca + P.crows[(size_t)kComp * (cbeg + r) + cc];
+            ca = add_rows_in_order(ca, P.crows + (size_t)kComp * cbeg, c0, c1, cc);
         } else {
             for (int r = c0; r < c1; ++r) {
                 V3 fc;
@@ -1099,11 +1119,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
             }
         }
         for (int r = q1, re = q1 + s_nspr[cs]; r < re; ++r) ca = ca + s_rows[r * kComp + cc];
-        if (P.has_self) {
-            const int pos = b0.x + cs;
-            for (int r = P.so_off[pos], re = P.so_off[pos + 1]; r < re; ++r)
-                ca = ca + P.so_rows[(size_t)kComp * r + cc];
-        }
+        if (P.has_self) ca = add_rows_in_order(ca, P.so_rows, s_so[cs], s_so[cs + 1], cc);
         acc = acc + ca;
         s_acc[cs * kComp + cc] = acc;
     }
@@ -1396,6 +1412,20 @@ __device__ __forceinline__ void contact_body_frame(const BodyPoses& B, int shape
 __global__ void k_fill_i32(int* __restrict__ p, int value, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
+}
+
+// up to kFillMax int arrays filled by one launch (grid-stride over each)
+constexpr int kFillMax = 6;
+struct FillList {
+    int* p[kFillMax];
+    int value[kFillMax];
+    int n[kFillMax];
+    int count;
+};
+__global__ void k_fill_many(FillList F) {
+    for (int k = 0; k < F.count; ++k)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n[k]; i += gridDim.x * blockDim.x)
+            F.p[k][i] = F.value[k];
 }
 
 // init_body_particle_contacts (rigid_vbd_kernels.py:3276-3321) + counts per color position: one entry
@@ -2468,7 +2498,7 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
     S.ee_filter = prm->ee_filter;
     S.ee_filter_off = prm->ee_filter_offsets;
     const long prims = std::max(std::max(T, E), 1);
-    S.nbucket = next_pow2(std::max(16L * prims, 1024L));
+    S.nbucket = next_pow2(std::max(4L * prims, 1024L));
     S.g_cap = (int)(27L * prims);
     hipError_t e;
     if ((e = sc_malloc(c, &S.vt_list, (size_t)N * vt_cap)) != hipSuccess ||
@@ -2478,8 +2508,8 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
         (e = sc_malloc(c, &S.ee_count, (size_t)E)) != hipSuccess || (e = sc_malloc(c, &S.ee_min, (size_t)E)) != hipSuccess ||
         (e = sc_malloc(c, &S.ext, 2)) != hipSuccess || (e = sc_malloc(c, &S.g_count, (size_t)S.nbucket + 1)) != hipSuccess ||
         (e = sc_malloc(c, &S.g_off, (size_t)S.nbucket + 1)) != hipSuccess ||
-        (e = sc_malloc(c, &S.g_cur, (size_t)S.nbucket)) != hipSuccess ||
-        (e = sc_malloc(c, &S.g_ent, (size_t)S.g_cap)) != hipSuccess || (e = sc_malloc(c, &S.g_overflow, 1)) != hipSuccess ||
+        (e = sc_malloc(c, &S.g_ent, (size_t)S.g_cap)) != hipSuccess ||
+        (e = sc_malloc(c, &S.g_box, 2 * (size_t)S.g_cap)) != hipSuccess || (e = sc_malloc(c, &S.g_overflow, 1)) != hipSuccess ||
         (e = sc_malloc(c, &S.it_cnt, (size_t)N + 1)) != hipSuccess ||
         (e = sc_malloc(c, &S.it_off, (size_t)N + 1)) != hipSuccess || (e = sc_malloc(c, &S.it_cur, (size_t)N)) != hipSuccess ||
         (e = sc_malloc(c, &S.ee_n, (size_t)N)) != hipSuccess ||
@@ -2503,6 +2533,9 @@ int32_t setup_self(VbdContext* c, const VbdModelDesc* m, const VbdSolverParams* 
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N, 256)), dim3(256), 0, 0, S.trunc, vbd_layout::fbits(1.0f), N);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, S.ncoll, 0, 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, 0, S.g_overflow, 0, 1);
+    // bucket counts start at zero; each detection's write pass counts them back down to zero
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(S.nbucket + 1, 256)), dim3(256), 0, 0, S.g_count, 0,
+                       S.nbucket + 1);
     if ((e = hipDeviceSynchronize()) != hipSuccess)
         return fail(VBD_ERR_HIP, std::string("vbd_create: self-contact init: ") + hipGetErrorString(e));
     c->self_on = true;
@@ -2528,33 +2561,37 @@ int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
     const int N = S.N, T = S.T, E = S.E;
     const long nslots = (long)E * S.ee_cap + (long)N * S.vt_cap;
     const int nb = S.nbucket;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, S.ext, 0, 2);
-    VBD_LAUNCH_CHECK("k_fill_i32");
+    // one launch clears what this detection accumulates into (the bucket counts are already zero)
+    FillList F{};
+    auto add_fill = [&F](int* p, int value, int n) {
+        F.p[F.count] = p;
+        F.value[F.count] = value;
+        F.n[F.count] = n;
+        ++F.count;
+    };
+    add_fill(S.ext, 0, 2);
+    add_fill(S.tri_min, vbd_layout::fbits(S.margin), T);
+    add_fill(S.it_cnt, 0, N + 1);
+    add_fill(S.ee_n, 0, N);
+    add_fill(S.ncoll, 0, 1);
+    hipLaunchKernelGGL(k_fill_many, dim3(blocks_for(std::max(std::max(T, N + 1), 1), 256)), dim3(256), 0, s, F);
+    VBD_LAUNCH_CHECK("k_fill_many");
     const int np = std::max(std::max(T, E), 1);
     hipLaunchKernelGGL(k_sc_extents, dim3(blocks_for(np, 256)), dim3(256), 0, s, S, q);
     VBD_LAUNCH_CHECK("k_sc_extents");
     for (int mode = 0; mode < 2; ++mode) {
         const int n = mode == 0 ? T : E;
-        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, s, S.g_count, 0, nb + 1);
-        VBD_LAUNCH_CHECK("k_fill_i32");
         if (n > 0) {
             hipLaunchKernelGGL(k_sc_grid_insert, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, q, mode, 0);
             VBD_LAUNCH_CHECK("k_sc_grid_insert");
         }
         int32_t rc = sc_scan(c, s, S.g_count, S.g_off, nb + 1);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_sc_copy_i32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, S.g_cur, S.g_off, nb);
-        VBD_LAUNCH_CHECK("k_sc_copy_i32");
         if (n > 0) {
             hipLaunchKernelGGL(k_sc_grid_insert, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, q, mode, 1);
             VBD_LAUNCH_CHECK("k_sc_grid_insert");
         }
         if (mode == 0) {
-            if (T > 0) {
-                hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(T, 256)), dim3(256), 0, s, S.tri_min,
-                                   vbd_layout::fbits(S.margin), T);
-                VBD_LAUNCH_CHECK("k_fill_i32");
-            }
             hipLaunchKernelGGL(k_sc_vt_query, dim3(blocks_for(N, kQPerBlock)), dim3(256), 0, s, S, q);
             VBD_LAUNCH_CHECK("k_sc_vt_query");
         } else if (E > 0) {
@@ -2563,10 +2600,6 @@ int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
         }
     }
     // item lists per color position
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N + 1, 256)), dim3(256), 0, s, S.it_cnt, 0, N + 1);
-    VBD_LAUNCH_CHECK("k_fill_i32");
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(N, 256)), dim3(256), 0, s, S.ee_n, 0, N);
-    VBD_LAUNCH_CHECK("k_fill_i32");
     const int gs = (int)std::max(1L, (nslots + 255) / 256);
     hipLaunchKernelGGL(k_sc_items_count, dim3(gs), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_count");
@@ -2580,8 +2613,6 @@ int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
     hipLaunchKernelGGL(k_sc_items_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_sort");
     // compact collision list for the truncation
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, S.ncoll, 0, 1);
-    VBD_LAUNCH_CHECK("k_fill_i32");
     hipLaunchKernelGGL(k_sc_collect, dim3(gs), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_collect");
     // displacements are measured from the positions of the last detection
@@ -3061,7 +3092,7 @@ int32_t launch_spring_rows(VbdContext* c, hipStream_t s, int j0, int j1, const f
 
 int32_t launch_contact_rows(VbdContext* c, hipStream_t s, int p_lo, int p_hi, const float* q, float dt) {
     if (!c->crows || !c->have_contacts || p_hi <= p_lo) return VBD_OK;
-    hipLaunchKernelGGL(k_contact_rows, dim3(kSelfGrid), dim3(256), 0, s, c->kp, q, p_lo, p_hi, dt);
+    hipLaunchKernelGGL(k_contact_rows, dim3(kContactRowsGrid), dim3(256), 0, s, c->kp, q, p_lo, p_hi, dt);
     VBD_LAUNCH_CHECK("k_contact_rows");
     return VBD_OK;
 }

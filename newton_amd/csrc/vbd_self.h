@@ -60,10 +60,10 @@ struct SelfParams {
     // grid
     int* ext;                    // [2] float bits: largest triangle / edge half-extent
     int nbucket;                 // power of two
-    int* g_count;                // [nbucket + 1]
+    int* g_count;                // [nbucket + 1] zero between detections (counted up, then back down)
     int* g_off;                  // [nbucket + 1]
-    int* g_cur;                  // [nbucket]
     int4* g_ent;                 // [g_cap] (prim, ix, iy, iz)
+    float4* g_box;               // [2 * g_cap] the entry's primitive box: (lo.xyz, -), (hi.xyz, -)
     int g_cap;
     int* g_overflow;             // [1]
     // per color position item lists and rows
@@ -82,7 +82,7 @@ struct SelfParams {
     int* ncoll;                  // [1]
 };
 
-constexpr int kSelfGrid = 1024;  // work-groups of the grid-stride kernels over device-sized ranges
+constexpr int kSelfGrid = 256;  // work-groups of the self-contact grid-stride kernels (one per CU)
 
 // ---------------------------------------------------------------------------
 // geometry and forces, in the oracle's (reference) operation order
@@ -513,7 +513,9 @@ __global__ __launch_bounds__(256) void k_sc_extents(SelfParams S, const float* _
     }
 }
 
-// pass 0: count entries per bucket; pass 1: write them
+// pass 0: count entries per bucket; pass 1: write them, each bucket filled from its end as its count
+// goes back down to zero (the order within a bucket does not matter: a query keeps the smallest ids
+// by rank, and counts and minima are order-free)
 __global__ void k_sc_grid_insert(SelfParams S, const float* __restrict__ q, int mode, int pass) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = mode == 0 ? S.T : S.E;
@@ -537,15 +539,14 @@ __global__ void k_sc_grid_insert(SelfParams S, const float* __restrict__ q, int 
                 if (pass == 0) {
                     atomicAdd(S.g_count + b, 1);
                 } else {
-                    const int slot = atomicAdd(S.g_cur + b, 1);
-                    if (slot < S.g_cap) S.g_ent[slot] = make_int4(p, ix, iy, iz);
+                    const int slot = S.g_off[b] + atomicSub(S.g_count + b, 1) - 1;
+                    if (slot < S.g_cap) {
+                        S.g_ent[slot] = make_int4(p, ix, iy, iz);
+                        S.g_box[2 * slot] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+                        S.g_box[2 * slot + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+                    }
                 }
             }
-}
-
-__global__ void k_sc_copy_i32(int* __restrict__ dst, const int* __restrict__ src, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
 }
 
 __device__ __forceinline__ bool sc_world_ok(int a, int b) { return a < 0 || b < 0 || a == b; }
@@ -593,16 +594,19 @@ __device__ __forceinline__ bool sc_vt_test(const SelfParams& S, const float* __r
     const V3 A = ld3(q, i0), B = ld3(q, i1), C = ld3(q, i2);
     if (!sc_box_overlap(lo, hi, sc_vmin(sc_vmin(A, B), C), sc_vmax(sc_vmax(A, B), C))) return false;
     if (v == i0 || v == i1 || v == i2) return false;
-    if (sc_filtered(S.vt_filter, S.vt_filter_off, v, t)) return false;
+    // the distance test before the filter list search and the rest-shape test (the tests are
+    // independent predicates, so their order leaves the result unchanged)
     V3 bary;
     const V3 cp = sc_tri_closest(A, B, C, x, bary);
     dist = length(sub(cp, x));
+    if (!(dist < S.margin)) return false;
+    if (sc_filtered(S.vt_filter, S.vt_filter_off, v, t)) return false;
     if (use_rest) {
         const V3 rp = ld3(S.q_rest, v);
         const V3 cr = sc_tri_closest(ld3(S.q_rest, i0), ld3(S.q_rest, i1), ld3(S.q_rest, i2), rp, bary);
         if (length(sub(cr, rp)) < S.rest_excl) return false;
     }
-    return dist < S.margin;
+    return true;
 }
 
 __device__ __forceinline__ bool sc_ee_test(const SelfParams& S, const float* __restrict__ q, int e, int a0, int a1,
@@ -612,15 +616,16 @@ __device__ __forceinline__ bool sc_ee_test(const SelfParams& S, const float* __r
     const V3 P2 = ld3(q, b0), Q2 = ld3(q, b1);
     if (!sc_box_overlap(lo, hi, sc_vmin(P2, Q2), sc_vmax(P2, Q2))) return false;
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
-    if (sc_filtered(S.ee_filter, S.ee_filter_off, e, g)) return false;
     const V3 st = sc_closest_edge_edge(P, Q, P2, Q2, S.par_eps);
     dist = st.z;
+    if (!(dist < S.margin)) return false;
+    if (sc_filtered(S.ee_filter, S.ee_filter_off, e, g)) return false;
     if (use_rest) {
         const V3 sr = sc_closest_edge_edge(ld3(S.q_rest, a0), ld3(S.q_rest, a1), ld3(S.q_rest, b0), ld3(S.q_rest, b1),
                                            S.par_eps);
         if (sr.z < S.rest_excl) return false;
     }
-    return dist < S.margin;
+    return true;
 }
 
 // A query runs on kQL lanes that stride over its cell's bucket; matches are appended to an LDS
@@ -668,7 +673,10 @@ __global__ __launch_bounds__(256) void k_sc_vt_query(SelfParams S, const float* 
         je = S.g_off[b + 1];
         for (int j = jb + lane; j < je; j += kQL) {
             const int4 en = S.g_ent[j];
+            const float4 bl = S.g_box[2 * j], bh = S.g_box[2 * j + 1];
             if (en.y != ix || en.z != iy || en.w != iz) continue;
+            // the primitive's box test (the first geometric test of the candidate) from the entry
+            if (!sc_box_overlap(lo, hi, v3(bl.x, bl.y, bl.z), v3(bh.x, bh.y, bh.z))) continue;
             float dist;
             if (!sc_vt_test(S, q, v, x, wv, lo, hi, use_rest, en.x, dist)) continue;
             mind = sc_min(mind, dist);
@@ -696,7 +704,10 @@ __global__ __launch_bounds__(256) void k_sc_vt_query(SelfParams S, const float* 
         int have = 0;
         for (int j = jb; j < je; ++j) {
             const int4 en = S.g_ent[j];
+            const float4 bl = S.g_box[2 * j], bh = S.g_box[2 * j + 1];
             if (en.y != ix || en.z != iy || en.w != iz) continue;
+            // the primitive's box test (the first geometric test of the candidate) from the entry
+            if (!sc_box_overlap(lo, hi, v3(bl.x, bl.y, bl.z), v3(bh.x, bh.y, bh.z))) continue;
             float dist;
             if (!sc_vt_test(S, q, v, x, wv, lo, hi, use_rest, en.x, dist)) continue;
             sc_keep_smallest(row, S.vt_cap, have, en.x);
@@ -744,7 +755,10 @@ __global__ __launch_bounds__(256) void k_sc_ee_query(SelfParams S, const float* 
         je = S.g_off[b + 1];
         for (int j = jb + lane; j < je; j += kQL) {
             const int4 en = S.g_ent[j];
+            const float4 bl = S.g_box[2 * j], bh = S.g_box[2 * j + 1];
             if (en.y != ix || en.z != iy || en.w != iz) continue;
+            // the primitive's box test (the first geometric test of the candidate) from the entry
+            if (!sc_box_overlap(lo, hi, v3(bl.x, bl.y, bl.z), v3(bh.x, bh.y, bh.z))) continue;
             float dist;
             if (!sc_ee_test(S, q, e, a0, a1, P, Q, we, lo, hi, use_rest, en.x, dist)) continue;
             mind = sc_min(mind, dist);
@@ -769,7 +783,10 @@ __global__ __launch_bounds__(256) void k_sc_ee_query(SelfParams S, const float* 
         int have = 0;
         for (int j = jb; j < je; ++j) {
             const int4 en = S.g_ent[j];
+            const float4 bl = S.g_box[2 * j], bh = S.g_box[2 * j + 1];
             if (en.y != ix || en.z != iy || en.w != iz) continue;
+            // the primitive's box test (the first geometric test of the candidate) from the entry
+            if (!sc_box_overlap(lo, hi, v3(bl.x, bl.y, bl.z), v3(bh.x, bh.y, bh.z))) continue;
             float dist;
             if (!sc_ee_test(S, q, e, a0, a1, P, Q, we, lo, hi, use_rest, en.x, dist)) continue;
             sc_keep_smallest(row, S.ee_cap, have, en.x);
@@ -852,11 +869,31 @@ __global__ void k_sc_items_vt(SelfParams S) {
     sc_vt_put(S, p, key, 3);
 }
 
-// insertion sort of each color position's vertex-triangle items by key
-__global__ void k_sc_items_sort(SelfParams S) {
+// insertion sort of each color position's vertex-triangle items by key (keys of one position are
+// distinct); lists of up to kSortCap items are sorted in the lane's LDS column, longer ones in place
+constexpr int kSortCap = 32;
+__global__ __launch_bounds__(256) void k_sc_items_sort(SelfParams S) {
+    __shared__ int2 s_it[kSortCap * 256];  // element k of lane l at k * 256 + l
     const int pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= S.N) return;
     const int b = S.it_off[pos] + S.ee_n[pos], e = S.it_off[pos + 1];
+    const int n = e - b;
+    if (n <= 1) return;
+    if (n <= kSortCap) {
+        int2* L = s_it + threadIdx.x;
+        for (int i = 0; i < n; ++i) L[i * 256] = S.items[b + i];
+        for (int i = 1; i < n; ++i) {
+            const int2 x = L[i * 256];
+            int j = i;
+            while (j > 0 && L[(j - 1) * 256].x > x.x) {
+                L[j * 256] = L[(j - 1) * 256];
+                --j;
+            }
+            L[j * 256] = x;
+        }
+        for (int i = 0; i < n; ++i) S.items[b + i] = L[i * 256];
+        return;
+    }
     for (int i = b + 1; i < e; ++i) {
         const int2 x = S.items[i];
         int j = i;
