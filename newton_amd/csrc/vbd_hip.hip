@@ -1668,12 +1668,15 @@ __global__ __launch_bounds__(kNormBlock) void k_residual_final(int nparts, const
 
 // Per-world norms: one work-group per world sums |g_p|^2 over the world's particle range
 // [off[w], off[w+1]) (thread-strided in double, then the fixed shuffle tree); the range is
-// clipped to [0, n).
+// clipped to [0, n).  With live flags (the batched driver), a frozen world -- whose iterate no
+// longer changes -- keeps the norm it was frozen with and is not summed again.
 __global__ __launch_bounds__(kNormBlock) void k_residual_worlds(int n, const int* __restrict__ off,
                                                                const float* __restrict__ g,
-                                                               double* __restrict__ out) {
+                                                               double* __restrict__ out,
+                                                               const int* __restrict__ live) {
     __shared__ double s_part[kNormBlock / 64];
     const int w = blockIdx.x;
+    if (live && !live[w]) return;  // uniform
     const int p0 = max(off[w], 0), p1 = min(off[w + 1], n);
     double v = 0.0;
     for (int i = p0 + (int)threadIdx.x; i < p1; i += kNormBlock) {
@@ -2134,7 +2137,6 @@ struct VbdContext {
     double* r_dev = nullptr;    // [1] squared residual norm (iterate-to-tolerance driver)
     int* zero_world = nullptr;  // [N] zeros: every particle in world 0 (single-world freezing)
     int* one_state = nullptr;   // [3] active, iterations, live of the single-world driver
-    unsigned int* ticket = nullptr;  // [1] work-group ticket of k_residual_worlds_freeze (zero between launches)
     double* r_host = nullptr;   // [1] pinned host copy of r_dev
     int32_t* w_host = nullptr;  // [kWorldPinned] pinned staging of the world driver's state
     void* cub_tmp = nullptr;
@@ -2763,9 +2765,7 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipMalloc(&c->r_dev, sizeof(double))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->zero_world, sizeof(int) * (size_t)std::max(N, 1))) != hipSuccess ||
         (e = hipMemset(c->zero_world, 0, sizeof(int) * (size_t)std::max(N, 1))) != hipSuccess ||
-        (e = hipMalloc(&c->one_state, sizeof(int) * 3)) != hipSuccess ||
-        (e = hipMalloc(&c->ticket, sizeof(unsigned int))) != hipSuccess ||
-        (e = hipMemset(c->ticket, 0, sizeof(unsigned int))) != hipSuccess)
+        (e = hipMalloc(&c->one_state, sizeof(int) * 3)) != hipSuccess)
         return cleanup("hipMalloc", e);
     if ((e = hipHostMalloc(&c->r_host, sizeof(double))) != hipSuccess) return cleanup("hipHostMalloc", e);
     if ((e = hipHostMalloc(&c->w_host, sizeof(int32_t) * kWorldPinned)) != hipSuccess)
@@ -2860,7 +2860,6 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->r_dev);
     hipFree(c->zero_world);
     hipFree(c->one_state);
-    hipFree(c->ticket);
     if (c->r_host) hipHostFree(c->r_host);
     if (c->w_host) hipHostFree(c->w_host);
     hipFree(c->cub_tmp);
@@ -3403,18 +3402,24 @@ int32_t vbd_set_iterate_range(VbdContext* c, void* stream, float* q_dst, const f
     return VBD_OK;
 }
 
+namespace {
+// the residual and the per-world norms (of the live worlds only when `live` is given)
+int32_t residual_worlds(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
+                        const int32_t* world_offsets, double* norm_sq_out, const int* live) {
+    int32_t rc = vbd_residual(c, stream, q, dt, nullptr, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual_worlds, dim3(world_count), dim3(kNormBlock), 0, (hipStream_t)stream,
+                       c->model.particle_count, world_offsets, c->g_tmp, norm_sq_out, live);
+    VBD_LAUNCH_CHECK("k_residual_worlds");
+    return VBD_OK;
+}
+}  // namespace
+
 int32_t vbd_residual_worlds(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
                             const int32_t* world_offsets, double* norm_sq_out) {
     if (!c || !q || !world_offsets || !norm_sq_out) return fail(VBD_ERR_ARG, "vbd_residual_worlds: null argument");
     if (world_count < 1) return fail(VBD_ERR_ARG, "vbd_residual_worlds: world_count must be >= 1");
-    hipStream_t s = (hipStream_t)stream;
-    int32_t rc = vbd_residual(c, stream, q, dt, nullptr, nullptr);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_residual_worlds, dim3(world_count), dim3(kNormBlock), 0, s, c->model.particle_count,
-                       world_offsets, c->g_tmp,
-                       norm_sq_out);
-    VBD_LAUNCH_CHECK("k_residual_worlds");
-    return VBD_OK;
+    return residual_worlds(c, stream, q, dt, world_count, world_offsets, norm_sq_out, nullptr);
 }
 
 namespace {
@@ -3473,67 +3478,43 @@ int32_t build_live_list(VbdContext* c, hipStream_t s, const std::vector<int32_t>
 }
 }  // namespace
 
-// Per-world norms and freezing in one launch for the batched driver: each work-group sums its world's
-// squared residual in the fixed order of k_residual_worlds (a frozen world, whose iterate no longer
-// changes, keeps the norm it was frozen with and is not summed again), and the last work-group to
-// finish freezes every live world within its target (recording the iterations it ran) and publishes
-// the live count, freezing all of them once at most stop_at_live (> 0) remain.  `ticket` is zero on
-// entry and is left zero.
-__global__ __launch_bounds__(kNormBlock) void k_residual_worlds_freeze(int n, const int* __restrict__ off,
-                                                                      const float* __restrict__ g,
-                                                                      double* __restrict__ norm,
-                                                                      const double* __restrict__ target,
-                                                                      int* __restrict__ st, int done,
-                                                                      int stop_at_live,
-                                                                      unsigned int* __restrict__ ticket) {
-    __shared__ double s_part[kNormBlock / 64];
-    __shared__ int s_live[kNormBlock / 64];
-    __shared__ int s_last;
-    const int w = blockIdx.x;
-    const int nw = (int)gridDim.x;
-    if (st[w]) {
-        const int p0 = max(off[w], 0), p1 = min(off[w + 1], n);
-        double v = 0.0;
-        for (int i = p0 + (int)threadIdx.x; i < p1; i += kNormBlock) {
-            double a = g[3 * (size_t)i], b = g[3 * (size_t)i + 1], c = g[3 * (size_t)i + 2];
-            v += a * a + b * b + c * c;
-        }
-        double sum = block_sum_f64(v, s_part);
-        if (threadIdx.x == 0) norm[w] = sum;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(ticket, 1u) == (unsigned int)(nw - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+// Freezes the worlds whose squared norm reached their target after `done` iterations (one
+// work-group; st = [active W | iterations W | live count]).  Frozen worlds stay frozen.  Once at
+// most stop_at_live (> 0) worlds are live, the rest are frozen too (the driver stops there).
+__global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __restrict__ norm,
+                                                      const double* __restrict__ target, int* __restrict__ st,
+                                                      int done, int stop_at_live) {
+    __shared__ int s_live[1024 / 64];
+    __shared__ int s_total;
     int live = 0;
-    for (int k = threadIdx.x; k < nw; k += kNormBlock) {
-        int a = st[k];
-        if (a && norm[k] <= target[k]) {
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+        int a = st[w];
+        if (a && norm[w] <= target[w]) {
             a = 0;
-            st[k] = 0;
-            st[nw + k] = done;
+            st[w] = 0;
+            st[nw + w] = done;
         }
         live += a;
     }
-    for (int o = 32; o > 0; o >>= 1) live += __shfl_down(live, o, 64);
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_down(live, off, 64);
     if ((threadIdx.x & 63) == 0) s_live[threadIdx.x >> 6] = live;
     __syncthreads();
-    int total = 0;
-    for (int k = 0; k < kNormBlock / 64; ++k) total += s_live[k];
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) sum += s_live[k];
+        s_total = sum;
+    }
+    __syncthreads();
+    const int total = s_total;
     const bool stop_all = stop_at_live > 0 && total <= stop_at_live;
     if (stop_all) {
-        for (int k = threadIdx.x; k < nw; k += kNormBlock)
-            if (st[k]) {
-                st[k] = 0;
-                st[nw + k] = done;
+        for (int w = threadIdx.x; w < nw; w += blockDim.x)
+            if (st[w]) {
+                st[w] = 0;
+                st[nw + w] = done;
             }
     }
-    if (threadIdx.x == 0) {
-        st[2 * nw] = stop_all ? 0 : total;
-        *ticket = 0u;
-    }
+    if (threadIdx.x == 0) st[2 * nw] = stop_all ? 0 : total;
 }
 
 __global__ void k_world_init(int nw, int* __restrict__ st) {
@@ -3583,18 +3564,10 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     }
     std::vector<int32_t> st_host((size_t)(2 * W + 1), 1), live_prev((size_t)W, 1);
     int32_t done = 0, live = W;
-    // the residual of the live worlds' particles, their norms and the freezing decisions
-    auto evaluate = [&](int32_t iters_done) -> int32_t {
-        int32_t r = vbd_residual(c, stream, q, dt, nullptr, nullptr);
-        if (r) return r;
-        hipLaunchKernelGGL(k_residual_worlds_freeze, dim3(W), dim3(kNormBlock), 0, s, c->model.particle_count,
-                           world_offsets, c->g_tmp, norm, target, world_state_dev, iters_done, stop_at_live,
-                           c->ticket);
-        VBD_LAUNCH_CHECK("k_residual_worlds_freeze");
-        return VBD_OK;
-    };
-    int32_t rc = evaluate(0);
+    int32_t rc = residual_worlds(c, stream, q, dt, W, world_offsets, norm, world_state_dev);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0, stop_at_live);
+    VBD_LAUNCH_CHECK("k_world_freeze");
     const int32_t poll_every = kWorldPoll;
     int32_t since_poll = poll_every;  // poll before the first iteration
     for (;;) {
@@ -3627,8 +3600,11 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
             if (rc) return rc;
         }
         done += n;
-        rc = evaluate(done);
+        rc = residual_worlds(c, stream, q, dt, W, world_offsets, norm, world_state_dev);
         if (rc) return rc;
+        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, done,
+                           stop_at_live);
+        VBD_LAUNCH_CHECK("k_world_freeze");
         ++since_poll;
     }
     VBD_HIP_TRY(hipMemcpyAsync(norm_sq_out, norm, sizeof(double) * (size_t)W, hipMemcpyDeviceToHost, s));
