@@ -207,14 +207,23 @@ class LearnedWarmStart:
         self.pred = torch.empty((n, 3), dtype=torch.float32, device=self.device)
         self.acc = torch.zeros((n, 3), dtype=torch.float32, device=self.device)
         self._tmp = torch.empty((n, 3), dtype=torch.float32, device=self.device)
-        # 1-ring mean operator for the smoothness term of the training loss
+        # 1-ring sums for the smoothness term of the training loss, from a padded neighbour table:
+        # each row is summed in neighbour order, so training is bit-reproducible (a scatter-add
+        # would add in the order its atomics land)
         deg = (self.nbr_off[1:] - self.nbr_off[:-1]).long()
-        self._rows = torch.repeat_interleave(torch.arange(n, device=self.device), deg)
+        cols = torch.arange(int(deg.max().item()) if n else 0, device=self.device)
+        valid = cols[None, :] < deg[:, None]
+        idx = (self.nbr_off[:-1, None].long() + cols[None, :]).clamp(max=max(self.nbr.numel() - 1, 0))
+        self._ring_idx = torch.where(valid, self.nbr.long()[idx], torch.zeros_like(idx))
+        self._ring_w = valid.float()[:, :, None]
         self._deg = deg.clamp(min=1).float()[:, None]
         self._act = self.active.float()[:, None]
 
+    def _ring_sum(self, e: torch.Tensor) -> torch.Tensor:
+        return (e[self._ring_idx] * self._ring_w).sum(dim=1)
+
     def _ring_mean(self, e: torch.Tensor) -> torch.Tensor:
-        return torch.zeros_like(e).index_add_(0, self._rows, e[self.nbr.long()]) / self._deg
+        return self._ring_sum(e) / self._deg
 
     def train_substep(self, feat: torch.Tensor, target: torch.Tensor, lr: float, smooth: float = 100.0):
         """One Adam step on one substep's rows (all particles) with a smoothness-weighted loss
@@ -229,7 +238,7 @@ class LearnedWarmStart:
         if smooth > 0.0:
             le = (e - self._ring_mean(e)) * self._act
             w = le / self._deg
-            lt = le - torch.zeros_like(le).index_add_(0, self._rows, w[self.nbr.long()])  # (I - A)^T (L e)
+            lt = le - self._ring_sum(w)  # (I - A)^T (L e)
             g = e + smooth * lt * self._act
         self.mlp.train_step(feat, (pred - 0.5 * g).contiguous(), self._act[:, 0].contiguous(), lr=lr)
         return (e * e).sum() / (3.0 * self._act.sum().clamp(min=1.0))
