@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-smooth", type=float, default=100.0,
                     help="weight of the 1-ring smoothness term of the warm-start training loss")
+    ap.add_argument("--train-target", choices=("converged", "tolerance"), default="converged",
+                    help="training target of the cloth warm start: the correction after train_iterations "
+                         "iterations, or the one the tolerance solve stops at")
     ap.add_argument("--train-protocol", choices=("deployed", "converged"), default="deployed",
                     help="trajectory of the training data: the deployed tolerance solve, or converged solves")
     ap.add_argument("--verts", type=int, default=256, help="cloth vertices per side of the fixed-iteration leg")
@@ -675,11 +678,16 @@ def residual_leg(args):
             if float(solver.residual(_trial(trial), dt).item()) < r0:
                 solver.set_iterate(s0, guess, commit=True)
             its = 0
-            if deployed:
+            tol_target = args.train_target == "tolerance"
+            if deployed or tol_target:
                 its, _ = solver.iterate_to_residual(s0, dt, args.rel_tol * args.rel_tol * r0, args.max_iterations)
                 x_tol = s0.particle_q.clone()
-            solver.iterate(s0, dt, max(args.train_iterations - its, 0))
-            targs.append(ws.target(solver, s0))
+            if tol_target:
+                targs.append(ws.target(solver, s0))
+            if not (tol_target and deployed):
+                solver.iterate(s0, dt, max(args.train_iterations - its, 0))
+            if not tol_target:
+                targs.append(ws.target(solver, s0))
             if deployed:
                 solver.set_iterate(s0, x_tol, commit=True)
             ws.observe(solver, s0)
