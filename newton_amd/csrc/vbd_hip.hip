@@ -2602,20 +2602,22 @@ int32_t self_detect(VbdContext* c, hipStream_t s, const float* q) {
         }
     }
     // item lists per color position
-    const int gs = (int)std::max(1L, (nslots + 255) / 256);
-    hipLaunchKernelGGL(k_sc_items_count, dim3(gs), dim3(256), 0, s, S);
+    hipLaunchKernelGGL(k_sc_items_count_ee, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
+    VBD_LAUNCH_CHECK("k_sc_items_count_ee");
+    const int gv = (int)std::max(1L, ((long)N * S.vt_cap + 255) / 256);
+    hipLaunchKernelGGL(k_sc_items_count, dim3(gv), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_count");
     int32_t rc = sc_scan(c, s, S.it_cnt, S.it_off, N + 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_sc_items_ee, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_ee");
-    const int gv = (int)std::max(1L, ((long)N * S.vt_cap + 255) / 256);
     hipLaunchKernelGGL(k_sc_items_vt, dim3(gv), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_vt");
     hipLaunchKernelGGL(k_sc_items_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_items_sort");
     // compact collision list for the truncation
-    hipLaunchKernelGGL(k_sc_collect, dim3(gs), dim3(256), 0, s, S);
+    hipLaunchKernelGGL(k_sc_collect, dim3((int)std::max(1L, (nslots + 256L * kCollectPer - 1) / (256L * kCollectPer))),
+                       dim3(256), 0, s, S);
     VBD_LAUNCH_CHECK("k_sc_collect");
     // displacements are measured from the positions of the last detection
     hipLaunchKernelGGL(k_sc_copy3, dim3(blocks_for(3L * N, 256)), dim3(256), 0, s, S.pos_cd, q, N);

@@ -807,21 +807,28 @@ __global__ __launch_bounds__(256) void k_sc_ee_query(SelfParams S, const float* 
 // particle's edges (its CSR lists them in ascending id) -- then the vertex-triangle items in
 // (p, slot) order, appended by atomics and sorted (short lists: a vertex's own contacts and
 // those of the few triangles around it).
-__global__ void k_sc_items_count(SelfParams S) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nee = (long)S.E * S.ee_cap;
-    if (i < nee) {
-        const int e1 = (int)(i / S.ee_cap), k = (int)(i - (long)e1 * S.ee_cap);
-        if (k >= S.ee_count[e1]) return;
-        const int* a = S.edge_idx + 4 * e1;
-        for (int r = 0; r < 2; ++r) {
-            const int pos = S.pos_of[a[2 + r]];
-            atomicAdd(S.it_cnt + pos, 1);
-            atomicAdd(S.ee_n + pos, 1);
+// Edge-edge item counts of each color position, by the walk over the particle's edges that
+// k_sc_items_ee writes the items with (no atomics: a contact-rich detection has a few hundred
+// thousand edge-edge contacts, two items each).
+__global__ void k_sc_items_count_ee(SelfParams S) {
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= S.N) return;
+    const int v = S.color_particles[pos];
+    int n = 0;
+    if (S.E > 0) {
+        for (int c = S.adj_edge_off[v]; c < S.adj_edge_off[v + 1]; c += 2) {
+            if (S.adj_edge[c + 1] < 2) continue;
+            const int e1 = S.adj_edge[c];
+            n += S.ee_count[e1] < S.ee_cap ? S.ee_count[e1] : S.ee_cap;
         }
-        return;
     }
-    const long j = i - nee;
+    S.it_cnt[pos] = n;
+    S.ee_n[pos] = n;
+}
+
+// vertex-triangle item counts (added to the edge-edge counts): one per corner and one for the vertex
+__global__ void k_sc_items_count(SelfParams S) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= (long)S.N * S.vt_cap) return;
     const int p = (int)(j / S.vt_cap), k = (int)(j - (long)p * S.vt_cap);
     if (k >= S.vt_count[p]) return;
@@ -950,19 +957,53 @@ __device__ void sc_row(const SelfParams& S, const float* __restrict__ q, const f
 // truncation
 // ---------------------------------------------------------------------------
 // compact list of the recorded collision slots (edge-edge slots first in the slot space)
-__global__ void k_sc_collect(SelfParams S) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ bool sc_slot_recorded(const SelfParams& S, long i) {
     const long nee = (long)S.E * S.ee_cap;
     if (i < nee) {
         const int e1 = (int)(i / S.ee_cap), k = (int)(i - (long)e1 * S.ee_cap);
-        if (k >= S.ee_count[e1]) return;
-    } else {
-        const long j = i - nee;
-        if (j >= (long)S.N * S.vt_cap) return;
-        const int p = (int)(j / S.vt_cap), k = (int)(j - (long)p * S.vt_cap);
-        if (k >= S.vt_count[p]) return;
+        return k < S.ee_count[e1];
     }
-    S.coll[atomicAdd(S.ncoll, 1)] = (int)i;
+    const long j = i - nee;
+    if (j >= (long)S.N * S.vt_cap) return false;
+    const int p = (int)(j / S.vt_cap), k = (int)(j - (long)p * S.vt_cap);
+    return k < S.vt_count[p];
+}
+
+// kCollectPer slots per lane, one counter atomic per work-group (a contact-rich detection records a
+// few hundred thousand slots, which one atomic each -- or one per wave, a wave mostly covering one
+// edge's list -- would serialise on the counter)
+constexpr int kCollectPer = 16;
+__global__ __launch_bounds__(256) void k_sc_collect(SelfParams S) {
+    __shared__ int s_cnt[256];
+    __shared__ int s_base;
+    const int t = threadIdx.x;
+    const long nslots = (long)S.E * S.ee_cap + (long)S.N * S.vt_cap;
+    const long b0 = (long)blockIdx.x * 256 * kCollectPer;
+    int mine = 0;
+    unsigned rec = 0u;
+    for (int k = 0; k < kCollectPer; ++k) {
+        const long i = b0 + (long)k * 256 + t;
+        if (i < nslots && sc_slot_recorded(S, i)) {
+            rec |= 1u << k;
+            ++mine;
+        }
+    }
+    // inclusive scan of the lanes' counts
+    s_cnt[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int v = t >= o ? s_cnt[t - o] : 0;
+        __syncthreads();
+        s_cnt[t] += v;
+        __syncthreads();
+    }
+    const int total = s_cnt[255];
+    if (total == 0) return;  // uniform
+    if (t == 0) s_base = atomicAdd(S.ncoll, total);
+    __syncthreads();
+    int w = s_base + s_cnt[t] - mine;
+    for (int k = 0; k < kCollectPer; ++k)
+        if ((rec >> k) & 1u) S.coll[w++] = (int)(b0 + (long)k * 256 + t);
 }
 
 __device__ void sc_trunc_one(const SelfParams& S, const float* __restrict__ disp, long i);
