@@ -340,11 +340,13 @@ def roofline_only(args):
                       "pass_shape": args.pass_shape, **roof}))
 
 
-def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
+def measure_traffic(args, workload="drop", timeout_s: float = 240.0, shape: str | None = None):
     """HBM bytes per color-pass launch from two rocprofv3 --pmc child runs (FETCH_SIZE, then WRITE_SIZE).
 
     Each counter is scaled by what it reads for the 256 MiB device copy made in
     the same run (bytes copied / counter bytes), so the figure is in real bytes.
+    ``shape`` ("lanes:waves:aligned") fixes the pass shape of the child run (default: the
+    fixed-iteration leg's ``--pass-shape`` for "drop", the autotuner's choice otherwise).
     Returns (bytes_per_launch, details) or (None, reason).
     """
     import csv
@@ -363,7 +365,7 @@ def measure_traffic(args, workload="drop", timeout_s: float = 240.0):
                    str(ROOT / "bench.py"), "--pmc-mode", "--pmc-workload", workload, "--verts", str(args.verts),
                    "--iterations", str(args.iterations), "--worlds", str(args.worlds), "--cube-worlds",
                    str(args.cube_worlds), "--residual-worlds", str(args.residual_worlds), "--roofline-reps", "5",
-                   "--pass-shape", args.pass_shape if workload == "drop" else "auto"]
+                   "--pass-shape", shape or (args.pass_shape if workload == "drop" else "auto")]
             try:
                 proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp",
                                       env={**os.environ, "TMPDIR": "/tmp"})
@@ -1320,6 +1322,16 @@ def main():
             legs[name] = batched_leg(args, {"batched_tets": "cubes"}.get(name, name), steps)
             _progress(f"{name}: {legs[name]['vertex_steps_per_s']:.4g} vertex-steps/s")
             torch.cuda.empty_cache()
+            if args.traffic and name in ("batched", "batched_tets"):
+                # HBM bytes per pass at the pass shape the leg ran (two rocprofv3 --pmc child runs)
+                _progress(f"HBM traffic of the {name} leg: two rocprofv3 --pmc child runs")
+                tune = legs[name]["autotune"]
+                shape = f"{tune['block']}:{tune['waves']}:{int(tune['aligned'])}"
+                traffic, detail = measure_traffic(args, {"batched_tets": "cubes"}.get(name, name), shape=shape)
+                legs[name]["roofline"]["traffic"] = None if traffic is None else round(traffic, 1)
+                legs[name]["roofline"]["traffic_detail"] = detail
+                legs[name]["roofline"]["traffic_vs_algorithmic"] = (
+                    None if traffic is None else round(traffic / legs[name]["roofline"]["bytes_per_launch"], 3))
     if args.moving_steps > 0:
         legs["moving_body_full_surface"] = moving_body_leg(args)
         _progress(f"moving body, full surface: {legs['moving_body_full_surface']['vertex_steps_per_s']:.4g} "
