@@ -1666,25 +1666,41 @@ __global__ __launch_bounds__(kNormBlock) void k_residual_final(int nparts, const
     if (threadIdx.x == 0) out[0] = s;
 }
 
-// Per-world norms: one work-group per world sums |g_p|^2 over the world's particle range
-// [off[w], off[w+1]) (thread-strided in double, then the fixed shuffle tree); the range is
-// clipped to [0, n).  With live flags (the batched driver), a frozen world -- whose iterate no
-// longer changes -- keeps the norm it was frozen with and is not summed again.
+// Per-world norms, first stage: `split` work-groups per world each sum |g_p|^2 over one contiguous
+// slice of the world's particle range [off[w], off[w+1]) (clipped to [0, n); thread-strided in
+// double, then the fixed shuffle tree) into part[w * split + k]; k_world_norms (or the batched
+// driver's freezing kernel) adds a world's slices in order.  Splitting keeps a few dozen large
+// worlds from leaving most CUs idle.  With live flags (the batched driver), a frozen world --
+// whose iterate no longer changes -- keeps the norm it was frozen with and is not summed again.
 __global__ __launch_bounds__(kNormBlock) void k_residual_worlds(int n, const int* __restrict__ off,
-                                                               const float* __restrict__ g,
-                                                               double* __restrict__ out,
+                                                               const float* __restrict__ g, int split,
+                                                               double* __restrict__ part,
                                                                const int* __restrict__ live) {
     __shared__ double s_part[kNormBlock / 64];
-    const int w = blockIdx.x;
+    const int w = blockIdx.x / split, k = blockIdx.x - w * split;
     if (live && !live[w]) return;  // uniform
-    const int p0 = max(off[w], 0), p1 = min(off[w + 1], n);
+    const int p0 = max(off[w], 0), p1 = max(min(off[w + 1], n), p0);
+    const int chunk = (p1 - p0 + split - 1) / split;
+    const int q0 = min(p0 + k * chunk, p1), q1 = min(q0 + chunk, p1);
     double v = 0.0;
-    for (int i = p0 + (int)threadIdx.x; i < p1; i += kNormBlock) {
+    for (int i = q0 + (int)threadIdx.x; i < q1; i += kNormBlock) {
         double a = g[3 * (size_t)i], b = g[3 * (size_t)i + 1], c = g[3 * (size_t)i + 2];
         v += a * a + b * b + c * c;
     }
     double sum = block_sum_f64(v, s_part);
-    if (threadIdx.x == 0) out[w] = sum;
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
+}
+
+// a world's squared norm: its slices' sums added in slice order
+__device__ __forceinline__ double world_norm(const double* __restrict__ part, int split, int w) {
+    double s = 0.0;
+    for (int k = 0; k < split; ++k) s += part[(size_t)w * split + k];
+    return s;
+}
+
+__global__ void k_world_norms(int nw, const double* __restrict__ part, int split, double* __restrict__ out) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) out[w] = world_norm(part, split, w);
 }
 
 // Iterate from a caller's guess: for solved particles (ACTIVE, inv_mass != 0)
@@ -3405,13 +3421,21 @@ int32_t vbd_set_iterate_range(VbdContext* c, void* stream, float* q_dst, const f
 }
 
 namespace {
-// the residual and the per-world norms (of the live worlds only when `live` is given)
-int32_t residual_worlds(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
-                        const int32_t* world_offsets, double* norm_sq_out, const int* live) {
+// slices per world of the per-world norms: up to 16, within the first-stage scratch (r_part)
+int world_split(VbdContext* c, int32_t world_count) {
+    const int cap = norm_parts(c->model.particle_count);
+    return std::max(1, std::min(16, cap / std::max(world_count, 1)));
+}
+
+// the residual and the per-world slice sums (of the live worlds only when `live` is given) in
+// c->r_part, world_split slices per world
+int32_t residual_world_parts(VbdContext* c, void* stream, const float* q, float dt, int32_t world_count,
+                             const int32_t* world_offsets, const int* live) {
     int32_t rc = vbd_residual(c, stream, q, dt, nullptr, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_residual_worlds, dim3(world_count), dim3(kNormBlock), 0, (hipStream_t)stream,
-                       c->model.particle_count, world_offsets, c->g_tmp, norm_sq_out, live);
+    const int split = world_split(c, world_count);
+    hipLaunchKernelGGL(k_residual_worlds, dim3(world_count * split), dim3(kNormBlock), 0, (hipStream_t)stream,
+                       c->model.particle_count, world_offsets, c->g_tmp, split, c->r_part, live);
     VBD_LAUNCH_CHECK("k_residual_worlds");
     return VBD_OK;
 }
@@ -3421,7 +3445,12 @@ int32_t vbd_residual_worlds(VbdContext* c, void* stream, const float* q, float d
                             const int32_t* world_offsets, double* norm_sq_out) {
     if (!c || !q || !world_offsets || !norm_sq_out) return fail(VBD_ERR_ARG, "vbd_residual_worlds: null argument");
     if (world_count < 1) return fail(VBD_ERR_ARG, "vbd_residual_worlds: world_count must be >= 1");
-    return residual_worlds(c, stream, q, dt, world_count, world_offsets, norm_sq_out, nullptr);
+    int32_t rc = residual_world_parts(c, stream, q, dt, world_count, world_offsets, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_world_norms, dim3(blocks_for(world_count, 256)), dim3(256), 0, (hipStream_t)stream,
+                       world_count, c->r_part, world_split(c, world_count), norm_sq_out);
+    VBD_LAUNCH_CHECK("k_world_norms");
+    return VBD_OK;
 }
 
 namespace {
@@ -3480,10 +3509,12 @@ int32_t build_live_list(VbdContext* c, hipStream_t s, const std::vector<int32_t>
 }
 }  // namespace
 
-// Freezes the worlds whose squared norm reached their target after `done` iterations (one
-// work-group; st = [active W | iterations W | live count]).  Frozen worlds stay frozen.  Once at
+// Adds each live world's norm slices (k_residual_worlds) into norm[w] and freezes the worlds whose
+// squared norm reached their target after `done` iterations (one work-group; st = [active W |
+// iterations W | live count]).  Frozen worlds stay frozen.  Once at
 // most stop_at_live (> 0) worlds are live, the rest are frozen too (the driver stops there).
-__global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __restrict__ norm,
+__global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __restrict__ part, int split,
+                                                      double* __restrict__ norm,
                                                       const double* __restrict__ target, int* __restrict__ st,
                                                       int done, int stop_at_live) {
     __shared__ int s_live[1024 / 64];
@@ -3491,6 +3522,7 @@ __global__ __launch_bounds__(1024) void k_world_freeze(int nw, const double* __r
     int live = 0;
     for (int w = threadIdx.x; w < nw; w += blockDim.x) {
         int a = st[w];
+        if (a) norm[w] = world_norm(part, split, w);  // a frozen world keeps its norm
         if (a && norm[w] <= target[w]) {
             a = 0;
             st[w] = 0;
@@ -3566,9 +3598,11 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
     }
     std::vector<int32_t> st_host((size_t)(2 * W + 1), 1), live_prev((size_t)W, 1);
     int32_t done = 0, live = W;
-    int32_t rc = residual_worlds(c, stream, q, dt, W, world_offsets, norm, world_state_dev);
+    const int split = world_split(c, W);
+    int32_t rc = residual_world_parts(c, stream, q, dt, W, world_offsets, world_state_dev);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, 0, stop_at_live);
+    hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, c->r_part, split, norm, target, world_state_dev,
+                       0, stop_at_live);
     VBD_LAUNCH_CHECK("k_world_freeze");
     const int32_t poll_every = kWorldPoll;
     int32_t since_poll = poll_every;  // poll before the first iteration
@@ -3602,9 +3636,10 @@ int32_t vbd_iterate_to_residual_worlds(VbdContext* c, void* stream, float* q, fl
             if (rc) return rc;
         }
         done += n;
-        rc = residual_worlds(c, stream, q, dt, W, world_offsets, norm, world_state_dev);
+        rc = residual_world_parts(c, stream, q, dt, W, world_offsets, world_state_dev);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, norm, target, world_state_dev, done,
+        hipLaunchKernelGGL(k_world_freeze, dim3(1), dim3(1024), 0, s, W, c->r_part, split, norm, target,
+                           world_state_dev, done,
                            stop_at_live);
         VBD_LAUNCH_CHECK("k_world_freeze");
         ++since_poll;
