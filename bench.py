@@ -1,6 +1,6 @@
 """Benchmark of the MI355X VBD solver: vertex-steps/s at a fixed force residual, HBM roofline, CPU baseline.
 
-Prints ONE JSON line.  Its ``value`` is the north-star metric: vertex-steps/s of the batched +
+Prints ONE JSON line.  Its ``value`` is the headline metric: vertex-steps/s of the batched +
 learned-warm-start solver at a fixed relative force residual, on the 128x128 wind cloth (top edge
 pinned, 8-mode Fourier wind; 16,384 vertices) run as 64 worlds with their own winds (1,048,576
 vertices).  One step = one substep (dt 1/600 s) of every world, each world iterated until its
