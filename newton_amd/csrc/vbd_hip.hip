@@ -2021,10 +2021,11 @@ __device__ __forceinline__ ContactGeo ef_load(const ColParams& C, int tid) {
 // ---- soft-contact generation from per-tile hit counts ------------------------
 // Records in pair order (a stable compaction of the hitting pairs; the reference's atomic counter
 // gives an arbitrary order), with no per-pair flag or index array: a tile of kColTile pairs is one
-// 256-lane work-group, four rounds of 256 consecutive pairs; a round's hits are ranked with a wave
-// ballot; one work-group scans the tile counts.
+// 1024-lane work-group, one pair per lane (an edge or face pair near a shape runs a few thousand
+// dependent SDF evaluations, so a lane must not carry several); hits are ranked with wave ballots
+// and per-wave sums; one work-group scans the tile counts.
 constexpr int kColTile = 1024;
-constexpr int kColRounds = kColTile / 256;
+constexpr int kColWaves = kColTile / 64;
 
 __device__ __forceinline__ int wave_rank(bool hit, int lane, int* total) {
     unsigned long long m = __ballot(hit ? 1 : 0);
@@ -2032,26 +2033,26 @@ __device__ __forceinline__ int wave_rank(bool hit, int lane, int* total) {
     return __popcll(m & ((1ull << lane) - 1ull));
 }
 
-__global__ __launch_bounds__(256) void k_collide_tile_count(ColParams C, const float* __restrict__ q, float margin,
-                                                            int* __restrict__ tile_count) {
-    __shared__ int wsum[4];
+__global__ __launch_bounds__(kColTile) void k_collide_tile_count(ColParams C, const float* __restrict__ q,
+                                                                 float margin, int* __restrict__ tile_count) {
+    __shared__ int wsum[kColWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int count = 0;
-    for (int r = 0; r < kColRounds; ++r) {
-        int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
-        bool hit = false;
-        if (tid < C.P + C.E + C.F) {
-            const ContactGeo g = soft_contact_geometry(C, q, margin, tid);
-            hit = g.hit;
-            if (C.ef_cache && tid >= C.P) ef_store(C, tid, g);
-        }
-        int t;
-        wave_rank(hit, lane, &t);
-        count += t;
+    const int tid = blockIdx.x * kColTile + threadIdx.x;
+    bool hit = false;
+    if (tid < C.P + C.E + C.F) {
+        const ContactGeo g = soft_contact_geometry(C, q, margin, tid);
+        hit = g.hit;
+        if (C.ef_cache && tid >= C.P) ef_store(C, tid, g);
     }
-    if (lane == 0) wsum[wave] = count;
+    int t;
+    wave_rank(hit, lane, &t);
+    if (lane == 0) wsum[wave] = t;
     __syncthreads();
-    if (threadIdx.x == 0) tile_count[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int w = 0; w < kColWaves; ++w) sum += wsum[w];
+        tile_count[blockIdx.x] = sum;
+    }
 }
 
 // Exclusive scan of the tile counts in place (one 1024-lane work-group, each lane a run of
@@ -2081,36 +2082,33 @@ __global__ __launch_bounds__(1024) void k_collide_tile_scan(int tiles, int* __re
     if (threadIdx.x == 1023) *count_out = part[1023];
 }
 
-__global__ __launch_bounds__(256) void k_collide_tile_write(ColParams C, const float* __restrict__ q, float margin,
-                                                            const int* __restrict__ tile_off, VbdContactsOut out) {
-    __shared__ int wsum[kColRounds][4];
+__global__ __launch_bounds__(kColTile) void k_collide_tile_write(ColParams C, const float* __restrict__ q,
+                                                                 float margin, const int* __restrict__ tile_off,
+                                                                 VbdContactsOut out) {
+    __shared__ int wsum[kColWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int run = tile_off[blockIdx.x];
-    for (int r = 0; r < kColRounds; ++r) {
-        int tid = blockIdx.x * kColTile + r * 256 + threadIdx.x;
-        ContactGeo g;
-        g.hit = false;
-        if (tid < C.P + C.E + C.F)
-            g = (C.ef_cache && tid >= C.P) ? ef_load(C, tid) : soft_contact_geometry(C, q, margin, tid);
-        int t;
-        int rank = wave_rank(g.hit, lane, &t);
-        if (lane == 0) wsum[r][wave] = t;
-        __syncthreads();
-        int before = 0;
-        for (int w = 0; w < wave; ++w) before += wsum[r][w];
-        int k = run + before + rank;
-        run += wsum[r][0] + wsum[r][1] + wsum[r][2] + wsum[r][3];
-        if (g.hit && k < out.soft_contact_max) {
-            out.soft_contact_shape[k] = g.shape;
-            st3(out.soft_contact_body_pos, k, g.body_pos);
-            st3(out.soft_contact_body_vel, k, g.body_vel);
-            if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
-            out.soft_contact_indices[3 * k] = g.i0;
-            out.soft_contact_indices[3 * k + 1] = g.i1;
-            out.soft_contact_indices[3 * k + 2] = g.i2;
-            if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, g.bary);
-            st3(out.soft_contact_normal, k, g.normal);
-        }
+    const int tid = blockIdx.x * kColTile + threadIdx.x;
+    ContactGeo g;
+    g.hit = false;
+    if (tid < C.P + C.E + C.F)
+        g = (C.ef_cache && tid >= C.P) ? ef_load(C, tid) : soft_contact_geometry(C, q, margin, tid);
+    int t;
+    const int rank = wave_rank(g.hit, lane, &t);
+    if (lane == 0) wsum[wave] = t;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    const int k = tile_off[blockIdx.x] + before + rank;
+    if (g.hit && k < out.soft_contact_max) {
+        out.soft_contact_shape[k] = g.shape;
+        st3(out.soft_contact_body_pos, k, g.body_pos);
+        st3(out.soft_contact_body_vel, k, g.body_vel);
+        if (out.soft_contact_particle) out.soft_contact_particle[k] = g.particle;
+        out.soft_contact_indices[3 * k] = g.i0;
+        out.soft_contact_indices[3 * k + 1] = g.i1;
+        out.soft_contact_indices[3 * k + 2] = g.i2;
+        if (out.soft_contact_barycentric) st3(out.soft_contact_barycentric, k, g.bary);
+        st3(out.soft_contact_normal, k, g.normal);
     }
 }
 
@@ -2690,11 +2688,11 @@ std::string noalloc_scratch(int32_t** out) {
 int32_t launch_collide_tiles(const ColParams& cp, hipStream_t s, const float* q, float margin, int* tiles_dev,
                              const VbdContactsOut& out) {
     const int tiles = blocks_for((long)cp.P + cp.E + cp.F, kColTile);
-    hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev);
+    hipLaunchKernelGGL(k_collide_tile_count, dim3(tiles), dim3(kColTile), 0, s, cp, q, margin, tiles_dev);
     VBD_LAUNCH_CHECK("k_collide_tile_count");
     hipLaunchKernelGGL(k_collide_tile_scan, dim3(1), dim3(1024), 0, s, tiles, tiles_dev, out.soft_contact_count);
     VBD_LAUNCH_CHECK("k_collide_tile_scan");
-    hipLaunchKernelGGL(k_collide_tile_write, dim3(tiles), dim3(256), 0, s, cp, q, margin, tiles_dev, out);
+    hipLaunchKernelGGL(k_collide_tile_write, dim3(tiles), dim3(kColTile), 0, s, cp, q, margin, tiles_dev, out);
     VBD_LAUNCH_CHECK("k_collide_tile_write");
     return VBD_OK;
 }
