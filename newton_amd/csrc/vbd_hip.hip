@@ -677,7 +677,7 @@ struct KParams {
     const float* c_body_pos;
     const float* c_body_vel;
     const float* c_normal;
-    // contacts packed per color position (k_contact_sort_pack)
+    // contacts packed per color position (k_contact_sort, k_contact_pack)
     const int* pos_coff;
     const float* cpack;
     const int* cpack_ef;  // per packed entry of an edge/face record: {c0, c1, c2, corner} {w0, w1, w2, 0}
@@ -1489,9 +1489,13 @@ __global__ void k_contact_place(const int* __restrict__ count_ptr, int cmax, con
     }
 }
 
-// Sorts each color position's entry keys ascending (contact-index order: the serial accumulation order)
-// and packs the entries there, so a batch's contacts are one contiguous, vertex-ordered range.  The body
-// point and its velocity are resolved here once per substep (the body poses are fixed during a step).
+// k_contact_sort sorts each color position's entry keys ascending (contact-index order: the serial
+// accumulation order; lists of up to kSortCols keys in the lane's LDS column) and records each entry's
+// position; k_contact_pack then packs every entry in place, so a batch's contacts are one contiguous,
+// vertex-ordered range.  The body point and its velocity are resolved there once per substep (the
+// body poses are fixed during a step).
+constexpr int kSortCols = 32;
+constexpr int kPackGrid = 1024;  // work-groups of the pack's grid-stride loop
 struct PackArgs {
     const float* pk;
     const float* mkd;
@@ -1509,14 +1513,30 @@ struct PackArgs {
     float dt;
 };
 
-__global__ void k_contact_sort_pack(int n, const int* __restrict__ color_particles, const int* __restrict__ vert_rec,
-                                    const int* __restrict__ pos_coff, int* __restrict__ ids, PackArgs A,
-                                    float* __restrict__ cpack, int* __restrict__ cpack_ef,
-                                    int* __restrict__ cid_to_pack) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_contact_sort(int n, const int* __restrict__ pos_coff, int* __restrict__ ids,
+                                                      int* __restrict__ ent_pos) {
+    __shared__ int s_ids[kSortCols * 256];  // element i of lane l at i * 256 + l
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int b = pos_coff[k], e = pos_coff[k + 1];
-    if (e == b) return;
+    for (int j = b; j < e; ++j) ent_pos[j] = k;
+    if (e - b <= 1) return;
+    if (e - b <= kSortCols) {
+        int* L = s_ids + threadIdx.x;
+        const int m = e - b;
+        for (int i = 0; i < m; ++i) L[i * 256] = ids[b + i];
+        for (int i = 1; i < m; ++i) {
+            const int key = L[i * 256];
+            int j = i - 1;
+            while (j >= 0 && L[j * 256] > key) {
+                L[(j + 1) * 256] = L[j * 256];
+                --j;
+            }
+            L[(j + 1) * 256] = key;
+        }
+        for (int i = 0; i < m; ++i) ids[b + i] = L[i * 256];
+        return;
+    }
     for (int i = b + 1; i < e; ++i) {
         int key = ids[i];
         int j = i - 1;
@@ -1526,9 +1546,21 @@ __global__ void k_contact_sort_pack(int n, const int* __restrict__ color_particl
         }
         ids[j + 1] = key;
     }
-    const int v = color_particles[k];
-    const int local = vert_rec[4 * k + 3] & 0xffff;
-    for (int j = b; j < e; ++j) {
+}
+
+// one packed record per entry (grid-stride over the device-sized entry count pos_coff[n]); an entry's
+// loads (contact, shape, body pose, corners) no longer wait on the entries before it at its position
+__global__ __launch_bounds__(256) void k_contact_pack(int n, const int* __restrict__ color_particles,
+                                                      const int* __restrict__ vert_rec,
+                                                      const int* __restrict__ pos_coff, const int* __restrict__ ids,
+                                                      const int* __restrict__ ent_pos, PackArgs A,
+                                                      float* __restrict__ cpack, int* __restrict__ cpack_ef,
+                                                      int* __restrict__ cid_to_pack) {
+    const int total = pos_coff[n];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        const int k = ent_pos[j];
+        const int v = color_particles[k];
+        const int local = vert_rec[4 * k + 3] & 0xffff;
         const int key = ids[j];
         const int cid = key >> 2, corner = key & 3;
         const int sh = A.c_shape[cid];
@@ -2140,6 +2172,7 @@ struct VbdContext {
     float* scratch3 = nullptr;  // q_prev | inertia | disp, each [N,3]
     int* c_cnt = nullptr;       // [N+1] contacts per color position; zero between substeps
     int* c_ids = nullptr;       // [entry_cap] entry keys (contact id << 2 | corner) in color-position order
+    int* c_ent_pos = nullptr;   // [entry_cap] color position of each entry (k_contact_sort, for the pack)
     float* c_state = nullptr;   // pk | mke | mkd | mmu, each [cap]
     int* pos_coff = nullptr;    // [N+1]
     float* cpack = nullptr;     // [entry_cap * 16]
@@ -2767,6 +2800,7 @@ int32_t vbd_create(const VbdModelDesc* m, const VbdSolverParams* prm, VbdContext
     if ((e = hipMalloc(&c->c_cnt, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->pos_coff, sizeof(int) * (size_t)(N + 1))) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_ids, sizeof(int) * c->entry_cap)) != hipSuccess) return cleanup("hipMalloc", e);
+    if ((e = hipMalloc(&c->c_ent_pos, sizeof(int) * c->entry_cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->cid_to_pack, sizeof(int) * 3 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->c_state, sizeof(float) * 4 * cap)) != hipSuccess) return cleanup("hipMalloc", e);
     if ((e = hipMalloc(&c->cpack, sizeof(float) * kContactWords * c->entry_cap)) != hipSuccess)
@@ -2866,6 +2900,7 @@ int32_t vbd_destroy(VbdContext* c) {
     hipFree(c->c_cnt);
     hipFree(c->pos_coff);
     hipFree(c->c_ids);
+    hipFree(c->c_ent_pos);
     hipFree(c->cid_to_pack);
     hipFree(c->c_state);
     hipFree(c->cpack);
@@ -3037,10 +3072,13 @@ int32_t vbd_contact_prepare(VbdContext* c, void* stream, const VbdContactsDesc* 
     pa.radius = c->model.particle_radius;
     pa.bodies = body_poses(c);
     pa.dt = cd->dt;
-    hipLaunchKernelGGL(k_contact_sort_pack, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->model.color_particles,
-                       c->lay_vert, c->pos_coff, c->c_ids, pa, c->cpack, c->cpack_ef,
+    hipLaunchKernelGGL(k_contact_sort, dim3(blocks_for(N, 256)), dim3(256), 0, s, N, c->pos_coff, c->c_ids,
+                       c->c_ent_pos);
+    VBD_LAUNCH_CHECK("k_contact_sort");
+    hipLaunchKernelGGL(k_contact_pack, dim3(kPackGrid), dim3(256), 0, s, N, c->model.color_particles, c->lay_vert,
+                       c->pos_coff, c->c_ids, c->c_ent_pos, pa, c->cpack, c->cpack_ef,
                        c->params.avbd_beta != 0.0f ? c->cid_to_pack : nullptr);
-    VBD_LAUNCH_CHECK("k_contact_sort_pack");
+    VBD_LAUNCH_CHECK("k_contact_pack");
     int32_t rc = launch_batch_contacts(c, s);
     if (rc) return rc;
     KParams& k = c->kp;
