@@ -82,7 +82,7 @@ struct SelfParams {
     int* ncoll;                  // [1]
 };
 
-constexpr int kSelfGrid = 256;  // work-groups of the self-contact grid-stride kernels (one per CU)
+constexpr int kSelfGrid = 2048;  // work-groups of the self-contact grid-stride kernels (a contact-rich pass has a few hundred thousand items: one per lane)
 
 // ---------------------------------------------------------------------------
 // geometry and forces, in the oracle's (reference) operation order
