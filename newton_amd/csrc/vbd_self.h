@@ -632,7 +632,7 @@ __device__ __forceinline__ bool sc_ee_test(const SelfParams& S, const float* __r
 // list (kQCap per query), counts and minima are reduced across the lanes, and the list's
 // `capacity` smallest ids are placed by rank (ids are distinct: a cell lists a primitive once).
 // A query with more than kQCap matches is redone by its first lane alone (sc_keep_smallest).
-constexpr int kQL = 8;
+constexpr int kQL = 16;
 constexpr int kQCap = 128;
 constexpr int kQPerBlock = 256 / kQL;
 
