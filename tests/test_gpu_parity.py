@@ -85,8 +85,11 @@ def test_generic_path_for_high_valence_vertex_bitexact(cuda_device):
     np.testing.assert_array_equal(qd_gpu, qd_ref)
 
 
-def test_serial_contact_mode_bitexact(cuda_device):
-    model, state, info = _many_planes_scene(cuda_device)
+@pytest.mark.parametrize("planes", [24, 40])
+def test_serial_contact_mode_bitexact(cuda_device, planes):
+    """Contacts that overflow the batch's LDS rows are summed serially; with 40 planes every particle
+    carries 40 entries (past the 32 its position's key sort keeps in LDS)."""
+    model, state, info = _many_planes_scene(cuda_device, planes)
     q0, qd0 = initial(state)
     q_gpu, qd_gpu, _ = gpu_run(model, state, info, 4)
     q_ref, qd_ref, _ = oracle_run(model, q0, qd0, info, 4)
